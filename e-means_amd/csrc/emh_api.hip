@@ -1,0 +1,362 @@
+// C ABI of the e-means nearest-centroid search (include/emeans_hip.h).
+//
+// Mirrors the reference's plugin traits Assigner / AssignRows<T>
+// (src/lib.rs:75-91) with the CUDA backend's behaviour (src/cuda.rs) as the
+// contract: set_centroids copies the table and computes norms, set_bias adds
+// to the scores until the next set_centroids, assign is synchronous and ties
+// go to the lowest index. Errors come back as codes plus a message instead of
+// the reference's panics.
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <new>
+
+#include "emh_assigner.h"
+
+namespace emh {
+
+static thread_local std::string g_last_error = "no error";
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+int check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return EMH_OK;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        return fail(EMH_E_MEMORY, std::string("not enough GPU memory: ") + what);
+    return fail(EMH_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static int op_of(int precision) {
+    switch (precision) {
+        case EMH_FP32: return OP_F32;
+        case EMH_BF16_RESCORE: return OP_BF16;
+        case EMH_FP8_RESCORE: return OP_FP8;
+        case EMH_I8: return OP_I8;
+        default: return -1;
+    }
+}
+
+static size_t row_bytes_of(int rows) { return rows == EMH_ROWS_F32 ? 4 : 1; }
+
+// Records a timing event on the stream when timing is on.
+static int mark(emh_assigner *h, hipEvent_t *slot) {
+    if (!h->timing) return EMH_OK;
+    *slot = h->timing_event();
+    if (!*slot) return fail(EMH_E_DEVICE, "timing event");
+    return check(hipEventRecord(*slot, h->stream), "hipEventRecord");
+}
+
+int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev, uint32_t *labels_dev,
+               float *dist_dev) {
+    if (m == 0) return EMH_OK;
+    const SearchGeom &g = h->g;
+    int rc;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if ((rc = mark(h, &ev[0]))) return rc;
+    if (h->op == OP_F32 || h->op == OP_I8) {
+        if ((rc = mark(h, &ev[1]))) return rc;
+        if ((rc = check(launch_search(g, 1, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
+                                      h->cn.as<int64_t>(), nullptr, nullptr, nullptr, labels_dev, dist_dev, nullptr,
+                                      h->stream),
+                        "search kernel")))
+            return rc;
+        if ((rc = mark(h, &ev[2]))) return rc;
+        return mark(h, &ev[3]);
+    }
+    if ((rc = check(h->cand.ensure(m * h->M * 4), "candidate buffer"))) return rc;
+    const uint8_t *x8 = nullptr;
+    const float *rs = nullptr;
+    if (h->op == OP_FP8) {
+        if ((rc = check(h->x8.ensure(m * g.d_pad), "fp8 row buffer"))) return rc;
+        if ((rc = check(h->row_scale.ensure(m * 4), "row scale buffer"))) return rc;
+        if ((rc = check(launch_rows_fp8(x_dev, rows, m, g.d, g.d_pad, ids_dev, h->x8.as<uint8_t>(),
+                                        h->row_scale.as<float>(), h->stream),
+                        "fp8 row encoding")))
+            return rc;
+        x8 = h->x8.as<uint8_t>();
+        rs = h->row_scale.as<float>();
+    }
+    if ((rc = mark(h, &ev[1]))) return rc;
+    if ((rc = check(launch_search(g, h->M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
+                                  h->cn.as<int64_t>(), x8, rs, h->cmax.as<unsigned>(), nullptr, nullptr,
+                                  h->cand.as<uint32_t>(), h->stream),
+                    "search kernel")))
+        return rc;
+    if ((rc = mark(h, &ev[2]))) return rc;
+    if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen_f32.as<float>(),
+                                   h->has_bias ? h->bias.as<float>() : nullptr, g.k, h->cand.as<uint32_t>(),
+                                   labels_dev, dist_dev, h->stream),
+                    "rescore kernel")))
+        return rc;
+    return mark(h, &ev[3]);
+}
+
+}  // namespace emh
+
+using namespace emh;
+
+hipEvent_t emh_assigner::timing_event() {
+    if (events_used == events.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        events.push_back(e);
+    }
+    return events[events_used++];
+}
+
+emh_assigner::~emh_assigner() {
+    for (DevBuf *b : {&cen_f32, &cen_op, &norm32, &norm64, &hn, &cn, &bias, &cmax, &x_stage, &labels_stage,
+                      &dist_stage, &cand, &x8, &row_scale})
+        b->release();
+    accum_free(acc);
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+extern "C" {
+
+const char *emh_last_error(void) { return g_last_error.c_str(); }
+
+int emh_create(int device, int precision, int rescore_m, emh_assigner **out) {
+    if (!out) return fail(EMH_E_ARG, "emh_create: out is NULL");
+    *out = nullptr;
+    if (precision == EMH_I16)
+        return fail(EMH_E_UNSUPPORTED, "I16 centroids are not offered by this build");
+    int op = op_of(precision);
+    if (op < 0) return fail(EMH_E_ARG, "emh_create: unknown precision");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0) return fail(EMH_E_DEVICE, "no HIP device available");
+    if (device < 0 || device >= count) return fail(EMH_E_ARG, "emh_create: device ordinal out of range");
+    int rc;
+    if ((rc = check(hipSetDevice(device), "hipSetDevice"))) return rc;
+    hipDeviceProp_t prop;
+    if ((rc = check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties"))) return rc;
+    emh_assigner *h = new (std::nothrow) emh_assigner();
+    if (!h) return fail(EMH_E_MEMORY, "host allocation failed");
+    h->device = device;
+    h->precision = precision;
+    h->op = op;
+    if (op == OP_BF16 || op == OP_FP8) {
+        int m = rescore_m <= 0 ? 4 : rescore_m;
+        if (m > MAX_M) m = MAX_M;
+        int p = 1;
+        while (p < m) p <<= 1;
+        h->M = p;
+    }
+    h->device_name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    if ((rc = check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate"))) {
+        delete h;
+        return rc;
+    }
+    h->own_stream = true;
+    *out = h;
+    return EMH_OK;
+}
+
+void emh_destroy(emh_assigner *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+}
+
+int emh_set_stream(emh_assigner *h, void *stream) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_stream: NULL handle");
+    if (h->own_stream && h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
+    h->own_stream = false;
+    h->stream = (hipStream_t)stream;
+    return EMH_OK;
+}
+
+int emh_synchronize(emh_assigner *h) {
+    if (!h) return fail(EMH_E_ARG, "emh_synchronize: NULL handle");
+    return check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+int emh_set_timing(emh_assigner *h, int on) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_timing: NULL handle");
+    int rc;
+    if ((rc = check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"))) return rc;
+    h->timing = on != 0;
+    h->events_used = 0;
+    return EMH_OK;
+}
+
+int emh_get_timing(emh_assigner *h, double *ms_out, uint64_t *calls) {
+    if (!h || !ms_out || !calls) return fail(EMH_E_ARG, "emh_get_timing: NULL argument");
+    int rc;
+    if ((rc = check(hipStreamSynchronize(h->stream), "hipStreamSynchronize"))) return rc;
+    ms_out[0] = ms_out[1] = ms_out[2] = 0.0;
+    const size_t n = h->events_used / 4;
+    for (size_t c = 0; c < n; c++) {
+        for (int s = 0; s < 3; s++) {
+            float ms = 0.0f;
+            if ((rc = check(hipEventElapsedTime(&ms, h->events[4 * c + s], h->events[4 * c + s + 1]),
+                            "hipEventElapsedTime")))
+                return rc;
+            ms_out[s] += ms;
+        }
+    }
+    *calls = n;
+    h->events_used = 0;
+    return EMH_OK;
+}
+
+int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t d) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_centroids: NULL handle");
+    if (k == 0 || d == 0 || !centroids) return fail(EMH_E_ARG, "set_centroids: need k > 0, d > 0 and a k x d table");
+    if (h->op == OP_I8 && d > 32768) return fail(EMH_E_ARG, "integer assignment supports d <= 32768");
+    if (k > 0xFFFFFFFEull) return fail(EMH_E_ARG, "set_centroids: k must fit in 32 bits");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    SearchGeom g = make_geom(h->op, k, d);
+    const size_t es = op_elem_bytes(h->op);
+    if ((rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
+    if ((rc = check(h->cen_op.ensure(g.k_pad * g.d_pad * es), "operand buffer"))) return rc;
+    if ((rc = check(h->norm32.ensure(g.k_pad * 4), "norm buffer"))) return rc;
+    if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
+    if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
+    if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
+    if ((rc = check(h->cmax.ensure(4), "scale buffer"))) return rc;
+    h->g = g;
+    h->has_centroids = false;
+    h->has_bias = false;
+    if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4, hipMemcpyHostToDevice, h->stream), "copy centroids")))
+        return rc;
+    if (h->op == OP_FP8) {
+        if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
+        if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
+            return rc;
+    }
+    if ((rc = check(launch_prep_centroids(g, h->cen_f32.as<float>(), h->cen_op.p, h->norm32.as<float>(),
+                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stream),
+                    "centroid prep")))
+        return rc;
+    if ((rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr, h->hn.as<float>(),
+                                    h->cn.as<int64_t>(), h->stream),
+                    "score offsets")))
+        return rc;
+    // The caller may reuse `centroids` on return (pageable source).
+    if ((rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;
+    h->has_centroids = true;
+    return EMH_OK;
+}
+
+int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_bias: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (!bias || k != h->g.k) return fail(EMH_E_ARG, "set_bias: bias must have k entries");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    if ((rc = check(h->bias.ensure(k * 4), "bias buffer"))) return rc;
+    if ((rc = check(hipMemcpyAsync(h->bias.p, bias, k * 4, hipMemcpyHostToDevice, h->stream), "copy bias"))) return rc;
+    if ((rc = check(launch_set_bias(h->g, h->norm32.as<float>(), h->norm64.as<int64_t>(), h->bias.as<float>(),
+                                    h->hn.as<float>(), h->cn.as<int64_t>(), h->stream),
+                    "score offsets")))
+        return rc;
+    if ((rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;
+    h->has_bias = true;
+    return EMH_OK;
+}
+
+int emh_name(const emh_assigner *h, char *buf, size_t cap) {
+    if (!h || !buf || cap == 0) return fail(EMH_E_ARG, "emh_name: bad arguments");
+    static const char *kind[] = {"FP32 fused MFMA argmin", "BF16 MFMA + FP32 rescore", "FP8-e4m3 MFMA + FP32 rescore",
+                                 "I8 MFMA exact", "I16"};
+    char tmp[512];
+    int n;
+    if (h->op == OP_BF16 || h->op == OP_FP8)
+        n = snprintf(tmp, sizeof tmp, "HIP %s (top-%d), %s", kind[h->precision], h->M, h->device_name.c_str());
+    else
+        n = snprintf(tmp, sizeof tmp, "HIP %s, %s", kind[h->precision], h->device_name.c_str());
+    if (h->has_centroids)
+        snprintf(tmp + n, sizeof tmp - n, ", k=%zu d=%zu (%s rows)", h->g.k, h->g.d,
+                 h->g.resident ? "LDS-resident" : "chunked");
+    strncpy(buf, tmp, cap - 1);
+    buf[cap - 1] = 0;
+    return EMH_OK;
+}
+
+static int assign_host(emh_assigner *h, const void *x, int rows, size_t m, uint32_t *labels, float *dist) {
+    if (!h) return fail(EMH_E_ARG, "assign: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (m == 0) return EMH_OK;
+    if (!x || !labels || !dist) return fail(EMH_E_ARG, "assign: NULL buffer");
+    if ((h->op == OP_I8) && rows != EMH_ROWS_I8)
+        return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    const size_t d = h->g.d, rb = row_bytes_of(rows);
+    // Rows per upload: up to 1 Mi rows or 1 GiB of staging, whichever is smaller.
+    size_t cap = std::min<size_t>(m, std::max<size_t>(1, std::min<size_t>(1u << 20, (1ull << 30) / (d * rb))));
+    if ((rc = check(h->x_stage.ensure(cap * d * rb), "row staging"))) return rc;
+    if ((rc = check(h->labels_stage.ensure(cap * 4), "label staging"))) return rc;
+    if ((rc = check(h->dist_stage.ensure(cap * 4), "distance staging"))) return rc;
+    for (size_t r0 = 0; r0 < m; r0 += cap) {
+        size_t mc = std::min(cap, m - r0);
+        if ((rc = check(hipMemcpyAsync(h->x_stage.p, (const char *)x + r0 * d * rb, mc * d * rb, hipMemcpyHostToDevice,
+                                       h->stream),
+                        "copy rows")))
+            return rc;
+        if ((rc = run_search(h, h->x_stage.p, rows, mc, nullptr, h->labels_stage.as<uint32_t>(),
+                             h->dist_stage.as<float>())))
+            return rc;
+        if ((rc = check(hipMemcpyAsync(labels + r0, h->labels_stage.p, mc * 4, hipMemcpyDeviceToHost, h->stream),
+                        "copy labels")))
+            return rc;
+        if ((rc = check(hipMemcpyAsync(dist + r0, h->dist_stage.p, mc * 4, hipMemcpyDeviceToHost, h->stream),
+                        "copy distances")))
+            return rc;
+        if ((rc = check(hipStreamSynchronize(h->stream), "assign"))) return rc;
+    }
+    return EMH_OK;
+}
+
+int emh_assign_f32(emh_assigner *h, const float *x, size_t m, uint32_t *labels, float *dist) {
+    return assign_host(h, x, EMH_ROWS_F32, m, labels, dist);
+}
+int emh_assign_i8(emh_assigner *h, const int8_t *x, size_t m, uint32_t *labels, float *dist) {
+    return assign_host(h, x, EMH_ROWS_I8, m, labels, dist);
+}
+int emh_assign_u8(emh_assigner *h, const uint8_t *x, size_t m, uint32_t *labels, float *dist) {
+    return assign_host(h, x, EMH_ROWS_U8, m, labels, dist);
+}
+
+int emh_assign_device(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                      uint32_t *labels_dev, float *dist_dev) {
+    if (!h) return fail(EMH_E_ARG, "assign: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (rows < EMH_ROWS_F32 || rows > EMH_ROWS_U8) return fail(EMH_E_ARG, "assign: unknown row type");
+    if (m == 0) return EMH_OK;
+    if (!x_dev || !labels_dev || !dist_dev) return fail(EMH_E_ARG, "assign: NULL buffer");
+    if (h->op == OP_I8 && rows != EMH_ROWS_I8)
+        return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    return run_search(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev);
+}
+
+int emh_accumulate_device(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                          const uint32_t *labels_dev, float *sums_dev, uint32_t *counts_dev) {
+    if (!h) return fail(EMH_E_ARG, "accumulate: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (rows < EMH_ROWS_F32 || rows > EMH_ROWS_U8) return fail(EMH_E_ARG, "accumulate: unknown row type");
+    if (!sums_dev || !counts_dev || (m && (!x_dev || !labels_dev))) return fail(EMH_E_ARG, "accumulate: NULL buffer");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    return check(launch_accumulate(h->acc, x_dev, rows, m, h->g.d, ids_dev, labels_dev, h->g.k, sums_dev, counts_dev,
+                                   h->stream),
+                 "accumulate");
+}
+
+}  // extern "C"

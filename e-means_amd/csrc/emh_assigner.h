@@ -1,0 +1,70 @@
+// The assigner object behind the C ABI handle (shared by the API and the
+// training driver).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "emh_internal.h"
+
+namespace emh {
+
+// Grow-only device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes) {
+        if (bytes <= cap && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(&p, bytes ? bytes : 16);
+        if (e == hipSuccess) cap = bytes;
+        return e;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <class T> T *as() const { return (T *)p; }
+};
+
+}  // namespace emh
+
+struct emh_assigner {
+    int device = 0;
+    int precision = EMH_FP32;
+    int op = emh::OP_F32;
+    int M = 1;  // candidates per row (1 = plain argmin)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string device_name;
+
+    emh::SearchGeom g{};
+    bool has_centroids = false;
+    bool has_bias = false;
+
+    // centroid side (set_centroids / set_bias)
+    emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax;
+    // row side (per assign call)
+    emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale;
+    // per-cluster accumulation
+    emh::AccumScratch acc;
+
+    // optional per-kernel timing: 4 events per search call (row prep, search, rescore)
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    size_t events_used = 0;
+    hipEvent_t timing_event();
+
+    ~emh_assigner();
+};
+
+namespace emh {
+// Search over device rows (x_dev of `rows` type, optionally gathered by ids).
+int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev, uint32_t *labels_dev,
+               float *dist_dev);
+int fail(int code, const std::string &msg);
+int check(hipError_t e, const char *what);
+}  // namespace emh

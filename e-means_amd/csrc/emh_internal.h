@@ -1,0 +1,80 @@
+// Internal declarations shared by the e-means HIP library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include <string>
+
+#include "../../include/emeans_hip.h"
+
+namespace emh {
+
+enum Op { OP_F32 = 0, OP_BF16 = 1, OP_FP8 = 2, OP_I8 = 3 };
+
+// Workgroup geometry of the search kernels: 4 waves, each owning 32 rows, and
+// centroid tiles of BN centroids (two 32x32 MFMA blocks per wave).
+constexpr int WAVES = 4;
+constexpr int BM = 32 * WAVES;  // rows per workgroup
+constexpr int BN = 64;          // centroids per tile
+constexpr int CHUNK_BYTES = 128;  // d-chunk per row when rows are not resident
+constexpr int ROW_LDS_BUDGET = 64 * 1024;  // bytes of LDS for a resident row block
+constexpr int MAX_M = 8;        // top-m candidates kept by the rescoring searches
+
+inline int op_elem_bytes(int op) { return op == OP_F32 ? 4 : (op == OP_BF16 ? 2 : 1); }
+// Bytes one lane feeds one MFMA, times the two lane halves.
+inline int op_granule(int op) {
+    int g = op == OP_F32 ? 8 : (op == OP_BF16 ? 32 : (op == OP_FP8 ? 64 : 32));
+    return g < 16 ? 16 : g;
+}
+
+struct SearchGeom {
+    int op;
+    bool resident;      // the workgroup's rows stay in LDS for the whole centroid sweep
+    size_t d, d_pad;    // elements; d_pad * elem_bytes is the operand row stride
+    size_t k, k_pad;
+    int row_bytes;      // bytes of one LDS row segment (d_pad*es when resident, else CHUNK_BYTES)
+};
+
+SearchGeom make_geom(int op, size_t k, size_t d);
+
+// Launchers (stream-ordered, no synchronisation).
+hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
+                                 const unsigned *cmax_bits, hipStream_t s);
+hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s);
+hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
+                           float *hn, int64_t *cn, hipStream_t s);
+hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
+                           uint8_t *x8, float *row_scale, hipStream_t s);
+hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
+                         const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
+                         const float *row_scale, const unsigned *cmax_bits, uint32_t *labels, float *dist,
+                         uint32_t *cand, hipStream_t s);
+hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,
+                          const float *bias, size_t k, const uint32_t *cand, uint32_t *labels, float *dist,
+                          hipStream_t s);
+
+// Device-side per-cluster accumulation (emh_accum.hip).
+struct AccumScratch {
+    uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
+    uint32_t *offsets = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    size_t cap_rows = 0, cap_k = 0;
+};
+hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k);
+void accum_free(AccumScratch &a);
+hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
+                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s);
+hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, size_t k, size_t d, double *pass_sums,
+                                  uint64_t *pass_counts, hipStream_t s);
+hipError_t launch_closing_mean(const double *pass_sums, const uint64_t *pass_counts, size_t k, size_t d,
+                               float *centroids, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s);
+hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
+
+// Error reporting shared by the C ABI.
+void set_error(const std::string &msg);
+
+}  // namespace emh

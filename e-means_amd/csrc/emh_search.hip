@@ -1,0 +1,704 @@
+// Nearest-centroid search kernels for gfx950 (MI355X).
+//
+// Replaces the reference's per-chunk cuBLAS sgemm + argmin kernel pair
+// (src/cuda.rs:20-48, 174-211) and its CPU searches (src/cpu.rs:108-167,
+// src/cpu_i8.rs:130-191) with one fused pass: a workgroup keeps a block of
+// BM rows resident in LDS, streams centroid tiles through LDS, computes the
+// BM x BN inner products on 32x32 MFMA tiles and folds them straight into a
+// per-row running argmin (or top-m), so no n x k matrix is ever written.
+//
+// MFMA layout used throughout (32x32xK): lane l feeds A row i = l % 32 and
+// B column j = l % 32 with the K elements of lane half h = l / 32; the output
+// register r of lane l holds D[8*(r/4) + 4*h + r%4][l % 32]. Centroids are
+// the A operand and rows the B operand, so every lane owns one row and sees
+// 16 centroids of each 32x32 tile in ascending index order, which gives the
+// reference's lowest-index tie rule with a strict '<' (src/cpu.rs:149).
+#include <float.h>
+#include <math.h>
+
+#include <algorithm>
+#include <type_traits>
+
+#include "emh_internal.h"
+
+namespace emh {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef int i32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+
+constexpr uint32_t NO_LABEL = 0xFFFFFFFFu;
+constexpr int64_t PAD_SCORE64 = INT64_MAX / 4;
+
+template <int OP> struct OpT;
+template <> struct OpT<OP_F32> { static constexpr int ES = 4, LB = 4; };
+template <> struct OpT<OP_BF16> { static constexpr int ES = 2, LB = 16; };
+template <> struct OpT<OP_FP8> { static constexpr int ES = 1, LB = 32; };
+template <> struct OpT<OP_I8> { static constexpr int ES = 1, LB = 16; };
+
+SearchGeom make_geom(int op, size_t k, size_t d) {
+    SearchGeom g;
+    g.op = op;
+    g.d = d;
+    g.k = k;
+    g.k_pad = (k + BN - 1) / BN * BN;
+    const size_t es = op_elem_bytes(op);
+    const size_t gran = op_granule(op);
+    size_t res_bytes = (d * es + gran - 1) / gran * gran;
+    g.resident = (size_t)BM * (res_bytes + 16) <= (size_t)ROW_LDS_BUDGET;
+    if (g.resident) {
+        g.d_pad = res_bytes / es;
+        g.row_bytes = (int)res_bytes;
+    } else {
+        g.d_pad = (d * es + CHUNK_BYTES - 1) / CHUNK_BYTES * CHUNK_BYTES / es;
+        g.row_bytes = CHUNK_BYTES;
+    }
+    return g;
+}
+
+// ---------------------------------------------------------------- helpers --
+
+__device__ inline uint16_t bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+template <int ROWS> __device__ inline float src_elem(const void *x, size_t idx) {
+    if constexpr (ROWS == EMH_ROWS_F32) return ((const float *)x)[idx];
+    else if constexpr (ROWS == EMH_ROWS_I8) return (float)((const int8_t *)x)[idx];
+    else return (float)((const uint8_t *)x)[idx];
+}
+
+// OCP e4m3 power-of-two scale mapping |v| <= amax into [0, 448].
+__device__ inline float fp8_scale_for(float amax) {
+    if (!(amax > 0.0f) || !isfinite(amax)) return 1.0f;
+    return exp2f(floorf(log2f(448.0f / amax)));
+}
+
+struct SearchArgs {
+    const void *x;
+    const uint32_t *ids;
+    size_t m, d, d_pad, k, k_pad;
+    int row_bytes;
+    const uint8_t *cen_op;
+    const float *hn;
+    const int64_t *cn;
+    const uint8_t *x8;
+    const float *row_scale;
+    const unsigned *cmax_bits;
+    uint32_t *labels;
+    float *dist;
+    uint32_t *cand;
+};
+
+// One 16-byte LDS segment of a row, converted to the operand type.
+// e0: first element of the segment; valid: the row exists.
+template <int OP, int ROWS>
+__device__ inline uint4 row_segment(const SearchArgs &a, size_t call_row, size_t src_row, size_t e0, bool valid) {
+    uint4 out = make_uint4(0, 0, 0, 0);
+    if (!valid) return out;
+    if constexpr (OP == OP_FP8) {
+        // pre-scaled e4m3 rows, already padded to d_pad
+        return *(const uint4 *)(a.x8 + call_row * a.d_pad + e0);
+    } else if constexpr (OP == OP_I8) {
+        const int8_t *p = (const int8_t *)a.x + src_row * a.d;
+        uint32_t w[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                size_t e = e0 + 4 * q + b;
+                uint32_t byte = e < a.d ? (uint8_t)p[e] : 0u;
+                v |= byte << (8 * b);
+            }
+            w[q] = v;
+        }
+        return make_uint4(w[0], w[1], w[2], w[3]);
+    } else if constexpr (OP == OP_F32) {
+        float v[4];
+        const size_t base = src_row * a.d;
+#pragma unroll
+        for (int q = 0; q < 4; q++) v[q] = (e0 + q < a.d) ? src_elem<ROWS>(a.x, base + e0 + q) : 0.0f;
+        return make_uint4(__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]), __float_as_uint(v[3]));
+    } else {  // OP_BF16
+        uint32_t w[4];
+        const size_t base = src_row * a.d;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            size_t e = e0 + 2 * q;
+            float lo = e < a.d ? src_elem<ROWS>(a.x, base + e) : 0.0f;
+            float hi = e + 1 < a.d ? src_elem<ROWS>(a.x, base + e + 1) : 0.0f;
+            w[q] = (uint32_t)bf16_rne(lo) | ((uint32_t)bf16_rne(hi) << 16);
+        }
+        return make_uint4(w[0], w[1], w[2], w[3]);
+    }
+}
+
+// Insert (v, vi) into the sorted list (bs, bi); strict '<' keeps the earlier
+// (lower-index) entry first among equal scores.
+template <int M, typename S>
+__device__ inline void topm_insert(S (&bs)[M], uint32_t (&bi)[M], S v, uint32_t vi) {
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        bool lt = v < bs[t];
+        S ts = bs[t];
+        uint32_t ti = bi[t];
+        bs[t] = lt ? v : ts;
+        bi[t] = lt ? vi : ti;
+        v = lt ? ts : v;
+        vi = lt ? ti : vi;
+    }
+}
+
+// Merge with (v, vi) from another lane: order by (score, index).
+template <int M, typename S>
+__device__ inline void topm_insert_lex(S (&bs)[M], uint32_t (&bi)[M], S v, uint32_t vi) {
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        bool lt = v < bs[t] || (v == bs[t] && vi < bi[t]);
+        S ts = bs[t];
+        uint32_t ti = bi[t];
+        bs[t] = lt ? v : ts;
+        bi[t] = lt ? vi : ti;
+        v = lt ? ts : v;
+        vi = lt ? ti : vi;
+    }
+}
+
+__device__ inline int64_t shfl_xor_i64(int64_t v, int mask) {
+    int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
+    int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), mask, 64);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// --------------------------------------------------------------- search ----
+
+template <int OP, int ROWS, int M, bool RES>
+__global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
+    using T = OpT<OP>;
+    constexpr int ES = T::ES, LB = T::LB;
+    using Score = typename std::conditional<OP == OP_I8, int64_t, float>::type;
+    extern __shared__ __align__(16) uint8_t smem[];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, j = lane & 31;
+    const size_t row0 = (size_t)blockIdx.x * BM;
+    const int RB = a.row_bytes, RS = RB + 16;
+    uint8_t *sR = smem;
+    uint8_t *sC = smem + BM * RS;
+    float *sHn = (float *)(sC + BN * RS);
+    int64_t *sCn = (int64_t *)(sC + BN * RS);
+    const int segs = RB / 16;
+    const int nchunks = RES ? 1 : (int)((a.d_pad * ES) / RB);
+    const size_t my_row = row0 + wave * 32 + j;
+
+    auto load_rows = [&](int chunk) {
+        for (int q = tid; q < BM * segs; q += 256) {
+            int r = q / segs, sg = q - r * segs;
+            size_t grow = row0 + r;
+            bool valid = grow < a.m;
+            size_t src = valid ? (a.ids ? (size_t)a.ids[grow] : grow) : 0;
+            size_t e0 = ((size_t)chunk * RB + sg * 16) / ES;
+            *(uint4 *)(sR + r * RS + sg * 16) = row_segment<OP, ROWS>(a, grow, src, e0, valid);
+        }
+    };
+    auto load_cents = [&](size_t c0, int chunk) {
+        const size_t stride = a.d_pad * ES;
+        for (int q = tid; q < BN * segs; q += 256) {
+            int r = q / segs, sg = q - r * segs;
+            const uint8_t *src = a.cen_op + (c0 + r) * stride + (size_t)chunk * RB + sg * 16;
+            *(uint4 *)(sC + r * RS + sg * 16) = *(const uint4 *)src;
+        }
+        if (tid < BN) {
+            if constexpr (OP == OP_I8) sCn[tid] = a.cn[c0 + tid];
+            else sHn[tid] = a.hn[c0 + tid];
+        }
+    };
+
+    // Per-lane running top-m over this lane's centroid subset.
+    Score bs[M];
+    uint32_t bi[M];
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        if constexpr (OP == OP_I8) bs[t] = INT64_MAX;
+        else bs[t] = INFINITY;
+        bi[t] = NO_LABEL;
+    }
+    // Squared row norm (f32 for FP32, exact int for I8), accumulated from LDS.
+    float xn32 = 0.0f;
+    int32_t xn_i = 0;
+    float srs = 1.0f;
+    if constexpr (OP == OP_FP8) {
+        float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
+        srs = (my_row < a.m ? a.row_scale[my_row] : 1.0f) * cs;
+    }
+    auto norm_part = [&]() {
+        const uint8_t *p = sR + (wave * 32 + j) * RS + h * (RB / 2);
+        for (int b = 0; b < RB / 2; b += 16) {
+            uint4 v = *(const uint4 *)(p + b);
+            if constexpr (OP == OP_F32) {
+                float f[4] = {__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
+#pragma unroll
+                for (int q = 0; q < 4; q++) xn32 = __fadd_rn(xn32, __fmul_rn(f[q], f[q]));
+            } else if constexpr (OP == OP_I8) {
+                uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+#pragma unroll
+                    for (int bb = 0; bb < 4; bb++) {
+                        int e = (int)(int8_t)((w[q] >> (8 * bb)) & 0xff);
+                        xn_i += e * e;
+                    }
+            }
+        }
+    };
+
+    if constexpr (RES) {
+        load_rows(0);
+        __syncthreads();
+        if constexpr (OP == OP_F32 || OP == OP_I8) norm_part();
+    }
+
+    for (size_t c0 = 0; c0 < a.k_pad; c0 += BN) {
+        using Acc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
+        Acc acc0 = {}, acc1 = {};
+        for (int ch = 0; ch < nchunks; ch++) {
+            if constexpr (!RES) load_rows(ch);
+            load_cents(c0, ch);
+            __syncthreads();
+            if constexpr (!RES) {
+                if (c0 == 0) {
+                    if constexpr (OP == OP_F32 || OP == OP_I8) norm_part();
+                }
+            }
+            const uint8_t *pb = sR + (wave * 32 + j) * RS + h * (RB / 2);
+            const uint8_t *pa0 = sC + j * RS + h * (RB / 2);
+            const uint8_t *pa1 = sC + (32 + j) * RS + h * (RB / 2);
+            if constexpr (OP == OP_F32) {
+                for (int b = 0; b < RB / 2; b += 16) {
+                    float4 xb = *(const float4 *)(pb + b);
+                    float4 c0v = *(const float4 *)(pa0 + b);
+                    float4 c1v = *(const float4 *)(pa1 + b);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(c0v.x, xb.x, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(c1v.x, xb.x, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(c0v.y, xb.y, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(c1v.y, xb.y, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(c0v.z, xb.z, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(c1v.z, xb.z, acc1, 0, 0, 0);
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(c0v.w, xb.w, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(c1v.w, xb.w, acc1, 0, 0, 0);
+                }
+            } else if constexpr (OP == OP_BF16) {
+                for (int b = 0; b < RB / 2; b += LB) {
+                    bf16x8 xb = __builtin_bit_cast(bf16x8, *(const uint4 *)(pb + b));
+                    bf16x8 ca = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa0 + b));
+                    bf16x8 cb = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa1 + b));
+                    acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ca, xb, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(cb, xb, acc1, 0, 0, 0);
+                }
+            } else if constexpr (OP == OP_I8) {
+                for (int b = 0; b < RB / 2; b += LB) {
+                    i32x4 xb = __builtin_bit_cast(i32x4, *(const uint4 *)(pb + b));
+                    i32x4 ca = __builtin_bit_cast(i32x4, *(const uint4 *)(pa0 + b));
+                    i32x4 cb = __builtin_bit_cast(i32x4, *(const uint4 *)(pa1 + b));
+                    acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca, xb, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cb, xb, acc1, 0, 0, 0);
+                }
+            } else {  // OP_FP8
+                for (int b = 0; b < RB / 2; b += LB) {
+                    uint4 x0 = *(const uint4 *)(pb + b), x1 = *(const uint4 *)(pb + b + 16);
+                    uint4 a0 = *(const uint4 *)(pa0 + b), a1 = *(const uint4 *)(pa0 + b + 16);
+                    uint4 b0 = *(const uint4 *)(pa1 + b), b1 = *(const uint4 *)(pa1 + b + 16);
+                    i32x8 xb = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+                    i32x8 ca = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                    i32x8 cb = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+                    acc0 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ca, xb, acc0, 0, 0, 0, 127, 0, 127);
+                    acc1 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cb, xb, acc1, 0, 0, 0, 127, 0, 127);
+                }
+            }
+            if (ch == nchunks - 1) {
+                // Epilogue: fold the 2 x 16 scores of this lane into its top-m,
+                // in ascending centroid index.
+#pragma unroll
+                for (int nb = 0; nb < 2; nb++) {
+#pragma unroll
+                    for (int r = 0; r < 16; r++) {
+                        const int il = nb * 32 + 8 * (r / 4) + 4 * h + (r % 4);
+                        const uint32_t ci = (uint32_t)(c0 + il);
+                        Score s;
+                        if constexpr (OP == OP_I8) {
+                            int32_t dv = nb == 0 ? acc0[r] : acc1[r];
+                            s = sCn[il] - 2 * (int64_t)dv;
+                        } else if constexpr (OP == OP_FP8) {
+                            float dv = nb == 0 ? acc0[r] : acc1[r];
+                            s = fmaf(srs, sHn[il], -dv);
+                        } else {
+                            float dv = nb == 0 ? acc0[r] : acc1[r];
+                            s = sHn[il] - dv;
+                        }
+                        if constexpr (M == 1) {
+                            if (s < bs[0]) {
+                                bs[0] = s;
+                                bi[0] = ci;
+                            }
+                        } else {
+                            if (s < bs[M - 1]) topm_insert<M, Score>(bs, bi, s, ci);
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    // Merge the two lane halves that hold the same row.
+#pragma unroll
+    for (int t = 0; t < M; t++) {
+        Score ps;
+        if constexpr (OP == OP_I8) ps = shfl_xor_i64(bs[t], 32);
+        else ps = __shfl_xor(bs[t], 32, 64);
+        uint32_t pi = (uint32_t)__shfl_xor((int)bi[t], 32, 64);
+        if (h == 0) topm_insert_lex<M, Score>(bs, bi, ps, pi);
+    }
+    if constexpr (OP == OP_F32) xn32 = __fadd_rn(xn32, __shfl_xor(xn32, 32, 64));
+    if constexpr (OP == OP_I8) xn_i += __shfl_xor(xn_i, 32, 64);
+
+    if (h == 0 && my_row < a.m) {
+        if constexpr (OP == OP_F32 && M == 1) {
+            // dist = |x|^2 + |c|^2 + bias - 2 x.c, clamped at 0 (src/cpu.rs:160-162)
+            float v = __fadd_rn(__fmul_rn(2.0f, bs[0]), xn32);
+            a.labels[my_row] = bi[0] == NO_LABEL ? 0u : bi[0];
+            a.dist[my_row] = v > 0.0f ? v : 0.0f;
+        } else if constexpr (OP == OP_I8) {
+            // dist = ((|x|^2 + best) as f64).max(0) as f32 (src/cpu_i8.rs:183-186)
+            double v = __dadd_rn((double)(int64_t)xn_i, (double)bs[0]);
+            a.labels[my_row] = bi[0] == NO_LABEL ? 0u : bi[0];
+            a.dist[my_row] = (float)(v > 0.0 ? v : 0.0);
+        } else {
+#pragma unroll
+            for (int t = 0; t < M; t++) a.cand[my_row * M + t] = bi[t];
+        }
+    }
+}
+
+// -------------------------------------------------------------- rescore ----
+
+// Exact FP32 distances to the m candidates of each row; lowest (score, index)
+// wins. dist = max(|x - c|^2 + bias, 0), summed in index order without FMA.
+template <int ROWS, int M>
+__global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, const uint32_t *ids,
+                                                      const float *c, const float *bias, const uint32_t *cand,
+                                                      uint32_t *labels, float *dist) {
+    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t row = g / M;
+    const int slot = (int)(g % M);
+    float s = INFINITY;
+    uint32_t ci = NO_LABEL;
+    if (row < m) {
+        ci = cand[row * M + slot];
+        if (ci != NO_LABEL) {
+            const size_t src = ids ? (size_t)ids[row] : row;
+            const float *cp = c + (size_t)ci * d;
+            float acc = 0.0f;
+            for (size_t e = 0; e < d; e++) {
+                float df = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
+                acc = __fadd_rn(acc, __fmul_rn(df, df));
+            }
+            if (bias) acc = __fadd_rn(acc, bias[ci]);
+            s = acc;
+        }
+    }
+#pragma unroll
+    for (int off = 1; off < M; off <<= 1) {
+        float os = __shfl_xor(s, off, 64);
+        uint32_t oi = (uint32_t)__shfl_xor((int)ci, off, 64);
+        if (os < s || (os == s && oi < ci)) {
+            s = os;
+            ci = oi;
+        }
+    }
+    if (slot == 0 && row < m) {
+        labels[row] = ci == NO_LABEL ? 0u : ci;
+        dist[row] = (ci != NO_LABEL && s > 0.0f) ? s : 0.0f;
+    }
+}
+
+// ---------------------------------------------------------- centroid prep --
+
+// One wave per centroid: operand row (zero padded), and the f32 squared norm
+// folded as src/cpu.rs:170-180 does (16 strided partial sums, then the tail),
+// or the exact i64 norm of the rounded i8 centroid (src/cpu_i8.rs:83-93).
+template <int OP>
+__global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
+                                                             size_t k_pad, uint8_t *cen_op, float *norm32,
+                                                             int64_t *norm64, const unsigned *cmax_bits) {
+    const int lane = threadIdx.x & 63;
+    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (jc >= k_pad) return;
+    const bool live = jc < k;
+    const float *cp = c + jc * d;
+    float cs = 1.0f;
+    if constexpr (OP == OP_FP8) cs = fp8_scale_for(__uint_as_float(*cmax_bits));
+    // operand row
+    if constexpr (OP == OP_F32) {
+        float *o = (float *)cen_op + jc * d_pad;
+        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? cp[e] : 0.0f;
+    } else if constexpr (OP == OP_BF16) {
+        uint16_t *o = (uint16_t *)cen_op + jc * d_pad;
+        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? bf16_rne(cp[e]) : (uint16_t)0;
+    } else if constexpr (OP == OP_FP8) {
+        uint32_t *o = (uint32_t *)(cen_op + jc * d_pad);
+        for (size_t w = lane; w < d_pad / 4; w += 64) {
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                size_t e = 4 * w + q;
+                v[q] = (live && e < d) ? cp[e] * cs : 0.0f;
+            }
+            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+            o[w] = (uint32_t)pk;
+        }
+    } else {  // OP_I8: round half away from zero, clamp (src/cpu_i8.rs:83)
+        int8_t *o = (int8_t *)cen_op + jc * d_pad;
+        int64_t part = 0;
+        for (size_t e = lane; e < d_pad; e += 64) {
+            int8_t q = 0;
+            if (live && e < d) {
+                float r = roundf(cp[e]);
+                r = r != r ? 0.0f : fminf(fmaxf(r, -128.0f), 127.0f);
+                q = (int8_t)r;
+            }
+            o[e] = q;
+            part += (int64_t)q * q;
+        }
+        for (int off = 32; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
+        if (lane == 0 && live) norm64[jc] = part;
+    }
+    if constexpr (OP != OP_I8) {
+        if (live) {
+            // acc[t] = sum over i = t mod 16 of c_i^2, in order; fold; add tail
+            const size_t body = d & ~(size_t)15;
+            float acc = 0.0f;
+            if (lane < 16)
+                for (size_t i = lane; i < body; i += 16) acc = __fadd_rn(acc, __fmul_rn(cp[i], cp[i]));
+            float s = 0.0f;
+            for (int t = 0; t < 16; t++) s = __fadd_rn(s, __shfl(acc, t, 64));
+            if (lane == 0) {
+                float tail = 0.0f;
+                for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
+                norm32[jc] = __fadd_rn(s, tail);
+            }
+        }
+    }
+}
+
+__global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) {
+    float mx = 0.0f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
+        float a = fabsf(v[i]);
+        if (a > mx && isfinite(a)) mx = a;
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(mx));
+}
+
+// hn_j = (|c_j|^2 + bias_j) / 2 for the float searches; for I8
+// cn_j = sum c8^2 + round(bias_j) (src/cpu_i8.rs:112-117, unit 1). Pad
+// centroids get a score that never wins.
+__global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *norm32, const int64_t *norm64,
+                                const float *bias, float *hn, int64_t *cn) {
+    size_t jc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (jc >= k_pad) return;
+    if (op == OP_I8) {
+        int64_t v = PAD_SCORE64;
+        if (jc < k) {
+            v = norm64[jc];
+            if (bias) {
+                double r = round((double)bias[jc]);
+                r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
+                v += (int64_t)r;
+            }
+        }
+        cn[jc] = v;
+    } else {
+        float v = INFINITY;
+        if (jc < k) v = __fmul_rn(0.5f, bias ? __fadd_rn(norm32[jc], bias[jc]) : norm32[jc]);
+        hn[jc] = v;
+    }
+}
+
+// Per-row e4m3 encoding with a power-of-two scale: one wave per row.
+template <int ROWS>
+__global__ __launch_bounds__(256) void rows_fp8_kernel(const void *x, size_t m, size_t d, size_t d_pad,
+                                                       const uint32_t *ids, uint8_t *x8, float *row_scale) {
+    const int lane = threadIdx.x & 63;
+    const size_t r = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= m) return;
+    const size_t src = ids ? (size_t)ids[r] : r;
+    float mx = 0.0f;
+    for (size_t e = lane; e < d; e += 64) {
+        float a = fabsf(src_elem<ROWS>(x, src * d + e));
+        if (a > mx && isfinite(a)) mx = a;
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    const float s = fp8_scale_for(mx);
+    uint32_t *o = (uint32_t *)(x8 + r * d_pad);
+    for (size_t w = lane; w < d_pad / 4; w += 64) {
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            size_t e = 4 * w + q;
+            v[q] = e < d ? src_elem<ROWS>(x, src * d + e) * s : 0.0f;
+        }
+        int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+        o[w] = (uint32_t)pk;
+    }
+    if (lane == 0) row_scale[r] = s;
+}
+
+// ------------------------------------------------------------- launchers ---
+
+static size_t search_lds(const SearchGeom &g) { return (size_t)(BM + BN) * (g.row_bytes + 16) + BN * 8; }
+
+template <int OP, int ROWS, int M, bool RES>
+static hipError_t launch_one(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
+    auto kern = search_kernel<OP, ROWS, M, RES>;
+    size_t lds = search_lds(g);
+    static size_t attr_bytes = 0;
+    if (lds > 64 * 1024 && lds > attr_bytes) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr_bytes = lds;
+    }
+    dim3 grid((unsigned)((a.m + BM - 1) / BM));
+    hipLaunchKernelGGL(kern, grid, dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int OP, int ROWS, int M>
+static hipError_t launch_res(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
+    return g.resident ? launch_one<OP, ROWS, M, true>(g, a, s) : launch_one<OP, ROWS, M, false>(g, a, s);
+}
+
+template <int OP, int ROWS>
+static hipError_t launch_m(const SearchGeom &g, int M, const SearchArgs &a, hipStream_t s) {
+    switch (M) {
+        case 1: return launch_res<OP, ROWS, 1>(g, a, s);
+        case 2: return launch_res<OP, ROWS, 2>(g, a, s);
+        case 4: return launch_res<OP, ROWS, 4>(g, a, s);
+        case 8: return launch_res<OP, ROWS, 8>(g, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
+                         const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
+                         const float *row_scale, const unsigned *cmax_bits, uint32_t *labels, float *dist,
+                         uint32_t *cand, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    SearchArgs a;
+    a.x = x;
+    a.ids = ids;
+    a.m = m;
+    a.d = g.d;
+    a.d_pad = g.d_pad;
+    a.k = g.k;
+    a.k_pad = g.k_pad;
+    a.row_bytes = g.row_bytes;
+    a.cen_op = (const uint8_t *)cen_op;
+    a.hn = hn;
+    a.cn = cn;
+    a.x8 = x8;
+    a.row_scale = row_scale;
+    a.cmax_bits = cmax_bits;
+    a.labels = labels;
+    a.dist = dist;
+    a.cand = cand;
+    switch (g.op) {
+        case OP_F32:
+            if (M != 1) return hipErrorInvalidValue;
+            if (rows == EMH_ROWS_F32) return launch_res<OP_F32, EMH_ROWS_F32, 1>(g, a, s);
+            if (rows == EMH_ROWS_I8) return launch_res<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
+            return launch_res<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
+        case OP_BF16:
+            if (rows == EMH_ROWS_F32) return launch_m<OP_BF16, EMH_ROWS_F32>(g, M, a, s);
+            if (rows == EMH_ROWS_I8) return launch_m<OP_BF16, EMH_ROWS_I8>(g, M, a, s);
+            return launch_m<OP_BF16, EMH_ROWS_U8>(g, M, a, s);
+        case OP_FP8:
+            return launch_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
+        case OP_I8:
+            if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
+            return launch_res<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int ROWS>
+static hipError_t rescore_rows(int M, const void *x, size_t m, size_t d, const uint32_t *ids, const float *c,
+                               const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
+                               hipStream_t s) {
+    dim3 grid((unsigned)((m * M + 255) / 256));
+    switch (M) {
+        case 1: hipLaunchKernelGGL((rescore_kernel<ROWS, 1>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
+        case 2: hipLaunchKernelGGL((rescore_kernel<ROWS, 2>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
+        case 4: hipLaunchKernelGGL((rescore_kernel<ROWS, 4>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
+        case 8: hipLaunchKernelGGL((rescore_kernel<ROWS, 8>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,
+                          const float *bias, size_t /*k*/, const uint32_t *cand, uint32_t *labels, float *dist,
+                          hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    if (rows == EMH_ROWS_F32) return rescore_rows<EMH_ROWS_F32>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
+    if (rows == EMH_ROWS_I8) return rescore_rows<EMH_ROWS_I8>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
+    return rescore_rows<EMH_ROWS_U8>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
+}
+
+hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
+                                 const unsigned *cmax, hipStream_t s) {
+    dim3 grid((unsigned)((g.k_pad + 3) / 4));
+    switch (g.op) {
+        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
+        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
+        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
+        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s) {
+    unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 1024);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, v, count, out_bits);
+    return hipGetLastError();
+}
+
+hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
+                           float *hn, int64_t *cn, hipStream_t s) {
+    dim3 grid((unsigned)((g.k_pad + 255) / 256));
+    hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
+                           uint8_t *x8, float *row_scale, hipStream_t s) {
+    if (m == 0) return hipSuccess;
+    dim3 grid((unsigned)((m + 3) / 4));
+    if (rows == EMH_ROWS_F32) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
+    else if (rows == EMH_ROWS_I8) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
+    else hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
+    return hipGetLastError();
+}
+
+}  // namespace emh

@@ -1,0 +1,283 @@
+"""Python mirror of the reference's plugin interface over libemeans_hip.so.
+
+``HipAssigner`` has the methods of the reference traits ``Assigner`` and
+``AssignRows<T>`` (src/lib.rs:75-91): ``set_centroids``, ``set_bias``,
+``name`` and ``assign``; ``train`` / ``assign`` / ``resolve_recipe`` mirror
+src/lib.rs:201-225 and 348-520. All compute runs in the HIP library; this
+module only marshals arguments. Failures raise ``EmeansError``, and loading
+the module without the built library raises too: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libemeans_hip.so")
+
+FP32, BF16_RESCORE, FP8_RESCORE, I8, I16 = 0, 1, 2, 3, 4
+ROWS_F32, ROWS_I8, ROWS_U8 = 0, 1, 2
+ROWS = {np.dtype(np.float32): ROWS_F32, np.dtype(np.int8): ROWS_I8, np.dtype(np.uint8): ROWS_U8}
+E_OK, E_ARG, E_STATE, E_MEMORY, E_DEVICE, E_UNSUPPORTED = 0, 1, 2, 3, 4, 5
+PRECISIONS = {"fp32": FP32, "bf16": BF16_RESCORE, "fp8": FP8_RESCORE, "i8": I8, "i16": I16}
+
+EXPORTS = [
+    "emh_create", "emh_destroy", "emh_set_centroids", "emh_set_bias", "emh_name", "emh_assign_f32",
+    "emh_assign_i8", "emh_assign_u8", "emh_last_error", "emh_set_stream", "emh_assign_device",
+    "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
+    "emh_assign_all", "emh_set_timing", "emh_get_timing",
+]
+
+
+class EmeansError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    """Mirror of ``Config`` (src/lib.rs:103-139); ``Config()`` is the default."""
+
+    _fields_ = [
+        ("nepochs", C.c_double),
+        ("batch_law", C.c_int32),
+        ("has_batch_size", C.c_int32),
+        ("batch_size", C.c_uint64),
+        ("has_alpha0", C.c_int32),
+        ("has_unused_threshold", C.c_int32),
+        ("alpha0", C.c_double),
+        ("unused_threshold", C.c_float),
+        ("balance", C.c_float),
+        ("seed", C.c_uint64),
+    ]
+
+    def __init__(self, nepochs=5.0, batch_law=0, batch_size=None, alpha0=None, unused_threshold=None,
+                 balance=0.0, seed=1234):
+        super().__init__()
+        self.nepochs = nepochs
+        self.batch_law = batch_law
+        self.has_batch_size = batch_size is not None
+        self.batch_size = batch_size or 0
+        self.has_alpha0 = alpha0 is not None
+        self.alpha0 = alpha0 or 0.0
+        self.has_unused_threshold = unused_threshold is not None
+        self.unused_threshold = unused_threshold or 0.0
+        self.balance = balance
+        self.seed = seed
+
+
+class Recipe(C.Structure):
+    _fields_ = [
+        ("batch_size", C.c_uint64),
+        ("alpha0", C.c_double),
+        ("total_steps", C.c_uint64),
+        ("unused_threshold", C.c_float),
+        ("n_epochs", C.c_uint32),
+        ("epoch_rows_", C.c_uint64 * 64),
+    ]
+
+    @property
+    def epoch_rows(self):
+        return [int(self.epoch_rows_[i]) for i in range(self.n_epochs)]
+
+
+class TrainOut(C.Structure):
+    _fields_ = [
+        ("n_splits", C.c_uint64),
+        ("n_objective", C.c_uint32),
+        ("objective_", C.c_double * 64),
+        ("recipe", Recipe),
+        ("seconds", C.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library; raise if it was not built (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: run `python e-means_amd/build.py` (or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp, sz, i32, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64
+    L.emh_create.argtypes = [i32, i32, i32, C.POINTER(C.c_void_p)]
+    L.emh_destroy.argtypes = [vp]
+    L.emh_destroy.restype = None
+    L.emh_set_centroids.argtypes = [vp, vp, sz, sz]
+    L.emh_set_bias.argtypes = [vp, vp, sz]
+    L.emh_name.argtypes = [vp, C.c_char_p, sz]
+    for f in ("emh_assign_f32", "emh_assign_i8", "emh_assign_u8"):
+        getattr(L, f).argtypes = [vp, vp, sz, vp, vp]
+    L.emh_last_error.restype = C.c_char_p
+    L.emh_set_stream.argtypes = [vp, vp]
+    L.emh_assign_device.argtypes = [vp, vp, i32, sz, vp, vp, vp]
+    L.emh_synchronize.argtypes = [vp]
+    L.emh_set_timing.argtypes = [vp, i32]
+    L.emh_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
+    L.emh_accumulate_device.argtypes = [vp, vp, i32, sz, vp, vp, vp, vp]
+    L.emh_config_default.argtypes = [C.POINTER(Config)]
+    L.emh_config_default.restype = None
+    L.emh_resolve_recipe.argtypes = [u64, u64, u64, C.POINTER(Config), C.POINTER(Recipe)]
+    L.emh_train.argtypes = [vp, vp, i32, sz, sz, sz, C.POINTER(Config), vp, vp, C.POINTER(TrainOut)]
+    L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise EmeansError(rc, lib().emh_last_error().decode())
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+def require_gpu():
+    """Fail loudly unless the HIP library loads and a GPU is visible."""
+    L = lib()
+    h = C.c_void_p()
+    rc = L.emh_create(0, FP32, 0, C.byref(h))
+    if rc != 0:
+        raise EmeansError(rc, L.emh_last_error().decode())
+    L.emh_destroy(h)
+
+
+class HipAssigner:
+    """``impl Assigner + AssignRows<T>`` on one GPU (src/lib.rs:75-91)."""
+
+    def __init__(self, precision=FP32, device: int = 0, rescore_m: int = 0):
+        if isinstance(precision, str):
+            precision = PRECISIONS[precision]
+        self._h = C.c_void_p()
+        _check(lib().emh_create(device, precision, rescore_m, C.byref(self._h)))
+        self.precision = precision
+        self.k = self.d = 0
+
+    def close(self):
+        if self._h:
+            lib().emh_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def set_centroids(self, centroids):
+        c = np.ascontiguousarray(centroids, np.float32)
+        if c.ndim != 2:
+            raise ValueError("centroids must be k x d")
+        _check(lib().emh_set_centroids(self._h, _p(c), c.shape[0], c.shape[1]))
+        self.k, self.d = c.shape
+
+    def set_bias(self, bias):
+        b = np.ascontiguousarray(bias, np.float32)
+        _check(lib().emh_set_bias(self._h, _p(b), b.size))
+
+    def name(self) -> str:
+        buf = C.create_string_buffer(512)
+        _check(lib().emh_name(self._h, buf, 512))
+        return buf.value.decode()
+
+    def assign(self, x):
+        x = np.ascontiguousarray(x)
+        if x.dtype not in ROWS:
+            raise TypeError(f"rows must be f32, i8 or u8, got {x.dtype}")
+        if x.ndim != 2 or (self.d and x.shape[1] != self.d):
+            raise ValueError("x must be m x d with the centroids' d")
+        m = x.shape[0]
+        labels = np.empty(m, np.uint32)
+        dist = np.empty(m, np.float32)
+        fn = {ROWS_F32: "emh_assign_f32", ROWS_I8: "emh_assign_i8", ROWS_U8: "emh_assign_u8"}[ROWS[x.dtype]]
+        _check(getattr(lib(), fn)(self._h, _p(x), m, _p(labels), _p(dist)))
+        return labels, dist
+
+    # ---- device-pointer extensions (torch tensors as HBM plumbing) ----
+    def set_stream(self, stream_handle: int):
+        _check(lib().emh_set_stream(self._h, C.c_void_p(stream_handle)))
+
+    def assign_device(self, x, labels, dist, ids=None):
+        """x, labels, dist, ids: torch CUDA tensors; asynchronous."""
+        rows = {"torch.float32": ROWS_F32, "torch.int8": ROWS_I8, "torch.uint8": ROWS_U8}[str(x.dtype)]
+        m = labels.shape[0]
+        _check(lib().emh_assign_device(self._h, C.c_void_p(x.data_ptr()), rows, m,
+                                       None if ids is None else C.c_void_p(ids.data_ptr()),
+                                       C.c_void_p(labels.data_ptr()), C.c_void_p(dist.data_ptr())))
+
+    def accumulate_device(self, x, labels, sums, counts, ids=None):
+        rows = {"torch.float32": ROWS_F32, "torch.int8": ROWS_I8, "torch.uint8": ROWS_U8}[str(x.dtype)]
+        _check(lib().emh_accumulate_device(self._h, C.c_void_p(x.data_ptr()), rows, labels.shape[0],
+                                           None if ids is None else C.c_void_p(ids.data_ptr()),
+                                           C.c_void_p(labels.data_ptr()), C.c_void_p(sums.data_ptr()),
+                                           C.c_void_p(counts.data_ptr())))
+
+    def synchronize(self):
+        _check(lib().emh_synchronize(self._h))
+
+    def set_timing(self, on: bool):
+        _check(lib().emh_set_timing(self._h, int(on)))
+
+    def get_timing(self):
+        """(row-encode ms, search ms, rescore ms, calls) summed since the last read."""
+        ms = (C.c_double * 3)()
+        calls = C.c_uint64()
+        _check(lib().emh_get_timing(self._h, ms, C.byref(calls)))
+        return ms[0], ms[1], ms[2], int(calls.value)
+
+
+def resolve_recipe(n: int, d: int, k: int, cfg: Config | None = None) -> Recipe:
+    r = Recipe()
+    _check(lib().emh_resolve_recipe(n, d, k, C.byref(cfg or Config()), C.byref(r)))
+    return r
+
+
+@dataclass
+class Output:
+    """Mirror of ``Output`` (src/lib.rs:151-163)."""
+
+    centroids: np.ndarray
+    labels: np.ndarray
+    objective: list
+    n_splits: int
+    recipe: Recipe = field(repr=False)
+    seconds: float = 0.0
+
+
+def train(x, k: int, cfg: Config | None = None, assigner: HipAssigner | None = None) -> Output:
+    """train (src/lib.rs:348-491) with the HIP assigner (FP32 by default)."""
+    x = np.ascontiguousarray(x)
+    if x.dtype not in ROWS or x.ndim != 2:
+        raise TypeError("x must be a 2-D f32, i8 or u8 array")
+    a = assigner or HipAssigner(FP32)
+    n, d = x.shape
+    cent = np.empty((k, d), np.float32)
+    labels = np.empty(n, np.uint32)
+    out = TrainOut()
+    _check(lib().emh_train(a.handle, _p(x), ROWS[x.dtype], n, d, k, C.byref(cfg or Config()), _p(cent),
+                           _p(labels), C.byref(out)))
+    obj = [out.objective_[i] for i in range(out.n_objective)]
+    return Output(cent, labels, obj, int(out.n_splits), out.recipe, float(out.seconds))
+
+
+def assign(x, centroids, assigner: HipAssigner | None = None):
+    """assign (src/lib.rs:512-520): exact labels and squared distances."""
+    x = np.ascontiguousarray(x)
+    c = np.ascontiguousarray(centroids, np.float32)
+    a = assigner or HipAssigner(FP32)
+    n, d = x.shape
+    labels = np.empty(n, np.uint32)
+    dist = np.empty(n, np.float32)
+    _check(lib().emh_assign_all(a.handle, _p(x), ROWS[x.dtype], n, d, _p(c), c.shape[0], _p(labels), _p(dist)))
+    return labels, dist

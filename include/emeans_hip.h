@@ -1,0 +1,164 @@
+/* emeans_hip.h - C ABI of the MI355X (gfx950) e-means nearest-centroid search.
+ *
+ * The first block is the drop-in boundary: one function per method of the
+ * reference crate's plugin traits (andrewluetgers/e-means)
+ *
+ *     trait Assigner            src/lib.rs:75-84
+ *     trait AssignRows<T>       src/lib.rs:87-91
+ *
+ * so a Rust `impl Assigner for HipAssigner` wraps exactly these functions
+ * (the binding is shown in INTEGRATION.md). The second block holds
+ * extensions the reference has no counterpart for: device-resident rows,
+ * device-side per-cluster sums/counts and the training driver of
+ * src/lib.rs:348-491 running over them.
+ *
+ * Conventions: every function returns 0 on success and a nonzero EMH_E_*
+ * code on failure; emh_last_error() then holds a message. Nothing panics,
+ * throws or aborts across this boundary. All pointers are plain host
+ * pointers unless the name says _dev. Calls on one handle must come from one
+ * thread at a time (the trait methods take &mut self).
+ */
+#ifndef EMEANS_HIP_H
+#define EMEANS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+#pragma GCC visibility push(default)
+
+typedef struct emh_assigner emh_assigner;
+
+/* Search precision of an assigner. */
+typedef enum {
+    EMH_FP32 = 0,          /* fused FP32 |x|^2 - 2x.c + |c|^2 argmin: the drop-in */
+    EMH_BF16_RESCORE = 1,  /* BF16 MFMA, FP32 accumulate, top-m kept, FP32 rescore */
+    EMH_FP8_RESCORE = 2,   /* FP8-e4m3 (per-row scaled) MFMA, top-m, FP32 rescore */
+    EMH_I8 = 3,            /* i8 MFMA, int32 dots: bit-exact CpuI8Assigner(I8) */
+    EMH_I16 = 4            /* i8 MFMA on split i16 centroids: CpuI8Assigner(I16) */
+} emh_precision;
+
+/* Row element types (the reference's Scalar impls, src/lib.rs:47-71). */
+typedef enum { EMH_ROWS_F32 = 0, EMH_ROWS_I8 = 1, EMH_ROWS_U8 = 2 } emh_rows;
+
+enum {
+    EMH_OK = 0,
+    EMH_E_ARG = 1,      /* bad argument or contract violation (the reference asserts) */
+    EMH_E_STATE = 2,    /* e.g. assign before set_centroids */
+    EMH_E_MEMORY = 3,   /* not enough device memory (src/cuda.rs:103-110) */
+    EMH_E_DEVICE = 4,   /* HIP runtime error */
+    EMH_E_UNSUPPORTED = 5 /* row type / precision combination not offered */
+};
+
+/* ---- Drop-in boundary: trait Assigner + AssignRows<T> ------------------- */
+
+/* Replaces CudaAssigner::new(ordinal) (src/cuda.rs:69-91) and
+ * CpuI8Assigner::new(bits) (src/cpu_i8.rs:61-69). rescore_m: candidates kept
+ * per row by the BF16/FP8 searches (1..8; 0 picks 4); ignored otherwise. */
+int emh_create(int device, int precision, int rescore_m, emh_assigner **out);
+void emh_destroy(emh_assigner *h);
+
+/* Assigner::set_centroids (src/lib.rs:77; src/cuda.rs:127-135). k x d
+ * row-major f32. Resets any bias (src/lib.rs:78-81). */
+int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t d);
+
+/* Assigner::set_bias (src/lib.rs:81; src/cpu.rs:95-99; src/cpu_i8.rs:112-117).
+ * bias has k entries; rows then minimise |x - c_j|^2 + bias_j and the returned
+ * distance includes the bias. */
+int emh_set_bias(emh_assigner *h, const float *bias, size_t k);
+
+/* Assigner::name (src/lib.rs:83). Writes a NUL-terminated string. */
+int emh_name(const emh_assigner *h, char *buf, size_t cap);
+
+/* AssignRows<T>::assign (src/lib.rs:90; src/cuda.rs:155-213): x is m x d
+ * row-major; labels[m], dist[m]. Ties go to the lowest index. Synchronous:
+ * outputs are valid on return. EMH_I8/EMH_I16 take i8 rows only, as
+ * CpuI8Assigner implements AssignRows<i8> only (src/cpu_i8.rs:130). */
+int emh_assign_f32(emh_assigner *h, const float *x, size_t m, uint32_t *labels, float *dist);
+int emh_assign_i8(emh_assigner *h, const int8_t *x, size_t m, uint32_t *labels, float *dist);
+int emh_assign_u8(emh_assigner *h, const uint8_t *x, size_t m, uint32_t *labels, float *dist);
+
+/* Message of the last failure on this thread (never NULL). */
+const char *emh_last_error(void);
+
+/* ---- Extensions (no counterpart in the reference traits) ---------------- */
+
+/* Run on this HIP stream (hipStream_t) instead of the assigner's own. */
+int emh_set_stream(emh_assigner *h, void *stream);
+
+/* Device-pointer search: x_dev (m x d, `rows` type) already in HBM; optional
+ * ids_dev (m u32) gathers row ids_dev[r] of x_dev for row r (the gather of
+ * src/lib.rs:421-423 done on device). Asynchronous on the assigner's stream;
+ * emh_synchronize() waits. */
+int emh_assign_device(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                      uint32_t *labels_dev, float *dist_dev);
+int emh_synchronize(emh_assigner *h);
+
+/* Per-kernel timing with HIP events on the assigner's stream. While on, each
+ * search call records events around its three stages; emh_get_timing waits,
+ * returns the summed milliseconds of {row encoding, search, rescore} in
+ * ms_out[0..2] and the number of search calls, and resets the totals. */
+int emh_set_timing(emh_assigner *h, int on);
+int emh_get_timing(emh_assigner *h, double *ms_out, uint64_t *calls);
+
+/* Per-cluster counts and f32 sums of a labelled batch, each sum folded in
+ * batch row order (bit-identical to the host loop of src/lib.rs:431-441).
+ * Device pointers; sums_dev is k x d f32, counts_dev k u32. */
+int emh_accumulate_device(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                          const uint32_t *labels_dev, float *sums_dev, uint32_t *counts_dev);
+
+/* Training configuration: mirror of Config (src/lib.rs:103-139). */
+typedef struct {
+    double nepochs;
+    int32_t batch_law;            /* 0 auto, 1 throughput, 2 quality */
+    int32_t has_batch_size;
+    uint64_t batch_size;
+    int32_t has_alpha0;
+    int32_t has_unused_threshold;
+    double alpha0;
+    float unused_threshold;
+    float balance;
+    uint64_t seed;
+} emh_config;
+
+/* Recipe (src/lib.rs:142-149). */
+typedef struct {
+    uint64_t batch_size;
+    double alpha0;
+    uint64_t total_steps;
+    float unused_threshold;
+    uint32_t n_epochs;
+    uint64_t epoch_rows[64];
+} emh_recipe;
+
+typedef struct {
+    uint64_t n_splits;
+    uint32_t n_objective;
+    double objective[64];
+    emh_recipe recipe;
+    double seconds;               /* wall time of the training loop */
+} emh_train_out;
+
+void emh_config_default(emh_config *cfg);
+/* resolve_recipe (src/lib.rs:201-225). */
+int emh_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const emh_config *cfg, emh_recipe *out);
+
+/* train (src/lib.rs:348-491) with this assigner: rows are uploaded once and
+ * stay in HBM; each batch's gather, search and per-cluster sums/counts run on
+ * the device; the EMA update, revival and schedule run on the host exactly
+ * as the reference. The last pass's per-cluster sums accumulate in FP64 on
+ * the device for the closing mean. centroids_out: k x d; labels_out: n. */
+int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size_t k, const emh_config *cfg,
+              float *centroids_out, uint32_t *labels_out, emh_train_out *out);
+
+/* assign (src/lib.rs:512-520): exact labels and distances for all n rows. */
+int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,
+                   uint32_t *labels, float *dist);
+
+#pragma GCC visibility pop
+#ifdef __cplusplus
+}
+#endif
+#endif

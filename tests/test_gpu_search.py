@@ -1,0 +1,191 @@
+"""GPU parity of the search kernels against the CPU oracle.
+
+FP32: the reference's own bounds (src/cpu.rs:298-306 vs FP64 brute force;
+src/cuda.rs:244-248 vs the CPU search). I8: bit-exact labels and distances
+vs the CpuI8Assigner restatement (src/cpu_i8.rs:294-324). BF16/FP8 + FP32
+rescore: label disagreement and distance error vs FP64 brute force, with the
+tolerances written in each test.
+"""
+import numpy as np
+import pytest
+
+import emeans_hip as E
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1, 1, 1), (5, 3, 7), (13, 9, 16), (37, 70, 33), (130, 257, 100), (64, 1030, 384), (9, 6, 1027)]
+I8_SHAPES = [(1, 1, 1), (5, 3, 7), (13, 9, 64), (37, 70, 33), (130, 257, 100), (20, 1030, 512), (9, 6, 1027)]
+
+
+def _bound_ok(x, c, labels, dist, bias=None):
+    """src/cpu.rs:298-306: chosen centroid within 1e-4(1+best) of the FP64
+    best, and the returned distance within the same bound."""
+    bl, bd = O.brute_f64(x, c, bias)
+    got = O.dist_f64(x, c, labels)
+    if bias is not None:
+        got = got + bias[labels].astype(np.float64)
+    tol = 1e-4 * (1.0 + np.abs(bd))
+    return bool(np.all(got <= bd + tol)), bool(np.all(np.abs(dist.astype(np.float64) - bd) <= tol))
+
+
+@pytest.mark.parametrize("n,k,d", SHAPES)
+def test_fp32_matches_brute_force(hip, n, k, d):
+    print(f"fp32 n={n} k={k} d={d}", flush=True)
+    x = O.lcg_f32(n * d, 1).reshape(n, d)
+    c = O.lcg_f32(k * d, 2).reshape(k, d)
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(c)
+    labels, dist = a.assign(x)
+    assert labels.max() < k
+    lab_ok, dist_ok = _bound_ok(x, c, labels, dist)
+    assert lab_ok and dist_ok
+
+
+def test_fp32_matches_cpu(hip):
+    # src/cuda.rs:221-249
+    n, k, d = 3000, 1031, 100
+    data = O.lcg_f32(n * d + k * d, 1)  # one generator: rows, then centroids
+    x = data[: n * d].reshape(n, d)
+    c = data[n * d:].reshape(k, d)
+    cl, cd = O.cpu_assign(x, c)
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(c)
+    gl, gd = a.assign(x)
+    same = int((cl == gl).sum())
+    assert same >= 0.999 * n, f"only {same} of {n} labels agree"
+    assert np.all(np.abs(cd - gd) <= 1e-3 * (1.0 + np.abs(cd)))
+
+
+@pytest.mark.parametrize("rows", ["i8", "u8"])
+def test_fp32_int_rows_equal_f32_rows(hip, rows):
+    rng = np.random.default_rng(5)
+    xi = (rng.integers(-128, 128, size=(300, 40)) if rows == "i8" else rng.integers(0, 256, size=(300, 40)))
+    xi = xi.astype(np.int8 if rows == "i8" else np.uint8)
+    c = rng.normal(scale=40, size=(70, 40)).astype(np.float32)
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(c)
+    l1, d1 = a.assign(xi)
+    l2, d2 = a.assign(xi.astype(np.float32))
+    assert np.array_equal(l1, l2) and np.array_equal(d1, d2)
+
+
+def test_fp32_bias(hip):
+    rng = np.random.default_rng(6)
+    x = rng.normal(size=(500, 24)).astype(np.float32)
+    c = rng.normal(size=(90, 24)).astype(np.float32)
+    bias = rng.uniform(0, 3, size=90).astype(np.float32)
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(c)
+    a.set_bias(bias)
+    labels, dist = a.assign(x)
+    lab_ok, dist_ok = _bound_ok(x, c, labels, dist, bias)
+    assert lab_ok and dist_ok
+    # set_centroids resets the bias (src/lib.rs:78-81)
+    a.set_centroids(c)
+    l2, d2 = a.assign(x)
+    assert _bound_ok(x, c, l2, d2) == (True, True)
+
+
+@pytest.mark.parametrize("n,k,d", I8_SHAPES)
+def test_i8_bit_exact(hip, n, k, d):
+    print(f"i8 n={n} k={k} d={d}", flush=True)
+    x = O.lcg_i8(n * d, 1).reshape(n, d)
+    ci = O.lcg_i8(k * d, 2).reshape(k, d).astype(np.float32)
+    el, ed = O.brute_i64(x, ci.astype(np.int8))
+    a = E.HipAssigner(E.I8)
+    a.set_centroids(ci)
+    labels, dist = a.assign(x)
+    assert np.array_equal(labels, el)
+    assert np.array_equal(dist, ed.astype(np.float32))
+
+
+def test_i8_fractional_centroids_and_bias_match_cpu(hip):
+    rng = np.random.default_rng(7)
+    x = rng.integers(-128, 128, size=(2000, 96)).astype(np.int8)
+    c = rng.normal(scale=60, size=(333, 96)).astype(np.float32)
+    c[:5] = np.round(c[:5]) + 0.5  # exact halves round away from zero
+    c[5, :3] = [300.0, -300.0, np.float32(127.49)]
+    bias = rng.uniform(-50, 50, size=333).astype(np.float32)
+    bias[:4] = [0.5, -0.5, 2.5, -2.5]
+    a = E.HipAssigner(E.I8)
+    for b in (None, bias):
+        a.set_centroids(c)
+        if b is not None:
+            a.set_bias(b)
+        gl, gd = a.assign(x)
+        cl, cd = O.i8_assign(x, c, bias=b)
+        assert np.array_equal(gl, cl) and np.array_equal(gd, cd)
+
+
+def test_i8_rejects_float_rows(hip):
+    a = E.HipAssigner(E.I8)
+    a.set_centroids(np.zeros((3, 4), np.float32))
+    with pytest.raises(E.EmeansError) as ei:
+        a.assign(np.zeros((2, 4), np.float32))
+    assert ei.value.code == E.E_UNSUPPORTED
+
+
+def _mixture(n, d, k_blobs, seed, scale=4.0):
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(scale=scale, size=(k_blobs, d)).astype(np.float32)
+    x = centres[rng.integers(0, k_blobs, n)] + rng.normal(size=(n, d)).astype(np.float32)
+    return x.astype(np.float32)
+
+
+@pytest.mark.parametrize("prec,m,max_dis", [("bf16", 4, 1e-3), ("bf16", 1, 2e-2), ("fp8", 8, 5e-3)])
+def test_rescored_search_vs_fp64(hip, prec, m, max_dis):
+    """Label disagreement vs FP64 at most max_dis (true errors only: a label
+    whose FP64 distance exceeds the best by more than 1e-6(1+best)), and the
+    returned distance within 1e-4(1+best) of FP64 at that label."""
+    print(f"{prec} m={m}", flush=True)
+    x = _mixture(20000, 128, 2048, 3)
+    c = x[O.permutation(1234, 20000)[:1024]]
+    a = E.HipAssigner(prec, rescore_m=m)
+    a.set_centroids(c)
+    labels, dist = a.assign(x)
+    bl, bd = O.brute_f64(x, c)
+    got = O.dist_f64(x, c, labels)
+    wrong = got > bd + 1e-6 * (1 + bd)
+    assert wrong.mean() <= max_dis, f"{prec} top-{m}: {wrong.mean():.2e} true label errors"
+    assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+
+
+def test_bf16_odd_shapes(hip):
+    for n, k, d in SHAPES:
+        x = O.lcg_f32(n * d, 1).reshape(n, d)
+        c = O.lcg_f32(k * d, 2).reshape(k, d)
+        a = E.HipAssigner("bf16", rescore_m=8)
+        a.set_centroids(c)
+        labels, dist = a.assign(x)
+        assert labels.max() < k
+        got = O.dist_f64(x, c, labels)
+        assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+
+
+def test_bf16_bias(hip):
+    rng = np.random.default_rng(8)
+    x = rng.normal(size=(800, 64)).astype(np.float32)
+    c = rng.normal(size=(200, 64)).astype(np.float32)
+    bias = rng.uniform(0, 5, size=200).astype(np.float32)
+    a = E.HipAssigner("bf16", rescore_m=8)
+    a.set_centroids(c)
+    a.set_bias(bias)
+    labels, dist = a.assign(x)
+    bl, bd = O.brute_f64(x, c, bias)
+    got = O.dist_f64(x, c, labels) + bias[labels]
+    assert (got > bd + 1e-6 * (1 + bd)).mean() < 0.01
+    assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+
+
+def test_name_and_state_errors(hip):
+    a = E.HipAssigner(E.FP32)
+    with pytest.raises(E.EmeansError) as ei:
+        a.assign(np.zeros((2, 3), np.float32))
+    assert ei.value.code == E.E_STATE
+    a.set_centroids(np.zeros((4, 3), np.float32))
+    assert "FP32" in a.name()
+    with pytest.raises(E.EmeansError):
+        a.set_bias(np.zeros(5, np.float32))
+    with pytest.raises(E.EmeansError):
+        E.HipAssigner(E.I8).set_centroids(np.zeros((1, 40000), np.float32))

@@ -1,0 +1,106 @@
+"""GPU training parity: device gather + search + per-cluster sums driven by
+the host recipe, against the CPU oracle's train (src/lib.rs:348-491)."""
+import numpy as np
+import pytest
+
+import emeans_hip as E
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _i8_blobs(n, d, blobs, seed):
+    rng = np.random.default_rng(seed)
+    centres = rng.integers(-90, 90, size=(blobs, d))
+    x = centres[rng.integers(0, blobs, n)] + rng.integers(-12, 13, size=(n, d))
+    return np.clip(x, -128, 127).astype(np.int8)
+
+
+@pytest.mark.parametrize("balance", [0.0, 0.1])
+def test_i8_training_is_bit_exact(hip, balance):
+    """int8 rows with the exact integer search: every label, distance and
+    per-cluster sum is an exact integer, so the whole trajectory (centroids,
+    last-pass labels, objective, revivals) must equal the CPU oracle's."""
+    x = _i8_blobs(6000, 48, 60, 1)
+    cfg_o = O.Config(nepochs=3, balance=balance, seed=99)
+    cfg_e = E.Config(nepochs=3, balance=balance, seed=99)
+    ref = O.train(x, 40, cfg_o, assigner=O.ASSIGNER_I8)
+    got = E.train(x, 40, cfg_e, E.HipAssigner(E.I8))
+    assert got.recipe.batch_size == ref.recipe.batch_size
+    assert np.array_equal(got.labels, ref.labels)
+    assert got.objective == ref.objective
+    assert got.n_splits == ref.n_splits
+    assert np.array_equal(got.centroids, ref.centroids)
+
+
+def _blobs_6d():
+    n, d = 20_000, 6
+    r = O.OracleRng(7)
+    u = r.open01_many(20 * d)
+    centers = (np.float32(8.0) * (u.astype(np.float32) - np.float32(0.5))).astype(np.float32)
+    noise_u = r.open01_many(n * d * 4).astype(np.float32).reshape(n, d, 4) - np.float32(0.5)
+    noise = np.zeros((n, d), np.float32)
+    for t in range(4):
+        noise = (noise + noise_u[:, :, t]).astype(np.float32)
+    return (centers.reshape(20, d)[np.arange(n) % 20] + noise).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_closing_mean_and_loss(hip, prec):
+    # src/lib.rs:575-614 on the GPU assigner
+    x = _blobs_6d()
+    n, d, k = x.shape[0], x.shape[1], 32
+    out = E.train(x, k, E.Config(), E.HipAssigner(prec))
+    counts = np.bincount(out.labels, minlength=k)
+    assert counts.sum() == n
+    sums = np.zeros((k, d))
+    np.add.at(sums, out.labels, x.astype(np.float64))
+    live = counts > 0
+    assert np.all(np.abs(out.centroids[live] - sums[live] / counts[live, None]) < 1e-3)
+    trained = E.assign(x, out.centroids)[1].astype(np.float64).mean()
+    start = E.assign(x, x[O.permutation(1234, n)[:k]])[1].astype(np.float64).mean()
+    assert trained < 0.75 * start and trained < 2.0
+    assert all(b < a for a, b in zip(out.objective, out.objective[1:]))
+
+
+def test_f32_training_tracks_oracle(hip):
+    """f32 rows: near-tie labels may differ from the CPU's (FP32 sums in a
+    different order), so the trajectories are compared statistically."""
+    rng = np.random.default_rng(2)
+    centres = rng.normal(scale=4, size=(64, 24)).astype(np.float32)
+    x = (centres[rng.integers(0, 64, 8000)] + rng.normal(size=(8000, 24))).astype(np.float32)
+    ref = O.train(x, 50, O.Config(nepochs=2))
+    got = E.train(x, 50, E.Config(nepochs=2))
+    mse_ref = O.assign(x, ref.centroids)[1].astype(np.float64).mean()
+    mse_got = O.assign(x, got.centroids)[1].astype(np.float64).mean()
+    assert abs(mse_got / mse_ref - 1) < 0.02
+    assert abs(got.objective[-1] / ref.objective[-1] - 1) < 0.02
+
+
+def test_device_accumulate_is_batch_order_f32(hip):
+    import torch
+
+    rng = np.random.default_rng(3)
+    n, d, k = 5000, 37, 50
+    x = rng.normal(scale=100, size=(n, d)).astype(np.float32)
+    ids = rng.permutation(n)[:3000].astype(np.uint32)
+    labels = rng.integers(0, k, 3000).astype(np.uint32)
+    labels[labels == 7] = 8  # an empty cluster
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(np.zeros((k, d), np.float32))
+    xd = torch.from_numpy(x).cuda()
+    idd = torch.from_numpy(ids.view(np.int32)).cuda()
+    ld = torch.from_numpy(labels.view(np.int32)).cuda()
+    sums = torch.full((k, d), 7.0, device="cuda")
+    counts = torch.full((k,), 9, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    a.accumulate_device(xd, ld, sums, counts, ids=idd)
+    a.synchronize()
+    # the host loop of src/lib.rs:431-441: sequential f32 adds in batch order
+    want = np.zeros((k, d), np.float32)
+    want_c = np.zeros(k, np.int64)
+    for r in range(3000):
+        want[labels[r]] += x[ids[r]]
+        want_c[labels[r]] += 1
+    assert np.array_equal(counts.cpu().numpy(), want_c)
+    assert np.array_equal(sums.cpu().numpy(), want)
