@@ -62,8 +62,8 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     if (h->op == OP_F32 || h->op == OP_I8) {
         if ((rc = mark(h, &ev[1]))) return rc;
         if ((rc = check(launch_search(g, 1, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
-                                      h->cn.as<int64_t>(), nullptr, nullptr, nullptr, labels_dev, dist_dev, nullptr,
-                                      h->stream),
+                                      h->cn.as<int64_t>(), nullptr, nullptr, nullptr, nullptr,
+                                      h->stats.as<unsigned>(), labels_dev, dist_dev, nullptr, h->stream),
                         "search kernel")))
             return rc;
         if ((rc = mark(h, &ev[2]))) return rc;
@@ -71,21 +71,23 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     }
     if ((rc = check(h->cand.ensure(m * h->M * 4), "candidate buffer"))) return rc;
     const uint8_t *x8 = nullptr;
-    const float *rs = nullptr;
+    const float *rs = nullptr, *rn = nullptr;
     if (h->op == OP_FP8) {
         if ((rc = check(h->x8.ensure(m * g.d_pad), "fp8 row buffer"))) return rc;
         if ((rc = check(h->row_scale.ensure(m * 4), "row scale buffer"))) return rc;
+        if ((rc = check(h->row_norm.ensure(m * 4), "row norm buffer"))) return rc;
         if ((rc = check(launch_rows_fp8(x_dev, rows, m, g.d, g.d_pad, ids_dev, h->x8.as<uint8_t>(),
-                                        h->row_scale.as<float>(), h->stream),
+                                        h->row_scale.as<float>(), h->row_norm.as<float>(), h->stream),
                         "fp8 row encoding")))
             return rc;
         x8 = h->x8.as<uint8_t>();
         rs = h->row_scale.as<float>();
+        rn = h->row_norm.as<float>();
     }
     if ((rc = mark(h, &ev[1]))) return rc;
     if ((rc = check(launch_search(g, h->M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
-                                  h->cn.as<int64_t>(), x8, rs, h->cmax.as<unsigned>(), nullptr, nullptr,
-                                  h->cand.as<uint32_t>(), h->stream),
+                                  h->cn.as<int64_t>(), x8, rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
+                                  nullptr, nullptr, h->cand.as<uint32_t>(), h->stream),
                     "search kernel")))
         return rc;
     if ((rc = mark(h, &ev[2]))) return rc;
@@ -111,8 +113,8 @@ hipEvent_t emh_assigner::timing_event() {
 }
 
 emh_assigner::~emh_assigner() {
-    for (DevBuf *b : {&cen_f32, &cen_op, &norm32, &norm64, &hn, &cn, &bias, &cmax, &x_stage, &labels_stage,
-                      &dist_stage, &cand, &x8, &row_scale})
+    for (DevBuf *b : {&cen_f32, &cen_op, &norm32, &norm64, &hn, &cn, &bias, &cmax, &stats, &x_stage, &labels_stage,
+                      &dist_stage, &cand, &x8, &row_scale, &row_norm})
         b->release();
     accum_free(acc);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -228,22 +230,25 @@ int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t 
     if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
     if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
     if ((rc = check(h->cmax.ensure(4), "scale buffer"))) return rc;
+    if ((rc = check(h->stats.ensure(8), "stats buffer"))) return rc;
     h->g = g;
     h->has_centroids = false;
     h->has_bias = false;
     if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4, hipMemcpyHostToDevice, h->stream), "copy centroids")))
         return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 2, 0u, h->stream), "fill"))) return rc;
     if (h->op == OP_FP8) {
         if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
         if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
             return rc;
     }
     if ((rc = check(launch_prep_centroids(g, h->cen_f32.as<float>(), h->cen_op.p, h->norm32.as<float>(),
-                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stream),
+                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
+                                          h->stream),
                     "centroid prep")))
         return rc;
     if ((rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr, h->hn.as<float>(),
-                                    h->cn.as<int64_t>(), h->stream),
+                                    h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
                     "score offsets")))
         return rc;
     // The caller may reuse `centroids` on return (pageable source).
@@ -260,8 +265,9 @@ int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
     if ((rc = check(h->bias.ensure(k * 4), "bias buffer"))) return rc;
     if ((rc = check(hipMemcpyAsync(h->bias.p, bias, k * 4, hipMemcpyHostToDevice, h->stream), "copy bias"))) return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>() + 1, 1, 0u, h->stream), "fill"))) return rc;
     if ((rc = check(launch_set_bias(h->g, h->norm32.as<float>(), h->norm64.as<int64_t>(), h->bias.as<float>(),
-                                    h->hn.as<float>(), h->cn.as<int64_t>(), h->stream),
+                                    h->hn.as<float>(), h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
                     "score offsets")))
         return rc;
     if ((rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;

@@ -46,9 +46,9 @@ struct emh_assigner {
     bool has_bias = false;
 
     // centroid side (set_centroids / set_bias)
-    emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax;
+    emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax, stats;
     // row side (per assign call)
-    emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale;
+    emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale, row_norm;
     // per-cluster accumulation
     emh::AccumScratch acc;
 

@@ -23,9 +23,11 @@ constexpr int ROW_LDS_BUDGET = 64 * 1024;  // bytes of LDS for a resident row bl
 constexpr int MAX_M = 8;        // top-m candidates kept by the rescoring searches
 
 inline int op_elem_bytes(int op) { return op == OP_F32 ? 4 : (op == OP_BF16 ? 2 : 1); }
-// Bytes one lane feeds one MFMA, times the two lane halves.
+// Row padding unit in bytes: at least what one lane feeds one MFMA, times
+// the two lane halves; the float rescoring searches pad to 64 bytes so a
+// half row is a whole number of 16-byte register quads.
 inline int op_granule(int op) {
-    int g = op == OP_F32 ? 8 : (op == OP_BF16 ? 32 : (op == OP_FP8 ? 64 : 32));
+    int g = op == OP_F32 ? 8 : (op == OP_BF16 ? 64 : (op == OP_FP8 ? 64 : 32));
     return g < 16 ? 16 : g;
 }
 
@@ -40,17 +42,20 @@ struct SearchGeom {
 SearchGeom make_geom(int op, size_t k, size_t d);
 
 // Launchers (stream-ordered, no synchronisation).
+// stats (2 x u32, f32 bits): [0] max_j |c_j|^2 (set by the prep kernel),
+// [1] max(0, -min_j bias_j) (set by the bias kernel); both start at 0.
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax_bits, hipStream_t s);
+                                 const unsigned *cmax_bits, unsigned *stats, hipStream_t s);
 hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s);
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
-                           float *hn, int64_t *cn, hipStream_t s);
+                           float *hn, int64_t *cn, unsigned *stats, hipStream_t s);
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
-                           uint8_t *x8, float *row_scale, hipStream_t s);
+                           uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s);
+bool reg_path(const SearchGeom &g);
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const unsigned *cmax_bits, uint32_t *labels, float *dist,
-                         uint32_t *cand, hipStream_t s);
+                         const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
+                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s);
 hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,
                           const float *bias, size_t k, const uint32_t *cand, uint32_t *labels, float *dist,
                           hipStream_t s);

@@ -90,6 +90,9 @@ struct SearchArgs {
     const uint8_t *x8;
     const float *row_scale;
     const unsigned *cmax_bits;
+    // [0]: max_j |c_j|^2, [1]: max(0, -min_j bias_j), both as f32 bits
+    const unsigned *stats;
+    const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
     uint32_t *labels;
     float *dist;
     uint32_t *cand;
@@ -335,11 +338,13 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                             int32_t dv = nb == 0 ? acc0[r] : acc1[r];
                             s = sCn[il] - 2 * (int64_t)dv;
                         } else if constexpr (OP == OP_FP8) {
+                            // operands are -c * s_C and x * s_row: acc ~ -s_row s_C x.c
                             float dv = nb == 0 ? acc0[r] : acc1[r];
-                            s = fmaf(srs, sHn[il], -dv);
+                            s = fmaf(srs, sHn[il], dv);
                         } else {
+                            // operands are -c and x: acc = -x.c
                             float dv = nb == 0 ? acc0[r] : acc1[r];
-                            s = sHn[il] - dv;
+                            s = sHn[il] + dv;
                         }
                         if constexpr (M == 1) {
                             if (s < bs[0]) {
@@ -382,6 +387,185 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         } else {
 #pragma unroll
             for (int t = 0; t < M; t++) a.cand[my_row * M + t] = bi[t];
+        }
+    }
+}
+
+// ------------------------------------------- register-resident rescoring ----
+//
+// The BF16 / FP8 candidate search for rows of at most 256 operand bytes.
+// Each wave holds FR 32-row fragments of B in VGPRs for the whole centroid
+// sweep; centroid tiles are double-buffered in LDS (one barrier per tile).
+// The accumulators start at |c|^2/2 + bias/2 + B, where B is a per-workgroup
+// offset that keeps every score positive (B >= max |x|^2/2 plus the bound on
+// the operands' rounding error), so a score's f32 bits order like the score.
+// Each lane then replaces the low 5 bits of its 32 scores of a tile by their
+// position, takes one integer min (value and position at once) and inserts
+// that tile minimum into its running top-m; the rescoring kernel settles the
+// exact FP32 order of the m candidates.
+template <int OP, int ROWS, int M, int NS, int FR>
+__global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
+    constexpr int ES = OpT<OP>::ES, LB = OpT<OP>::LB;
+    constexpr int RB = 32 * NS;    // operand bytes per row
+    constexpr int RS = RB + 16;    // LDS row stride
+    constexpr int HB = RB / 2;     // bytes per lane half
+    constexpr int STEPS = HB / LB;
+    constexpr int PER_T = NS / 2;  // uint4 of a 64-centroid tile per thread
+    static_assert(NS % 2 == 0 && HB % LB == 0, "shape");
+    __shared__ __align__(16) uint8_t sC[2][BN * RS];
+    __shared__ float sH[2][BN];
+    __shared__ float sRed[4];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = lane >> 5, j = lane & 31;
+    const size_t row0 = (size_t)blockIdx.x * (128 * FR);
+
+    // Row fragments into registers, and the largest |x|^2 of the workgroup.
+    uint4 xb[FR][NS];
+    int sb[FR];
+    float nmax = 0.0f;
+#pragma unroll
+    for (int fr = 0; fr < FR; fr++) {
+        const size_t r = row0 + (size_t)(wave * FR + fr) * 32 + j;
+        const bool valid = r < a.m;
+        const size_t src = valid ? (a.ids ? (size_t)a.ids[r] : r) : 0;
+#pragma unroll
+        for (int q = 0; q < NS; q++) xb[fr][q] = row_segment<OP, ROWS>(a, r, src, (size_t)(h * HB + 16 * q) / ES, valid);
+        float part = 0.0f;
+        if constexpr (OP == OP_FP8) {
+            if (valid && h == 0) part = a.row_norm[r];
+        } else if (valid) {
+            const size_t e0 = (size_t)h * (HB / ES), e1 = std::min(a.d, e0 + HB / ES);
+            for (size_t e = e0; e < e1; e++) {
+                float v = src_elem<ROWS>(a.x, src * a.d + e);
+                part = fmaf(v, v, part);
+            }
+        }
+        part += __shfl_xor(part, 32, 64);
+        nmax = fmaxf(nmax, part);
+        sb[fr] = 127;
+        if constexpr (OP == OP_FP8) {
+            // E8M0 scale 1 / s_row: undoes the row's power-of-two encoding scale
+            const uint32_t eb = valid ? ((__float_as_uint(a.row_scale[r]) >> 23) & 0xff) : 127u;
+            sb[fr] = 254 - (int)eb;
+        }
+    }
+    for (int off = 16; off > 0; off >>= 1) nmax = fmaxf(nmax, __shfl_xor(nmax, off, 64));
+    if (lane == 0) sRed[wave] = nmax;
+    __syncthreads();
+    nmax = fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3]));
+    const float cmaxn = __uint_as_float(a.stats[0]);
+    const float negb = __uint_as_float(a.stats[1]);
+    constexpr float coef = OP == OP_FP8 ? 0.25f : 0.0078125f;
+    const float B = 0.5f * nmax + coef * sqrtf(nmax * cmaxn) + 0.5f * negb + 1.0f;
+    int sa = 127;
+    if constexpr (OP == OP_FP8) {
+        const float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
+        sa = 254 - (int)((__float_as_uint(cs) >> 23) & 0xff);
+    }
+
+    const size_t stride = a.d_pad * ES;  // == RB
+    uint4 st[PER_T];
+    float hv = 0.0f;
+    auto gload = [&](size_t c0) {
+#pragma unroll
+        for (int q = 0; q < PER_T; q++) {
+            const int u = tid + 256 * q, r = u / (RB / 16), sg = u - r * (RB / 16);
+            st[q] = *(const uint4 *)(a.cen_op + (c0 + r) * stride + sg * 16);
+        }
+        hv = tid < BN ? a.hn[c0 + tid] : 0.0f;
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PER_T; q++) {
+            const int u = tid + 256 * q, r = u / (RB / 16), sg = u - r * (RB / 16);
+            *(uint4 *)(&sC[buf][r * RS + sg * 16]) = st[q];
+        }
+        if (tid < BN) sH[buf][tid] = hv + B;
+    };
+
+    uint32_t bs[FR][M], bi[FR][M];
+#pragma unroll
+    for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            bs[fr][t] = 0xFFFFFFFFu;
+            bi[fr][t] = NO_LABEL;
+        }
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    const int ntiles = (int)(a.k_pad / BN);
+    for (int t = 0; t < ntiles; t++) {
+        const int buf = t & 1;
+        const size_t c0 = (size_t)t * BN;
+        if (t + 1 < ntiles) gload(c0 + BN);
+        f32x16 acc[FR][2];
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const float v = sH[buf][nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
+            }
+        const uint8_t *pa0 = &sC[buf][j * RS + h * HB];
+        const uint8_t *pa1 = &sC[buf][(32 + j) * RS + h * HB];
+#pragma unroll
+        for (int s = 0; s < STEPS; s++) {
+            if constexpr (OP == OP_BF16) {
+                const bf16x8 c0v = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa0 + s * LB));
+                const bf16x8 c1v = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa1 + s * LB));
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++) {
+                    const bf16x8 xv = __builtin_bit_cast(bf16x8, xb[fr][s]);
+                    acc[fr][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c0v, xv, acc[fr][0], 0, 0, 0);
+                    acc[fr][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c1v, xv, acc[fr][1], 0, 0, 0);
+                }
+            } else {  // OP_FP8
+                const uint4 a0 = *(const uint4 *)(pa0 + s * LB), a1 = *(const uint4 *)(pa0 + s * LB + 16);
+                const uint4 b0 = *(const uint4 *)(pa1 + s * LB), b1 = *(const uint4 *)(pa1 + s * LB + 16);
+                const i32x8 c0v = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+                const i32x8 c1v = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++) {
+                    const uint4 x0 = xb[fr][2 * s], x1 = xb[fr][2 * s + 1];
+                    const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+                    acc[fr][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c0v, xv, acc[fr][0], 0, 0, 0, sa, 0, sb[fr]);
+                    acc[fr][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c1v, xv, acc[fr][1], 0, 0, 0, sa, 0, sb[fr]);
+                }
+            }
+        }
+        // tile minimum per row: value and position in one unsigned min
+#pragma unroll
+        for (int fr = 0; fr < FR; fr++) {
+            uint32_t g = 0xFFFFFFFFu;
+#pragma unroll
+            for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                    g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
+            const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
+            const uint32_t gi = (uint32_t)c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+        }
+        if (t + 1 < ntiles) sstore(buf ^ 1);
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int fr = 0; fr < FR; fr++) {
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            const uint32_t ps = (uint32_t)__shfl_xor((int)bs[fr][t], 32, 64);
+            const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][t], 32, 64);
+            if (h == 0) topm_insert_lex<M, uint32_t>(bs[fr], bi[fr], ps, pi);
+        }
+        const size_t r = row0 + (size_t)(wave * FR + fr) * 32 + j;
+        if (h == 0 && r < a.m) {
+#pragma unroll
+            for (int t = 0; t < M; t++) a.cand[r * M + t] = bi[fr][t];
         }
     }
 }
@@ -436,7 +620,8 @@ __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, s
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
-                                                             int64_t *norm64, const unsigned *cmax_bits) {
+                                                             int64_t *norm64, const unsigned *cmax_bits,
+                                                             unsigned *stats) {
     const int lane = threadIdx.x & 63;
     const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (jc >= k_pad) return;
@@ -444,13 +629,14 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     const float *cp = c + jc * d;
     float cs = 1.0f;
     if constexpr (OP == OP_FP8) cs = fp8_scale_for(__uint_as_float(*cmax_bits));
-    // operand row
+    // operand row: the float searches store -c, so that an MFMA accumulator
+    // started at |c|^2/2 ends at |c|^2/2 - x.c
     if constexpr (OP == OP_F32) {
         float *o = (float *)cen_op + jc * d_pad;
-        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? cp[e] : 0.0f;
+        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? -cp[e] : 0.0f;
     } else if constexpr (OP == OP_BF16) {
         uint16_t *o = (uint16_t *)cen_op + jc * d_pad;
-        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? bf16_rne(cp[e]) : (uint16_t)0;
+        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? bf16_rne(-cp[e]) : (uint16_t)0;
     } else if constexpr (OP == OP_FP8) {
         uint32_t *o = (uint32_t *)(cen_op + jc * d_pad);
         for (size_t w = lane; w < d_pad / 4; w += 64) {
@@ -458,7 +644,7 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 size_t e = 4 * w + q;
-                v[q] = (live && e < d) ? cp[e] * cs : 0.0f;
+                v[q] = (live && e < d) ? -cp[e] * cs : 0.0f;
             }
             int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
             pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
@@ -492,7 +678,9 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
             if (lane == 0) {
                 float tail = 0.0f;
                 for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
-                norm32[jc] = __fadd_rn(s, tail);
+                const float nv = __fadd_rn(s, tail);
+                norm32[jc] = nv;
+                if (nv >= 0.0f) atomicMax(&stats[0], __float_as_uint(nv));  // NaN norms are skipped
             }
         }
     }
@@ -512,9 +700,10 @@ __global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) 
 // cn_j = sum c8^2 + round(bias_j) (src/cpu_i8.rs:112-117, unit 1). Pad
 // centroids get a score that never wins.
 __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *norm32, const int64_t *norm64,
-                                const float *bias, float *hn, int64_t *cn) {
+                                const float *bias, float *hn, int64_t *cn, unsigned *stats) {
     size_t jc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (jc >= k_pad) return;
+    if (bias && jc < k && bias[jc] < 0.0f) atomicMax(&stats[1], __float_as_uint(-bias[jc]));
     if (op == OP_I8) {
         int64_t v = PAD_SCORE64;
         if (jc < k) {
@@ -536,17 +725,24 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
 // Per-row e4m3 encoding with a power-of-two scale: one wave per row.
 template <int ROWS>
 __global__ __launch_bounds__(256) void rows_fp8_kernel(const void *x, size_t m, size_t d, size_t d_pad,
-                                                       const uint32_t *ids, uint8_t *x8, float *row_scale) {
+                                                       const uint32_t *ids, uint8_t *x8, float *row_scale,
+                                                       float *row_norm) {
     const int lane = threadIdx.x & 63;
     const size_t r = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= m) return;
     const size_t src = ids ? (size_t)ids[r] : r;
-    float mx = 0.0f;
+    float mx = 0.0f, sq = 0.0f;
     for (size_t e = lane; e < d; e += 64) {
-        float a = fabsf(src_elem<ROWS>(x, src * d + e));
+        const float v = src_elem<ROWS>(x, src * d + e);
+        const float a = fabsf(v);
         if (a > mx && isfinite(a)) mx = a;
+        sq = fmaf(v, v, sq);
     }
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    for (int off = 32; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        sq += __shfl_xor(sq, off, 64);
+    }
+    if (lane == 0) row_norm[r] = sq;
     const float s = fp8_scale_for(mx);
     uint32_t *o = (uint32_t *)(x8 + r * d_pad);
     for (size_t w = lane; w < d_pad / 4; w += 64) {
@@ -598,12 +794,49 @@ static hipError_t launch_m(const SearchGeom &g, int M, const SearchArgs &a, hipS
     }
 }
 
+// Register-resident BF16/FP8 search: rows of 64..256 operand bytes.
+bool reg_path(const SearchGeom &g) {
+    return (g.op == OP_BF16 || g.op == OP_FP8) && g.resident && g.row_bytes <= 256 && g.row_bytes % 64 == 0;
+}
+
+template <int OP, int ROWS, int M, int NS>
+static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
+    constexpr int FR = 2;
+    dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)));
+    hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int OP, int ROWS, int M>
+static hipError_t launch_reg_ns(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
+    switch (g.row_bytes / 32) {
+        case 2: return launch_reg<OP, ROWS, M, 2>(a, s);
+        case 4: return launch_reg<OP, ROWS, M, 4>(a, s);
+        case 6: return launch_reg<OP, ROWS, M, 6>(a, s);
+        case 8: return launch_reg<OP, ROWS, M, 8>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int OP, int ROWS>
+static hipError_t launch_reg_m(const SearchGeom &g, int M, const SearchArgs &a, hipStream_t s) {
+    switch (M) {
+        case 1: return launch_reg_ns<OP, ROWS, 1>(g, a, s);
+        case 2: return launch_reg_ns<OP, ROWS, 2>(g, a, s);
+        case 4: return launch_reg_ns<OP, ROWS, 4>(g, a, s);
+        case 8: return launch_reg_ns<OP, ROWS, 8>(g, a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const unsigned *cmax_bits, uint32_t *labels, float *dist,
-                         uint32_t *cand, hipStream_t s) {
+                         const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
+                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s) {
     if (m == 0) return hipSuccess;
     SearchArgs a;
+    a.stats = stats;
+    a.row_norm = row_norm;
     a.x = x;
     a.ids = ids;
     a.m = m;
@@ -628,10 +861,16 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
             if (rows == EMH_ROWS_I8) return launch_res<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
             return launch_res<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
         case OP_BF16:
+            if (reg_path(g)) {
+                if (rows == EMH_ROWS_F32) return launch_reg_m<OP_BF16, EMH_ROWS_F32>(g, M, a, s);
+                if (rows == EMH_ROWS_I8) return launch_reg_m<OP_BF16, EMH_ROWS_I8>(g, M, a, s);
+                return launch_reg_m<OP_BF16, EMH_ROWS_U8>(g, M, a, s);
+            }
             if (rows == EMH_ROWS_F32) return launch_m<OP_BF16, EMH_ROWS_F32>(g, M, a, s);
             if (rows == EMH_ROWS_I8) return launch_m<OP_BF16, EMH_ROWS_I8>(g, M, a, s);
             return launch_m<OP_BF16, EMH_ROWS_U8>(g, M, a, s);
         case OP_FP8:
+            if (reg_path(g)) return launch_reg_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
             return launch_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
         case OP_I8:
             if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
@@ -665,13 +904,14 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 }
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax, hipStream_t s) {
+                                 const unsigned *cmax, unsigned *stats, hipStream_t s) {
     dim3 grid((unsigned)((g.k_pad + 3) / 4));
+    uint8_t *op = (uint8_t *)cen_op;
     switch (g.op) {
-        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
-        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
-        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
-        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, (uint8_t *)cen_op, norm32, norm64, cmax); break;
+        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
+        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
+        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
+        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -685,19 +925,19 @@ hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipSt
 }
 
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
-                           float *hn, int64_t *cn, hipStream_t s) {
+                           float *hn, int64_t *cn, unsigned *stats, hipStream_t s) {
     dim3 grid((unsigned)((g.k_pad + 255) / 256));
-    hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn);
+    hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn, stats);
     return hipGetLastError();
 }
 
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
-                           uint8_t *x8, float *row_scale, hipStream_t s) {
+                           uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s) {
     if (m == 0) return hipSuccess;
     dim3 grid((unsigned)((m + 3) / 4));
-    if (rows == EMH_ROWS_F32) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
-    else if (rows == EMH_ROWS_I8) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
-    else hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale);
+    if (rows == EMH_ROWS_F32) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
+    else if (rows == EMH_ROWS_I8) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
+    else hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
     return hipGetLastError();
 }
 
