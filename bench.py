@@ -116,11 +116,12 @@ def judge(x, c, res, rows):
     d64 = ((xs - cd[lab]) ** 2).sum(1)
     best_exact = ((xs - cd[best.indices]) ** 2).sum(1)
     wrong = d64 > best_exact + 1e-6 * (1 + best_exact)
-    rel = (res["dist"][ids].double() - d64).abs() / d64.clamp_min(1e-12)
+    # error relative to 1 + d, the form of the reference's tolerance (src/cpu.rs:303)
+    rel = (res["dist"][ids].double() - d64).abs() / (1.0 + d64)
     return {
         "label_disagreement": float((lab != best.indices).double().mean()),
         "true_label_errors": float(wrong.double().mean()),
-        "dist_rel_err_max": float(rel.max()),
+        "dist_err_max_rel_1pd": float(rel.max()),
         "judged_rows": rows,
     }
 

@@ -32,6 +32,9 @@ CFLAGS = [
     "-Wno-unused-value",
     # packed FP32 VALU ops stay off (hipcc's host pass ignores the feature)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in arch VGPRs: the search epilogues read every score
+    # with VALU ops, which cannot take AGPR operands
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
 ]
 
 

@@ -23,13 +23,10 @@ constexpr int ROW_LDS_BUDGET = 64 * 1024;  // bytes of LDS for a resident row bl
 constexpr int MAX_M = 8;        // top-m candidates kept by the rescoring searches
 
 inline int op_elem_bytes(int op) { return op == OP_F32 ? 4 : (op == OP_BF16 ? 2 : 1); }
-// Row padding unit in bytes: at least what one lane feeds one MFMA, times
-// the two lane halves; the float rescoring searches pad to 64 bytes so a
-// half row is a whole number of 16-byte register quads.
-inline int op_granule(int op) {
-    int g = op == OP_F32 ? 8 : (op == OP_BF16 ? 64 : (op == OP_FP8 ? 64 : 32));
-    return g < 16 ? 16 : g;
-}
+// Row padding unit in bytes: each lane half of a row must be a whole number
+// of 16-byte quads (the MFMA loops read halves quad by quad), and the BF16 /
+// FP8 searches pad to 64 bytes so a half is an even number of quads.
+inline int op_granule(int op) { return (op == OP_BF16 || op == OP_FP8) ? 64 : 32; }
 
 struct SearchGeom {
     int op;

@@ -93,6 +93,7 @@ struct SearchArgs {
     // [0]: max_j |c_j|^2, [1]: max(0, -min_j bias_j), both as f32 bits
     const unsigned *stats;
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
+    int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
     uint32_t *labels;
     float *dist;
     uint32_t *cand;
@@ -109,6 +110,7 @@ __device__ inline uint4 row_segment(const SearchArgs &a, size_t call_row, size_t
         return *(const uint4 *)(a.x8 + call_row * a.d_pad + e0);
     } else if constexpr (OP == OP_I8) {
         const int8_t *p = (const int8_t *)a.x + src_row * a.d;
+        if (a.vec && e0 + 16 <= a.d) return *(const uint4 *)(p + e0);
         uint32_t w[4];
 #pragma unroll
         for (int q = 0; q < 4; q++) {
@@ -125,12 +127,25 @@ __device__ inline uint4 row_segment(const SearchArgs &a, size_t call_row, size_t
     } else if constexpr (OP == OP_F32) {
         float v[4];
         const size_t base = src_row * a.d;
+        if constexpr (ROWS == EMH_ROWS_F32) {
+            if (a.vec && e0 + 4 <= a.d) return *(const uint4 *)((const float *)a.x + base + e0);
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) v[q] = (e0 + q < a.d) ? src_elem<ROWS>(a.x, base + e0 + q) : 0.0f;
         return make_uint4(__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]), __float_as_uint(v[3]));
     } else {  // OP_BF16
         uint32_t w[4];
         const size_t base = src_row * a.d;
+        if constexpr (ROWS == EMH_ROWS_F32) {
+            if (a.vec && e0 + 8 <= a.d) {
+                const float4 lo = *(const float4 *)((const float *)a.x + base + e0);
+                const float4 hi = *(const float4 *)((const float *)a.x + base + e0 + 4);
+                return make_uint4((uint32_t)bf16_rne(lo.x) | ((uint32_t)bf16_rne(lo.y) << 16),
+                                  (uint32_t)bf16_rne(lo.z) | ((uint32_t)bf16_rne(lo.w) << 16),
+                                  (uint32_t)bf16_rne(hi.x) | ((uint32_t)bf16_rne(hi.y) << 16),
+                                  (uint32_t)bf16_rne(hi.z) | ((uint32_t)bf16_rne(hi.w) << 16));
+            }
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
             size_t e = e0 + 2 * q;
@@ -393,27 +408,96 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 
 // ------------------------------------------- register-resident rescoring ----
 //
-// The BF16 / FP8 candidate search for rows of at most 256 operand bytes.
+// The BF16 / FP8 candidate search for rows of 64..256 operand bytes.
 // Each wave holds FR 32-row fragments of B in VGPRs for the whole centroid
-// sweep; centroid tiles are double-buffered in LDS (one barrier per tile).
-// The accumulators start at |c|^2/2 + bias/2 + B, where B is a per-workgroup
-// offset that keeps every score positive (B >= max |x|^2/2 plus the bound on
-// the operands' rounding error), so a score's f32 bits order like the score.
-// Each lane then replaces the low 5 bits of its 32 scores of a tile by their
-// position, takes one integer min (value and position at once) and inserts
-// that tile minimum into its running top-m; the rescoring kernel settles the
-// exact FP32 order of the m candidates.
+// sweep. Centroid tiles stream into two LDS buffers with direct global->LDS
+// loads (no staging registers); each wave-load writes 1 KiB of LDS
+// contiguously, so rows are not padded but XOR-swizzled by quad
+// (quad q of row r sits in slot q ^ (r & (G-1))) to spread a fragment read
+// over the banks. The accumulators start at |c|^2/2 + bias/2 + B, where B is
+// a per-workgroup offset that keeps every score positive (B >= max |x|^2/2
+// plus a bound on the operands' rounding error), so a score's f32 bits order
+// like the score. Each lane replaces the low 5 bits of its 32 scores of a
+// tile by their position, takes one integer min (value and position at
+// once) and inserts that tile minimum into its running top-m; the rescoring
+// kernel settles the exact FP32 order of the m candidates.
+template <int OP, int NQ, int LBQ> struct RegTile {
+    static constexpr int G = (NQ & -NQ) < 16 ? (NQ & -NQ) : 16;  // swizzle span (power of two dividing NQ)
+    __device__ static inline int slot(int row, int q) { return q ^ (row & (G - 1)); }
+};
+
+template <int OP, int M, int NS, int FR, typename Acc>
+__device__ __forceinline__ void reg_tile(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
+                                         const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
+                                         uint32_t (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
+    constexpr int LB = OpT<OP>::LB, RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2, LBQ = LB / 16;
+    constexpr int STEPS = HQ / LBQ;
+    using T = RegTile<OP, NQ, LBQ>;
+    Acc acc[FR][2];
+#pragma unroll
+    for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const float v = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
+        }
+    const uint8_t *row0 = sC + j * RB;          // centroid j of block 0
+    const uint8_t *row1 = sC + (32 + j) * RB;   // centroid j of block 1 (same swizzle: 32 is a multiple of G)
+#pragma unroll
+    for (int s = 0; s < STEPS; s++) {
+        const int q0 = T::slot(j, h * HQ + s * LBQ);
+        if constexpr (OP == OP_BF16) {
+            const bf16x8 c0v = __builtin_bit_cast(bf16x8, *(const uint4 *)(row0 + 16 * q0));
+            const bf16x8 c1v = __builtin_bit_cast(bf16x8, *(const uint4 *)(row1 + 16 * q0));
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) {
+                const bf16x8 xv = __builtin_bit_cast(bf16x8, xb[fr][s]);
+                acc[fr][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c0v, xv, acc[fr][0], 0, 0, 0);
+                acc[fr][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c1v, xv, acc[fr][1], 0, 0, 0);
+            }
+        } else {  // OP_FP8: two quads per lane per MFMA
+            const int q1 = T::slot(j, h * HQ + s * LBQ + 1);
+            const uint4 a0 = *(const uint4 *)(row0 + 16 * q0), a1 = *(const uint4 *)(row0 + 16 * q1);
+            const uint4 b0 = *(const uint4 *)(row1 + 16 * q0), b1 = *(const uint4 *)(row1 + 16 * q1);
+            const i32x8 c0v = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+            const i32x8 c1v = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) {
+                const uint4 x0 = xb[fr][2 * s], x1 = xb[fr][2 * s + 1];
+                const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+                acc[fr][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c0v, xv, acc[fr][0], 0, 0, 0, sa, 0, sb[fr]);
+                acc[fr][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c1v, xv, acc[fr][1], 0, 0, 0, sa, 0, sb[fr]);
+            }
+        }
+    }
+    // tile minimum per row: value and position in one unsigned min
+#pragma unroll
+    for (int fr = 0; fr < FR; fr++) {
+        uint32_t g = 0xFFFFFFFFu;
+#pragma unroll
+        for (int nb = 0; nb < 2; nb++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
+        const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
+        const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+    }
+}
+
 template <int OP, int ROWS, int M, int NS, int FR>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     constexpr int ES = OpT<OP>::ES, LB = OpT<OP>::LB;
     constexpr int RB = 32 * NS;    // operand bytes per row
-    constexpr int RS = RB + 16;    // LDS row stride
+    constexpr int NQ = RB / 16;    // 16-byte quads per row
     constexpr int HB = RB / 2;     // bytes per lane half
-    constexpr int STEPS = HB / LB;
-    constexpr int PER_T = NS / 2;  // uint4 of a 64-centroid tile per thread
-    static_assert(NS % 2 == 0 && HB % LB == 0, "shape");
-    __shared__ __align__(16) uint8_t sC[2][BN * RS];
-    __shared__ float sH[2][BN];
+    constexpr int PER_T = NQ * BN / 256;  // quads of a tile per thread
+    using T = RegTile<OP, NQ, LB / 16>;
+    static_assert(NS % 2 == 0 && HB % LB == 0 && PER_T >= 1, "shape");
+    __shared__ __align__(16) uint8_t sC0[BN * RB];
+    __shared__ __align__(16) uint8_t sC1[BN * RB];
+    __shared__ float sH0[BN], sH1[BN];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -464,24 +548,23 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
         sa = 254 - (int)((__float_as_uint(cs) >> 23) & 0xff);
     }
 
-    const size_t stride = a.d_pad * ES;  // == RB
-    uint4 st[PER_T];
-    float hv = 0.0f;
-    auto gload = [&](size_t c0) {
+    // Direct global->LDS tile loads: wave-load q of wave w fills LDS quads
+    // [(4q + w) * 64, +64); lane L fetches the centroid quad that belongs in
+    // LDS quad p = (4q + w) * 64 + L (row p / NQ, slot p % NQ, quad slot ^ swz).
+    int goff[PER_T];
 #pragma unroll
-        for (int q = 0; q < PER_T; q++) {
-            const int u = tid + 256 * q, r = u / (RB / 16), sg = u - r * (RB / 16);
-            st[q] = *(const uint4 *)(a.cen_op + (c0 + r) * stride + sg * 16);
-        }
-        hv = tid < BN ? a.hn[c0 + tid] : 0.0f;
-    };
-    auto sstore = [&](int buf) {
+    for (int q = 0; q < PER_T; q++) {
+        const int p = (4 * q + wave) * 64 + lane, r = p / NQ, sl = p - r * NQ;
+        goff[q] = (r * NQ + T::slot(r, sl)) * 16;
+    }
+    const int ntiles = (int)(a.k_pad / BN);
+    auto dma = [&](uint8_t *dst, int t) {
+        const uint8_t *g = a.cen_op + (size_t)t * (BN * RB);
 #pragma unroll
-        for (int q = 0; q < PER_T; q++) {
-            const int u = tid + 256 * q, r = u / (RB / 16), sg = u - r * (RB / 16);
-            *(uint4 *)(&sC[buf][r * RS + sg * 16]) = st[q];
-        }
-        if (tid < BN) sH[buf][tid] = hv + B;
+        for (int q = 0; q < PER_T; q++)
+            __builtin_amdgcn_global_load_lds((const void *)(g + goff[q]),
+                                             (__attribute__((address_space(3))) void *)(dst + (4 * q + wave) * 1024),
+                                             16, 0, 0);
     };
 
     uint32_t bs[FR][M], bi[FR][M];
@@ -493,64 +576,36 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             bi[fr][t] = NO_LABEL;
         }
 
-    gload(0);
-    sstore(0);
+    float hv = tid < BN ? a.hn[tid] : 0.0f;
+    dma(sC0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+    if (tid < BN) sH0[tid] = hv + B;
     __syncthreads();
-    const int ntiles = (int)(a.k_pad / BN);
-    for (int t = 0; t < ntiles; t++) {
-        const int buf = t & 1;
-        const size_t c0 = (size_t)t * BN;
-        if (t + 1 < ntiles) gload(c0 + BN);
-        f32x16 acc[FR][2];
-#pragma unroll
-        for (int nb = 0; nb < 2; nb++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const float v = sH[buf][nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
-#pragma unroll
-                for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
-            }
-        const uint8_t *pa0 = &sC[buf][j * RS + h * HB];
-        const uint8_t *pa1 = &sC[buf][(32 + j) * RS + h * HB];
-#pragma unroll
-        for (int s = 0; s < STEPS; s++) {
-            if constexpr (OP == OP_BF16) {
-                const bf16x8 c0v = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa0 + s * LB));
-                const bf16x8 c1v = __builtin_bit_cast(bf16x8, *(const uint4 *)(pa1 + s * LB));
-#pragma unroll
-                for (int fr = 0; fr < FR; fr++) {
-                    const bf16x8 xv = __builtin_bit_cast(bf16x8, xb[fr][s]);
-                    acc[fr][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c0v, xv, acc[fr][0], 0, 0, 0);
-                    acc[fr][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c1v, xv, acc[fr][1], 0, 0, 0);
-                }
-            } else {  // OP_FP8
-                const uint4 a0 = *(const uint4 *)(pa0 + s * LB), a1 = *(const uint4 *)(pa0 + s * LB + 16);
-                const uint4 b0 = *(const uint4 *)(pa1 + s * LB), b1 = *(const uint4 *)(pa1 + s * LB + 16);
-                const i32x8 c0v = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-                const i32x8 c1v = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-#pragma unroll
-                for (int fr = 0; fr < FR; fr++) {
-                    const uint4 x0 = xb[fr][2 * s], x1 = xb[fr][2 * s + 1];
-                    const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
-                    acc[fr][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c0v, xv, acc[fr][0], 0, 0, 0, sa, 0, sb[fr]);
-                    acc[fr][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c1v, xv, acc[fr][1], 0, 0, 0, sa, 0, sb[fr]);
-                }
-            }
+    for (int t = 0; t < ntiles; t += 2) {
+        // even tile from buffer 0; the next tile streams into buffer 1
+        const bool more1 = t + 1 < ntiles;
+        if (more1) {
+            dma(sC1, t + 1);
+            hv = tid < BN ? a.hn[(size_t)(t + 1) * BN + tid] : 0.0f;
         }
-        // tile minimum per row: value and position in one unsigned min
-#pragma unroll
-        for (int fr = 0; fr < FR; fr++) {
-            uint32_t g = 0xFFFFFFFFu;
-#pragma unroll
-            for (int nb = 0; nb < 2; nb++)
-#pragma unroll
-                for (int r = 0; r < 16; r++)
-                    g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
-            const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
-            const uint32_t gi = (uint32_t)c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+        reg_tile<OP, M, NS, FR, f32x16>(sC0, sH0, xb, sb, sa, h, j, (uint32_t)(t * BN), bs, bi);
+        if (more1) {
+            __builtin_amdgcn_s_waitcnt(0);
+            if (tid < BN) sH1[tid] = hv + B;
         }
-        if (t + 1 < ntiles) sstore(buf ^ 1);
+        __syncthreads();
+        if (!more1) break;
+        // odd tile from buffer 1; the next tile streams into buffer 0
+        const bool more2 = t + 2 < ntiles;
+        if (more2) {
+            dma(sC0, t + 2);
+            hv = tid < BN ? a.hn[(size_t)(t + 2) * BN + tid] : 0.0f;
+        }
+        reg_tile<OP, M, NS, FR, f32x16>(sC1, sH1, xb, sb, sa, h, j, (uint32_t)((t + 1) * BN), bs, bi);
+        if (more2) {
+            __builtin_amdgcn_s_waitcnt(0);
+            if (tid < BN) sH0[tid] = hv + B;
+        }
         __syncthreads();
     }
 
@@ -572,43 +627,71 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
 
 // -------------------------------------------------------------- rescore ----
 
-// Exact FP32 distances to the m candidates of each row; lowest (score, index)
-// wins. dist = max(|x - c|^2 + bias, 0), summed in index order without FMA.
+// Exact FP32 distances |x - c|^2 + bias to the m candidates of each row;
+// the lowest (score, index) wins and dist = max(score, 0). One wave per row
+// at a time: 16 lanes per (row, candidate) pair stride over d with 16-byte
+// loads, each lane folds its elements in order without FMA, and a fixed
+// butterfly adds the 16 partial sums, so every lane holds identical bits.
 template <int ROWS, int M>
 __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, const uint32_t *ids,
                                                       const float *c, const float *bias, const uint32_t *cand,
                                                       uint32_t *labels, float *dist) {
-    const size_t g = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t row = g / M;
-    const int slot = (int)(g % M);
-    float s = INFINITY;
-    uint32_t ci = NO_LABEL;
-    if (row < m) {
-        ci = cand[row * M + slot];
-        if (ci != NO_LABEL) {
-            const size_t src = ids ? (size_t)ids[row] : row;
-            const float *cp = c + (size_t)ci * d;
-            float acc = 0.0f;
-            for (size_t e = 0; e < d; e++) {
-                float df = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
-                acc = __fadd_rn(acc, __fmul_rn(df, df));
-            }
-            if (bias) acc = __fadd_rn(acc, bias[ci]);
-            s = acc;
-        }
-    }
+    constexpr int G = 16, PPW = 64 / G, PASSES = (M + PPW - 1) / PPW;
+    const int lane = threadIdx.x & 63, grp = lane / G, gl = lane % G;
+    const size_t nwaves = (size_t)gridDim.x * 4;
+    const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0;
+    for (size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < m; row += nwaves) {
+        const size_t src = ids ? (size_t)ids[row] : row;
+        float best = INFINITY;
+        uint32_t bidx = NO_LABEL;
 #pragma unroll
-    for (int off = 1; off < M; off <<= 1) {
-        float os = __shfl_xor(s, off, 64);
-        uint32_t oi = (uint32_t)__shfl_xor((int)ci, off, 64);
-        if (os < s || (os == s && oi < ci)) {
-            s = os;
-            ci = oi;
+        for (int p = 0; p < PASSES; p++) {
+            const int slot = p * PPW + grp;
+            const uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
+            float acc = 0.0f;
+            if (ci != NO_LABEL) {
+                const float *cp = c + (size_t)ci * d;
+                if (vec) {
+                    const float *xr = (const float *)x + src * d;
+                    for (size_t e = 4 * gl; e < d; e += 4 * G) {
+                        const float4 xv = *(const float4 *)(xr + e), cv = *(const float4 *)(cp + e);
+                        float t0 = __fsub_rn(xv.x, cv.x), t1 = __fsub_rn(xv.y, cv.y);
+                        float t2 = __fsub_rn(xv.z, cv.z), t3 = __fsub_rn(xv.w, cv.w);
+                        acc = __fadd_rn(acc, __fmul_rn(t0, t0));
+                        acc = __fadd_rn(acc, __fmul_rn(t1, t1));
+                        acc = __fadd_rn(acc, __fmul_rn(t2, t2));
+                        acc = __fadd_rn(acc, __fmul_rn(t3, t3));
+                    }
+                } else {
+                    for (size_t e = gl; e < d; e += G) {
+                        const float t = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
+                        acc = __fadd_rn(acc, __fmul_rn(t, t));
+                    }
+                }
+            }
+#pragma unroll
+            for (int off = G / 2; off > 0; off >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, off, 64));
+            float s = INFINITY;
+            if (ci != NO_LABEL) s = bias ? __fadd_rn(acc, bias[ci]) : acc;
+            uint32_t si = ci;
+#pragma unroll
+            for (int off = G; off < 64; off <<= 1) {
+                const float os = __shfl_xor(s, off, 64);
+                const uint32_t oi = (uint32_t)__shfl_xor((int)si, off, 64);
+                if (os < s || (os == s && oi < si)) {
+                    s = os;
+                    si = oi;
+                }
+            }
+            if (s < best || (s == best && si < bidx)) {
+                best = s;
+                bidx = si;
+            }
         }
-    }
-    if (slot == 0 && row < m) {
-        labels[row] = ci == NO_LABEL ? 0u : ci;
-        dist[row] = (ci != NO_LABEL && s > 0.0f) ? s : 0.0f;
+        if (lane == 0) {
+            labels[row] = bidx == NO_LABEL ? 0u : bidx;
+            dist[row] = (bidx != NO_LABEL && best > 0.0f) ? best : 0.0f;
+        }
     }
 }
 
@@ -837,6 +920,8 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
     SearchArgs a;
     a.stats = stats;
     a.row_norm = row_norm;
+    const size_t rb = rows == EMH_ROWS_F32 ? 4 : 1;
+    a.vec = ((uintptr_t)x % 16 == 0) && ((g.d * rb) % 16 == 0);
     a.x = x;
     a.ids = ids;
     a.m = m;
@@ -883,7 +968,7 @@ template <int ROWS>
 static hipError_t rescore_rows(int M, const void *x, size_t m, size_t d, const uint32_t *ids, const float *c,
                                const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
                                hipStream_t s) {
-    dim3 grid((unsigned)((m * M + 255) / 256));
+    dim3 grid((unsigned)std::min<size_t>((m + 3) / 4, 16384));
     switch (M) {
         case 1: hipLaunchKernelGGL((rescore_kernel<ROWS, 1>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
         case 2: hipLaunchKernelGGL((rescore_kernel<ROWS, 2>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;

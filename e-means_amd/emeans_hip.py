@@ -104,6 +104,14 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: run `python e-means_amd/build.py` (or __graft_entry__.build())")
+    # PyTorch-ROCm ships its own HIP runtime under the same soname
+    # (libamdhip64.so.7); the first one loaded serves the whole process and
+    # torch only initialises its GPU with its own. Load torch's first when it
+    # is installed, so that torch tensors and this library share one runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, sz, i32, u64 = C.c_void_p, C.c_size_t, C.c_int, C.c_uint64
     L.emh_create.argtypes = [i32, i32, i32, C.POINTER(C.c_void_p)]

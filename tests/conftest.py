@@ -26,6 +26,9 @@ def gpu_available() -> bool:
 @pytest.fixture(scope="session")
 def hip():
     """The product library on cuda:0; a GPU test fails (not skips) without it."""
+    import torch
+
+    torch.cuda.init()  # torch's HIP runtime first: the library then shares it
     import emeans_hip
 
     emeans_hip.require_gpu()
