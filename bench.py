@@ -43,7 +43,22 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the FP32 / FP8 comparison passes")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(nbytes: int):
+    """A plain copy of known size: what FETCH_SIZE / WRITE_SIZE read for it
+    calibrates the counters of the search kernels."""
+    import torch
+
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
 
 
 def log(msg):
@@ -168,6 +183,9 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a GPU")
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     dev = torch.device("cuda:0")
     log(f"data: n={args.n} d={args.d} k={args.k}")
     x, c = make_data(args.n, args.d, args.k, args.blobs, dev)
