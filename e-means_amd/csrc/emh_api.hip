@@ -121,6 +121,18 @@ emh_assigner::~emh_assigner() {
     if (own_stream && stream) (void)hipStreamDestroy(stream);
 }
 
+// I8: whether every score fits the packed int32 key of the register-resident
+// kernel (flag written by the bias kernel); otherwise the i64 kernel runs.
+static int refresh_i8_range(emh_assigner *h) {
+    h->g.i8_packed = false;
+    if (h->op != OP_I8 || !h->g.reg || (int64_t)h->g.d * 32768 >= ((int64_t)1 << 26)) return EMH_OK;
+    uint32_t flag = 1;
+    int rc = check(hipMemcpy(&flag, h->stats.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost), "read range flag");
+    if (rc) return rc;
+    h->g.i8_packed = flag == 0;
+    return EMH_OK;
+}
+
 extern "C" {
 
 const char *emh_last_error(void) { return g_last_error.c_str(); }
@@ -230,13 +242,13 @@ int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t 
     if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
     if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
     if ((rc = check(h->cmax.ensure(4), "scale buffer"))) return rc;
-    if ((rc = check(h->stats.ensure(8), "stats buffer"))) return rc;
+    if ((rc = check(h->stats.ensure(12), "stats buffer"))) return rc;
     h->g = g;
     h->has_centroids = false;
     h->has_bias = false;
     if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4, hipMemcpyHostToDevice, h->stream), "copy centroids")))
         return rc;
-    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 2, 0u, h->stream), "fill"))) return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
     if (h->op == OP_FP8) {
         if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
         if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
@@ -253,6 +265,7 @@ int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t 
         return rc;
     // The caller may reuse `centroids` on return (pageable source).
     if ((rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;
+    if ((rc = refresh_i8_range(h))) return rc;
     h->has_centroids = true;
     return EMH_OK;
 }
@@ -265,12 +278,13 @@ int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
     if ((rc = check(h->bias.ensure(k * 4), "bias buffer"))) return rc;
     if ((rc = check(hipMemcpyAsync(h->bias.p, bias, k * 4, hipMemcpyHostToDevice, h->stream), "copy bias"))) return rc;
-    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>() + 1, 1, 0u, h->stream), "fill"))) return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>() + 1, 2, 0u, h->stream), "fill"))) return rc;
     if ((rc = check(launch_set_bias(h->g, h->norm32.as<float>(), h->norm64.as<int64_t>(), h->bias.as<float>(),
                                     h->hn.as<float>(), h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
                     "score offsets")))
         return rc;
     if ((rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;
+    if ((rc = refresh_i8_range(h))) return rc;
     h->has_bias = true;
     return EMH_OK;
 }

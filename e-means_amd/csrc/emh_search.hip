@@ -45,6 +45,21 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     g.k = k;
     g.k_pad = (k + BN - 1) / BN * BN;
     const size_t es = op_elem_bytes(op);
+    // Register-resident kernel: operand rows padded to 64 bytes, at most 256
+    // (BF16, FP8) or 512 (FP32, I8) bytes. The LDS kernel can run on the same
+    // layout (rows resident in LDS), which the I8 search falls back to when
+    // its scores exceed the packed-key range.
+    // (rows past 256 bytes use 32-centroid tiles, which need 128-byte multiples)
+    size_t rb64 = (d * es + 63) / 64 * 64;
+    if (rb64 > 256) rb64 = (rb64 + 127) / 128 * 128;
+    if (rb64 <= ((op == OP_BF16 || op == OP_FP8) ? 256u : 512u)) {
+        g.reg = true;
+        g.resident = true;
+        g.d_pad = rb64 / es;
+        g.row_bytes = (int)rb64;
+        return g;
+    }
+    g.reg = false;
     const size_t gran = op_granule(op);
     size_t res_bytes = (d * es + gran - 1) / gran * gran;
     g.resident = (size_t)BM * (res_bytes + 16) <= (size_t)ROW_LDS_BUDGET;
@@ -362,7 +377,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                             s = sHn[il] + dv;
                         }
                         if constexpr (M == 1) {
-                            if (s < bs[0]) {
+                            if (s < bs[0] && ci < a.k) {
                                 bs[0] = s;
                                 bi[0] = ci;
                             }
@@ -406,107 +421,194 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     }
 }
 
-// ------------------------------------------- register-resident rescoring ----
+// ------------------------------------------- register-resident searches ----
 //
-// The BF16 / FP8 candidate search for rows of 64..256 operand bytes.
-// Each wave holds FR 32-row fragments of B in VGPRs for the whole centroid
-// sweep. Centroid tiles stream into two LDS buffers with direct global->LDS
-// loads (no staging registers); each wave-load writes 1 KiB of LDS
-// contiguously, so rows are not padded but XOR-swizzled by quad
+// For operand rows of at most 256 (BF16, FP8) or 512 (FP32, I8) bytes. Each
+// wave holds FR 32-row fragments of B in VGPRs for the whole centroid sweep.
+// Centroid tiles of 32 * NB centroids stream into two LDS buffers with direct
+// global->LDS loads (no staging registers); each wave-load writes 1 KiB of
+// LDS contiguously, so rows are not padded but XOR-swizzled by 16-byte quad
 // (quad q of row r sits in slot q ^ (r & (G-1))) to spread a fragment read
-// over the banks. The accumulators start at |c|^2/2 + bias/2 + B, where B is
-// a per-workgroup offset that keeps every score positive (B >= max |x|^2/2
-// plus a bound on the operands' rounding error), so a score's f32 bits order
-// like the score. Each lane replaces the low 5 bits of its 32 scores of a
-// tile by their position, takes one integer min (value and position at
-// once) and inserts that tile minimum into its running top-m; the rescoring
-// kernel settles the exact FP32 order of the m candidates.
-template <int OP, int NQ, int LBQ> struct RegTile {
+// over the banks.
+//
+// BF16 / FP8 (candidate search): the accumulators start at
+// |c|^2/2 + bias/2 + B, where B is a per-workgroup offset keeping every score
+// positive (B >= max |x|^2/2 plus a bound on the operands' rounding error),
+// so a score's f32 bits order like the score. Each lane replaces the low 5
+// bits of its 16*NB scores of a tile by their position, takes one integer
+// min (value and position at once) and inserts that tile minimum into its
+// running top-m; the rescoring kernel settles the exact FP32 order.
+//
+// FP32 (exact): accumulators start at (|c|^2 + bias)/2 and the per-lane
+// running argmin takes every score with a strict '<' in ascending index.
+//
+// I8 (exact): the score is the integer 2 x.c8 - (sum c8^2 + round(bias)),
+// maximised; key = (score << 5) | (31 - position) orders by score, then by
+// lowest index, so one integer max per value settles the tile.
+template <int NQ> struct Swz {
     static constexpr int G = (NQ & -NQ) < 16 ? (NQ & -NQ) : 16;  // swizzle span (power of two dividing NQ)
     __device__ static inline int slot(int row, int q) { return q ^ (row & (G - 1)); }
 };
 
-template <int OP, int M, int NS, int FR, typename Acc>
+// Per-lane running result: raw score bits of the candidate searches, the
+// f32 score of FP32, the int32 score of I8.
+template <int OP> struct RegScore { using T = uint32_t; };
+template <> struct RegScore<OP_F32> { using T = float; };
+template <> struct RegScore<OP_I8> { using T = int32_t; };
+
+template <int OP, int M, int NS, int FR, int NB>
 __device__ __forceinline__ void reg_tile(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
                                          const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
-                                         uint32_t (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
-    constexpr int LB = OpT<OP>::LB, RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2, LBQ = LB / 16;
-    constexpr int STEPS = HQ / LBQ;
-    using T = RegTile<OP, NQ, LBQ>;
-    Acc acc[FR][2];
+                                         typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
+    constexpr int LB = OpT<OP>::LB, RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2;
+    using S = Swz<NQ>;
+    using Acc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
+    Acc acc[FR][NB];
+    if constexpr (OP == OP_I8) {
 #pragma unroll
-    for (int nb = 0; nb < 2; nb++)
+        for (int fr = 0; fr < FR; fr++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const float v = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+            for (int nb = 0; nb < NB; nb++) acc[fr][nb] = Acc{};
+    } else {
 #pragma unroll
-            for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
-        }
-    const uint8_t *row0 = sC + j * RB;          // centroid j of block 0
-    const uint8_t *row1 = sC + (32 + j) * RB;   // centroid j of block 1 (same swizzle: 32 is a multiple of G)
+        for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-    for (int s = 0; s < STEPS; s++) {
-        const int q0 = T::slot(j, h * HQ + s * LBQ);
-        if constexpr (OP == OP_BF16) {
-            const bf16x8 c0v = __builtin_bit_cast(bf16x8, *(const uint4 *)(row0 + 16 * q0));
-            const bf16x8 c1v = __builtin_bit_cast(bf16x8, *(const uint4 *)(row1 + 16 * q0));
+            for (int r = 0; r < 16; r++) {
+                const float v = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
+            }
+    }
+    const uint8_t *crow[NB];
+#pragma unroll
+    for (int nb = 0; nb < NB; nb++) crow[nb] = sC + (nb * 32 + j) * RB;  // same swizzle: 32 is a multiple of G
+    if constexpr (OP == OP_F32) {
+#pragma unroll
+        for (int s = 0; s < HQ; s++) {
+            float4 cv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) cv[nb] = *(const float4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s));
 #pragma unroll
             for (int fr = 0; fr < FR; fr++) {
-                const bf16x8 xv = __builtin_bit_cast(bf16x8, xb[fr][s]);
-                acc[fr][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c0v, xv, acc[fr][0], 0, 0, 0);
-                acc[fr][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(c1v, xv, acc[fr][1], 0, 0, 0);
+                const float4 xv = __builtin_bit_cast(float4, xb[fr][s]);
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) {
+                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].x, xv.x, acc[fr][nb], 0, 0, 0);
+                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].y, xv.y, acc[fr][nb], 0, 0, 0);
+                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].z, xv.z, acc[fr][nb], 0, 0, 0);
+                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].w, xv.w, acc[fr][nb], 0, 0, 0);
+                }
             }
-        } else {  // OP_FP8: two quads per lane per MFMA
-            const int q1 = T::slot(j, h * HQ + s * LBQ + 1);
-            const uint4 a0 = *(const uint4 *)(row0 + 16 * q0), a1 = *(const uint4 *)(row0 + 16 * q1);
-            const uint4 b0 = *(const uint4 *)(row1 + 16 * q0), b1 = *(const uint4 *)(row1 + 16 * q1);
-            const i32x8 c0v = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
-            const i32x8 c1v = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
+        }
+    } else if constexpr (OP == OP_BF16 || OP == OP_I8) {
+#pragma unroll
+        for (int s = 0; s < HQ; s++) {
+            uint4 cv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) cv[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s));
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) {
+                    if constexpr (OP == OP_BF16)
+                        acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                            __builtin_bit_cast(bf16x8, cv[nb]), __builtin_bit_cast(bf16x8, xb[fr][s]), acc[fr][nb], 0, 0, 0);
+                    else
+                        acc[fr][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                            __builtin_bit_cast(i32x4, cv[nb]), __builtin_bit_cast(i32x4, xb[fr][s]), acc[fr][nb], 0, 0, 0);
+                }
+        }
+    } else {  // OP_FP8: two quads per lane per MFMA
+#pragma unroll
+        for (int s = 0; s < HQ / 2; s++) {
+            i32x8 cv[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++) {
+                const uint4 a0 = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + 2 * s));
+                const uint4 a1 = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + 2 * s + 1));
+                cv[nb] = i32x8{(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+            }
 #pragma unroll
             for (int fr = 0; fr < FR; fr++) {
                 const uint4 x0 = xb[fr][2 * s], x1 = xb[fr][2 * s + 1];
                 const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
-                acc[fr][0] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c0v, xv, acc[fr][0], 0, 0, 0, sa, 0, sb[fr]);
-                acc[fr][1] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(c1v, xv, acc[fr][1], 0, 0, 0, sa, 0, sb[fr]);
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++)
+                    acc[fr][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv[nb], xv, acc[fr][nb], 0, 0, 0, sa, 0, sb[fr]);
             }
         }
     }
-    // tile minimum per row: value and position in one unsigned min
+    (void)LB;
+    // fold the tile into each row's running result
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
-        uint32_t g = 0xFFFFFFFFu;
+        if constexpr (OP == OP_BF16 || OP == OP_FP8) {
+            uint32_t g = 0xFFFFFFFFu;
 #pragma unroll
-        for (int nb = 0; nb < 2; nb++)
+            for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-            for (int r = 0; r < 16; r++)
-                g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
-        const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
-        const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+                for (int r = 0; r < 16; r++)
+                    g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
+            const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
+            const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+        } else if constexpr (OP == OP_F32) {
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const float s = acc[fr][nb][r];
+                    if (s < bs[fr][0]) {
+                        bs[fr][0] = s;
+                        bi[fr][0] = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+                    }
+                }
+        } else {  // OP_I8: maximise key = ((2 dot - cn) << 5) | (31 - position)
+            const int32_t *sN = (const int32_t *)sH;  // -cn per centroid
+            int32_t g = INT32_MIN;
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    const int32_t sc = (acc[fr][nb][r] << 1) + sN[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+                    g = max(g, (int32_t)((uint32_t)sc << 5) | (31 - (nb * 16 + r)));
+                }
+            const int32_t sc = g >> 5;
+            const uint32_t li = 31u - (uint32_t)(g & 31), nb = li >> 4, r = li & 15u;
+            if (sc > bs[fr][0]) {
+                bs[fr][0] = sc;
+                bi[fr][0] = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            }
+        }
     }
 }
 
-template <int OP, int ROWS, int M, int NS, int FR>
+template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
-    constexpr int ES = OpT<OP>::ES, LB = OpT<OP>::LB;
+    constexpr int ES = OpT<OP>::ES;
     constexpr int RB = 32 * NS;    // operand bytes per row
     constexpr int NQ = RB / 16;    // 16-byte quads per row
     constexpr int HB = RB / 2;     // bytes per lane half
-    constexpr int PER_T = NQ * BN / 256;  // quads of a tile per thread
-    using T = RegTile<OP, NQ, LB / 16>;
-    static_assert(NS % 2 == 0 && HB % LB == 0 && PER_T >= 1, "shape");
-    __shared__ __align__(16) uint8_t sC0[BN * RB];
-    __shared__ __align__(16) uint8_t sC1[BN * RB];
-    __shared__ float sH0[BN], sH1[BN];
+    constexpr int TB = 32 * NB;    // centroids per tile
+    constexpr int PER_T = NQ * TB / 256;  // quads of a tile per thread
+    constexpr bool CAND = OP == OP_BF16 || OP == OP_FP8;
+    using S = Swz<NQ>;
+    using Score = typename RegScore<OP>::T;
+    static_assert(NS % 2 == 0 && PER_T >= 1 && (NQ * TB) % 256 == 0, "shape");
+    __shared__ __align__(16) uint8_t sC0[TB * RB];
+    __shared__ __align__(16) uint8_t sC1[TB * RB];
+    __shared__ float sH0[TB], sH1[TB];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, j = lane & 31;
     const size_t row0 = (size_t)blockIdx.x * (128 * FR);
 
-    // Row fragments into registers, and the largest |x|^2 of the workgroup.
+    // Row fragments into registers; squared row norms (exact for I8).
     uint4 xb[FR][NS];
     int sb[FR];
+    float xn[FR];
+    int32_t xni[FR];
     float nmax = 0.0f;
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
@@ -515,17 +617,39 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
         const size_t src = valid ? (a.ids ? (size_t)a.ids[r] : r) : 0;
 #pragma unroll
         for (int q = 0; q < NS; q++) xb[fr][q] = row_segment<OP, ROWS>(a, r, src, (size_t)(h * HB + 16 * q) / ES, valid);
+        // |x|^2 from the fragment just loaded (pad elements are zero): exact
+        // for I8, the f32 values for FP32, the bf16-rounded values for BF16
+        // (only the positivity offset uses it), the encoding pass's for FP8
         float part = 0.0f;
+        int32_t ipart = 0;
         if constexpr (OP == OP_FP8) {
             if (valid && h == 0) part = a.row_norm[r];
-        } else if (valid) {
-            const size_t e0 = (size_t)h * (HB / ES), e1 = std::min(a.d, e0 + HB / ES);
-            for (size_t e = e0; e < e1; e++) {
-                float v = src_elem<ROWS>(a.x, src * a.d + e);
-                part = fmaf(v, v, part);
+        } else {
+#pragma unroll
+            for (int q = 0; q < NS; q++) {
+                const uint32_t w[4] = {xb[fr][q].x, xb[fr][q].y, xb[fr][q].z, xb[fr][q].w};
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if constexpr (OP == OP_F32) {
+                        const float v = __uint_as_float(w[u]);
+                        part = fmaf(v, v, part);
+                    } else if constexpr (OP == OP_BF16) {
+                        const float lo = __uint_as_float(w[u] << 16), hi = __uint_as_float(w[u] & 0xFFFF0000u);
+                        part = fmaf(hi, hi, fmaf(lo, lo, part));
+                    } else {
+#pragma unroll
+                        for (int b = 0; b < 4; b++) {
+                            const int32_t v = (int32_t)(int8_t)(w[u] >> (8 * b));
+                            ipart += v * v;
+                        }
+                    }
+                }
             }
         }
         part += __shfl_xor(part, 32, 64);
+        ipart += __shfl_xor(ipart, 32, 64);
+        xn[fr] = part;
+        xni[fr] = ipart;
         nmax = fmaxf(nmax, part);
         sb[fr] = 127;
         if constexpr (OP == OP_FP8) {
@@ -534,18 +658,22 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             sb[fr] = 254 - (int)eb;
         }
     }
-    for (int off = 16; off > 0; off >>= 1) nmax = fmaxf(nmax, __shfl_xor(nmax, off, 64));
-    if (lane == 0) sRed[wave] = nmax;
-    __syncthreads();
-    nmax = fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3]));
-    const float cmaxn = __uint_as_float(a.stats[0]);
-    const float negb = __uint_as_float(a.stats[1]);
-    constexpr float coef = OP == OP_FP8 ? 0.25f : 0.0078125f;
-    const float B = 0.5f * nmax + coef * sqrtf(nmax * cmaxn) + 0.5f * negb + 1.0f;
+    float B = 0.0f;
     int sa = 127;
-    if constexpr (OP == OP_FP8) {
-        const float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
-        sa = 254 - (int)((__float_as_uint(cs) >> 23) & 0xff);
+    if constexpr (CAND) {
+        for (int off = 16; off > 0; off >>= 1) nmax = fmaxf(nmax, __shfl_xor(nmax, off, 64));
+        if (lane == 0) sRed[wave] = nmax;
+        __syncthreads();
+        nmax = fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3]));
+        const float cmaxn = __uint_as_float(a.stats[0]);
+        const float negb = __uint_as_float(a.stats[1]);
+        constexpr float coef = OP == OP_FP8 ? 0.25f : 0.0078125f;
+        // (1 + 2^-6) covers the bf16 rounding of the norm itself
+        B = 0.5078125f * nmax + coef * sqrtf(nmax * cmaxn) + 0.5f * negb + 1.0f;
+        if constexpr (OP == OP_FP8) {
+            const float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
+            sa = 254 - (int)((__float_as_uint(cs) >> 23) & 0xff);
+        }
     }
 
     // Direct global->LDS tile loads: wave-load q of wave w fills LDS quads
@@ -555,43 +683,53 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
 #pragma unroll
     for (int q = 0; q < PER_T; q++) {
         const int p = (4 * q + wave) * 64 + lane, r = p / NQ, sl = p - r * NQ;
-        goff[q] = (r * NQ + T::slot(r, sl)) * 16;
+        goff[q] = (r * NQ + S::slot(r, sl)) * 16;
     }
-    const int ntiles = (int)(a.k_pad / BN);
+    const int ntiles = (int)(a.k_pad / TB);
     auto dma = [&](uint8_t *dst, int t) {
-        const uint8_t *g = a.cen_op + (size_t)t * (BN * RB);
+        const uint8_t *g = a.cen_op + (size_t)t * (TB * RB);
 #pragma unroll
         for (int q = 0; q < PER_T; q++)
             __builtin_amdgcn_global_load_lds((const void *)(g + goff[q]),
                                              (__attribute__((address_space(3))) void *)(dst + (4 * q + wave) * 1024),
                                              16, 0, 0);
     };
+    // per-centroid score offsets for the tile: float hn (+ B), or int32 -cn (I8)
+    auto hload = [&](int t) -> float {
+        if (tid >= TB) return 0.0f;
+        const size_t ci = (size_t)t * TB + tid;
+        if constexpr (OP == OP_I8) return __int_as_float((int32_t)(-a.cn[ci]));
+        else return a.hn[ci] + B;
+    };
 
-    uint32_t bs[FR][M], bi[FR][M];
+    Score bs[FR][M];
+    uint32_t bi[FR][M];
 #pragma unroll
     for (int fr = 0; fr < FR; fr++)
 #pragma unroll
         for (int t = 0; t < M; t++) {
-            bs[fr][t] = 0xFFFFFFFFu;
+            if constexpr (OP == OP_I8) bs[fr][t] = INT32_MIN;
+            else if constexpr (OP == OP_F32) bs[fr][t] = INFINITY;
+            else bs[fr][t] = 0xFFFFFFFFu;
             bi[fr][t] = NO_LABEL;
         }
 
-    float hv = tid < BN ? a.hn[tid] : 0.0f;
+    float hv = hload(0);
     dma(sC0, 0);
     __builtin_amdgcn_s_waitcnt(0);
-    if (tid < BN) sH0[tid] = hv + B;
+    if (tid < TB) sH0[tid] = hv;
     __syncthreads();
     for (int t = 0; t < ntiles; t += 2) {
         // even tile from buffer 0; the next tile streams into buffer 1
         const bool more1 = t + 1 < ntiles;
         if (more1) {
             dma(sC1, t + 1);
-            hv = tid < BN ? a.hn[(size_t)(t + 1) * BN + tid] : 0.0f;
+            hv = hload(t + 1);
         }
-        reg_tile<OP, M, NS, FR, f32x16>(sC0, sH0, xb, sb, sa, h, j, (uint32_t)(t * BN), bs, bi);
+        reg_tile<OP, M, NS, FR, NB>(sC0, sH0, xb, sb, sa, h, j, (uint32_t)(t * TB), bs, bi);
         if (more1) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (tid < BN) sH1[tid] = hv + B;
+            if (tid < TB) sH1[tid] = hv;
         }
         __syncthreads();
         if (!more1) break;
@@ -599,28 +737,51 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
         const bool more2 = t + 2 < ntiles;
         if (more2) {
             dma(sC0, t + 2);
-            hv = tid < BN ? a.hn[(size_t)(t + 2) * BN + tid] : 0.0f;
+            hv = hload(t + 2);
         }
-        reg_tile<OP, M, NS, FR, f32x16>(sC1, sH1, xb, sb, sa, h, j, (uint32_t)((t + 1) * BN), bs, bi);
+        reg_tile<OP, M, NS, FR, NB>(sC1, sH1, xb, sb, sa, h, j, (uint32_t)((t + 1) * TB), bs, bi);
         if (more2) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (tid < BN) sH0[tid] = hv + B;
+            if (tid < TB) sH0[tid] = hv;
         }
         __syncthreads();
     }
 
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
-#pragma unroll
-        for (int t = 0; t < M; t++) {
-            const uint32_t ps = (uint32_t)__shfl_xor((int)bs[fr][t], 32, 64);
-            const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][t], 32, 64);
-            if (h == 0) topm_insert_lex<M, uint32_t>(bs[fr], bi[fr], ps, pi);
-        }
         const size_t r = row0 + (size_t)(wave * FR + fr) * 32 + j;
-        if (h == 0 && r < a.m) {
+        if constexpr (CAND) {
 #pragma unroll
-            for (int t = 0; t < M; t++) a.cand[r * M + t] = bi[fr][t];
+            for (int t = 0; t < M; t++) {
+                const uint32_t ps = (uint32_t)__shfl_xor((int)bs[fr][t], 32, 64);
+                const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][t], 32, 64);
+                if (h == 0) topm_insert_lex<M, uint32_t>(bs[fr], bi[fr], ps, pi);
+            }
+            if (h == 0 && r < a.m) {
+#pragma unroll
+                for (int t = 0; t < M; t++) a.cand[r * M + t] = bi[fr][t];
+            }
+        } else {
+            const Score ps = __shfl_xor(bs[fr][0], 32, 64);
+            const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][0], 32, 64);
+            const bool better = OP == OP_I8 ? (ps > bs[fr][0] || (ps == bs[fr][0] && pi < bi[fr][0]))
+                                            : (ps < bs[fr][0] || (ps == bs[fr][0] && pi < bi[fr][0]));
+            if (better) {
+                bs[fr][0] = ps;
+                bi[fr][0] = pi;
+            }
+            if (h == 0 && r < a.m) {
+                a.labels[r] = bi[fr][0] == NO_LABEL ? 0u : bi[fr][0];
+                if constexpr (OP == OP_F32) {
+                    // dist = |x|^2 + |c|^2 + bias - 2 x.c, clamped at 0 (src/cpu.rs:160-162)
+                    const float v = __fadd_rn(__fmul_rn(2.0f, (float)bs[fr][0]), xn[fr]);
+                    a.dist[r] = v > 0.0f ? v : 0.0f;
+                } else {
+                    // dist = ((|x|^2 + best) as f64).max(0) as f32 (src/cpu_i8.rs:183-186)
+                    const double v = __dadd_rn((double)xni[fr], -(double)(int32_t)bs[fr][0]);
+                    a.dist[r] = (float)(v > 0.0 ? v : 0.0);
+                }
+            }
         }
     }
 }
@@ -783,12 +944,15 @@ __global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) 
 // cn_j = sum c8^2 + round(bias_j) (src/cpu_i8.rs:112-117, unit 1). Pad
 // centroids get a score that never wins.
 __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *norm32, const int64_t *norm64,
-                                const float *bias, float *hn, int64_t *cn, unsigned *stats) {
+                                const float *bias, float *hn, int64_t *cn, unsigned *stats, int64_t cn_limit) {
     size_t jc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (jc >= k_pad) return;
     if (bias && jc < k && bias[jc] < 0.0f) atomicMax(&stats[1], __float_as_uint(-bias[jc]));
     if (op == OP_I8) {
-        int64_t v = PAD_SCORE64;
+        // pad centroids: 2^26 puts their packed score 2 x.0 - 2^26 below every
+        // real one (> -2^26) and still inside int32 once shifted; the i64
+        // kernel skips indices >= k instead
+        int64_t v = (int64_t)1 << 26;
         if (jc < k) {
             v = norm64[jc];
             if (bias) {
@@ -796,6 +960,7 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
                 r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
                 v += (int64_t)r;
             }
+            if (v >= cn_limit || v <= -cn_limit) atomicMax(&stats[2], 1u);
         }
         cn[jc] = v;
     } else {
@@ -877,16 +1042,13 @@ static hipError_t launch_m(const SearchGeom &g, int M, const SearchArgs &a, hipS
     }
 }
 
-// Register-resident BF16/FP8 search: rows of 64..256 operand bytes.
-bool reg_path(const SearchGeom &g) {
-    return (g.op == OP_BF16 || g.op == OP_FP8) && g.resident && g.row_bytes <= 256 && g.row_bytes % 64 == 0;
-}
+bool reg_path(const SearchGeom &g) { return g.reg; }
 
 template <int OP, int ROWS, int M, int NS>
 static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
-    constexpr int FR = 2;
+    constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
     dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)));
-    hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR>), grid, dim3(256), 0, s, a);
+    hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -897,6 +1059,12 @@ static hipError_t launch_reg_ns(const SearchGeom &g, const SearchArgs &a, hipStr
         case 4: return launch_reg<OP, ROWS, M, 4>(a, s);
         case 6: return launch_reg<OP, ROWS, M, 6>(a, s);
         case 8: return launch_reg<OP, ROWS, M, 8>(a, s);
+        case 12:
+            if constexpr (OP == OP_F32 || OP == OP_I8) return launch_reg<OP, ROWS, M, 12>(a, s);
+            return hipErrorInvalidValue;
+        case 16:
+            if constexpr (OP == OP_F32 || OP == OP_I8) return launch_reg<OP, ROWS, M, 16>(a, s);
+            return hipErrorInvalidValue;
         default: return hipErrorInvalidValue;
     }
 }
@@ -942,6 +1110,11 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
     switch (g.op) {
         case OP_F32:
             if (M != 1) return hipErrorInvalidValue;
+            if (reg_path(g)) {
+                if (rows == EMH_ROWS_F32) return launch_reg_ns<OP_F32, EMH_ROWS_F32, 1>(g, a, s);
+                if (rows == EMH_ROWS_I8) return launch_reg_ns<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
+                return launch_reg_ns<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
+            }
             if (rows == EMH_ROWS_F32) return launch_res<OP_F32, EMH_ROWS_F32, 1>(g, a, s);
             if (rows == EMH_ROWS_I8) return launch_res<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
             return launch_res<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
@@ -959,6 +1132,7 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
             return launch_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
         case OP_I8:
             if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
+            if (reg_path(g) && g.i8_packed) return launch_reg_ns<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
             return launch_res<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
     }
     return hipErrorInvalidValue;
@@ -1012,7 +1186,10 @@ hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipSt
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
                            float *hn, int64_t *cn, unsigned *stats, hipStream_t s) {
     dim3 grid((unsigned)((g.k_pad + 255) / 256));
-    hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn, stats);
+    // packed I8 key: |2 x.c - cn| < 2^26 needs |cn| < 2^26 - 2 * d * 2^14
+    const int64_t cn_limit = ((int64_t)1 << 26) - (int64_t)g.d * 32768;
+    hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn, stats,
+                       cn_limit);
     return hipGetLastError();
 }
 

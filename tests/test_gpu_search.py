@@ -118,6 +118,24 @@ def test_i8_fractional_centroids_and_bias_match_cpu(hip):
         assert np.array_equal(gl, cl) and np.array_equal(gd, cd)
 
 
+@pytest.mark.parametrize("d", [64, 512])
+def test_i8_large_bias_takes_the_i64_path(hip, d):
+    """Biases beyond the packed int32 key range switch the search to the i64
+    kernel; results must stay bit-exact either way."""
+    rng = np.random.default_rng(9)
+    x = rng.integers(-128, 128, size=(700, d)).astype(np.int8)
+    c = rng.normal(scale=40, size=(150, d)).astype(np.float32)
+    for scale in (1.0, 3e7):
+        bias = (rng.uniform(0, 1, size=150) * scale).astype(np.float32)
+        bias[3] = -bias[3]
+        a = E.HipAssigner(E.I8)
+        a.set_centroids(c)
+        a.set_bias(bias)
+        gl, gd = a.assign(x)
+        cl, cd = O.i8_assign(x, c, bias=bias)
+        assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias scale {scale}"
+
+
 def test_i8_rejects_float_rows(hip):
     a = E.HipAssigner(E.I8)
     a.set_centroids(np.zeros((3, 4), np.float32))
