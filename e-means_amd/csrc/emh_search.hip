@@ -456,14 +456,38 @@ template <int OP> struct RegScore { using T = uint32_t; };
 template <> struct RegScore<OP_F32> { using T = float; };
 template <> struct RegScore<OP_I8> { using T = int32_t; };
 
-template <int OP, int M, int NS, int FR, int NB>
-__device__ __forceinline__ void reg_tile(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
-                                         const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
-                                         typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
-    constexpr int LB = OpT<OP>::LB, RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2;
+template <int OP> using RegAcc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
+
+// Scheduling pattern for one pipelined tile: per MFMA step, its operand
+// LDS reads, then each MFMA followed by a slice of the previous tile's
+// epilogue, so the VALU work issues while the matrix core is busy.
+template <int OP, int NS, int FR, int NB>
+__device__ __forceinline__ void reg_interleave() {
+    constexpr int NQ = 2 * NS, HQ = NQ / 2;
+    constexpr int STEPS = OP == OP_F32 ? HQ : (OP == OP_FP8 ? HQ / 2 : HQ);
+    constexpr int MF_PER_STEP = FR * NB * (OP == OP_F32 ? 4 : 1);
+    constexpr int READS = NB * (OP == OP_FP8 ? 2 : 1);
+    constexpr int FOLD = OP == OP_F32 ? FR * NB * 16 * 3 : (OP == OP_I8 ? FR * (NB * 16 * 3 + 12) : FR * (NB * 16 * 2 + 40));
+    constexpr int NMF = STEPS * MF_PER_STEP;
+    constexpr int VPM = (FOLD + NMF - 1) / NMF;
+#pragma unroll
+    for (int s = 0; s < STEPS; s++) {
+        __builtin_amdgcn_sched_group_barrier(0x100, READS, 0);  // DS reads
+#pragma unroll
+        for (int q = 0; q < MF_PER_STEP; q++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // one MFMA
+            __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);  // VALU
+        }
+    }
+}
+
+// The MFMA half of a tile: acc = (score offsets) + centroid tile . row fragments.
+template <int OP, int NS, int FR, int NB>
+__device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
+                                         const int (&sb)[FR], int sa, int h, int j, RegAcc<OP> (&acc)[FR][NB]) {
+    constexpr int RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2;
     using S = Swz<NQ>;
-    using Acc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
-    Acc acc[FR][NB];
+    using Acc = RegAcc<OP>;
     if constexpr (OP == OP_I8) {
 #pragma unroll
         for (int fr = 0; fr < FR; fr++)
@@ -538,8 +562,12 @@ __device__ __forceinline__ void reg_tile(const uint8_t *sC, const float *sH, con
             }
         }
     }
-    (void)LB;
-    // fold the tile into each row's running result
+}
+
+// The epilogue half: fold a tile's scores into each row's running result.
+template <int OP, int M, int FR, int NB>
+__device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const float *sN, int h, uint32_t c0,
+                                         typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
         if constexpr (OP == OP_BF16 || OP == OP_FP8) {
@@ -564,13 +592,13 @@ __device__ __forceinline__ void reg_tile(const uint8_t *sC, const float *sH, con
                     }
                 }
         } else {  // OP_I8: maximise key = ((2 dot - cn) << 5) | (31 - position)
-            const int32_t *sN = (const int32_t *)sH;  // -cn per centroid
+            const int32_t *sNi = (const int32_t *)sN;  // -cn per centroid
             int32_t g = INT32_MIN;
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const int32_t sc = (acc[fr][nb][r] << 1) + sN[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+                    const int32_t sc = (acc[fr][nb][r] << 1) + sNi[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
                     g = max(g, (int32_t)((uint32_t)sc << 5) | (31 - (nb * 16 + r)));
                 }
             const int32_t sc = g >> 5;
@@ -597,7 +625,7 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     static_assert(NS % 2 == 0 && PER_T >= 1 && (NQ * TB) % 256 == 0, "shape");
     __shared__ __align__(16) uint8_t sC0[TB * RB];
     __shared__ __align__(16) uint8_t sC1[TB * RB];
-    __shared__ float sH0[TB], sH1[TB];
+    __shared__ float sH[3][TB];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -714,35 +742,66 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             bi[fr][t] = NO_LABEL;
         }
 
+    // Software pipeline: the MFMAs of tile t and the epilogue of tile t-1
+    // share one basic block (two accumulator sets), so the scheduler can
+    // interleave the epilogue's VALU work with the matrix cores. Score
+    // offsets rotate through three LDS slots: the I8 epilogue still reads
+    // tile t-1's while tile t+1's are written.
+    using Acc = RegAcc<OP>;
+    Acc accA[FR][NB], accB[FR][NB];
     float hv = hload(0);
     dma(sC0, 0);
     __builtin_amdgcn_s_waitcnt(0);
-    if (tid < TB) sH0[tid] = hv;
+    if (tid < TB) sH[0][tid] = hv;
     __syncthreads();
-    for (int t = 0; t < ntiles; t += 2) {
-        // even tile from buffer 0; the next tile streams into buffer 1
-        const bool more1 = t + 1 < ntiles;
-        if (more1) {
-            dma(sC1, t + 1);
-            hv = hload(t + 1);
+    {
+        const bool more = 1 < ntiles;
+        if (more) {
+            dma(sC1, 1);
+            hv = hload(1);
         }
-        reg_tile<OP, M, NS, FR, NB>(sC0, sH0, xb, sb, sa, h, j, (uint32_t)(t * TB), bs, bi);
-        if (more1) {
+        reg_mfma<OP, NS, FR, NB>(sC0, sH[0], xb, sb, sa, h, j, accA);
+        if (more) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH1[tid] = hv;
+            if (tid < TB) sH[1][tid] = hv;
         }
         __syncthreads();
-        if (!more1) break;
-        // odd tile from buffer 1; the next tile streams into buffer 0
-        const bool more2 = t + 2 < ntiles;
-        if (more2) {
-            dma(sC0, t + 2);
+    }
+    for (int t = 1;; t += 2) {
+        // tile t from buffer 1; tile t-1 (accA) folds meanwhile
+        if (t >= ntiles) {
+            reg_fold<OP, M, FR, NB>(accA, sH[(t - 1) % 3], h, (uint32_t)((t - 1) * TB), bs, bi);
+            break;
+        }
+        bool more = t + 1 < ntiles;
+        if (more) {
+            dma(sC0, t + 1);
+            hv = hload(t + 1);
+        }
+        reg_mfma<OP, NS, FR, NB>(sC1, sH[t % 3], xb, sb, sa, h, j, accB);
+        reg_fold<OP, M, FR, NB>(accA, sH[(t - 1) % 3], h, (uint32_t)((t - 1) * TB), bs, bi);
+        reg_interleave<OP, NS, FR, NB>();
+        if (more) {
+            __builtin_amdgcn_s_waitcnt(0);
+            if (tid < TB) sH[(t + 1) % 3][tid] = hv;
+        }
+        __syncthreads();
+        // tile t+1 from buffer 0; tile t (accB) folds meanwhile
+        if (t + 1 >= ntiles) {
+            reg_fold<OP, M, FR, NB>(accB, sH[t % 3], h, (uint32_t)(t * TB), bs, bi);
+            break;
+        }
+        more = t + 2 < ntiles;
+        if (more) {
+            dma(sC1, t + 2);
             hv = hload(t + 2);
         }
-        reg_tile<OP, M, NS, FR, NB>(sC1, sH1, xb, sb, sa, h, j, (uint32_t)((t + 1) * TB), bs, bi);
-        if (more2) {
+        reg_mfma<OP, NS, FR, NB>(sC0, sH[(t + 1) % 3], xb, sb, sa, h, j, accA);
+        reg_fold<OP, M, FR, NB>(accB, sH[t % 3], h, (uint32_t)(t * TB), bs, bi);
+        reg_interleave<OP, NS, FR, NB>();
+        if (more) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH0[tid] = hv;
+            if (tid < TB) sH[(t + 2) % 3][tid] = hv;
         }
         __syncthreads();
     }
@@ -794,7 +853,7 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
 // loads, each lane folds its elements in order without FMA, and a fixed
 // butterfly adds the 16 partial sums, so every lane holds identical bits.
 template <int ROWS, int M>
-__global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, const uint32_t *ids,
+__global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, size_t k, const uint32_t *ids,
                                                       const float *c, const float *bias, const uint32_t *cand,
                                                       uint32_t *labels, float *dist) {
     constexpr int G = 16, PPW = 64 / G, PASSES = (M + PPW - 1) / PPW;
@@ -808,7 +867,8 @@ __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, s
 #pragma unroll
         for (int p = 0; p < PASSES; p++) {
             const int slot = p * PPW + grp;
-            const uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
+            uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
+            if (ci >= k) ci = NO_LABEL;  // pad centroids can fill a list when k is small
             float acc = 0.0f;
             if (ci != NO_LABEL) {
                 const float *cp = c + (size_t)ci * d;
@@ -1139,27 +1199,27 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
 }
 
 template <int ROWS>
-static hipError_t rescore_rows(int M, const void *x, size_t m, size_t d, const uint32_t *ids, const float *c,
+static hipError_t rescore_rows(int M, const void *x, size_t m, size_t d, size_t k, const uint32_t *ids, const float *c,
                                const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
                                hipStream_t s) {
     dim3 grid((unsigned)std::min<size_t>((m + 3) / 4, 16384));
     switch (M) {
-        case 1: hipLaunchKernelGGL((rescore_kernel<ROWS, 1>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
-        case 2: hipLaunchKernelGGL((rescore_kernel<ROWS, 2>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
-        case 4: hipLaunchKernelGGL((rescore_kernel<ROWS, 4>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
-        case 8: hipLaunchKernelGGL((rescore_kernel<ROWS, 8>), grid, dim3(256), 0, s, x, m, d, ids, c, bias, cand, labels, dist); break;
+        case 1: hipLaunchKernelGGL((rescore_kernel<ROWS, 1>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
+        case 2: hipLaunchKernelGGL((rescore_kernel<ROWS, 2>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
+        case 4: hipLaunchKernelGGL((rescore_kernel<ROWS, 4>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
+        case 8: hipLaunchKernelGGL((rescore_kernel<ROWS, 8>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
 hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,
-                          const float *bias, size_t /*k*/, const uint32_t *cand, uint32_t *labels, float *dist,
+                          const float *bias, size_t k, const uint32_t *cand, uint32_t *labels, float *dist,
                           hipStream_t s) {
     if (m == 0) return hipSuccess;
-    if (rows == EMH_ROWS_F32) return rescore_rows<EMH_ROWS_F32>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
-    if (rows == EMH_ROWS_I8) return rescore_rows<EMH_ROWS_I8>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
-    return rescore_rows<EMH_ROWS_U8>(M, x, m, d, ids, c, bias, cand, labels, dist, s);
+    if (rows == EMH_ROWS_F32) return rescore_rows<EMH_ROWS_F32>(M, x, m, d, k, ids, c, bias, cand, labels, dist, s);
+    if (rows == EMH_ROWS_I8) return rescore_rows<EMH_ROWS_I8>(M, x, m, d, k, ids, c, bias, cand, labels, dist, s);
+    return rescore_rows<EMH_ROWS_U8>(M, x, m, d, k, ids, c, bias, cand, labels, dist, s);
 }
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
