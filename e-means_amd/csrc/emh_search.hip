@@ -458,29 +458,6 @@ template <> struct RegScore<OP_I8> { using T = int32_t; };
 
 template <int OP> using RegAcc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
 
-// Scheduling pattern for one pipelined tile: per MFMA step, its operand
-// LDS reads, then each MFMA followed by a slice of the previous tile's
-// epilogue, so the VALU work issues while the matrix core is busy.
-template <int OP, int NS, int FR, int NB>
-__device__ __forceinline__ void reg_interleave() {
-    constexpr int NQ = 2 * NS, HQ = NQ / 2;
-    constexpr int STEPS = OP == OP_F32 ? HQ : (OP == OP_FP8 ? HQ / 2 : HQ);
-    constexpr int MF_PER_STEP = FR * NB * (OP == OP_F32 ? 4 : 1);
-    constexpr int READS = NB * (OP == OP_FP8 ? 2 : 1);
-    constexpr int FOLD = OP == OP_F32 ? FR * NB * 16 * 3 : (OP == OP_I8 ? FR * (NB * 16 * 3 + 12) : FR * (NB * 16 * 2 + 40));
-    constexpr int NMF = STEPS * MF_PER_STEP;
-    constexpr int VPM = (FOLD + NMF - 1) / NMF;
-#pragma unroll
-    for (int s = 0; s < STEPS; s++) {
-        __builtin_amdgcn_sched_group_barrier(0x100, READS, 0);  // DS reads
-#pragma unroll
-        for (int q = 0; q < MF_PER_STEP; q++) {
-            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);    // one MFMA
-            __builtin_amdgcn_sched_group_barrier(0x002, VPM, 0);  // VALU
-        }
-    }
-}
-
 // The MFMA half of a tile: acc = (score offsets) + centroid tile . row fragments.
 template <int OP, int NS, int FR, int NB>
 __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
@@ -488,20 +465,17 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
     constexpr int RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2;
     using S = Swz<NQ>;
     using Acc = RegAcc<OP>;
-    if constexpr (OP == OP_I8) {
+    // The offsets enter as the C operand of each fragment's first MFMA
+    // (no per-fragment copies); I8 starts from zero.
+    Acc init[NB];
 #pragma unroll
-        for (int fr = 0; fr < FR; fr++)
+    for (int nb = 0; nb < NB; nb++) {
+        if constexpr (OP == OP_I8) {
+            init[nb] = Acc{};
+        } else {
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) acc[fr][nb] = Acc{};
-    } else {
-#pragma unroll
-        for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const float v = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
-#pragma unroll
-                for (int fr = 0; fr < FR; fr++) acc[fr][nb][r] = v;
-            }
+            for (int r = 0; r < 16; r++) init[nb][r] = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+        }
     }
     const uint8_t *crow[NB];
 #pragma unroll
@@ -517,7 +491,7 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
                 const float4 xv = __builtin_bit_cast(float4, xb[fr][s]);
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++) {
-                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].x, xv.x, acc[fr][nb], 0, 0, 0);
+                    acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].x, xv.x, s == 0 ? init[nb] : acc[fr][nb], 0, 0, 0);
                     acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].y, xv.y, acc[fr][nb], 0, 0, 0);
                     acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].z, xv.z, acc[fr][nb], 0, 0, 0);
                     acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[nb].w, xv.w, acc[fr][nb], 0, 0, 0);
@@ -536,10 +510,12 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
                 for (int nb = 0; nb < NB; nb++) {
                     if constexpr (OP == OP_BF16)
                         acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            __builtin_bit_cast(bf16x8, cv[nb]), __builtin_bit_cast(bf16x8, xb[fr][s]), acc[fr][nb], 0, 0, 0);
+                            __builtin_bit_cast(bf16x8, cv[nb]), __builtin_bit_cast(bf16x8, xb[fr][s]),
+                            s == 0 ? init[nb] : acc[fr][nb], 0, 0, 0);
                     else
                         acc[fr][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            __builtin_bit_cast(i32x4, cv[nb]), __builtin_bit_cast(i32x4, xb[fr][s]), acc[fr][nb], 0, 0, 0);
+                            __builtin_bit_cast(i32x4, cv[nb]), __builtin_bit_cast(i32x4, xb[fr][s]),
+                            s == 0 ? init[nb] : acc[fr][nb], 0, 0, 0);
                 }
         }
     } else {  // OP_FP8: two quads per lane per MFMA
@@ -558,7 +534,8 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
                 const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++)
-                    acc[fr][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv[nb], xv, acc[fr][nb], 0, 0, 0, sa, 0, sb[fr]);
+                    acc[fr][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv[nb], xv, s == 0 ? init[nb] : acc[fr][nb],
+                                                                                  0, 0, 0, sa, 0, sb[fr]);
             }
         }
     }
@@ -625,7 +602,7 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     static_assert(NS % 2 == 0 && PER_T >= 1 && (NQ * TB) % 256 == 0, "shape");
     __shared__ __align__(16) uint8_t sC0[TB * RB];
     __shared__ __align__(16) uint8_t sC1[TB * RB];
-    __shared__ float sH[3][TB];
+    __shared__ float sH[2][TB];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -742,66 +719,43 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             bi[fr][t] = NO_LABEL;
         }
 
-    // Software pipeline: the MFMAs of tile t and the epilogue of tile t-1
-    // share one basic block (two accumulator sets), so the scheduler can
-    // interleave the epilogue's VALU work with the matrix cores. Score
-    // offsets rotate through three LDS slots: the I8 epilogue still reads
-    // tile t-1's while tile t+1's are written.
+    // Tile loop: while tile t is multiplied and folded from one LDS buffer,
+    // tile t+1 streams into the other. (An epilogue overlapped with the next
+    // tile's MFMAs needs a second accumulator set, which halves occupancy and
+    // measured slower than two workgroups per SIMD interleaving.)
     using Acc = RegAcc<OP>;
-    Acc accA[FR][NB], accB[FR][NB];
+    Acc acc[FR][NB];
     float hv = hload(0);
     dma(sC0, 0);
     __builtin_amdgcn_s_waitcnt(0);
     if (tid < TB) sH[0][tid] = hv;
     __syncthreads();
-    {
-        const bool more = 1 < ntiles;
-        if (more) {
-            dma(sC1, 1);
-            hv = hload(1);
+    for (int t = 0; t < ntiles; t += 2) {
+        // even tile from buffer 0; the next tile streams into buffer 1
+        const bool more1 = t + 1 < ntiles;
+        if (more1) {
+            dma(sC1, t + 1);
+            hv = hload(t + 1);
         }
-        reg_mfma<OP, NS, FR, NB>(sC0, sH[0], xb, sb, sa, h, j, accA);
-        if (more) {
+        reg_mfma<OP, NS, FR, NB>(sC0, sH[0], xb, sb, sa, h, j, acc);
+        reg_fold<OP, M, FR, NB>(acc, sH[0], h, (uint32_t)(t * TB), bs, bi);
+        if (more1) {
             __builtin_amdgcn_s_waitcnt(0);
             if (tid < TB) sH[1][tid] = hv;
         }
         __syncthreads();
-    }
-    for (int t = 1;; t += 2) {
-        // tile t from buffer 1; tile t-1 (accA) folds meanwhile
-        if (t >= ntiles) {
-            reg_fold<OP, M, FR, NB>(accA, sH[(t - 1) % 3], h, (uint32_t)((t - 1) * TB), bs, bi);
-            break;
-        }
-        bool more = t + 1 < ntiles;
-        if (more) {
-            dma(sC0, t + 1);
-            hv = hload(t + 1);
-        }
-        reg_mfma<OP, NS, FR, NB>(sC1, sH[t % 3], xb, sb, sa, h, j, accB);
-        reg_fold<OP, M, FR, NB>(accA, sH[(t - 1) % 3], h, (uint32_t)((t - 1) * TB), bs, bi);
-        reg_interleave<OP, NS, FR, NB>();
-        if (more) {
-            __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH[(t + 1) % 3][tid] = hv;
-        }
-        __syncthreads();
-        // tile t+1 from buffer 0; tile t (accB) folds meanwhile
-        if (t + 1 >= ntiles) {
-            reg_fold<OP, M, FR, NB>(accB, sH[t % 3], h, (uint32_t)(t * TB), bs, bi);
-            break;
-        }
-        more = t + 2 < ntiles;
-        if (more) {
-            dma(sC1, t + 2);
+        if (!more1) break;
+        // odd tile from buffer 1; the next tile streams into buffer 0
+        const bool more2 = t + 2 < ntiles;
+        if (more2) {
+            dma(sC0, t + 2);
             hv = hload(t + 2);
         }
-        reg_mfma<OP, NS, FR, NB>(sC0, sH[(t + 1) % 3], xb, sb, sa, h, j, accA);
-        reg_fold<OP, M, FR, NB>(accB, sH[t % 3], h, (uint32_t)(t * TB), bs, bi);
-        reg_interleave<OP, NS, FR, NB>();
-        if (more) {
+        reg_mfma<OP, NS, FR, NB>(sC1, sH[1], xb, sb, sa, h, j, acc);
+        reg_fold<OP, M, FR, NB>(acc, sH[1], h, (uint32_t)((t + 1) * TB), bs, bi);
+        if (more2) {
             __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH[(t + 2) % 3][tid] = hv;
+            if (tid < TB) sH[0][tid] = hv;
         }
         __syncthreads();
     }
