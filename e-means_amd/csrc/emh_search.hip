@@ -499,11 +499,19 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
             }
         }
     } else if constexpr (OP == OP_BF16 || OP == OP_I8) {
+        // A fragments are read one step ahead of the MFMAs that use them
+        uint4 nxt[NB];
+#pragma unroll
+        for (int nb = 0; nb < NB; nb++) nxt[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ));
 #pragma unroll
         for (int s = 0; s < HQ; s++) {
             uint4 cv[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) cv[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s));
+            for (int nb = 0; nb < NB; nb++) cv[nb] = nxt[nb];
+            if (s + 1 < HQ) {
+#pragma unroll
+                for (int nb = 0; nb < NB; nb++) nxt[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s + 1));
+            }
 #pragma unroll
             for (int fr = 0; fr < FR; fr++)
 #pragma unroll
@@ -602,7 +610,8 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     static_assert(NS % 2 == 0 && PER_T >= 1 && (NQ * TB) % 256 == 0, "shape");
     __shared__ __align__(16) uint8_t sC0[TB * RB];
     __shared__ __align__(16) uint8_t sC1[TB * RB];
-    __shared__ float sH[2][TB];
+    __shared__ __align__(16) uint8_t sC2[TB * RB];
+    __shared__ float sH0[256], sH1[256], sH2[256];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -699,12 +708,22 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
                                              (__attribute__((address_space(3))) void *)(dst + (4 * q + wave) * 1024),
                                              16, 0, 0);
     };
-    // per-centroid score offsets for the tile: float hn (+ B), or int32 -cn (I8)
-    auto hload = [&](int t) -> float {
-        if (tid >= TB) return 0.0f;
-        const size_t ci = (size_t)t * TB + tid;
-        if constexpr (OP == OP_I8) return __int_as_float((int32_t)(-a.cn[ci]));
-        else return a.hn[ci] + B;
+    // Per-centroid score offsets of a tile, loaded raw: any arithmetic on the
+    // loaded value would make the compiler wait for every outstanding load of
+    // the wave (the tile DMA included) right there. Converted at the store:
+    // float hn + B, or int32 -cn (I8).
+    using HRaw = typename std::conditional<OP == OP_I8, int64_t, float>::type;
+    auto hload = [&](int t) -> HRaw {
+        // every thread loads (threads past TB repeat an entry): no branch
+        const size_t ci = (size_t)t * TB + (tid & (TB - 1));
+        if constexpr (OP == OP_I8) return a.cn[ci];
+        else return a.hn[ci];
+    };
+    // every thread stores (slots past TB are never read): no branch, so the
+    // compiler's wait tracking stays exact across the stage
+    auto hstore = [&](float *dst, HRaw v) {
+        if constexpr (OP == OP_I8) dst[tid] = __int_as_float((int32_t)(-v));
+        else dst[tid] = v + B;
     };
 
     Score bs[FR][M];
@@ -719,46 +738,54 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             bi[fr][t] = NO_LABEL;
         }
 
-    // Tile loop: while tile t is multiplied and folded from one LDS buffer,
-    // tile t+1 streams into the other. (An epilogue overlapped with the next
-    // tile's MFMAs needs a second accumulator set, which halves occupancy and
-    // measured slower than two workgroups per SIMD interleaving.)
+    // Tile loop over three LDS buffers: while tile t is multiplied and
+    // folded, tiles t+1 and t+2 are in flight, so a DMA has two tiles of
+    // compute to land. Each stage waits only for the older DMA (vmcnt counts
+    // this wave's outstanding loads, which complete in order).
     using Acc = RegAcc<OP>;
     Acc acc[FR][NB];
-    float hv = hload(0);
+    constexpr int WAIT_ONE = (PER_T + 1) | 0x0F70;  // vmcnt(PER_T + 1): the newest tile may stay in flight
+    // Workgroup barrier that waits for LDS traffic only: __syncthreads()'s
+    // release fence would also wait for the newest tile's DMA. The DMA each
+    // wave needs next has landed by its own vmcnt wait before this point.
+    auto tile_barrier = []() {
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+        asm volatile("" ::: "memory");
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+    };
+    HRaw hv0 = hload(0), hv1 = 0, hv2 = 0;
     dma(sC0, 0);
-    __builtin_amdgcn_s_waitcnt(0);
-    if (tid < TB) sH[0][tid] = hv;
+    dma(sC1, 1 < ntiles ? 1 : 0);
+    hv1 = hload(1 < ntiles ? 1 : 0);
+    __builtin_amdgcn_s_waitcnt(WAIT_ONE);
+    hstore(sH0, hv0);
     __syncthreads();
-    for (int t = 0; t < ntiles; t += 2) {
-        // even tile from buffer 0; the next tile streams into buffer 1
-        const bool more1 = t + 1 < ntiles;
-        if (more1) {
-            dma(sC1, t + 1);
-            hv = hload(t + 1);
-        }
-        reg_mfma<OP, NS, FR, NB>(sC0, sH[0], xb, sb, sa, h, j, acc);
-        reg_fold<OP, M, FR, NB>(acc, sH[0], h, (uint32_t)(t * TB), bs, bi);
-        if (more1) {
-            __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH[1][tid] = hv;
-        }
-        __syncthreads();
-        if (!more1) break;
-        // odd tile from buffer 1; the next tile streams into buffer 0
-        const bool more2 = t + 2 < ntiles;
-        if (more2) {
-            dma(sC0, t + 2);
-            hv = hload(t + 2);
-        }
-        reg_mfma<OP, NS, FR, NB>(sC1, sH[1], xb, sb, sa, h, j, acc);
-        reg_fold<OP, M, FR, NB>(acc, sH[1], h, (uint32_t)((t + 1) * TB), bs, bi);
-        if (more2) {
-            __builtin_amdgcn_s_waitcnt(0);
-            if (tid < TB) sH[0][tid] = hv;
-        }
-        __syncthreads();
+    // One stage: compute tile T from (BC, BH); start tile T+2 into BN2;
+    // publish tile T+1's offsets into BH1. The LDS arrays are named directly
+    // so the compiler can tell the buffer being read from the ones in flight.
+    // Every stage issues the same loads (past the end it re-fetches the last
+    // tile into the free buffer), so the wait counts are the same each time.
+#define EMH_STAGE(T, BC, BH, BN2, BH1, HV1, HV2)                                   \
+    {                                                                             \
+        const int tn = (T) + 2 < ntiles ? (T) + 2 : ntiles - 1;                   \
+        dma(BN2, tn);                                                             \
+        HV2 = hload(tn);                                                          \
+        reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);                  \
+        reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * TB), bs, bi);        \
+        __builtin_amdgcn_sched_barrier(0); /* keep the wait after the compute */  \
+        __builtin_amdgcn_s_waitcnt(WAIT_ONE);                                     \
+        hstore(BH1, HV1);                                                         \
+        tile_barrier();                                                           \
     }
+    for (int t = 0; t < ntiles; t += 3) {
+        EMH_STAGE(t, sC0, sH0, sC2, sH1, hv1, hv2);
+        if (t + 1 >= ntiles) break;
+        EMH_STAGE(t + 1, sC1, sH1, sC0, sH2, hv2, hv0);
+        if (t + 2 >= ntiles) break;
+        EMH_STAGE(t + 2, sC2, sH2, sC1, sH0, hv0, hv1);
+    }
+#undef EMH_STAGE
 
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
