@@ -1,14 +1,24 @@
-"""Benchmark of the nearest-centroid search (assign-only search at scale).
+"""Benchmark of the e-means nearest-centroid search on one MI355X.
 
-Workload: 2,000,000 x 128 f32 rows from a Gaussian mixture (16384 blob
-centres ~ N(0, 16 I), each row = a uniformly chosen centre + N(0, I), seed 0),
+Default workload (`--workload search`): assign-only search at scale,
+2,000,000 x 128 f32 rows from a Gaussian mixture (16384 blob centres
+~ N(0, 16 I), each row = a uniformly chosen centre + N(0, I), seed 0),
 k = 16384 centroids drawn from the rows; every row resident in HBM before the
 timed region. One step = one pass of the search over all n rows: for the
-BF16/FP8 searches that is the MFMA search kernel plus the FP32 rescoring of
-the top-m candidates (plus the per-row FP8 encoding), for FP32 the fused
-argmin kernel. Prints one JSON line.
+BF16/FP8 searches the MFMA search kernel plus the FP32 rescoring of the top-m
+candidates (plus the per-row FP8 encoding), for FP32 the fused argmin kernel.
 
-    python bench.py [--steps K] [--warmup W] [--precision bf16|fp8|fp32|i8]
+Other workloads (reported in DESIGN.md, not the headline line):
+  search-i8          1,000,000 x 512 int8 codes (rank-64 signal + 0.1 noise,
+                     L2-normalised, faiss calibration), k = 4096, exact I8 search
+  search-fp8-large-k 2,000,000 x 256 heavy-tailed rows (Student-t nu=3, random
+                     rotation), k = 65536, FP8 top-8 + FP32 rescore vs FP32
+  train-c1           e-means training, 200,000 x 96 mixture of 4096 blobs,
+                     k = 1024, 5 epochs, Config::default(); GPU vs CPU port
+  train-c3           int8 training, 1,000,000 x 512 codes, k = 4096, 5 epochs,
+                     I8 search; GPU vs CPU port (bit-identical results)
+
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -25,7 +35,8 @@ for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
 
 # MI355X dense peaks (AMD published): TFLOP/s (TOP/s for int8)
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2}
-HBM_PEAK_GBS = 8000.0
+METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
+WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3"]
 
 
 def parse():
@@ -33,19 +44,25 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="bf16", choices=["bf16", "fp8", "fp32", "i8"])
-    ap.add_argument("--rescore-m", type=int, default=4)
-    ap.add_argument("--n", type=int, default=2_000_000)
-    ap.add_argument("--d", type=int, default=128)
-    ap.add_argument("--k", type=int, default=16384)
+    ap.add_argument("--workload", default="search", choices=WORKLOADS)
+    ap.add_argument("--precision", default=None, choices=["bf16", "fp8", "fp32", "i8"],
+                    help="search precision of the headline line (workload default if omitted)")
+    ap.add_argument("--rescore-m", type=int, default=None)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--d", type=int, default=None)
+    ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--blobs", type=int, default=16384)
     ap.add_argument("--judge-rows", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-compare", action="store_true", help="skip the FP32 / FP8 comparison passes")
+    ap.add_argument("--no-compare", action="store_true", help="skip the comparison passes")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def copy_probe(nbytes: int):
@@ -61,23 +78,65 @@ def copy_probe(nbytes: int):
     print(json.dumps({"copy_probe_bytes": nbytes, "copies": 3}), flush=True)
 
 
-def log(msg):
-    print(msg, file=sys.stderr, flush=True)
-
-
-def make_data(n, d, k, blobs, device):
+# ------------------------------------------------------------------ data ----
+def mixture(n, d, blobs, device, seed=0):
+    """Rows = a uniformly chosen centre ~ N(0, 16 I) + N(0, I)."""
     import torch
 
-    g = torch.Generator(device=device).manual_seed(0)
+    g = torch.Generator(device=device).manual_seed(seed)
     centres = torch.randn(blobs, d, generator=g, device=device) * 4.0
     which = torch.randint(0, blobs, (n,), generator=g, device=device)
-    x = centres[which] + torch.randn(n, d, generator=g, device=device)
-    pick = torch.randperm(n, generator=torch.Generator().manual_seed(1234))[:k]
-    c = x[pick.to(device)].contiguous()
-    return x.contiguous(), c
+    return (centres[which] + torch.randn(n, d, generator=g, device=device)).contiguous()
 
 
-def run_search(prec, m, x, c_host, steps, warmup, rows_i8=False):
+def pick_rows(x, k, seed=1234):
+    import torch
+
+    pick = torch.randperm(x.shape[0], generator=torch.Generator().manual_seed(seed))[:k]
+    return x[pick.to(x.device)].contiguous()
+
+
+def int8_codes(n, d, device, seed=0):
+    """Rank-64 Gaussian signal + 0.1 isotropic noise, L2-normalised, then
+    int8 codes with the faiss calibration (per-dimension mean shift, one
+    power-of-two scale putting the largest centred magnitude <= 112, round
+    half away from zero, clamp), as Int8Quantizer::fit/encode do
+    (src/quantize.rs:45-73)."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    w = torch.randn(64, d, generator=g, device=device)
+    x = torch.empty(n, d, device=device)
+    for r0 in range(0, n, 262144):
+        r1 = min(n, r0 + 262144)
+        z = torch.randn(r1 - r0, 64, generator=g, device=device)
+        v = z @ w + 0.1 * torch.randn(r1 - r0, d, generator=g, device=device)
+        x[r0:r1] = v / v.norm(dim=1, keepdim=True)
+    shift = x.double().mean(0).float()
+    peak = (x - shift).abs().max().item()
+    scale = 2.0 ** __import__("math").floor(__import__("math").log2(112.0 / peak))
+    v = (x - shift) * scale
+    codes = (torch.sign(v) * torch.floor(v.abs() + 0.5)).clamp(-128, 127).to(torch.int8)
+    return codes.contiguous(), shift, scale
+
+
+def heavy_tailed(n, d, device, seed=0):
+    """i.i.d. Student-t (nu = 3) per dimension times a random orthogonal Q."""
+    import torch
+
+    g = torch.Generator(device=device).manual_seed(seed)
+    q, _ = torch.linalg.qr(torch.randn(d, d, generator=g, device=device))
+    x = torch.empty(n, d, device=device)
+    for r0 in range(0, n, 262144):
+        r1 = min(n, r0 + 262144)
+        z = torch.randn(r1 - r0, d, generator=g, device=device)
+        chi = (torch.randn(r1 - r0, d, 3, generator=g, device=device) ** 2).sum(-1)
+        x[r0:r1] = (z / torch.sqrt(chi / 3.0)) @ q
+    return x.contiguous()
+
+
+# ---------------------------------------------------------------- search ----
+def run_search(prec, m, x, c_host, steps, warmup):
     """Time `steps` searches of all rows; per-kernel times from HIP events."""
     import torch
 
@@ -104,14 +163,13 @@ def run_search(prec, m, x, c_host, steps, warmup, rows_i8=False):
     ms = t0.elapsed_time(t1) / steps
     enc, search, rescore, calls = a.get_timing()
     a.set_timing(False)
-    name = a.name()
     return {
         "ms_per_step": ms,
         "rows_per_s": n / (ms * 1e-3),
         "search_ms": search / max(calls, 1),
         "rescore_ms": rescore / max(calls, 1),
         "encode_ms": enc / max(calls, 1),
-        "name": name,
+        "name": a.name(),
         "labels": labels,
         "dist": dist,
     }
@@ -125,119 +183,103 @@ def judge(x, c, res, rows):
     ids = torch.randperm(x.shape[0], generator=g)[:rows].to(x.device)
     xs = x[ids].double()
     cd = c.double()
-    d2 = (xs * xs).sum(1, keepdim=True) - 2.0 * (xs @ cd.T) + (cd * cd).sum(1)[None]
-    best = d2.min(1)
+    best_idx = torch.empty(rows, dtype=torch.long, device=x.device)
+    for r0 in range(0, rows, 4096):
+        xb = xs[r0:r0 + 4096]
+        d2 = (xb * xb).sum(1, keepdim=True) - 2.0 * (xb @ cd.T) + (cd * cd).sum(1)[None]
+        best_idx[r0:r0 + 4096] = d2.argmin(1)
     lab = res["labels"][ids].long()
     d64 = ((xs - cd[lab]) ** 2).sum(1)
-    best_exact = ((xs - cd[best.indices]) ** 2).sum(1)
+    best_exact = ((xs - cd[best_idx]) ** 2).sum(1)
     wrong = d64 > best_exact + 1e-6 * (1 + best_exact)
     # error relative to 1 + d, the form of the reference's tolerance (src/cpu.rs:303)
     rel = (res["dist"][ids].double() - d64).abs() / (1.0 + d64)
     return {
-        "label_disagreement": float((lab != best.indices).double().mean()),
+        "label_disagreement": float((lab != best_idx).double().mean()),
         "true_label_errors": float(wrong.double().mean()),
         "dist_err_max_rel_1pd": float(rel.max()),
         "judged_rows": rows,
     }
 
 
-def cpu_baseline(x, c, seconds):
-    """The oracle's restatement of the reference CPU search (src/cpu.rs,
-    AVX-512 4x4 micro-kernel, 256 KiB blocks) on the host cores, on a bounded
-    sample of the same rows."""
-    import numpy as np
-
+def cpu_search_baseline(x, c, seconds, i8=False):
+    """The oracle's restatement of the reference CPU search (src/cpu.rs:
+    AVX-512 4x4 micro-kernel, 256 KiB blocks; src/cpu_i8.rs: VNNI) on the
+    host cores, on a bounded sample of the same rows."""
     import oracle as O
 
     cores = O.default_threads()
-    simd = O.lib().or_simd_best()
     xh = x.cpu().numpy()
     ch = c.cpu().numpy()
-    sample = 512
+
+    def run(rows):
+        if i8:
+            return O.i8_assign(xh[:rows], ch, nthreads=cores)
+        return O.cpu_assign(xh[:rows], ch, nthreads=cores)
+
+    sample = 256
     t = time.perf_counter()
-    O.cpu_assign(xh[:sample], ch, nthreads=cores)
+    run(sample)
     dt = time.perf_counter() - t
     sample = int(min(xh.shape[0], max(sample, sample * seconds / max(dt, 1e-3))))
     t = time.perf_counter()
-    O.cpu_assign(xh[:sample], ch, nthreads=cores)
+    run(sample)
     dt = time.perf_counter() - t
+    kind = ("AVX-512 VNNI" if O.has_vnni() else "portable") if i8 else \
+        ("AVX-512" if O.lib().or_simd_best() == O.SIMD_AVX512 else "portable")
     return {
         "value": sample / dt,
         "unit": "rows/s",
         "cores": cores,
         "kind": "port",
         "sample": f"first {sample} rows of the same workload against all {ch.shape[0]} centroids "
-                  f"({'AVX-512' if simd == O.SIMD_AVX512 else 'portable'} kernel, {cores} threads, {dt:.1f} s)",
-        "gflops": 2.0 * sample * ch.shape[0] * ch.shape[1] / dt / 1e9,
+                  f"({kind} kernel, {cores} threads, {dt:.1f} s)",
+        "gops": 2.0 * sample * ch.shape[0] * ch.shape[1] / dt / 1e9,
     }
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rows would shard by "
-              "range with no exchange; see the sharding note in DESIGN.md). Run with --gpus 1.")
-        sys.exit(2)
+def search_line(args, x, c, prec, m, compare, i8_rows=None):
     import torch
 
-    if not torch.cuda.is_available():
-        print("bench.py needs a GPU")
-        sys.exit(2)
-    if args.copy_probe:
-        copy_probe(args.copy_probe)
-        return
-    dev = torch.device("cuda:0")
-    log(f"data: n={args.n} d={args.d} k={args.k}")
-    x, c = make_data(args.n, args.d, args.k, args.blobs, dev)
+    n, d = x.shape
+    k = c.shape[0]
     c_host = c.cpu().numpy()
-    torch.cuda.synchronize()
-
+    xq = i8_rows if i8_rows is not None else x
     variants = {}
-    prec = args.precision
-    m = args.rescore_m if prec in ("bf16", "fp8") else 1
-    xq = x
-    if prec == "i8":
-        xq = x.clamp(-128, 127).round().to(torch.int8)
     log(f"timing {prec} (top-{m})")
     head = run_search(prec, m, xq, c_host, args.steps, args.warmup)
     variants[prec] = head
-    if not args.no_compare and prec != "fp32":
-        log("timing fp32")
-        variants["fp32"] = run_search("fp32", 1, x, c_host, max(2, args.steps // 4), 1)
-    if not args.no_compare and prec == "bf16":
-        log("timing fp8 (top-8)")
-        variants["fp8"] = run_search("fp8", 8, x, c_host, args.steps, args.warmup)
-
+    for cprec, cm in compare:
+        log(f"timing {cprec} (top-{cm})")
+        steps = max(2, args.steps // 4) if cprec == "fp32" else args.steps
+        variants[cprec] = run_search(cprec, cm, x, c_host, steps, 1)
     quality = {}
     if prec != "i8":
         for name, v in variants.items():
-            q = judge(x, c, v, min(args.judge_rows, args.n))
+            q = judge(x, c, v, min(args.judge_rows, n))
             q["mse"] = float(v["dist"].double().mean())
             quality[name] = q
         if "fp32" in quality:
             for name, q in quality.items():
                 q["mse_rel_vs_fp32"] = q["mse"] / quality["fp32"]["mse"] - 1.0
-
-    flops = 2.0 * args.n * args.k * args.d
+    flops = 2.0 * n * k * d
     achieved = flops / (head["search_ms"] * 1e-3) / 1e12
-    peak = PEAK[prec]
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 2),
-        "peak": peak,
+        "peak": PEAK[prec],
         "unit": "TFLOP/s" if prec != "i8" else "TOP/s",
-        "frac": round(achieved / peak, 4),
-        "traffic": None,
-        "kernel": f"search_kernel ({prec})",
+        "frac": round(achieved / PEAK[prec], 4),
+        "traffic": traffic_from_profile(args.workload, prec),
+        "kernel": f"search_reg_kernel ({prec})",
         "kernel_ms": round(head["search_ms"], 4),
     }
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_baseline(x, c, args.cpu_seconds)
-
+        cpu = cpu_search_baseline(xq if prec == "i8" else x, c, args.cpu_seconds, i8=prec == "i8")
     out = {
-        "metric": "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE",
+        "metric": METRIC,
         "value": round(head["rows_per_s"], 1),
         "unit": "rows/s",
         "n_gpus": 1,
@@ -248,15 +290,10 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32", "i8": "int8"}[prec],
-        "data": "synthetic (Gaussian mixture, generated on the GPU with torch, seed 0)",
-        "config": {
-            "workload": f"assign-only search at scale: {args.n}x{args.d} f32 rows, mixture of {args.blobs} blobs, "
-                        f"k={args.k} centroids drawn from the rows",
-            "n": args.n, "d": args.d, "k": args.k,
-            "precision": prec + ("+fp32 rescore" if prec in ("bf16", "fp8") else ""),
-            "rescore_m": m,
-            "rows_resident_in_hbm": True,
-        },
+        "data": "synthetic (generated on the GPU with torch, seed 0)",
+        "config": {"workload": args.workload_desc, "n": n, "d": d, "k": k,
+                   "precision": prec + ("+fp32 rescore" if prec in ("bf16", "fp8") else ""),
+                   "rescore_m": m, "rows_resident_in_hbm": True},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "variants": {
@@ -275,6 +312,139 @@ def main():
     }
     if "fp32" in variants and prec != "fp32":
         out["speedup_vs_fp32"] = round(head["rows_per_s"] / variants["fp32"]["rows_per_s"], 3)
+    return out
+
+
+def traffic_from_profile(workload, prec):
+    """HBM bytes per launch of the search kernel, from the committed rocprof
+    counter summary of the same workload (null when there is none)."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(p) as f:
+            return json.load(f).get(f"{workload}/{prec}")
+    except (OSError, ValueError):
+        return None
+
+
+# ----------------------------------------------------------------- train ----
+def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
+    """GPU training (rows resident in HBM; per batch: id upload, gather +
+    search + per-cluster sums on the device, EMA/revival on the host) vs the
+    CPU port of the same train loop."""
+    import numpy as np
+
+    import emeans_hip as E
+
+    xh = x.cpu().numpy()
+    n, d = xh.shape
+    cfg = E.Config(nepochs=nepochs)
+    log(f"train {prec} warmup")
+    E.train(xh[: min(n, 50 * k)], k, E.Config(nepochs=1), E.HipAssigner(prec))
+    log(f"train {prec}")
+    t = time.perf_counter()
+    out = E.train(xh, k, cfg, E.HipAssigner(prec))
+    gpu_s = time.perf_counter() - t
+    rows_per_s = nepochs * n / gpu_s
+
+    def mse(cent):
+        lab, dist = E.assign(mse_x if mse_x is not None else xh, cent, E.HipAssigner("fp32"))
+        return float(dist.astype(np.float64).mean())
+
+    res = {
+        "metric": "train rows/s (epochs x n / wall time)",
+        "value": round(rows_per_s, 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": int(out.recipe.total_steps),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * gpu_s / max(1, out.recipe.total_steps), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16": "bf16", "fp32": "f32", "i8": "int8"}[prec],
+        "data": "synthetic (generated on the GPU with torch, seed 0)",
+        "config": {"workload": args.workload_desc, "n": n, "d": d, "k": k, "epochs": nepochs,
+                   "batch": int(out.recipe.batch_size), "precision": prec},
+        "gpu_seconds": round(gpu_s, 3),
+        "gpu_loop_seconds": round(out.seconds, 3),
+        "final_mse_exact_reassign": mse(out.centroids),
+        "n_splits": out.n_splits,
+        "objective": out.objective,
+    }
+    if not args.no_cpu:
+        import oracle as O
+
+        log("cpu port train")
+        t = time.perf_counter()
+        ref = O.train(xh, k, O.Config(nepochs=nepochs), assigner=oracle_assigner)
+        cpu_s = time.perf_counter() - t
+        res["cpu_baseline"] = {
+            "value": round(nepochs * n / cpu_s, 1),
+            "unit": "rows/s",
+            "cores": O.default_threads(),
+            "kind": "port",
+            "sample": f"the full training run ({cpu_s:.1f} s) with the oracle's train and "
+                      f"{'int8 VNNI' if oracle_assigner else 'AVX-512 f32'} search",
+        }
+        res["cpu_final_mse_exact_reassign"] = mse(ref.centroids)
+        res["identical_to_cpu"] = bool(np.array_equal(ref.centroids, out.centroids)
+                                       and np.array_equal(ref.labels, out.labels))
+    return res
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (rows would shard by "
+              "range with no exchange; see the sharding note in DESIGN.md). Run with --gpus 1.")
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU")
+        sys.exit(2)
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
+    dev = torch.device("cuda:0")
+    w = args.workload
+    if w == "search":
+        n, d, k = args.n or 2_000_000, args.d or 128, args.k or 16384
+        args.workload_desc = (f"assign-only search at scale: {n}x{d} f32 rows, mixture of {args.blobs} blobs, "
+                              f"k={k} centroids drawn from the rows")
+        log(f"data: n={n} d={d} k={k}")
+        x = mixture(n, d, args.blobs, dev)
+        c = pick_rows(x, k)
+        prec = args.precision or "bf16"
+        m = args.rescore_m or (4 if prec == "bf16" else 8 if prec == "fp8" else 1)
+        compare = [] if args.no_compare else [c_ for c_ in (("fp32", 1), ("fp8", 8)) if c_[0] != prec]
+        out = search_line(args, x, c, prec, m, compare)
+    elif w == "search-i8":
+        n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
+        args.workload_desc = (f"int8 search: {n}x{d} int8 codes (rank-64 signal + 0.1 noise, L2-normalised, "
+                              f"faiss calibration), k={k} centroids drawn from the codes, exact I8 search")
+        codes, _, _ = int8_codes(n, d, dev)
+        c = pick_rows(codes.float(), k)
+        out = search_line(args, codes.float(), c, "i8", 1, [], i8_rows=codes)
+    elif w == "search-fp8-large-k":
+        n, d, k = args.n or 2_000_000, args.d or 256, args.k or 65536
+        args.workload_desc = (f"large-k search: {n}x{d} heavy-tailed rows (Student-t nu=3, random rotation), "
+                              f"k={k} centroids drawn from the rows, FP8 top-8 + FP32 rescore")
+        x = heavy_tailed(n, d, dev)
+        c = pick_rows(x, k)
+        compare = [] if args.no_compare else [("fp32", 1)]
+        out = search_line(args, x, c, args.precision or "fp8", args.rescore_m or 8, compare)
+    elif w == "train-c1":
+        n, d, k = args.n or 200_000, args.d or 96, args.k or 1024
+        args.workload_desc = f"e-means training: {n}x{d} mixture of 4096 blobs, k={k}, 5 epochs, default config"
+        x = mixture(n, d, 4096, dev)
+        out = train_line(args, x, k, 5.0, args.precision or "fp32", 0)
+    else:  # train-c3
+        n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
+        args.workload_desc = (f"int8 training: {n}x{d} int8 codes (faiss calibration), k={k}, 5 epochs, "
+                              f"I8 search (CpuI8Assigner semantics)")
+        codes, _, _ = int8_codes(n, d, dev)
+        out = train_line(args, codes, k, 5.0, "i8", 1)
     print(json.dumps(out), flush=True)
 
 
