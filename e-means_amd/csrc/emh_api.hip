@@ -234,9 +234,8 @@ int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t 
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
     SearchGeom g = make_geom(h->op, k, d);
-    const size_t es = op_elem_bytes(h->op);
     if ((rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
-    if ((rc = check(h->cen_op.ensure(g.k_pad * g.d_pad * es), "operand buffer"))) return rc;
+    if ((rc = check(h->cen_op.ensure(operand_bytes(g)), "operand buffer"))) return rc;
     if ((rc = check(h->norm32.ensure(g.k_pad * 4), "norm buffer"))) return rc;
     if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
     if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;

@@ -40,9 +40,25 @@ struct SearchGeom {
 
 SearchGeom make_geom(int op, size_t k, size_t d);
 
+// Centroids per tile of the register-resident search (one or two 32-wide
+// MFMA blocks).
+inline int reg_tile(const SearchGeom &g) { return g.row_bytes <= 256 ? 64 : 32; }
+// Bytes of the operand centroid buffer: the row-major table (k_pad rows of
+// d_pad elements; row_bytes is only the LDS segment when rows stream in
+// chunks), followed on the register-resident path (where a row is
+// row_bytes) by the same table in tile-major order (reg_operand).
+inline size_t operand_table_bytes(const SearchGeom &g) { return g.k_pad * g.d_pad * (size_t)op_elem_bytes(g.op); }
+inline size_t operand_bytes(const SearchGeom &g) {
+    return g.reg ? 2 * operand_table_bytes(g) : operand_table_bytes(g);
+}
+inline const void *reg_operand(const SearchGeom &g, const void *cen_op) {
+    return (const uint8_t *)cen_op + operand_table_bytes(g);
+}
+
 // Launchers (stream-ordered, no synchronisation).
 // stats (2 x u32, f32 bits): [0] max_j |c_j|^2 (set by the prep kernel),
 // [1] max(0, -min_j bias_j) (set by the bias kernel); both start at 0.
+// cen_op holds operand_bytes(g): the prep launcher fills both layouts.
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
                                  const unsigned *cmax_bits, unsigned *stats, hipStream_t s);
 hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s);

@@ -52,7 +52,8 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     // (rows past 256 bytes use 32-centroid tiles, which need 128-byte multiples)
     size_t rb64 = (d * es + 63) / 64 * 64;
     if (rb64 > 256) rb64 = (rb64 + 127) / 128 * 128;
-    if (rb64 <= ((op == OP_BF16 || op == OP_FP8) ? 256u : 512u)) {
+    // (the tile-major table is addressed through a buffer resource: < 2 GiB)
+    if (rb64 <= ((op == OP_BF16 || op == OP_FP8) ? 256u : 512u) && g.k_pad * rb64 <= 0x7FFFFFFFu) {
         g.reg = true;
         g.resident = true;
         g.d_pad = rb64 / es;
@@ -425,11 +426,16 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 //
 // For operand rows of at most 256 (BF16, FP8) or 512 (FP32, I8) bytes. Each
 // wave holds FR 32-row fragments of B in VGPRs for the whole centroid sweep.
-// Centroid tiles of 32 * NB centroids stream into two LDS buffers with direct
-// global->LDS loads (no staging registers); each wave-load writes 1 KiB of
-// LDS contiguously, so rows are not padded but XOR-swizzled by 16-byte quad
-// (quad q of row r sits in slot q ^ (r & (G-1))) to spread a fragment read
-// over the banks.
+// The operand centroids are also stored tile-major (tile_layout_kernel):
+// tile t holds TB = 32 * NB centroids, and 16-byte quad q of its centroid c
+// sits at byte (q * TB + c) * 16 of the tile. A tile is one contiguous block,
+// so it streams into LDS through buffer loads that write LDS directly (no
+// staging registers, 1 KiB per wave-load), and a fragment read, quad q of
+// 32 consecutive centroids, is 512 contiguous bytes: free of bank conflicts,
+// and every read of a tile sits at an immediate offset from one lane address.
+// (Buffer loads rather than global ones: the compiler tracks a global load
+// into LDS as a possible LDS access and then waits for every outstanding LDS
+// read wherever it needs any one of them.)
 //
 // BF16 / FP8 (candidate search): the accumulators start at
 // |c|^2/2 + bias/2 + B, where B is a per-workgroup offset keeping every score
@@ -444,11 +450,9 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 //
 // I8 (exact): the score is the integer 2 x.c8 - (sum c8^2 + round(bias)),
 // maximised; key = (score << 5) | (31 - position) orders by score, then by
-// lowest index, so one integer max per value settles the tile.
-template <int NQ> struct Swz {
-    static constexpr int G = (NQ & -NQ) < 16 ? (NQ & -NQ) : 16;  // swizzle span (power of two dividing NQ)
-    __device__ static inline int slot(int row, int q) { return q ^ (row & (G - 1)); }
-};
+// lowest index. The per-centroid part ((-cn) << 5) + 31 - position is
+// tabulated in LDS with the tile, so a key is one shift-add of the dot
+// product and one integer max per value settles the tile.
 
 // Per-lane running result: raw score bits of the candidate searches, the
 // f32 score of FP32, the int32 score of I8.
@@ -458,12 +462,16 @@ template <> struct RegScore<OP_I8> { using T = int32_t; };
 
 template <int OP> using RegAcc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
 
-// The MFMA half of a tile: acc = (score offsets) + centroid tile . row fragments.
+// Operand quads a lane reads ahead of the MFMAs that use them.
+template <int OP> struct RegPrefetch { static constexpr int D = (OP == OP_BF16 || OP == OP_I8) ? 2 : 1; };
+
+// The MFMA half of a tile: acc = (score offsets) + centroid tile . row
+// fragments. sT is the tile in LDS, sH its per-centroid offsets.
 template <int OP, int NS, int FR, int NB>
-__device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, const uint4 (&xb)[FR][NS],
+__device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, const uint4 (&xb)[FR][NS],
                                          const int (&sb)[FR], int sa, int h, int j, RegAcc<OP> (&acc)[FR][NB]) {
-    constexpr int RB = 32 * NS, NQ = RB / 16, HQ = NQ / 2;
-    using S = Swz<NQ>;
+    constexpr int TB = 32 * NB;
+    constexpr int QS = TB * 16;  // bytes from one quad of a centroid to its next
     using Acc = RegAcc<OP>;
     // The offsets enter as the C operand of each fragment's first MFMA
     // (no per-fragment copies); I8 starts from zero.
@@ -477,15 +485,15 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
             for (int r = 0; r < 16; r++) init[nb][r] = sH[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
         }
     }
-    const uint8_t *crow[NB];
-#pragma unroll
-    for (int nb = 0; nb < NB; nb++) crow[nb] = sC + (nb * 32 + j) * RB;  // same swizzle: 32 is a multiple of G
+    // this lane's quad 0 (of its row half) of centroid j of block 0
+    const uint8_t *ab = sT + (h * NS * TB + j) * 16;
+    auto ld = [&](int s, int nb) -> uint4 { return *(const uint4 *)(ab + s * QS + nb * 512); };
     if constexpr (OP == OP_F32) {
 #pragma unroll
-        for (int s = 0; s < HQ; s++) {
+        for (int s = 0; s < NS; s++) {
             float4 cv[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) cv[nb] = *(const float4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s));
+            for (int nb = 0; nb < NB; nb++) cv[nb] = __builtin_bit_cast(float4, ld(s, nb));
 #pragma unroll
             for (int fr = 0; fr < FR; fr++) {
                 const float4 xv = __builtin_bit_cast(float4, xb[fr][s]);
@@ -499,41 +507,42 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sC, const float *sH, con
             }
         }
     } else if constexpr (OP == OP_BF16 || OP == OP_I8) {
-        // A fragments are read one step ahead of the MFMAs that use them
-        uint4 nxt[NB];
+        // a ring of PD + 1 operand slots: step s reads step s + PD's quads
+        constexpr int PD = RegPrefetch<OP>::D;
+        uint4 ring[PD + 1][NB];
 #pragma unroll
-        for (int nb = 0; nb < NB; nb++) nxt[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ));
+        for (int p = 0; p < PD; p++)
 #pragma unroll
-        for (int s = 0; s < HQ; s++) {
-            uint4 cv[NB];
+            for (int nb = 0; nb < NB; nb++)
+                if (p < NS) ring[p][nb] = ld(p, nb);
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++) cv[nb] = nxt[nb];
-            if (s + 1 < HQ) {
+        for (int s = 0; s < NS; s++) {
+            if (s + PD < NS) {
 #pragma unroll
-                for (int nb = 0; nb < NB; nb++) nxt[nb] = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + s + 1));
+                for (int nb = 0; nb < NB; nb++) ring[(s + PD) % (PD + 1)][nb] = ld(s + PD, nb);
             }
 #pragma unroll
             for (int fr = 0; fr < FR; fr++)
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++) {
+                    const uint4 cv = ring[s % (PD + 1)][nb];
                     if constexpr (OP == OP_BF16)
                         acc[fr][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
-                            __builtin_bit_cast(bf16x8, cv[nb]), __builtin_bit_cast(bf16x8, xb[fr][s]),
+                            __builtin_bit_cast(bf16x8, cv), __builtin_bit_cast(bf16x8, xb[fr][s]),
                             s == 0 ? init[nb] : acc[fr][nb], 0, 0, 0);
                     else
                         acc[fr][nb] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                            __builtin_bit_cast(i32x4, cv[nb]), __builtin_bit_cast(i32x4, xb[fr][s]),
+                            __builtin_bit_cast(i32x4, cv), __builtin_bit_cast(i32x4, xb[fr][s]),
                             s == 0 ? init[nb] : acc[fr][nb], 0, 0, 0);
                 }
         }
     } else {  // OP_FP8: two quads per lane per MFMA
 #pragma unroll
-        for (int s = 0; s < HQ / 2; s++) {
+        for (int s = 0; s < NS / 2; s++) {
             i32x8 cv[NB];
 #pragma unroll
             for (int nb = 0; nb < NB; nb++) {
-                const uint4 a0 = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + 2 * s));
-                const uint4 a1 = *(const uint4 *)(crow[nb] + 16 * S::slot(j, h * HQ + 2 * s + 1));
+                const uint4 a0 = ld(2 * s, nb), a1 = ld(2 * s + 1, nb);
                 cv[nb] = i32x8{(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
             }
 #pragma unroll
@@ -571,27 +580,27 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
                     const float s = acc[fr][nb][r];
-                    if (s < bs[fr][0]) {
-                        bs[fr][0] = s;
-                        bi[fr][0] = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-                    }
+                    const bool u = s < bs[fr][0];
+                    bs[fr][0] = u ? s : bs[fr][0];
+                    bi[fr][0] = u ? c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3) : bi[fr][0];
                 }
-        } else {  // OP_I8: maximise key = ((2 dot - cn) << 5) | (31 - position)
-            const int32_t *sNi = (const int32_t *)sN;  // -cn per centroid
+        } else {  // OP_I8: maximise key = (dot << 6) + ((-cn) << 5) + 31 - position
+            const int32_t *sK = (const int32_t *)sN;
             int32_t g = INT32_MIN;
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
 #pragma unroll
                 for (int r = 0; r < 16; r++) {
-                    const int32_t sc = (acc[fr][nb][r] << 1) + sNi[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)];
-                    g = max(g, (int32_t)((uint32_t)sc << 5) | (31 - (nb * 16 + r)));
+                    const int32_t key = (int32_t)(((uint32_t)acc[fr][nb][r] << 6) +
+                                                  (uint32_t)sK[nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3)]);
+                    g = max(g, key);
                 }
             const int32_t sc = g >> 5;
             const uint32_t li = 31u - (uint32_t)(g & 31), nb = li >> 4, r = li & 15u;
-            if (sc > bs[fr][0]) {
-                bs[fr][0] = sc;
-                bi[fr][0] = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-            }
+            const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            const bool u = sc > bs[fr][0];
+            bs[fr][0] = u ? sc : bs[fr][0];
+            bi[fr][0] = u ? gi : bi[fr][0];
         }
     }
 }
@@ -599,22 +608,22 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     constexpr int ES = OpT<OP>::ES;
-    constexpr int RB = 32 * NS;    // operand bytes per row
-    constexpr int NQ = RB / 16;    // 16-byte quads per row
-    constexpr int HB = RB / 2;     // bytes per lane half
-    constexpr int TB = 32 * NB;    // centroids per tile
-    constexpr int PER_T = NQ * TB / 256;  // quads of a tile per thread
+    constexpr int RB = 32 * NS;           // operand bytes per row
+    constexpr int HB = RB / 2;            // bytes per lane half
+    constexpr int TB = 32 * NB;           // centroids per tile
+    constexpr int TILE = TB * RB;         // bytes per tile
+    constexpr int PER_T = TILE / 4096;    // 1 KiB wave-loads per wave per tile
     constexpr bool CAND = OP == OP_BF16 || OP == OP_FP8;
-    using S = Swz<NQ>;
     using Score = typename RegScore<OP>::T;
-    static_assert(NS % 2 == 0 && PER_T >= 1 && (NQ * TB) % 256 == 0, "shape");
-    __shared__ __align__(16) uint8_t sC0[TB * RB];
-    __shared__ __align__(16) uint8_t sC1[TB * RB];
-    __shared__ __align__(16) uint8_t sC2[TB * RB];
+    static_assert(NS % 2 == 0 && PER_T >= 1 && TILE % 4096 == 0, "shape");
+    __shared__ __align__(16) uint8_t sC0[TILE];
+    __shared__ __align__(16) uint8_t sC1[TILE];
+    __shared__ __align__(16) uint8_t sC2[TILE];
     __shared__ float sH0[256], sH1[256], sH2[256];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int h = lane >> 5, j = lane & 31;
     const size_t row0 = (size_t)blockIdx.x * (128 * FR);
 
@@ -690,40 +699,47 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
         }
     }
 
-    // Direct global->LDS tile loads: wave-load q of wave w fills LDS quads
-    // [(4q + w) * 64, +64); lane L fetches the centroid quad that belongs in
-    // LDS quad p = (4q + w) * 64 + L (row p / NQ, slot p % NQ, quad slot ^ swz).
-    int goff[PER_T];
-#pragma unroll
-    for (int q = 0; q < PER_T; q++) {
-        const int p = (4 * q + wave) * 64 + lane, r = p / NQ, sl = p - r * NQ;
-        goff[q] = (r * NQ + S::slot(r, sl)) * 16;
-    }
+    // Tile DMA: wave-load q of wave w copies tile bytes [(4q + w) KiB, +1 KiB)
+    // to the same place in LDS; lane L moves 16 bytes at L * 16. The buffer
+    // resource spans the tile-major table (its size is checked on the host).
     const int ntiles = (int)(a.k_pad / TB);
+    const __amdgpu_buffer_rsrc_t rsrc =
+        __builtin_amdgcn_make_buffer_rsrc((void *)a.cen_op, 0, (int)(a.k_pad * (size_t)RB), 0x00020000);
     auto dma = [&](uint8_t *dst, int t) {
-        const uint8_t *g = a.cen_op + (size_t)t * (TB * RB);
+        const uint32_t so = (uint32_t)t * TILE + (uint32_t)wv * 1024u;
 #pragma unroll
         for (int q = 0; q < PER_T; q++)
-            __builtin_amdgcn_global_load_lds((const void *)(g + goff[q]),
-                                             (__attribute__((address_space(3))) void *)(dst + (4 * q + wave) * 1024),
-                                             16, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (__attribute__((address_space(3))) void *)(dst + q * 4096 + wv * 1024),
+                                                     16, (uint32_t)lane * 16u, so + q * 4096u, 0, 0);
     };
     // Per-centroid score offsets of a tile, loaded raw: any arithmetic on the
     // loaded value would make the compiler wait for every outstanding load of
     // the wave (the tile DMA included) right there. Converted at the store:
-    // float hn + B, or int32 -cn (I8).
-    using HRaw = typename std::conditional<OP == OP_I8, int64_t, float>::type;
+    // float hn + B, or the I8 key part ((-cn) << 5) + 31 - position.
+    // I8 loads the low word of cn only (|cn| <= 2^26 on this path): a dead
+    // high word still being written by a load would make the compiler wait
+    // for it, and with it for the tile DMA, before reusing its register.
+    using HRaw = typename std::conditional<OP == OP_I8, int32_t, float>::type;
     auto hload = [&](int t) -> HRaw {
         // every thread loads (threads past TB repeat an entry): no branch
         const size_t ci = (size_t)t * TB + (tid & (TB - 1));
-        if constexpr (OP == OP_I8) return a.cn[ci];
+        if constexpr (OP == OP_I8) return ((const int32_t *)a.cn)[2 * ci];  // little-endian low word
         else return a.hn[ci];
     };
     // every thread stores (slots past TB are never read): no branch, so the
     // compiler's wait tracking stays exact across the stage
     auto hstore = [&](float *dst, HRaw v) {
-        if constexpr (OP == OP_I8) dst[tid] = __int_as_float((int32_t)(-v));
-        else dst[tid] = v + B;
+        if constexpr (OP == OP_I8) {
+            // centroid c of the tile is value r of block nb for lanes of half
+            // (c >> 2) & 1, with position nb * 16 + r in the fold
+            const int c = tid & (TB - 1), cl = c & 31;
+            const int pos = (c >> 5) * 16 + ((cl >> 3) << 2) + (cl & 3);
+            // |cn| <= 2^26 on this path, so -cn << 5 fits in int32
+            const uint32_t part = ((uint32_t)(-v) << 5) + (uint32_t)(31 - pos);
+            dst[tid] = __int_as_float((int32_t)part);
+        } else {
+            dst[tid] = v + B;
+        }
     };
 
     Score bs[FR][M];
@@ -824,6 +840,18 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
             }
         }
     }
+}
+
+// Copies the row-major operand table into the tile-major order of the
+// register-resident search: quad q of centroid c of tile t goes to quad
+// (t * nq + q) * tb + c. One thread per 16-byte quad.
+__global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, int nq, int tb) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= k_pad * (size_t)nq) return;
+    const size_t per_tile = (size_t)nq * tb;
+    const size_t t = i / per_tile, rem = i - t * per_tile;
+    const size_t q = rem / tb, c = rem - q * tb;
+    dst[i] = src[(t * tb + c) * nq + q];
 }
 
 // -------------------------------------------------------------- rescore ----
@@ -1148,32 +1176,44 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
     a.labels = labels;
     a.dist = dist;
     a.cand = cand;
+    if (reg_path(g)) {
+        // the register-resident kernels read the tile-major copy
+        SearchArgs r = a;
+        r.cen_op = (const uint8_t *)reg_operand(g, cen_op);
+        switch (g.op) {
+            case OP_F32:
+                if (M != 1) return hipErrorInvalidValue;
+                if (rows == EMH_ROWS_F32) return launch_reg_ns<OP_F32, EMH_ROWS_F32, 1>(g, r, s);
+                if (rows == EMH_ROWS_I8) return launch_reg_ns<OP_F32, EMH_ROWS_I8, 1>(g, r, s);
+                return launch_reg_ns<OP_F32, EMH_ROWS_U8, 1>(g, r, s);
+            case OP_BF16:
+                if (rows == EMH_ROWS_F32) return launch_reg_m<OP_BF16, EMH_ROWS_F32>(g, M, r, s);
+                if (rows == EMH_ROWS_I8) return launch_reg_m<OP_BF16, EMH_ROWS_I8>(g, M, r, s);
+                return launch_reg_m<OP_BF16, EMH_ROWS_U8>(g, M, r, s);
+            case OP_FP8:
+                return launch_reg_m<OP_FP8, EMH_ROWS_F32>(g, M, r, s);
+            case OP_I8:
+                if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
+                // scores past the packed key's range take the LDS kernel's i64 path
+                if (g.i8_packed) return launch_reg_ns<OP_I8, EMH_ROWS_I8, 1>(g, r, s);
+                break;
+        }
+    }
+    // LDS kernel (rows resident in LDS, or streamed in d-chunks)
     switch (g.op) {
         case OP_F32:
             if (M != 1) return hipErrorInvalidValue;
-            if (reg_path(g)) {
-                if (rows == EMH_ROWS_F32) return launch_reg_ns<OP_F32, EMH_ROWS_F32, 1>(g, a, s);
-                if (rows == EMH_ROWS_I8) return launch_reg_ns<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
-                return launch_reg_ns<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
-            }
             if (rows == EMH_ROWS_F32) return launch_res<OP_F32, EMH_ROWS_F32, 1>(g, a, s);
             if (rows == EMH_ROWS_I8) return launch_res<OP_F32, EMH_ROWS_I8, 1>(g, a, s);
             return launch_res<OP_F32, EMH_ROWS_U8, 1>(g, a, s);
         case OP_BF16:
-            if (reg_path(g)) {
-                if (rows == EMH_ROWS_F32) return launch_reg_m<OP_BF16, EMH_ROWS_F32>(g, M, a, s);
-                if (rows == EMH_ROWS_I8) return launch_reg_m<OP_BF16, EMH_ROWS_I8>(g, M, a, s);
-                return launch_reg_m<OP_BF16, EMH_ROWS_U8>(g, M, a, s);
-            }
             if (rows == EMH_ROWS_F32) return launch_m<OP_BF16, EMH_ROWS_F32>(g, M, a, s);
             if (rows == EMH_ROWS_I8) return launch_m<OP_BF16, EMH_ROWS_I8>(g, M, a, s);
             return launch_m<OP_BF16, EMH_ROWS_U8>(g, M, a, s);
         case OP_FP8:
-            if (reg_path(g)) return launch_reg_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
             return launch_m<OP_FP8, EMH_ROWS_F32>(g, M, a, s);
         case OP_I8:
             if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
-            if (reg_path(g) && g.i8_packed) return launch_reg_ns<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
             return launch_res<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
     }
     return hipErrorInvalidValue;
@@ -1214,6 +1254,12 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
         case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         default: return hipErrorInvalidValue;
     }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !g.reg) return e;
+    const int nq = g.row_bytes / 16, tb = reg_tile(g);
+    const size_t quads = g.k_pad * (size_t)nq;
+    hipLaunchKernelGGL(tile_layout_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s,
+                       (const uint4 *)cen_op, (uint4 *)reg_operand(g, cen_op), g.k_pad, nq, tb);
     return hipGetLastError();
 }
 
