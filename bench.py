@@ -36,7 +36,7 @@ for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
 # MI355X dense peaks (AMD published): TFLOP/s (TOP/s for int8)
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
-WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3"]
+WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4"]
 
 
 def parse():
@@ -209,8 +209,8 @@ def cpu_search_baseline(x, c, seconds, i8=False):
     import oracle as O
 
     cores = O.default_threads()
-    xh = x.cpu().numpy()
-    ch = c.cpu().numpy()
+    xh = x.cpu().numpy() if hasattr(x, "cpu") else x
+    ch = c.cpu().numpy() if hasattr(c, "cpu") else c
 
     def run(rows):
         if i8:
@@ -327,6 +327,69 @@ def traffic_from_profile(workload, prec):
 
 
 # ----------------------------------------------------------------- train ----
+def train_large_line(args, x, k, nepochs, prec, m):
+    """Large-k training: the reduced-precision search (top-m + FP32 rescore)
+    against the plain FP32 search, both on this GPU, compared by the exact
+    (FP32 search) re-assignment MSE of their final centroids. The CPU port
+    cannot train this in minutes: its leg times the same search on a sample
+    of rows, which bounds a CPU training run's rows/s from above."""
+    import numpy as np
+    import torch
+
+    import emeans_hip as E
+
+    xh = x.cpu().numpy()
+    del x
+    torch.cuda.empty_cache()
+    n, d = xh.shape
+    log(f"train {prec} warmup")
+    E.train(xh[:65536], 1024, E.Config(nepochs=1), E.HipAssigner(prec, rescore_m=m))
+    runs = {}
+    for name, p, mm in ((prec, prec, m), ("fp32", "fp32", 1)):
+        log(f"train {name}")
+        t = time.perf_counter()
+        out = E.train(xh, k, E.Config(nepochs=nepochs), E.HipAssigner(p, rescore_m=mm))
+        secs = time.perf_counter() - t
+        log(f"exact re-assignment MSE ({name})")
+        _, dist = E.assign(xh, out.centroids, E.HipAssigner("fp32"))
+        runs[name] = {"seconds": secs, "loop_seconds": out.seconds, "out": out,
+                      "mse": float(dist.astype(np.float64).mean())}
+    head = runs[prec]
+    out = head["out"]
+    res = {
+        "metric": "train rows/s (epochs x n / wall time)",
+        "value": round(nepochs * n / head["seconds"], 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": int(out.recipe.total_steps),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * head["seconds"] / max(1, out.recipe.total_steps), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16": "bf16", "fp8": "fp8"}[prec],
+        "data": "synthetic (generated on the GPU with torch, seed 0)",
+        "config": {"workload": args.workload_desc, "n": n, "d": d, "k": k, "epochs": nepochs,
+                   "batch": int(out.recipe.batch_size), "precision": f"{prec}+fp32 rescore", "rescore_m": m},
+        "gpu_seconds": round(head["seconds"], 3),
+        "gpu_loop_seconds": round(head["loop_seconds"], 3),
+        "final_mse_exact_reassign": head["mse"],
+        "fp32_search_run": {"rows_per_s": round(nepochs * n / runs["fp32"]["seconds"], 1),
+                            "gpu_seconds": round(runs["fp32"]["seconds"], 3),
+                            "final_mse_exact_reassign": runs["fp32"]["mse"]},
+        "mse_rel_vs_fp32": head["mse"] / runs["fp32"]["mse"] - 1.0,
+        "speedup_vs_fp32": round(runs["fp32"]["seconds"] / head["seconds"], 3),
+        "n_splits": out.n_splits,
+        "objective": out.objective,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (search on a sample)")
+        cpu = cpu_search_baseline(xh, out.centroids, args.cpu_seconds)
+        cpu["sample"] = "search only: " + cpu["sample"]
+        res["cpu_baseline"] = cpu
+    return res
+
+
 def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     """GPU training (rows resident in HBM; per batch: id upload, gather +
     search + per-cluster sums on the device, EMA/revival on the host) vs the
@@ -439,6 +502,15 @@ def main():
         args.workload_desc = f"e-means training: {n}x{d} mixture of 4096 blobs, k={k}, 5 epochs, default config"
         x = mixture(n, d, 4096, dev)
         out = train_line(args, x, k, 5.0, args.precision or "fp32", 0)
+    elif w == "train-c4":
+        n, d, k = args.n or 10_000_000, args.d or 256, args.k or 65536
+        prec = args.precision or "fp8"
+        m = args.rescore_m or 8
+        args.workload_desc = (f"large-k training: {n}x{d} heavy-tailed rows (Student-t nu=3, random rotation), "
+                              f"k={k}, 2 epochs, {prec.upper()} top-{m} + FP32 rescore, FP64 closing-mean sums on "
+                              f"the device; MSE vs the FP32 search on the same GPU")
+        x = heavy_tailed(n, d, dev)
+        out = train_large_line(args, x, k, 2.0, prec, m)
     else:  # train-c3
         n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
         args.workload_desc = (f"int8 training: {n}x{d} int8 codes (faiss calibration), k={k}, 5 epochs, "

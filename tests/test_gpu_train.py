@@ -45,7 +45,7 @@ def _blobs_6d():
     return (centers.reshape(20, d)[np.arange(n) % 20] + noise).astype(np.float32)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_closing_mean_and_loss(hip, prec):
     # src/lib.rs:575-614 on the GPU assigner
     x = _blobs_6d()
@@ -61,6 +61,27 @@ def test_closing_mean_and_loss(hip, prec):
     start = E.assign(x, x[O.permutation(1234, n)[:k]])[1].astype(np.float64).mean()
     assert trained < 0.75 * start and trained < 2.0
     assert all(b < a for a, b in zip(out.objective, out.objective[1:]))
+
+
+@pytest.mark.parametrize("prec,m", [("bf16", 4), ("fp8", 8)])
+def test_rescored_training_matches_fp32(hip, prec, m):
+    """Training on the candidate search (top-m + FP32 rescore) ends within
+    0.5% of the FP32 search's exact re-assignment MSE, and its closing mean
+    is exact on its own labels."""
+    rng = np.random.default_rng(4)
+    n, d, k = 60_000, 64, 1024
+    centres = rng.normal(scale=4, size=(2048, d)).astype(np.float32)
+    x = (centres[rng.integers(0, 2048, n)] + rng.normal(size=(n, d))).astype(np.float32)
+    ref = E.train(x, k, E.Config(nepochs=2), E.HipAssigner(E.FP32))
+    got = E.train(x, k, E.Config(nepochs=2), E.HipAssigner(prec, rescore_m=m))
+    mse_ref = E.assign(x, ref.centroids)[1].astype(np.float64).mean()
+    mse_got = E.assign(x, got.centroids)[1].astype(np.float64).mean()
+    assert abs(mse_got / mse_ref - 1) < 5e-3, (mse_got, mse_ref)
+    counts = np.bincount(got.labels, minlength=k)
+    sums = np.zeros((k, d))
+    np.add.at(sums, got.labels, x.astype(np.float64))
+    live = counts > 0
+    assert np.all(np.abs(got.centroids[live] - sums[live] / counts[live, None]) < 1e-3 * (1 + np.abs(got.centroids[live])))
 
 
 def test_f32_training_tracks_oracle(hip):
