@@ -3,11 +3,16 @@
 // The reference folds each batch on the host, row by row in batch order:
 // counts[l] += 1; sums[l] += row (f32), src/lib.rs:431-441, and keeps the
 // last pass's totals for the closing mean, src/lib.rs:446-454, 481-489.
-// Here the batch is grouped by label with a stable radix sort (row positions
-// keep their batch order inside each label), and one lane per (cluster,
-// dimension) folds that cluster's rows in that order, so every f32 sum is
-// bit-identical to the host loop. The last pass's totals accumulate in FP64.
-#include <rocprim/device/device_radix_sort.hpp>
+// Here the batch is grouped by label with a stable LSD radix sort (row
+// positions keep their batch order inside each label), and one lane per
+// (cluster, dimension) folds that cluster's rows in that order, so every f32
+// sum is bit-identical to the host loop. The last pass's totals accumulate in
+// FP64.
+//
+// The sort is this file's own (8-bit digits: block histograms, one-workgroup
+// scan, stable scatter with wave ballots). It issues kernels only, no memset
+// or allocation, so a captured graph can replay it.
+#include <algorithm>
 
 #include "emh_internal.h"
 
@@ -23,9 +28,8 @@ __global__ void fill_f64_kernel(double *p, size_t count) {
         p[i] = 0.0;
 }
 
-__global__ void iota_hist_kernel(const uint32_t *labels, size_t m, size_t k, uint32_t *iota, uint32_t *counts) {
+__global__ void count_kernel(const uint32_t *labels, size_t m, size_t k, uint32_t *counts) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
-        iota[i] = (uint32_t)i;
         uint32_t l = labels[i];
         if (l < k) atomicAdd(&counts[l], 1u);
     }
@@ -96,24 +100,106 @@ static int key_bits(size_t k) {
     return b;
 }
 
+// ---- stable LSD radix sort of (key, position) pairs, 8-bit digits ----------
+constexpr int SORT_BLOCK = 1024;  // elements per workgroup (256 threads x 4 rounds)
+
+// hist[digit * nblocks + block] = how many of the block's keys have the digit
+__global__ __launch_bounds__(256) void digit_hist_kernel(const uint32_t *keys, size_t m, int shift, uint32_t *hist,
+                                                         size_t nblocks) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
+    for (int r = 0; r < SORT_BLOCK / 256; r++) {
+        const size_t i = b0 + (size_t)r * 256 + threadIdx.x;
+        if (i < m) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// In-place exclusive scan of count entries by one workgroup of 1024 threads:
+// each thread scans a contiguous slice, then the slice totals are scanned.
+__global__ __launch_bounds__(1024) void exclusive_scan_kernel(uint32_t *v, size_t count) {
+    __shared__ uint32_t tot[1024];
+    const size_t per = (count + 1023) / 1024;
+    const size_t a = (size_t)threadIdx.x * per, b = std::min(count, a + per);
+    uint32_t s = 0;
+    for (size_t i = a; i < b; i++) s += v[i];
+    tot[threadIdx.x] = s;
+    __syncthreads();
+    // Hillis-Steele over the 1024 slice totals
+    for (int off = 1; off < 1024; off <<= 1) {
+        const uint32_t add = threadIdx.x >= (unsigned)off ? tot[threadIdx.x - off] : 0u;
+        __syncthreads();
+        tot[threadIdx.x] += add;
+        __syncthreads();
+    }
+    uint32_t run = tot[threadIdx.x] - s;  // exclusive prefix of this slice
+    for (size_t i = a; i < b; i++) {
+        const uint32_t c = v[i];
+        v[i] = run;
+        run += c;
+    }
+}
+
+// Stable scatter: an element's destination is the start of its digit in its
+// block (scanned histogram) plus the number of equal digits before it in the
+// block. Inside a wave the equal digits are found with one ballot per digit
+// bit; waves and rounds are ordered through LDS counters, so the block keeps
+// its input order.
+__global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
+                                                            int shift, const uint32_t *offs, size_t nblocks,
+                                                            uint32_t *keys_out, uint32_t *vals_out) {
+    __shared__ uint32_t run[256];      // equal digits in the rounds before this one
+    __shared__ uint32_t wcnt[4][256];  // per wave, this round
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    run[tid] = 0;
+    const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int r = 0; r < SORT_BLOCK / 256; r++) {
+        for (int q = 0; q < 4; q++) wcnt[q][tid] = 0;
+        __syncthreads();
+        const size_t i = b0 + (size_t)r * 256 + tid;
+        const bool live = i < m;
+        const uint32_t key = live ? keys[i] : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+        uint64_t same = __ballot(live);
+        for (int bit = 0; bit < 8; bit++) {
+            const uint64_t bl = __ballot(live && ((dg >> bit) & 1u));
+            same &= ((dg >> bit) & 1u) ? bl : ~bl;
+        }
+        const uint32_t before = (uint32_t)__popcll(same & lt);
+        if (live && before == 0) wcnt[w][dg] = (uint32_t)__popcll(same);
+        __syncthreads();
+        if (live) {
+            uint32_t pos = offs[(size_t)dg * nblocks + blockIdx.x] + run[dg] + before;
+            for (int q = 0; q < w; q++) pos += wcnt[q][dg];
+            keys_out[pos] = key;
+            vals_out[pos] = vals ? vals[i] : (uint32_t)i;
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+    }
+}
+
 hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
     hipError_t e = hipSuccess;
     if (m > a.cap_rows) {
+        hipFree(a.keys);
         hipFree(a.keys_alt);
         hipFree(a.vals);
         hipFree(a.vals_alt);
         hipFree(a.sort_tmp);
-        a.keys_alt = a.vals = a.vals_alt = nullptr;
+        a.keys = a.keys_alt = a.vals = a.vals_alt = nullptr;
         a.sort_tmp = nullptr;
+        if ((e = hipMalloc(&a.keys, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.keys_alt, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.vals, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.vals_alt, m * 4)) != hipSuccess) return e;
-        size_t bytes = 0;
-        e = rocprim::radix_sort_pairs((void *)nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr,
-                                      (const uint32_t *)nullptr, (uint32_t *)nullptr, m, 0, 32);
-        if (e != hipSuccess) return e;
-        a.sort_tmp_bytes = bytes;
-        if ((e = hipMalloc(&a.sort_tmp, bytes ? bytes : 16)) != hipSuccess) return e;
+        const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
+        a.sort_tmp_bytes = 256 * nblocks * 4;
+        if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
         a.cap_rows = m;
     }
     if (k > a.cap_k) {
@@ -126,12 +212,35 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
 }
 
 void accum_free(AccumScratch &a) {
+    hipFree(a.keys);
     hipFree(a.keys_alt);
     hipFree(a.vals);
     hipFree(a.vals_alt);
     hipFree(a.sort_tmp);
     hipFree(a.offsets);
     a = AccumScratch();
+}
+
+hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, size_t m, int bits, const uint32_t **keys_out,
+                             const uint32_t **vals_out, hipStream_t s) {
+    const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
+    uint32_t *hist = (uint32_t *)a.sort_tmp;
+    // ping-pong: pass 0 reads the input keys and writes positions
+    const uint32_t *kin = keys_in, *vin = nullptr;
+    uint32_t *kb[2] = {a.keys, a.keys_alt}, *vb[2] = {a.vals, a.vals_alt};
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 8) {
+        hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, shift, hist, nblocks);
+        hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, s, hist, 256 * nblocks);
+        hipLaunchKernelGGL(digit_scatter_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m, shift,
+                           (const uint32_t *)hist, nblocks, kb[cur], vb[cur]);
+        kin = kb[cur];
+        vin = vb[cur];
+        cur ^= 1;
+    }
+    *keys_out = kin;
+    *vals_out = vin;
+    return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s) {
@@ -153,25 +262,23 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
     hipError_t e;
     if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
     if ((e = launch_fill_u32(counts, k, 0u, s)) != hipSuccess) return e;
+    const uint32_t *sorted_pos = a.vals;
     if (m > 0) {
         unsigned blocks = (unsigned)std::min<size_t>((m + 255) / 256, 4096);
-        hipLaunchKernelGGL(iota_hist_kernel, dim3(blocks), dim3(256), 0, s, labels, m, k, a.vals, counts);
+        hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(256), 0, s, labels, m, k, counts);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        size_t bytes = a.sort_tmp_bytes;
-        e = rocprim::radix_sort_pairs(a.sort_tmp, bytes, labels, a.keys_alt, (const uint32_t *)a.vals, a.vals_alt, m,
-                                      0, key_bits(k), s);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, (const uint32_t *)a.keys_alt, m, k,
-                           a.offsets);
+        const uint32_t *sorted_keys;
+        if ((e = launch_sort_pairs(a, labels, m, key_bits(k), &sorted_keys, &sorted_pos, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, sorted_keys, m, k, a.offsets);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     dim3 grid((unsigned)((k + 3) / 4));
     if (rows == EMH_ROWS_F32)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, ids, a.vals_alt, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
     else if (rows == EMH_ROWS_I8)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, ids, a.vals_alt, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
     else
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, ids, a.vals_alt, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
     return hipGetLastError();
 }
 

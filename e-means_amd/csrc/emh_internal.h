@@ -30,7 +30,7 @@ inline int op_granule(int op) { return (op == OP_BF16 || op == OP_FP8) ? 64 : 32
 
 struct SearchGeom {
     int op;
-    bool reg;           // rows fit the register-resident kernel (operand row <= 256 / 512 bytes)
+    bool reg;           // rows fit the register-resident kernel (operand row <= 512 bytes)
     bool i8_packed = false;  // I8: every |2 x.c - cn| < 2^26, so scores fit the packed int32 key
     bool resident;      // the workgroup's rows stay in LDS for the whole centroid sweep
     size_t d, d_pad;    // elements; d_pad * elem_bytes is the operand row stride

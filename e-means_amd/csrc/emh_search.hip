@@ -45,15 +45,15 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     g.k = k;
     g.k_pad = (k + BN - 1) / BN * BN;
     const size_t es = op_elem_bytes(op);
-    // Register-resident kernel: operand rows padded to 64 bytes, at most 256
-    // (BF16, FP8) or 512 (FP32, I8) bytes. The LDS kernel can run on the same
-    // layout (rows resident in LDS), which the I8 search falls back to when
-    // its scores exceed the packed-key range.
+    // Register-resident kernel: operand rows padded to 64 bytes, at most 512
+    // bytes. The LDS kernel can run on the same layout (rows resident in
+    // LDS), which the I8 search falls back to when its scores exceed the
+    // packed-key range.
     // (rows past 256 bytes use 32-centroid tiles, which need 128-byte multiples)
     size_t rb64 = (d * es + 63) / 64 * 64;
     if (rb64 > 256) rb64 = (rb64 + 127) / 128 * 128;
     // (the tile-major table is addressed through a buffer resource: < 2 GiB)
-    if (rb64 <= ((op == OP_BF16 || op == OP_FP8) ? 256u : 512u) && g.k_pad * rb64 <= 0x7FFFFFFFu) {
+    if (rb64 <= 512u && g.k_pad * rb64 <= 0x7FFFFFFFu) {
         g.reg = true;
         g.resident = true;
         g.d_pad = rb64 / es;
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 
 // ------------------------------------------- register-resident searches ----
 //
-// For operand rows of at most 256 (BF16, FP8) or 512 (FP32, I8) bytes. Each
+// For operand rows of at most 512 bytes. Each
 // wave holds FR 32-row fragments of B in VGPRs for the whole centroid sweep.
 // The operand centroids are also stored tile-major (tile_layout_kernel):
 // tile t holds TB = 32 * NB centroids, and 16-byte quad q of its centroid c
@@ -1128,12 +1128,8 @@ static hipError_t launch_reg_ns(const SearchGeom &g, const SearchArgs &a, hipStr
         case 4: return launch_reg<OP, ROWS, M, 4>(a, s);
         case 6: return launch_reg<OP, ROWS, M, 6>(a, s);
         case 8: return launch_reg<OP, ROWS, M, 8>(a, s);
-        case 12:
-            if constexpr (OP == OP_F32 || OP == OP_I8) return launch_reg<OP, ROWS, M, 12>(a, s);
-            return hipErrorInvalidValue;
-        case 16:
-            if constexpr (OP == OP_F32 || OP == OP_I8) return launch_reg<OP, ROWS, M, 16>(a, s);
-            return hipErrorInvalidValue;
+        case 12: return launch_reg<OP, ROWS, M, 12>(a, s);
+        case 16: return launch_reg<OP, ROWS, M, 16>(a, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -151,13 +151,14 @@ def _mixture(n, d, k_blobs, seed, scale=4.0):
     return x.astype(np.float32)
 
 
-@pytest.mark.parametrize("prec,m,max_dis", [("bf16", 4, 1e-3), ("bf16", 1, 2e-2), ("fp8", 8, 5e-3)])
-def test_rescored_search_vs_fp64(hip, prec, m, max_dis):
+@pytest.mark.parametrize("prec,m,max_dis,d", [("bf16", 4, 1e-3, 128), ("bf16", 1, 2e-2, 128), ("fp8", 8, 5e-3, 128),
+                                              ("bf16", 4, 1e-3, 256), ("fp8", 4, 5e-3, 512), ("fp8", 8, 5e-3, 256)])
+def test_rescored_search_vs_fp64(hip, prec, m, max_dis, d):
     """Label disagreement vs FP64 at most max_dis (true errors only: a label
     whose FP64 distance exceeds the best by more than 1e-6(1+best)), and the
     returned distance within 1e-4(1+best) of FP64 at that label."""
-    print(f"{prec} m={m}", flush=True)
-    x = _mixture(20000, 128, 2048, 3)
+    print(f"{prec} m={m} d={d}", flush=True)
+    x = _mixture(20000, d, 2048, 3)
     c = x[O.permutation(1234, 20000)[:1024]]
     a = E.HipAssigner(prec, rescore_m=m)
     a.set_centroids(c)

@@ -98,15 +98,20 @@ def test_f32_training_tracks_oracle(hip):
     assert abs(got.objective[-1] / ref.objective[-1] - 1) < 0.02
 
 
-def test_device_accumulate_is_batch_order_f32(hip):
+@pytest.mark.parametrize("n,m,k", [(5000, 3000, 50), (20000, 7001, 300), (70000, 66000, 70000), (9000, 1, 1)])
+def test_device_accumulate_is_batch_order_f32(hip, n, m, k):
+    """The stable sort + segmented fold equals the sequential host loop bit
+    for bit: one, two and three 8-bit sort passes, blocks with a ragged end,
+    a single row, empty clusters."""
     import torch
 
+    print(f"accumulate n={n} m={m} k={k}", flush=True)
     rng = np.random.default_rng(3)
-    n, d, k = 5000, 37, 50
+    d = 37
     x = rng.normal(scale=100, size=(n, d)).astype(np.float32)
-    ids = rng.permutation(n)[:3000].astype(np.uint32)
-    labels = rng.integers(0, k, 3000).astype(np.uint32)
-    labels[labels == 7] = 8  # an empty cluster
+    ids = rng.permutation(n)[:m].astype(np.uint32)
+    labels = rng.integers(0, k, m).astype(np.uint32)
+    labels[labels == 7] = 8  # an empty cluster (when k > 8)
     a = E.HipAssigner(E.FP32)
     a.set_centroids(np.zeros((k, d), np.float32))
     xd = torch.from_numpy(x).cuda()
@@ -120,7 +125,7 @@ def test_device_accumulate_is_batch_order_f32(hip):
     # the host loop of src/lib.rs:431-441: sequential f32 adds in batch order
     want = np.zeros((k, d), np.float32)
     want_c = np.zeros(k, np.int64)
-    for r in range(3000):
+    for r in range(m):
         want[labels[r]] += x[ids[r]]
         want_c[labels[r]] += 1
     assert np.array_equal(counts.cpu().numpy(), want_c)
