@@ -36,7 +36,7 @@ for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
 # MI355X dense peaks (AMD published): TFLOP/s (TOP/s for int8)
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
-WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4"]
+WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5"]
 
 
 def parse():
@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--d", type=int, default=None)
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--blobs", type=int, default=16384)
+    ap.add_argument("--jobs", type=int, default=64, help="train-c5: number of problems")
     ap.add_argument("--judge-rows", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -327,6 +328,88 @@ def traffic_from_profile(workload, prec):
 
 
 # ----------------------------------------------------------------- train ----
+def c5_problems(njobs, dev, seed=5):
+    """Uneven problems: n log-uniform in [5e3, 5e5], d in {64, 96, 128, 256},
+    k log-uniform in [64, 4096], Gaussian-mixture rows (per-job seed),
+    3 epochs, balance 0.1 on the odd jobs."""
+    import numpy as np
+
+    import emeans_hip as E
+
+    rng = np.random.default_rng(seed)
+    probs = []
+    for j in range(njobs):
+        n = int(round(np.exp(rng.uniform(np.log(5e3), np.log(5e5)))))
+        d = int(rng.choice([64, 96, 128, 256]))
+        k = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
+        x = mixture(n, d, max(2 * k, 64), dev, seed=1000 + j).cpu().numpy()
+        probs.append((x, k, E.Config(nepochs=3.0, balance=0.1 if j % 2 else 0.0, seed=1234 + j)))
+    return probs
+
+
+def train_batch_line(args, njobs, prec, m):
+    """Many uneven jobs: the batched driver (one captured-graph replay per
+    lockstep step, reduced-precision search + FP32 rescore) against a loop of
+    single-job FP32 trainings, compared by aggregate rows/s and by each job's
+    exact (FP32 search) re-assignment MSE."""
+    import numpy as np
+
+    import emeans_hip as E
+
+    dev = __import__("torch").device("cuda:0")
+    probs = c5_problems(njobs, dev)
+    rows = sum(cfg.nepochs * x.shape[0] for x, _, cfg in probs)
+    work = sum(2.0 * cfg.nepochs * x.shape[0] * k * x.shape[1] for x, k, cfg in probs)
+    log("batched warmup")
+    E.train_batch([(x[:2 * k], k, E.Config(nepochs=1.0)) for x, k, _ in probs[:4]], E.HipAssigner(prec, rescore_m=m))
+    log(f"batched {prec} ({njobs} jobs)")
+    t = time.perf_counter()
+    got, st = E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
+    bs = time.perf_counter() - t
+    log("per-job fp32 loop")
+    t = time.perf_counter()
+    ref = [E.train(x, k, cfg, E.HipAssigner("fp32")) for x, k, cfg in probs]
+    ls = time.perf_counter() - t
+    log("exact re-assignment MSE")
+    rel = []
+    for (x, k, cfg), g, r in zip(probs, got, ref):
+        a = E.assign(x, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+        b = E.assign(x, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+        rel.append(a / b - 1.0)
+    loop_steps = sum(int(r.recipe.total_steps) for r in ref)
+    res = {
+        "metric": "train rows/s (sum over jobs of epochs x n / wall time)",
+        "value": round(rows / bs, 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": int(st.steps),
+        "warmup": 1,
+        "ms_per_step": round(1e3 * bs / max(1, st.steps), 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"bf16": "bf16", "fp32": "f32"}[prec],
+        "data": "synthetic (generated on the GPU with torch, per-job seeds)",
+        "config": {"workload": args.workload_desc, "jobs": njobs, "precision": f"{prec}" + ("+fp32 rescore" if prec != "fp32" else ""),
+                   "rescore_m": m, "rows_total": int(sum(x.shape[0] for x, _, _ in probs)),
+                   "search_tflop": round(work / 1e12, 2)},
+        "gpu_seconds": round(bs, 3),
+        "graph_replays": int(st.steps),
+        "graph_nodes_per_replay": int(st.graph_nodes),
+        "fp32_job_loop": {"rows_per_s": round(rows / ls, 1), "gpu_seconds": round(ls, 3),
+                          "train_steps": loop_steps},
+        "speedup_vs_fp32_loop": round(ls / bs, 3),
+        "mse_rel_vs_fp32_loop": {"mean": float(np.mean(rel)), "max": float(np.max(rel)), "min": float(np.min(rel))},
+    }
+    if not args.no_cpu:
+        log("cpu baseline (search on a sample of the largest job)")
+        jmax = max(range(len(probs)), key=lambda i: probs[i][0].shape[0] * probs[i][1] * probs[i][0].shape[1])
+        cpu = cpu_search_baseline(probs[jmax][0], got[jmax].centroids, args.cpu_seconds)
+        cpu["sample"] = "search only, largest job: " + cpu["sample"]
+        res["cpu_baseline"] = cpu
+    return res
+
+
 def train_large_line(args, x, k, nepochs, prec, m):
     """Large-k training: the reduced-precision search (top-m + FP32 rescore)
     against the plain FP32 search, both on this GPU, compared by the exact
@@ -511,6 +594,14 @@ def main():
                               f"the device; MSE vs the FP32 search on the same GPU")
         x = heavy_tailed(n, d, dev)
         out = train_large_line(args, x, k, 2.0, prec, m)
+    elif w == "train-c5":
+        njobs = args.jobs
+        prec = args.precision or "bf16"
+        m = args.rescore_m or (4 if prec == "bf16" else 1)
+        args.workload_desc = (f"batched uneven problems: {njobs} jobs, n log-uniform in [5000, 500000], d in "
+                              f"{{64, 96, 128, 256}}, k log-uniform in [64, 4096], Gaussian-mixture rows, 3 epochs, "
+                              f"balance 0.1 on odd jobs; one graph replay per lockstep step vs a per-job FP32 loop")
+        out = train_batch_line(args, njobs, prec, m)
     else:  # train-c3
         n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
         args.workload_desc = (f"int8 training: {n}x{d} int8 codes (faiss calibration), k={k}, 5 epochs, "

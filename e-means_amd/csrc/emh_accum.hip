@@ -282,6 +282,91 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
     return hipGetLastError();
 }
 
+// ---- many jobs per launch ----
+
+// key of batch slot i: its job's first global cluster + label, or nclusters
+// for a slot past the job's rows this step (sorts last, never folded)
+__global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots, uint32_t nclusters,
+                                 uint32_t *keys, uint32_t *counts_all) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (size_t)gridDim.x * blockDim.x) {
+        const uint2 r = rmap[i];
+        const JobDev &j = jobs[r.x];
+        uint32_t key = nclusters;
+        if (r.y < j.m) {
+            const uint32_t l = j.labels[r.y];
+            if (l < j.k) {
+                key = j.cluster_base + l;
+                atomicAdd(&counts_all[key], 1u);
+            }
+        }
+        keys[i] = key;
+    }
+}
+
+// The last epoch's first batch zeroes its job's FP64 totals.
+__global__ void jobs_pass_clear_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters) {
+    const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= nclusters) return;
+    const JobDev &j = jobs[cjob[g]];
+    if (!j.clear_pass || j.m == 0) return;
+    const size_t c = g - j.cluster_base;
+    for (size_t t = lane; t < j.d; t += 64) j.pass_sums[c * j.d + t] = 0.0;
+    if (lane == 0) j.pass_counts[c] = 0;
+}
+
+// One wave per global cluster: the job's rows folded in batch order (the
+// sorted slots), then the last epoch's FP64 totals.
+__global__ __launch_bounds__(256) void jobs_segment_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
+                                                               size_t nclusters, const uint32_t *sorted_slot,
+                                                               const uint32_t *start, const uint32_t *counts_all) {
+    const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (g >= nclusters) return;
+    const JobDev &j = jobs[cjob[g]];
+    if (j.m == 0) return;
+    const size_t c = g - j.cluster_base, d = j.d;
+    const uint32_t n = counts_all[g];
+    const uint32_t s0 = n ? start[g] : 0;
+    for (size_t t = lane; t < d; t += 64) {
+        float acc = 0.0f;
+        for (uint32_t q = 0; q < n; q++) {
+            const size_t r = sorted_slot[s0 + q] - j.row_base;
+            acc = __fadd_rn(acc, j.x[(size_t)j.ids[r] * d + t]);
+        }
+        j.sums[c * d + t] = acc;
+        if (j.last_epoch && n) j.pass_sums[c * d + t] += (double)acc;
+    }
+    if (lane == 0 && j.last_epoch && n) j.pass_counts[c] += n;
+}
+
+hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
+                                  const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                                  hipStream_t s) {
+    hipError_t e;
+    if ((e = accum_reserve(a, slots > 0 ? slots : 1, nclusters)) != hipSuccess) return e;
+    dim3 cgrid((unsigned)((nclusters + 3) / 4));
+    hipLaunchKernelGGL(jobs_pass_clear_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters);
+    if ((e = launch_fill_u32(counts_all, nclusters, 0u, s)) != hipSuccess) return e;
+    const uint32_t *sorted_slot = a.vals;
+    if (slots > 0) {
+        unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
+        hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters,
+                           keys, counts_all);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        const uint32_t *sorted_keys;
+        if ((e = launch_sort_pairs(a, keys, slots, key_bits(nclusters + 1), &sorted_keys, &sorted_slot, s)) !=
+            hipSuccess)
+            return e;
+        hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, sorted_keys, slots, nclusters,
+                           a.offsets);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(jobs_segment_sum_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters, sorted_slot,
+                       (const uint32_t *)a.offsets, (const uint32_t *)counts_all);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, size_t k, size_t d, double *pass_sums,
                                   uint64_t *pass_counts, hipStream_t s) {
     dim3 grid((unsigned)((k + 3) / 4));

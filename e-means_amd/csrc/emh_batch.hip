@@ -1,0 +1,274 @@
+// Batched training: many independent e-means problems in lockstep.
+//
+// Each job keeps the host recipe of src/lib.rs:348-491 exactly (JobRun);
+// what is shared is the device step. Every job's batch of one training step
+// is a slot range of one concatenated batch, every job's clusters a range of
+// one concatenated cluster list, and the step (upload of the staged
+// snapshots, prep of every job's centroid operand, the search per row-width
+// class, the FP32 rescore, the per-cluster sums with one stable sort, the
+// download) is captured once as a hipGraph and replayed once per step. The
+// kernels read the step's row counts and flags from a device job table that
+// the replay's first copy refreshes, so the graph never changes; jobs that
+// have finished keep their slots and idle.
+#include <chrono>
+
+#include "emh_recipe.h"
+
+namespace emh {
+
+namespace {
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Byte layout of a block made of aligned segments.
+struct Layout {
+    size_t bytes = 0;
+    size_t add(size_t n) {
+        const size_t at = align_up(bytes, 256);
+        bytes = at + (n ? n : 1);
+        return at;
+    }
+};
+
+struct GraphGuard {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphGuard() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+}  // namespace
+
+static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats) {
+    if (!jobs || njobs == 0) return fail(EMH_E_ARG, "train_batch: need at least one job");
+    if (njobs > 0xFFFFu) return fail(EMH_E_ARG, "train_batch: at most 65535 jobs per call");
+    if (h->op != OP_F32 && h->op != OP_BF16)
+        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32 and BF16 searches");
+    const int op = h->op, M = op == OP_BF16 ? h->M : 1;
+    const auto t0 = std::chrono::steady_clock::now();
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+
+    // Host runners, geometry and slot ranges.
+    std::vector<JobRun> run(njobs);
+    std::vector<SearchGeom> geo(njobs);
+    std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs);
+    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0;
+    for (size_t j = 0; j < njobs; j++) {
+        const emh_job &jb = jobs[j];
+        if (jb.rows != EMH_ROWS_F32) return fail(EMH_E_UNSUPPORTED, "train_batch: f32 rows only");
+        if ((rc = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out)))
+            return rc;
+        geo[j] = make_geom(op, jb.k, jb.d);
+        if (!geo[j].reg)
+            return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
+                                               " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256)");
+        row_base[j] = slots;
+        slots += run[j].R.batch_size;
+        cl_base[j] = nclusters;
+        nclusters += jb.k;
+        cen_base[j] = cen_floats;
+        cen_floats += jb.k * jb.d;
+        cpad_base[j] = cpad;
+        cpad += geo[j].k_pad;
+        op_base[j] = op_bytes;
+        op_bytes += geo[j].k_pad * (size_t)geo[j].row_bytes;
+    }
+    if (slots > 0xFFFFFFFFull || nclusters >= 0xFFFFFFFFull)
+        return fail(EMH_E_ARG, "train_batch: more than 2^32 - 1 batch rows or clusters in one call");
+
+    // Resident rows, one table per job.
+    std::vector<DevBuf> rows_d(njobs);
+    for (size_t j = 0; j < njobs; j++) {
+        const size_t bytes = jobs[j].n * jobs[j].d * 4;
+        if ((rc = check(rows_d[j].ensure(bytes), "resident rows"))) return rc;
+        if ((rc = check(hipMemcpyAsync(rows_d[j].p, jobs[j].x, bytes, hipMemcpyHostToDevice, h->stream), "upload rows")))
+            return rc;
+    }
+
+    // Staged per-step input (host pinned -> device, one copy) and output
+    // (device -> host pinned, one copy), same layout on both sides.
+    const size_t sa_bytes = search_args_bytes();
+    Layout in;
+    const size_t in_job = in.add(njobs * sizeof(JobDev)), in_sa = in.add(njobs * sa_bytes),
+                 in_cen = in.add(cen_floats * 4), in_bias = in.add(nclusters * 4), in_ids = in.add(slots * 4);
+    Layout outl;
+    const size_t out_lab = outl.add(slots * 4), out_dist = outl.add(slots * 4), out_cnt = outl.add(nclusters * 4),
+                 out_sums = outl.add(cen_floats * 4);
+    DevBuf in_d, out_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts, maps;
+    HostBuf in_h, out_h;
+    if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
+        (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
+        (rc = check(out_h.alloc(outl.bytes), "pinned step output")) ||
+        (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
+        (rc = check(hn.ensure(cpad * 4), "offsets")) || (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
+        (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
+        (rc = check(keys.ensure(slots * 4), "keys")) || (rc = check(pass_sums.ensure(cen_floats * 8), "pass sums")) ||
+        (rc = check(pass_counts.ensure(nclusters * 8), "pass counts")))
+        return rc;
+    memset(in_h.p, 0, in.bytes);
+    uint8_t *in_dp = (uint8_t *)in_d.p, *out_dp = (uint8_t *)out_d.p;
+    uint8_t *in_hp = (uint8_t *)in_h.p, *out_hp = (uint8_t *)out_h.p;
+
+    // Static maps: search workgroups per row-width class, prep entries, batch
+    // slots, clusters.
+    std::vector<int> classes;
+    for (size_t j = 0; j < njobs; j++)
+        if (std::find(classes.begin(), classes.end(), geo[j].row_bytes) == classes.end())
+            classes.push_back(geo[j].row_bytes);
+    std::sort(classes.begin(), classes.end());
+    std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
+    std::vector<size_t> wg_off, wg_cnt;
+    for (int rb : classes) {
+        wg_off.push_back(wg_all.size());
+        for (size_t j = 0; j < njobs; j++) {
+            if (geo[j].row_bytes != rb) continue;
+            for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += 256) wg_all.push_back(make_uint2((unsigned)j, (unsigned)r0));
+        }
+        wg_cnt.push_back(wg_all.size() - wg_off.back());
+    }
+    std::vector<uint32_t> cjob(nclusters);
+    for (size_t j = 0; j < njobs; j++) {
+        for (size_t c = 0; c < geo[j].k_pad; c++) cmap[cpad_base[j] + c] = make_uint2((unsigned)j, (unsigned)c);
+        for (size_t r = 0; r < run[j].R.batch_size; r++) rmap[row_base[j] + r] = make_uint2((unsigned)j, (unsigned)r);
+        for (size_t c = 0; c < jobs[j].k; c++) cjob[cl_base[j] + c] = (uint32_t)j;
+    }
+    Layout ml;
+    const size_t m_wg = ml.add(wg_all.size() * 8), m_c = ml.add(cpad * 8), m_r = ml.add(slots * 8),
+                 m_j = ml.add(nclusters * 4);
+    if ((rc = check(maps.ensure(ml.bytes), "maps"))) return rc;
+    uint8_t *mp = (uint8_t *)maps.p;
+    if ((rc = check(hipMemcpyAsync(mp + m_wg, wg_all.data(), wg_all.size() * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
+        (rc = check(hipMemcpyAsync(mp + m_c, cmap.data(), cpad * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
+        (rc = check(hipMemcpyAsync(mp + m_r, rmap.data(), slots * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
+        (rc = check(hipMemcpyAsync(mp + m_j, cjob.data(), nclusters * 4, hipMemcpyHostToDevice, h->stream), "maps")) ||
+        (rc = check(hipStreamSynchronize(h->stream), "upload")))
+        return rc;
+
+    // Static part of the job table (device addresses).
+    std::vector<JobDev> jd(njobs);
+    for (size_t j = 0; j < njobs; j++) {
+        JobDev &t = jd[j];
+        memset(&t, 0, sizeof t);
+        t.x = rows_d[j].as<float>();
+        t.ids = (const uint32_t *)(in_dp + in_ids) + row_base[j];
+        t.cen = (const float *)(in_dp + in_cen) + cen_base[j];
+        t.bias = (const float *)(in_dp + in_bias) + cl_base[j];
+        t.cen_op = (uint8_t *)cen_op.p + op_base[j];
+        t.norm32 = norm32.as<float>() + cpad_base[j];
+        t.hn = hn.as<float>() + cpad_base[j];
+        t.stats = stats_d.as<unsigned>() + 4 * j;
+        t.labels = (uint32_t *)(out_dp + out_lab) + row_base[j];
+        t.dist = (float *)(out_dp + out_dist) + row_base[j];
+        t.cand = cand.as<uint32_t>() + row_base[j] * M;
+        t.sums = (float *)(out_dp + out_sums) + cen_base[j];
+        t.counts = (uint32_t *)(out_dp + out_cnt) + cl_base[j];
+        t.pass_sums = pass_sums.as<double>() + cen_base[j];
+        t.pass_counts = pass_counts.as<uint64_t>() + cl_base[j];
+        t.d = (uint32_t)jobs[j].d;
+        t.k = (uint32_t)jobs[j].k;
+        t.k_pad = (uint32_t)geo[j].k_pad;
+        t.d_pad = (uint32_t)geo[j].d_pad;
+        t.row_bytes = (uint32_t)geo[j].row_bytes;
+        t.tb = (uint32_t)reg_tile(geo[j]);
+        t.row_base = (uint32_t)row_base[j];
+        t.cluster_base = (uint32_t)cl_base[j];
+    }
+    if ((rc = check(accum_reserve(h->acc, slots, nclusters), "sort scratch"))) return rc;
+
+    // Capture the step.
+    const JobDev *jobs_dev = (const JobDev *)(in_dp + in_job);
+    const void *sa_dev = in_dp + in_sa;
+    GraphGuard gg;
+    if ((rc = check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return rc;
+    hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in.bytes, hipMemcpyHostToDevice, h->stream);
+    if (ce == hipSuccess) ce = launch_fill_u32(stats_d.as<uint32_t>(), 4 * njobs, 0u, h->stream);
+    if (ce == hipSuccess) ce = launch_jobs_prep(op, jobs_dev, (const uint2 *)(mp + m_c), cpad, h->stream);
+    for (size_t c = 0; c < classes.size() && ce == hipSuccess; c++)
+        ce = launch_jobs_search(op, M, classes[c], sa_dev, (const uint2 *)(mp + m_wg) + wg_off[c], wg_cnt[c], h->stream);
+    if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    if (ce == hipSuccess)
+        ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
+                                    nclusters, keys.as<uint32_t>(), (uint32_t *)(out_dp + out_cnt), h->stream);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(out_hp, out_dp, outl.bytes, hipMemcpyDeviceToHost, h->stream);
+    hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);
+    if ((rc = check(ce, "capture step")) || (rc = check(ee, "end capture"))) return rc;
+    if ((rc = check(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0), "instantiate step"))) return rc;
+    size_t nodes = 0;
+    (void)hipGraphGetNodes(gg.graph, nullptr, &nodes);
+
+    // Lockstep steps.
+    uint64_t steps = 0;
+    std::vector<char> live(njobs);
+    JobDev *jd_h = (JobDev *)(in_hp + in_job);
+    for (;;) {
+        bool any = false;
+        for (size_t j = 0; j < njobs; j++) {
+            JobRun &J = run[j];
+            JobDev t = jd[j];
+            live[j] = J.next();
+            if (live[j]) {
+                any = true;
+                memcpy(in_hp + in_cen + cen_base[j] * 4, J.st.centroids.data(), J.k * J.d * 4);
+                if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
+                memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
+                t.m = (uint32_t)J.nb;
+                t.penalized = J.penalized;
+                t.last_epoch = J.is_last;
+                t.clear_pass = J.last_epoch_start;
+            }
+            jd_h[j] = t;
+            fill_search_args(in_hp + in_sa + j * sa_bytes, geo[j], t);
+        }
+        if (!any) break;
+        if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch")) ||
+            (rc = check(hipStreamSynchronize(h->stream), "step")))
+            return rc;
+        steps++;
+        for (size_t j = 0; j < njobs; j++) {
+            if (!live[j]) continue;
+            run[j].finish_batch((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j],
+                                (const uint32_t *)(out_hp + out_cnt) + cl_base[j], (const float *)(out_hp + out_sums) + cen_base[j]);
+        }
+    }
+
+    // Closing means (src/lib.rs:481-489) from the FP64 pass totals, per job;
+    // the staged centroid area holds each job's final EMA centroids.
+    float *cen_dev = (float *)(in_dp + in_cen);
+    for (size_t j = 0; j < njobs; j++) {
+        const size_t k = jobs[j].k, d = jobs[j].d;
+        if ((rc = check(hipMemcpyAsync(cen_dev + cen_base[j], run[j].st.centroids.data(), k * d * 4,
+                                       hipMemcpyHostToDevice, h->stream),
+                        "copy centroids")) ||
+            (rc = check(launch_closing_mean(jd[j].pass_sums, jd[j].pass_counts, k, d, cen_dev + cen_base[j], h->stream),
+                        "closing mean")) ||
+            (rc = check(hipMemcpyAsync(jobs[j].centroids_out, cen_dev + cen_base[j], k * d * 4, hipMemcpyDeviceToHost,
+                                       h->stream),
+                        "copy centroids")))
+            return rc;
+    }
+    if ((rc = check(hipStreamSynchronize(h->stream), "closing mean"))) return rc;
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (size_t j = 0; j < njobs; j++) {
+        jobs[j].out->n_splits = run[j].st.n_splits;
+        jobs[j].out->recipe = run[j].R;
+        jobs[j].out->seconds = secs;
+    }
+    if (stats) {
+        stats->steps = steps;
+        stats->graph_nodes = nodes;
+        stats->seconds = secs;
+    }
+    return EMH_OK;
+}
+
+}  // namespace emh
+
+using namespace emh;
+
+extern "C" int emh_train_batch(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats) {
+    if (!h) return fail(EMH_E_ARG, "train_batch: NULL handle");
+    return batch_impl(h, jobs, njobs, stats);
+}

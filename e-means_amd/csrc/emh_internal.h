@@ -75,6 +75,47 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
                           const float *bias, size_t k, const uint32_t *cand, uint32_t *labels, float *dist,
                           hipStream_t s);
 
+// ---- many independent problems per launch (batched training) -------------
+// One problem's device view for one step; the host rewrites the per-step
+// fields before each replay of the step graph. Pointers address this job's
+// slot of the shared per-step arrays.
+struct JobDev {
+    const float *x;          // resident rows, n x d f32
+    const uint32_t *ids;     // this step's batch row positions
+    const float *cen;        // this step's centroid snapshot, k x d
+    const float *bias;       // this step's balance bias, k (read when penalized)
+    uint8_t *cen_op;         // tile-major operand table (register-resident search)
+    float *norm32, *hn;      // k_pad each
+    unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias (f32 bits)
+    uint32_t *labels;        // batch slot
+    float *dist;             // batch slot
+    uint32_t *cand;          // batch slot x M
+    float *sums;             // k x d
+    uint32_t *counts;        // k
+    double *pass_sums;       // k x d
+    uint64_t *pass_counts;   // k
+    uint32_t d, k, k_pad, d_pad, row_bytes, tb;
+    uint32_t row_base, cluster_base;  // offsets in the concatenated batch / cluster ranges
+    // per step
+    uint32_t m;              // rows this step (0: the job is done)
+    uint32_t penalized;      // balance bias applies this step
+    uint32_t last_epoch;     // fold this step's sums into the FP64 pass totals
+    uint32_t clear_pass;     // first batch of the last epoch: zero the pass totals first
+};
+
+// prep: one wave per (job, centroid < k_pad) of cmap; operand, norms,
+// offsets with this step's bias, stats (zeroed before).
+hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s);
+// search of every job in one launch per row-width class: wg holds
+// (job, first row) per workgroup; all jobs of the call share row_bytes.
+hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_args, const uint2 *wg, size_t nwg,
+                              hipStream_t s);
+// rescoring of every job's candidates: rmap holds (job, row) per batch slot.
+hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
+// Builds a SearchArgs table entry (opaque bytes) for one job's step.
+size_t search_args_bytes();
+void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
+
 // Device-side per-cluster accumulation (emh_accum.hip).
 struct AccumScratch {
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
@@ -92,6 +133,12 @@ hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, siz
 hipError_t launch_closing_mean(const double *pass_sums, const uint64_t *pass_counts, size_t k, size_t d,
                                float *centroids, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s);
+// Batched: keys = cluster_base + label for live slots (nclusters for spare
+// slots), counts per global cluster, stable sort, then per-job f32 sums and
+// FP64 pass totals. cjob maps a global cluster to its job.
+hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
+                                  const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                                  hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
 
 // Error reporting shared by the C ABI.

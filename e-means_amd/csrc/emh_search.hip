@@ -14,6 +14,7 @@
 // 16 centroids of each 32x32 tile in ascending index order, which gives the
 // reference's lowest-index tie rule with a strict '<' (src/cpu.rs:149).
 #include <float.h>
+#include <string.h>
 #include <math.h>
 
 #include <algorithm>
@@ -605,8 +606,10 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
     }
 }
 
+// One workgroup's sweep: rows [row0, row0 + 128 FR) of a.x against every
+// centroid tile.
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
-__global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
+__device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_t row0) {
     constexpr int ES = OpT<OP>::ES;
     constexpr int RB = 32 * NS;           // operand bytes per row
     constexpr int HB = RB / 2;            // bytes per lane half
@@ -625,7 +628,6 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const int h = lane >> 5, j = lane & 31;
-    const size_t row0 = (size_t)blockIdx.x * (128 * FR);
 
     // Row fragments into registers; squared row norms (exact for I8).
     uint4 xb[FR][NS];
@@ -842,6 +844,23 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     }
 }
 
+template <int OP, int ROWS, int M, int NS, int FR, int NB>
+__global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR));
+}
+
+// Many independent problems in one launch: workgroup b sweeps rows
+// [wg[b].y, + 128 FR) of problem wg[b].x, whose arguments (row count of this
+// step included) sit in a device table. Problems with fewer rows this step
+// leave their spare workgroups idle.
+template <int OP, int ROWS, int M, int NS, int FR, int NB>
+__global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *jobs, const uint2 *wg) {
+    const uint2 w = wg[blockIdx.x];
+    const SearchArgs a = jobs[w.x];
+    if ((size_t)w.y >= a.m) return;  // whole workgroup: before any barrier
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)w.y);
+}
+
 // Copies the row-major operand table into the tile-major order of the
 // register-resident search: quad q of centroid c of tile t goes to quad
 // (t * nq + q) * tb + c. One thread per 16-byte quad.
@@ -861,71 +880,109 @@ __global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, i
 // at a time: 16 lanes per (row, candidate) pair stride over d with 16-byte
 // loads, each lane folds its elements in order without FMA, and a fixed
 // butterfly adds the 16 partial sums, so every lane holds identical bits.
+// One row, by one wave.
+template <int ROWS, int M>
+__device__ __forceinline__ void rescore_row(const void *x, size_t d, size_t k, const uint32_t *ids, const float *c,
+                                            const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
+                                            size_t row, int lane) {
+    constexpr int G = 16, PPW = 64 / G, PASSES = (M + PPW - 1) / PPW;
+    const int grp = lane / G, gl = lane % G;
+    const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0;
+    const size_t src = ids ? (size_t)ids[row] : row;
+    float best = INFINITY;
+    uint32_t bidx = NO_LABEL;
+#pragma unroll
+    for (int p = 0; p < PASSES; p++) {
+        const int slot = p * PPW + grp;
+        uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
+        if (ci >= k) ci = NO_LABEL;  // pad centroids can fill a list when k is small
+        float acc = 0.0f;
+        if (ci != NO_LABEL) {
+            const float *cp = c + (size_t)ci * d;
+            if (vec) {
+                const float *xr = (const float *)x + src * d;
+                for (size_t e = 4 * gl; e < d; e += 4 * G) {
+                    const float4 xv = *(const float4 *)(xr + e), cv = *(const float4 *)(cp + e);
+                    float t0 = __fsub_rn(xv.x, cv.x), t1 = __fsub_rn(xv.y, cv.y);
+                    float t2 = __fsub_rn(xv.z, cv.z), t3 = __fsub_rn(xv.w, cv.w);
+                    acc = __fadd_rn(acc, __fmul_rn(t0, t0));
+                    acc = __fadd_rn(acc, __fmul_rn(t1, t1));
+                    acc = __fadd_rn(acc, __fmul_rn(t2, t2));
+                    acc = __fadd_rn(acc, __fmul_rn(t3, t3));
+                }
+            } else {
+                for (size_t e = gl; e < d; e += G) {
+                    const float t = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
+                    acc = __fadd_rn(acc, __fmul_rn(t, t));
+                }
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, off, 64));
+        float s = INFINITY;
+        if (ci != NO_LABEL) s = bias ? __fadd_rn(acc, bias[ci]) : acc;
+        uint32_t si = ci;
+#pragma unroll
+        for (int off = G; off < 64; off <<= 1) {
+            const float os = __shfl_xor(s, off, 64);
+            const uint32_t oi = (uint32_t)__shfl_xor((int)si, off, 64);
+            if (os < s || (os == s && oi < si)) {
+                s = os;
+                si = oi;
+            }
+        }
+        if (s < best || (s == best && si < bidx)) {
+            best = s;
+            bidx = si;
+        }
+    }
+    if (lane == 0) {
+        labels[row] = bidx == NO_LABEL ? 0u : bidx;
+        dist[row] = (bidx != NO_LABEL && best > 0.0f) ? best : 0.0f;
+    }
+}
+
 template <int ROWS, int M>
 __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, size_t k, const uint32_t *ids,
                                                       const float *c, const float *bias, const uint32_t *cand,
                                                       uint32_t *labels, float *dist) {
-    constexpr int G = 16, PPW = 64 / G, PASSES = (M + PPW - 1) / PPW;
-    const int lane = threadIdx.x & 63, grp = lane / G, gl = lane % G;
+    const int lane = threadIdx.x & 63;
     const size_t nwaves = (size_t)gridDim.x * 4;
-    const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0;
-    for (size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < m; row += nwaves) {
-        const size_t src = ids ? (size_t)ids[row] : row;
-        float best = INFINITY;
-        uint32_t bidx = NO_LABEL;
-#pragma unroll
-        for (int p = 0; p < PASSES; p++) {
-            const int slot = p * PPW + grp;
-            uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
-            if (ci >= k) ci = NO_LABEL;  // pad centroids can fill a list when k is small
-            float acc = 0.0f;
-            if (ci != NO_LABEL) {
-                const float *cp = c + (size_t)ci * d;
-                if (vec) {
-                    const float *xr = (const float *)x + src * d;
-                    for (size_t e = 4 * gl; e < d; e += 4 * G) {
-                        const float4 xv = *(const float4 *)(xr + e), cv = *(const float4 *)(cp + e);
-                        float t0 = __fsub_rn(xv.x, cv.x), t1 = __fsub_rn(xv.y, cv.y);
-                        float t2 = __fsub_rn(xv.z, cv.z), t3 = __fsub_rn(xv.w, cv.w);
-                        acc = __fadd_rn(acc, __fmul_rn(t0, t0));
-                        acc = __fadd_rn(acc, __fmul_rn(t1, t1));
-                        acc = __fadd_rn(acc, __fmul_rn(t2, t2));
-                        acc = __fadd_rn(acc, __fmul_rn(t3, t3));
-                    }
-                } else {
-                    for (size_t e = gl; e < d; e += G) {
-                        const float t = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
-                        acc = __fadd_rn(acc, __fmul_rn(t, t));
-                    }
-                }
-            }
-#pragma unroll
-            for (int off = G / 2; off > 0; off >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, off, 64));
-            float s = INFINITY;
-            if (ci != NO_LABEL) s = bias ? __fadd_rn(acc, bias[ci]) : acc;
-            uint32_t si = ci;
-#pragma unroll
-            for (int off = G; off < 64; off <<= 1) {
-                const float os = __shfl_xor(s, off, 64);
-                const uint32_t oi = (uint32_t)__shfl_xor((int)si, off, 64);
-                if (os < s || (os == s && oi < si)) {
-                    s = os;
-                    si = oi;
-                }
-            }
-            if (s < best || (s == best && si < bidx)) {
-                best = s;
-                bidx = si;
-            }
-        }
-        if (lane == 0) {
-            labels[row] = bidx == NO_LABEL ? 0u : bidx;
-            dist[row] = (bidx != NO_LABEL && best > 0.0f) ? best : 0.0f;
-        }
+    for (size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < m; row += nwaves)
+        rescore_row<ROWS, M>(x, d, k, ids, c, bias, cand, labels, dist, row, lane);
+}
+
+// Every job's rows: slot i of the concatenated batch is row rmap[i].y of job
+// rmap[i].x (live when below that job's row count this step).
+template <int M>
+__global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
+    const int lane = threadIdx.x & 63;
+    const size_t nwaves = (size_t)gridDim.x * 4;
+    for (size_t i = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < slots; i += nwaves) {
+        const uint2 r = rmap[i];
+        const JobDev &j = jobs[r.x];
+        if (r.y >= j.m) continue;
+        rescore_row<EMH_ROWS_F32, M>(j.x, j.d, j.k, j.ids, j.cen, j.penalized ? j.bias : nullptr, j.cand, j.labels,
+                                     j.dist, r.y, lane);
     }
 }
 
 // ---------------------------------------------------------- centroid prep --
+
+// |c|^2 in f32 as src/cpu.rs:170-180 folds it: 16 strided partial sums in
+// order, summed in order, then the tail. Call with a whole wave; every lane
+// returns the value.
+__device__ inline float centroid_norm32(const float *cp, size_t d, int lane) {
+    const size_t body = d & ~(size_t)15;
+    float acc = 0.0f;
+    if (lane < 16)
+        for (size_t i = lane; i < body; i += 16) acc = __fadd_rn(acc, __fmul_rn(cp[i], cp[i]));
+    float s = 0.0f;
+    for (int t = 0; t < 16; t++) s = __fadd_rn(s, __shfl(acc, t, 64));
+    float tail = 0.0f;
+    for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
+    return __fadd_rn(s, tail);
+}
 
 // One wave per centroid: operand row (zero padded), and the f32 squared norm
 // folded as src/cpu.rs:170-180 does (16 strided partial sums, then the tail),
@@ -981,21 +1038,59 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     }
     if constexpr (OP != OP_I8) {
         if (live) {
-            // acc[t] = sum over i = t mod 16 of c_i^2, in order; fold; add tail
-            const size_t body = d & ~(size_t)15;
-            float acc = 0.0f;
-            if (lane < 16)
-                for (size_t i = lane; i < body; i += 16) acc = __fadd_rn(acc, __fmul_rn(cp[i], cp[i]));
-            float s = 0.0f;
-            for (int t = 0; t < 16; t++) s = __fadd_rn(s, __shfl(acc, t, 64));
+            const float nv = centroid_norm32(cp, d, lane);
             if (lane == 0) {
-                float tail = 0.0f;
-                for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
-                const float nv = __fadd_rn(s, tail);
                 norm32[jc] = nv;
                 if (nv >= 0.0f) atomicMax(&stats[0], __float_as_uint(nv));  // NaN norms are skipped
             }
         }
+    }
+}
+
+// Batched prep: one wave per (job, centroid < k_pad) of cmap. Writes the
+// negated operand straight into the tile-major table, |c|^2, the score
+// offset (|c|^2 + bias) / 2 with this step's bias, and the job's stats, as
+// prep_centroids_kernel + set_bias_kernel do for one job.
+template <int OP>
+__global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
+    const int lane = threadIdx.x & 63;
+    const size_t e = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (e >= entries) return;
+    const uint2 cm = cmap[e];
+    const JobDev &jb = jobs[cm.x];
+    if (jb.m == 0) return;  // job done: nothing reads its tables this step
+    const size_t jc = cm.y, d = jb.d;
+    const bool live = jc < jb.k;
+    const float *cp = jb.cen + jc * d;
+    // tile-major: byte b of centroid jc lives in tile jc / tb at quad b / 16
+    const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
+    uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
+    if constexpr (OP == OP_F32) {
+        for (size_t el = lane; el < jb.d_pad; el += 64) {
+            const size_t b = el * 4;
+            *(float *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? -cp[el] : 0.0f;
+        }
+    } else {
+        for (size_t el = lane; el < jb.d_pad; el += 64) {
+            const size_t b = el * 2;
+            *(uint16_t *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? bf16_rne(-cp[el]) : (uint16_t)0;
+        }
+    }
+    if (!live) {
+        if (lane == 0) jb.hn[jc] = INFINITY;
+        return;
+    }
+    const float nv = centroid_norm32(cp, d, lane);
+    if (lane == 0) {
+        jb.norm32[jc] = nv;
+        if (nv >= 0.0f) atomicMax(&jb.stats[0], __float_as_uint(nv));
+        float v = nv;
+        if (jb.penalized) {
+            const float b = jb.bias[jc];
+            if (b < 0.0f) atomicMax(&jb.stats[1], __float_as_uint(-b));
+            v = __fadd_rn(nv, b);
+        }
+        jb.hn[jc] = __fmul_rn(0.5f, v);
     }
 }
 
@@ -1256,6 +1351,91 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     const size_t quads = g.k_pad * (size_t)nq;
     hipLaunchKernelGGL(tile_layout_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s,
                        (const uint4 *)cen_op, (uint4 *)reg_operand(g, cen_op), g.k_pad, nq, tb);
+    return hipGetLastError();
+}
+
+// ---- batched launchers ----
+
+size_t search_args_bytes() { return sizeof(SearchArgs); }
+
+void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
+    SearchArgs a;
+    memset(&a, 0, sizeof a);
+    a.x = j.x;
+    a.ids = j.ids;
+    a.m = j.m;
+    a.d = g.d;
+    a.d_pad = g.d_pad;
+    a.k = g.k;
+    a.k_pad = g.k_pad;
+    a.row_bytes = g.row_bytes;
+    a.cen_op = j.cen_op;
+    a.hn = j.hn;
+    a.stats = j.stats;
+    a.vec = (g.d * 4) % 16 == 0;  // rows are hipMalloc'd f32 tables
+    a.labels = j.labels;
+    a.dist = j.dist;
+    a.cand = j.cand;
+    memcpy(dst, &a, sizeof a);
+}
+
+hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s) {
+    if (!entries) return hipSuccess;
+    dim3 grid((unsigned)((entries + 3) / 4));
+    if (op == OP_F32) hipLaunchKernelGGL(prep_jobs_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    else if (op == OP_BF16) hipLaunchKernelGGL(prep_jobs_kernel<OP_BF16>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <int OP, int M, int NS>
+static hipError_t launch_jobs_search_ns(const void *args, const uint2 *wg, size_t nwg, hipStream_t s) {
+    constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
+    hipLaunchKernelGGL((search_reg_jobs_kernel<OP, EMH_ROWS_F32, M, NS, FR, NB>), dim3((unsigned)nwg), dim3(256), 0, s,
+                       (const SearchArgs *)args, wg);
+    return hipGetLastError();
+}
+
+template <int OP, int M>
+static hipError_t launch_jobs_search_m(int row_bytes, const void *args, const uint2 *wg, size_t nwg, hipStream_t s) {
+    switch (row_bytes / 32) {
+        case 2: return launch_jobs_search_ns<OP, M, 2>(args, wg, nwg, s);
+        case 4: return launch_jobs_search_ns<OP, M, 4>(args, wg, nwg, s);
+        case 6: return launch_jobs_search_ns<OP, M, 6>(args, wg, nwg, s);
+        case 8: return launch_jobs_search_ns<OP, M, 8>(args, wg, nwg, s);
+        case 12: return launch_jobs_search_ns<OP, M, 12>(args, wg, nwg, s);
+        case 16: return launch_jobs_search_ns<OP, M, 16>(args, wg, nwg, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_args, const uint2 *wg, size_t nwg,
+                              hipStream_t s) {
+    if (!nwg) return hipSuccess;
+    if (op == OP_F32) {
+        if (M != 1) return hipErrorInvalidValue;
+        return launch_jobs_search_m<OP_F32, 1>(row_bytes, search_args, wg, nwg, s);
+    }
+    if (op != OP_BF16) return hipErrorInvalidValue;
+    switch (M) {
+        case 1: return launch_jobs_search_m<OP_BF16, 1>(row_bytes, search_args, wg, nwg, s);
+        case 2: return launch_jobs_search_m<OP_BF16, 2>(row_bytes, search_args, wg, nwg, s);
+        case 4: return launch_jobs_search_m<OP_BF16, 4>(row_bytes, search_args, wg, nwg, s);
+        case 8: return launch_jobs_search_m<OP_BF16, 8>(row_bytes, search_args, wg, nwg, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
+    if (!slots) return hipSuccess;
+    dim3 grid((unsigned)std::min<size_t>((slots + 3) / 4, 16384));
+    switch (M) {
+        case 1: hipLaunchKernelGGL(rescore_jobs_kernel<1>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
+        case 2: hipLaunchKernelGGL(rescore_jobs_kernel<2>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
+        case 4: hipLaunchKernelGGL(rescore_jobs_kernel<4>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
+        case 8: hipLaunchKernelGGL(rescore_jobs_kernel<8>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -7,238 +7,23 @@
 // per row) go up; the gather, the search and the per-cluster sums/counts run
 // on the GPU, and the per-cluster totals of the last pass accumulate in FP64
 // on the GPU.
-#include <math.h>
-#include <string.h>
-
-#include <algorithm>
 #include <chrono>
-#include <vector>
 
-#include "emh_assigner.h"
+#include "emh_recipe.h"
 
 namespace emh {
 
-// ---- constants of the reference (src/lib.rs:31-43) ----------------------
-static const double ALPHA_MIN_FACTOR = 0.1;
-static const size_t CHECK_UNUSED_EVERY = 5;
-static const float INIT_USAGE = 1.0f / 65536.0f;
-static const float SPLIT_EPSILON = 1e-4f;
-static const float UNUSED_THRESHOLD = 0.1f;
-static const size_t NO_REVIVAL_MAX_D = 2;
-static const size_t AUTO_QUALITY_MAX_D = 16;
-static const double BATCH_THROUGHPUT[4] = {0.035134, 0.134749, 0.876473, 0.813344};
-static const double BATCH_QUALITY[4] = {0.00439175, 0.134749, 0.876473, 0.813344};
-static const double ALPHA_LAW[5] = {0.8244, -0.4617, 0.9003, -0.4498, -0.1057};
-static const size_t ASSIGN_CHUNK = 65536;
-
-// ---- xoshiro256++ seeded by splitmix64 (src/lib.rs:227-274) -------------
-class Rng {
-  public:
-    explicit Rng(uint64_t seed) {
-        uint64_t z = seed;
-        for (auto &w : s_) {
-            z += 0x9E3779B97F4A7C15ull;
-            uint64_t x = z;
-            x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
-            x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
-            w = x ^ (x >> 31);
-        }
-    }
-    uint64_t next() {
-        const uint64_t r = rotl(s_[0] + s_[3], 23) + s_[0];
-        const uint64_t t = s_[1] << 17;
-        s_[2] ^= s_[0];
-        s_[3] ^= s_[1];
-        s_[1] ^= s_[2];
-        s_[0] ^= s_[3];
-        s_[2] ^= t;
-        s_[3] = rotl(s_[3], 45);
-        return r;
-    }
-    uint64_t below(uint64_t bound) {  // Lemire
-        for (;;) {
-            unsigned __int128 m = (unsigned __int128)next() * bound;
-            if ((uint64_t)m >= (0 - bound) % bound) return (uint64_t)(m >> 64);
-        }
-    }
-    double open01() { return ((double)(next() >> 11) + 0.5) / 9007199254740992.0; }
-    void permutation(uint32_t n, std::vector<uint32_t> &p) {
-        p.resize(n);
-        for (uint32_t i = 0; i < n; i++) p[i] = i;
-        for (uint32_t i = n; i-- > 1;) std::swap(p[i], p[below((uint64_t)i + 1)]);
-    }
-
-  private:
-    static uint64_t rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
-    uint64_t s_[4];
-};
-
-// ---- recipe (src/lib.rs:165-225) -----------------------------------------
-static uint64_t round_pow2(double v) {
-    if (v <= 1.0) return 1;
-    const double low = ldexp(1.0, (int)floor(log2(v)));
-    const double high = 2.0 * low;
-    return (uint64_t)(v / low < high / v ? low : high);
-}
-
-static int resolve(uint64_t n, uint64_t d, uint64_t k, const emh_config &cfg, emh_recipe &r) {
-    if (!(cfg.nepochs > 0.0) || cfg.nepochs >= 63.0) return -1;
-    int law = cfg.batch_law;
-    if (law == 0) law = d <= AUTO_QUALITY_MAX_D ? 2 : 1;
-    uint64_t batch;
-    if (cfg.has_batch_size) {
-        batch = std::min<uint64_t>(std::max<uint64_t>(cfg.batch_size, 1), n);
-    } else {
-        const double *c = law == 1 ? BATCH_THROUGHPUT : BATCH_QUALITY;
-        const double raw = c[0] * pow((double)k, c[1]) * pow((double)n, c[2]) * pow(cfg.nepochs, c[3]);
-        const uint64_t cap = law == 1 ? round_pow2((double)n / 2.0) : n / 2;
-        batch = std::max<uint64_t>(std::min(std::min(round_pow2(raw), cap), n), 1);
-    }
-    r.batch_size = batch;
-    if (cfg.has_alpha0) {
-        r.alpha0 = cfg.alpha0;
-    } else {
-        const double a = ALPHA_LAW[0] * pow((double)k, ALPHA_LAW[1]) * pow((double)batch, ALPHA_LAW[2]) *
-                         pow((double)n, ALPHA_LAW[3]) * pow(cfg.nepochs, ALPHA_LAW[4]);
-        r.alpha0 = std::min(std::max(a, 1e-4), 1.0);
-    }
-    const uint64_t whole = (uint64_t)floor(cfg.nepochs);
-    const uint64_t rem = (uint64_t)nearbyint((cfg.nepochs - (double)whole) * (double)n);
-    r.n_epochs = 0;
-    if (whole == 0) {
-        r.epoch_rows[r.n_epochs++] = rem;
-    } else {
-        for (uint64_t i = 0; i + 1 < whole; i++) r.epoch_rows[r.n_epochs++] = n;
-        if (rem > 0) r.epoch_rows[r.n_epochs++] = rem;
-        r.epoch_rows[r.n_epochs++] = n;
-    }
-    r.total_steps = 0;
-    for (uint32_t e = 0; e < r.n_epochs; e++) r.total_steps += (r.epoch_rows[e] + batch - 1) / batch;
-    r.unused_threshold = cfg.has_unused_threshold ? cfg.unused_threshold
-                                                  : (d <= NO_REVIVAL_MAX_D ? 0.0f : UNUSED_THRESHOLD);
-    return 0;
-}
-
-static double scheduled_alpha(double alpha0, uint64_t step, uint64_t total) {
-    const double frac = (double)std::min(step, total) / (double)std::max<uint64_t>(total, 1);
-    const double cosine = 0.5 * (1.0 + cos(M_PI * frac));
-    return alpha0 * (ALPHA_MIN_FACTOR + (1.0 - ALPHA_MIN_FACTOR) * cosine);
-}
-
-// ---- model state (src/lib.rs:276-345) -------------------------------------
-struct State {
-    size_t k, d;
-    std::vector<float> centroids, usage;
-    size_t n_splits = 0;
-
-    // N' = (1-a) N + a n ; c' = c + (a n / N') (s/n - c), all f32 (src/lib.rs:287-309)
-    void apply(const std::vector<uint32_t> &touched, const uint32_t *counts, const float *sums, float alpha) {
-        const bool replace = alpha == 1.0f;
-        if (!replace) {
-            const float decay = 1.0f - alpha;
-            for (float &u : usage) u *= decay;
-        }
-        for (uint32_t jj : touched) {
-            const size_t j = jj;
-            const float n = (float)counts[j];
-            const float retained = replace ? 0.0f : usage[j];
-            const float an = alpha * n;
-            const float mass = retained + an;
-            const float weight = an / mass;
-            float *c = &centroids[j * d];
-            const float *s = sums + j * d;
-            if (weight == 1.0f) {
-                for (size_t i = 0; i < d; i++) c[i] = s[i] / n;
-            } else {
-                for (size_t i = 0; i < d; i++) {
-                    const float mean = s[i] / n;
-                    c[i] = c[i] + weight * (mean - c[i]);
-                }
-            }
-            usage[j] = mass;
-        }
-    }
-
-    // Gumbel-top-k donor split of clusters below tau * mean mass (src/lib.rs:314-344)
-    void revive(float tau, Rng &rng) {
-        float total = 0.0f;
-        for (float u : usage) total += u;
-        const float threshold = total * tau / (float)k;
-        std::vector<size_t> unused;
-        for (size_t j = 0; j < k; j++)
-            if (usage[j] < threshold) unused.push_back(j);
-        if (unused.empty()) return;
-        std::vector<std::pair<double, size_t>> donors;
-        for (size_t j = 0; j < k; j++)
-            if (usage[j] >= threshold && usage[j] > 0.0f)
-                donors.emplace_back(log((double)usage[j]) - log(-log(rng.open01())), j);
-        const size_t nsplit = std::min(unused.size(), donors.size());
-        std::sort(donors.begin(), donors.end(),
-                  [](const std::pair<double, size_t> &a, const std::pair<double, size_t> &b) { return a.first > b.first; });
-        for (size_t t = 0; t < nsplit; t++) {
-            const size_t src = donors[t].second, dst = unused[t];
-            const float half = usage[src] / 2.0f;
-            for (size_t i = 0; i < d; i++) {
-                const float offset = (i % 2 == 0) ? SPLIT_EPSILON : -SPLIT_EPSILON;
-                const float c = centroids[src * d + i];
-                centroids[src * d + i] = c * (1.0f - offset);
-                centroids[dst * d + i] = c * (1.0f + offset);
-            }
-            bool same = true;
-            for (size_t i = 0; i < d; i++) same = same && centroids[src * d + i] == centroids[dst * d + i];
-            if (same) centroids[dst * d] = nextafterf(centroids[dst * d], INFINITY);
-            usage[src] = half;
-            usage[dst] = half;
-        }
-        n_splits += nsplit;
-    }
-};
-
-static float widen(const void *x, int rows, size_t idx) {
-    if (rows == EMH_ROWS_I8) return (float)((const int8_t *)x)[idx];
-    if (rows == EMH_ROWS_U8) return (float)((const uint8_t *)x)[idx];
-    return ((const float *)x)[idx];
-}
-
-// Pinned host buffer (hipHostMalloc).
-struct HostBuf {
-    void *p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault); }
-    ~HostBuf() {
-        if (p) (void)hipHostFree(p);
-    }
-    template <class T> T *as() const { return (T *)p; }
-};
-
 static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size_t k, const emh_config &cfg,
                       float *centroids_out, uint32_t *labels_out, emh_train_out *out) {
-    // input checks (src/lib.rs:349-361)
-    if (d == 0 || k == 0 || !x) return fail(EMH_E_ARG, "x must be a non-empty row-major n x d array");
-    if (n < k) return fail(EMH_E_ARG, "need at least k = " + std::to_string(k) + " rows, got " + std::to_string(n));
-    if (n > 0xFFFFFFFFull) return fail(EMH_E_ARG, "more than 2^32 - 1 rows: subsample or shard the input");
-    if (!(cfg.nepochs > 0.0)) return fail(EMH_E_ARG, "nepochs must be positive");
-    if (rows < EMH_ROWS_F32 || rows > EMH_ROWS_U8) return fail(EMH_E_ARG, "unknown row type");
+    JobRun J;
+    int rc;
+    if ((rc = J.init(x, rows, n, d, k, cfg, centroids_out, labels_out, out))) return rc;
     if (h->op == OP_I8 && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
-    if (!centroids_out || !labels_out || !out) return fail(EMH_E_ARG, "train: NULL output");
-    emh_recipe R;
-    memset(&R, 0, sizeof R);
-    if (resolve(n, d, k, cfg, R)) return fail(EMH_E_ARG, "nepochs out of range");
-    const size_t batch = R.batch_size;
+    const size_t batch = J.R.batch_size;
     const size_t rb = rows == EMH_ROWS_F32 ? 4 : 1;
     const auto t0 = std::chrono::steady_clock::now();
-    int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
-
-    // Random data rows as starting centroids (src/lib.rs:374-381).
-    State st{k, d, std::vector<float>(k * d), std::vector<float>(k, INIT_USAGE)};
-    {
-        std::vector<uint32_t> init;
-        Rng(cfg.seed).permutation((uint32_t)n, init);
-        for (size_t j = 0; j < k; j++)
-            for (size_t i = 0; i < d; i++) st.centroids[j * d + i] = widen(x, rows, (size_t)init[j] * d + i);
-    }
-    Rng split_rng(cfg.seed ^ 0x5EED5EED5EED5EEDull);
 
     // Device-resident rows and per-batch buffers.
     DevBuf data, ids_d, lab_d, dist_d, sums_d, counts_d, pass_d, passn_d, cent_d;
@@ -259,97 +44,42 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     if ((rc = check(sums_h.alloc(k * d * 4), "pinned sums"))) return rc;
     if ((rc = check(counts_h.alloc(k * 4), "pinned counts"))) return rc;
 
-    const size_t chunk = std::min(ASSIGN_CHUNK, batch);
-    std::vector<float> bias(cfg.balance > 0.0f ? k : 0);
-    std::vector<uint32_t> order, touched;
-    touched.reserve(std::min(k, batch));
-    for (size_t i = 0; i < n; i++) labels_out[i] = 0xFFFFFFFFu;
-    double mean_sqdist = 0.0;
-    uint64_t step = 0;
-    const uint32_t last = R.n_epochs - 1;
-    out->n_objective = 0;
-
-    for (uint32_t epoch = 0; epoch < R.n_epochs; epoch++) {
-        const size_t epoch_n = R.epoch_rows[epoch];
-        Rng(cfg.seed * 1000003ull + (uint64_t)epoch + 1).permutation((uint32_t)n, order);
-        const bool is_last = epoch == last;
-        if (is_last) {
+    while (J.next()) {
+        const size_t nb = J.nb;
+        if (J.last_epoch_start) {
             if ((rc = check(launch_fill_f64(pass_d.as<double>(), k * d, h->stream), "clear pass sums"))) return rc;
             if ((rc = check(launch_fill_f64((double *)passn_d.p, k, h->stream), "clear pass counts"))) return rc;
         }
-        double epoch_obj = 0.0;
-        for (size_t b0 = 0; b0 < epoch_n; b0 += batch) {
-            const size_t nb = std::min(batch, epoch_n - b0);
-            const uint32_t *ids = order.data() + b0;
-            // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
-            if ((rc = emh_set_centroids(h, st.centroids.data(), k, d))) return rc;
-            const bool penalized = cfg.balance > 0.0f && mean_sqdist > 0.0;
-            if (penalized) {
-                double mm = 0.0;
-                for (float u : st.usage) mm += (double)u;
-                const double mean_mass = mm / (double)k;
-                const float scale = (float)((double)cfg.balance * mean_sqdist / mean_mass);
-                for (size_t j = 0; j < k; j++) bias[j] = scale * st.usage[j];
-                if ((rc = emh_set_bias(h, bias.data(), k))) return rc;
-            }
-            memcpy(ids_h.p, ids, nb * 4);
-            if ((rc = check(hipMemcpyAsync(ids_d.p, ids_h.p, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
-                return rc;
-            // gather + search on the device (src/lib.rs:419-424)
-            if ((rc = run_search(h, data.p, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
-                return rc;
-            // per-cluster counts and sums in batch order (src/lib.rs:431-441)
-            if ((rc = check(launch_accumulate(h->acc, data.p, rows, nb, d, ids_d.as<uint32_t>(),
-                                              lab_d.as<uint32_t>(), k, sums_d.as<float>(), counts_d.as<uint32_t>(),
-                                              h->stream),
-                            "accumulate")))
-                return rc;
-            if (is_last &&
-                (rc = check(launch_pass_accumulate(sums_d.as<float>(), counts_d.as<uint32_t>(), k, d,
-                                                   pass_d.as<double>(), passn_d.as<uint64_t>(), h->stream),
-                            "pass accumulate")))
-                return rc;
-            if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
-                (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
-                (rc = check(hipMemcpyAsync(counts_h.p, counts_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "counts")) ||
-                (rc = check(hipMemcpyAsync(sums_h.p, sums_d.p, k * d * 4, hipMemcpyDeviceToHost, h->stream), "sums")) ||
-                (rc = check(hipStreamSynchronize(h->stream), "batch")))
-                return rc;
-            const uint32_t *lab = lab_h.as<uint32_t>();
-            float *dist = dist_h.as<float>();
-            const uint32_t *counts = counts_h.as<uint32_t>();
-            double batch_obj = 0.0;
-            for (size_t p0 = 0; p0 < nb; p0 += chunk) {
-                const size_t m = std::min(chunk, nb - p0);
-                if (penalized) {
-                    // report true squared distances (src/lib.rs:425-430)
-                    for (size_t r = p0; r < p0 + m; r++) {
-                        const float v = dist[r] - bias[lab[r]];
-                        dist[r] = v > 0.0f ? v : 0.0f;
-                    }
-                }
-                double cs = 0.0;
-                for (size_t r = p0; r < p0 + m; r++) cs += (double)dist[r];
-                batch_obj += cs;
-            }
-            if (is_last)
-                for (size_t r = 0; r < nb; r++) labels_out[ids[r]] = lab[r];
-            epoch_obj += batch_obj;
-            mean_sqdist = batch_obj / (double)nb;
-            touched.clear();
-            for (size_t j = 0; j < k; j++)
-                if (counts[j]) touched.push_back((uint32_t)j);
-            const float alpha = (float)scheduled_alpha(R.alpha0, step, R.total_steps);
-            st.apply(touched, counts, sums_h.as<float>(), alpha);
-            if ((step + 1) % CHECK_UNUSED_EVERY == 0 && R.unused_threshold > 0.0f)
-                st.revive(R.unused_threshold, split_rng);
-            step++;
-        }
-        if (out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+        // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
+        if ((rc = emh_set_centroids(h, J.st.centroids.data(), k, d))) return rc;
+        if (J.penalized && (rc = emh_set_bias(h, J.bias.data(), k))) return rc;
+        memcpy(ids_h.p, J.ids, nb * 4);
+        if ((rc = check(hipMemcpyAsync(ids_d.p, ids_h.p, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
+            return rc;
+        // gather + search on the device (src/lib.rs:419-424)
+        if ((rc = run_search(h, data.p, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
+            return rc;
+        // per-cluster counts and sums in batch order (src/lib.rs:431-441)
+        if ((rc = check(launch_accumulate(h->acc, data.p, rows, nb, d, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), k,
+                                          sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream),
+                        "accumulate")))
+            return rc;
+        if (J.is_last &&
+            (rc = check(launch_pass_accumulate(sums_d.as<float>(), counts_d.as<uint32_t>(), k, d, pass_d.as<double>(),
+                                               passn_d.as<uint64_t>(), h->stream),
+                        "pass accumulate")))
+            return rc;
+        if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
+            (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
+            (rc = check(hipMemcpyAsync(counts_h.p, counts_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "counts")) ||
+            (rc = check(hipMemcpyAsync(sums_h.p, sums_d.p, k * d * 4, hipMemcpyDeviceToHost, h->stream), "sums")) ||
+            (rc = check(hipStreamSynchronize(h->stream), "batch")))
+            return rc;
+        J.finish_batch(lab_h.as<uint32_t>(), dist_h.as<float>(), counts_h.as<uint32_t>(), sums_h.as<float>());
     }
 
     // Closing mean on the device from the FP64 pass totals (src/lib.rs:481-489).
-    if ((rc = check(hipMemcpyAsync(cent_d.p, st.centroids.data(), k * d * 4, hipMemcpyHostToDevice, h->stream),
+    if ((rc = check(hipMemcpyAsync(cent_d.p, J.st.centroids.data(), k * d * 4, hipMemcpyHostToDevice, h->stream),
                     "copy centroids")) ||
         (rc = check(launch_closing_mean(pass_d.as<double>(), passn_d.as<uint64_t>(), k, d, cent_d.as<float>(),
                                         h->stream),
@@ -358,8 +88,8 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                     "copy centroids")) ||
         (rc = check(hipStreamSynchronize(h->stream), "closing mean")))
         return rc;
-    out->n_splits = st.n_splits;
-    out->recipe = R;
+    out->n_splits = J.st.n_splits;
+    out->recipe = J.R;
     out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (DevBuf *b : {&data, &ids_d, &lab_d, &dist_d, &sums_d, &counts_d, &pass_d, &passn_d, &cent_d}) b->release();
     return EMH_OK;

@@ -28,7 +28,7 @@ EXPORTS = [
     "emh_create", "emh_destroy", "emh_set_centroids", "emh_set_bias", "emh_name", "emh_assign_f32",
     "emh_assign_i8", "emh_assign_u8", "emh_last_error", "emh_set_stream", "emh_assign_device",
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
-    "emh_assign_all", "emh_set_timing", "emh_get_timing",
+    "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch",
 ]
 
 
@@ -134,6 +134,7 @@ def lib():
     L.emh_resolve_recipe.argtypes = [u64, u64, u64, C.POINTER(Config), C.POINTER(Recipe)]
     L.emh_train.argtypes = [vp, vp, i32, sz, sz, sz, C.POINTER(Config), vp, vp, C.POINTER(TrainOut)]
     L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
+    L.emh_train_batch.argtypes = [vp, C.POINTER(Job), sz, C.POINTER(BatchStats)]
     _lib = L
     return L
 
@@ -277,6 +278,42 @@ def train(x, k: int, cfg: Config | None = None, assigner: HipAssigner | None = N
                            _p(labels), C.byref(out)))
     obj = [out.objective_[i] for i in range(out.n_objective)]
     return Output(cent, labels, obj, int(out.n_splits), out.recipe, float(out.seconds))
+
+
+class Job(C.Structure):
+    """emh_job: one train() call's arguments for emh_train_batch."""
+
+    _fields_ = [("x", C.c_void_p), ("rows", C.c_int), ("n", C.c_uint64), ("d", C.c_uint64), ("k", C.c_uint64),
+                ("cfg", Config), ("centroids_out", C.c_void_p), ("labels_out", C.c_void_p),
+                ("out", C.POINTER(TrainOut))]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("graph_nodes", C.c_uint64), ("seconds", C.c_double)]
+
+
+def train_batch(problems, assigner: HipAssigner | None = None):
+    """Trains independent problems in lockstep, one captured-graph replay per
+    training step (emh_train_batch). problems: (x, k, cfg) tuples with f32
+    rows. Each result equals train(x, k, cfg, assigner) on the same assigner
+    precision. Returns (list of Output, BatchStats)."""
+    a = assigner or HipAssigner(FP32)
+    keep, jobs = [], (Job * len(problems))()
+    for i, (x, k, cfg) in enumerate(problems):
+        x = np.ascontiguousarray(x, np.float32)
+        n, d = x.shape
+        cent = np.empty((k, d), np.float32)
+        labels = np.empty(n, np.uint32)
+        out = TrainOut()
+        keep.append((x, cent, labels, out))
+        jobs[i] = Job(_p(x), ROWS_F32, n, d, k, cfg or Config(), _p(cent), _p(labels), C.pointer(out))
+    st = BatchStats()
+    _check(lib().emh_train_batch(a.handle, jobs, len(problems), C.byref(st)))
+    res = []
+    for x, cent, labels, out in keep:
+        obj = [out.objective_[i] for i in range(out.n_objective)]
+        res.append(Output(cent, labels, obj, int(out.n_splits), out.recipe, float(out.seconds)))
+    return res, st
 
 
 def assign(x, centroids, assigner: HipAssigner | None = None):

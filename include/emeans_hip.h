@@ -157,6 +157,33 @@ int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size
 int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,
                    uint32_t *labels, float *dist);
 
+/* Many independent problems, trained in lockstep (no counterpart in the
+ * reference, which trains one problem per train() call, src/lib.rs:348).
+ * Each job is one train() call's arguments; every job ends with the results
+ * emh_train gives it on the same assigner (same precision and top-m). One
+ * training step of all jobs still running is one replay of a captured
+ * hipGraph: upload of every job's centroid snapshot, bias and batch ids,
+ * then the gather + search (+ FP32 rescore) + per-cluster sums of all jobs,
+ * then one download. Needs f32 rows and operand rows of at most 512 bytes
+ * (FP32: d <= 128, BF16: d <= 256); precision FP32 or BF16_RESCORE. */
+typedef struct emh_job {
+    const void *x; /* n x d rows, row-major */
+    int rows;      /* enum emh_rows: EMH_ROWS_F32 */
+    uint64_t n, d, k;
+    emh_config cfg;
+    float *centroids_out; /* k x d */
+    uint32_t *labels_out; /* n */
+    emh_train_out *out;
+} emh_job;
+
+typedef struct emh_batch_stats {
+    uint64_t steps;       /* graph replays: one per lockstep training step */
+    uint64_t graph_nodes; /* nodes of the per-step graph */
+    double seconds;       /* wall time of the call (row uploads included) */
+} emh_batch_stats;
+
+int emh_train_batch(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats);
+
 #pragma GCC visibility pop
 #ifdef __cplusplus
 }

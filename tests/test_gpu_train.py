@@ -130,3 +130,32 @@ def test_device_accumulate_is_batch_order_f32(hip, n, m, k):
         want_c[labels[r]] += 1
     assert np.array_equal(counts.cpu().numpy(), want_c)
     assert np.array_equal(sums.cpu().numpy(), want)
+
+
+def _job_data(seed, n, d, blobs):
+    rng = np.random.default_rng(seed)
+    centres = rng.normal(scale=4, size=(blobs, d)).astype(np.float32)
+    return (centres[rng.integers(0, blobs, n)] + rng.normal(size=(n, d))).astype(np.float32)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batched_training_equals_single(hip, prec):
+    """Every job of a lockstep batch (one graph replay per step, uneven n, d,
+    k and step counts, balance on some) ends with exactly the result of its
+    own single-problem training on the same assigner."""
+    specs = [(3000, 24, 40, 0.0), (9000, 64, 200, 0.1), (700, 96, 64, 0.0), (20000, 128, 300, 0.1),
+             (5000, 33, 17, 0.0)]
+    if prec == "bf16":
+        specs.append((4000, 256, 90, 0.1))
+    problems = []
+    for i, (n, d, k, bal) in enumerate(specs):
+        problems.append((_job_data(10 + i, n, d, 2 * k), k, E.Config(nepochs=2 + (i % 2), balance=bal, seed=7 + i)))
+    print(f"batch {prec}: {len(problems)} jobs", flush=True)
+    got, st = E.train_batch(problems, E.HipAssigner(prec))
+    assert st.steps == max(E.resolve_recipe(p[0].shape[0], p[0].shape[1], p[1], p[2]).total_steps for p in problems)
+    for (x, k, cfg), g in zip(problems, got):
+        ref = E.train(x, k, cfg, E.HipAssigner(prec))
+        assert np.array_equal(g.labels, ref.labels)
+        assert np.array_equal(g.centroids, ref.centroids)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
