@@ -218,14 +218,16 @@ def cpu_search_baseline(x, c, seconds, i8=False):
             return O.i8_assign(xh[:rows], ch, nthreads=cores)
         return O.cpu_assign(xh[:rows], ch, nthreads=cores)
 
-    sample = 256
-    t = time.perf_counter()
-    run(sample)
-    dt = time.perf_counter() - t
-    sample = int(min(xh.shape[0], max(sample, sample * seconds / max(dt, 1e-3))))
-    t = time.perf_counter()
-    run(sample)
-    dt = time.perf_counter() - t
+    # grow the sample until one timed run takes at least half the target
+    # (the first runs also warm the thread pool)
+    sample, dt = 256, 0.0
+    while True:
+        t = time.perf_counter()
+        run(sample)
+        dt = time.perf_counter() - t
+        if dt >= 0.5 * seconds or sample >= xh.shape[0]:
+            break
+        sample = int(min(xh.shape[0], max(2 * sample, sample * 0.9 * seconds / max(dt, 1e-3))))
     kind = ("AVX-512 VNNI" if O.has_vnni() else "portable") if i8 else \
         ("AVX-512" if O.lib().or_simd_best() == O.SIMD_AVX512 else "portable")
     return {
