@@ -38,6 +38,7 @@ static int op_of(int precision) {
         case EMH_BF16_RESCORE: return OP_BF16;
         case EMH_FP8_RESCORE: return OP_FP8;
         case EMH_I8: return OP_I8;
+        case EMH_I16: return OP_I16;
         default: return -1;
     }
 }
@@ -59,7 +60,7 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     int rc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if ((rc = mark(h, &ev[0]))) return rc;
-    if (h->op == OP_F32 || h->op == OP_I8) {
+    if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
         if ((rc = mark(h, &ev[1]))) return rc;
         if ((rc = check(launch_search(g, 1, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
                                       h->cn.as<int64_t>(), nullptr, nullptr, nullptr, nullptr,
@@ -140,8 +141,6 @@ const char *emh_last_error(void) { return g_last_error.c_str(); }
 int emh_create(int device, int precision, int rescore_m, emh_assigner **out) {
     if (!out) return fail(EMH_E_ARG, "emh_create: out is NULL");
     *out = nullptr;
-    if (precision == EMH_I16)
-        return fail(EMH_E_UNSUPPORTED, "I16 centroids are not offered by this build");
     int op = op_of(precision);
     if (op < 0) return fail(EMH_E_ARG, "emh_create: unknown precision");
     int count = 0;
@@ -229,7 +228,8 @@ int emh_get_timing(emh_assigner *h, double *ms_out, uint64_t *calls) {
 int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t d) {
     if (!h) return fail(EMH_E_ARG, "emh_set_centroids: NULL handle");
     if (k == 0 || d == 0 || !centroids) return fail(EMH_E_ARG, "set_centroids: need k > 0, d > 0 and a k x d table");
-    if (h->op == OP_I8 && d > 32768) return fail(EMH_E_ARG, "integer assignment supports d <= 32768");
+    if ((h->op == OP_I8 || h->op == OP_I16) && d > 32768)
+        return fail(EMH_E_ARG, "integer assignment supports d <= 32768");
     if (k > 0xFFFFFFFEull) return fail(EMH_E_ARG, "set_centroids: k must fit in 32 bits");
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
@@ -291,7 +291,7 @@ int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
 int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     if (!h || !buf || cap == 0) return fail(EMH_E_ARG, "emh_name: bad arguments");
     static const char *kind[] = {"FP32 fused MFMA argmin", "BF16 MFMA + FP32 rescore", "FP8-e4m3 MFMA + FP32 rescore",
-                                 "I8 MFMA exact", "I16"};
+                                 "I8 MFMA exact", "I16 (three i8 MFMA planes) exact"};
     char tmp[512];
     int n;
     if (h->op == OP_BF16 || h->op == OP_FP8)
@@ -311,7 +311,7 @@ static int assign_host(emh_assigner *h, const void *x, int rows, size_t m, uint3
     if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
     if (m == 0) return EMH_OK;
     if (!x || !labels || !dist) return fail(EMH_E_ARG, "assign: NULL buffer");
-    if ((h->op == OP_I8) && rows != EMH_ROWS_I8)
+    if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
@@ -358,7 +358,7 @@ int emh_assign_device(emh_assigner *h, const void *x_dev, int rows, size_t m, co
     if (rows < EMH_ROWS_F32 || rows > EMH_ROWS_U8) return fail(EMH_E_ARG, "assign: unknown row type");
     if (m == 0) return EMH_OK;
     if (!x_dev || !labels_dev || !dist_dev) return fail(EMH_E_ARG, "assign: NULL buffer");
-    if (h->op == OP_I8 && rows != EMH_ROWS_I8)
+    if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;

@@ -11,7 +11,9 @@
 
 namespace emh {
 
-enum Op { OP_F32 = 0, OP_BF16 = 1, OP_FP8 = 2, OP_I8 = 3 };
+// OP_I16: CentroidBits::I16, the centroid stored as three i8 planes (see
+// search_kernel).
+enum Op { OP_F32 = 0, OP_BF16 = 1, OP_FP8 = 2, OP_I8 = 3, OP_I16 = 4 };
 
 // Workgroup geometry of the search kernels: 4 waves, each owning 32 rows, and
 // centroid tiles of BN centroids (two 32x32 MFMA blocks per wave).
@@ -36,6 +38,7 @@ struct SearchGeom {
     size_t d, d_pad;    // elements; d_pad * elem_bytes is the operand row stride
     size_t k, k_pad;
     int row_bytes;      // bytes of one LDS row segment (d_pad*es when resident, else CHUNK_BYTES)
+    int shift = 0;      // I16: centroids are round(c * 2^shift) (src/cpu_i8.rs:95-107)
 };
 
 SearchGeom make_geom(int op, size_t k, size_t d);
@@ -47,7 +50,9 @@ inline int reg_tile(const SearchGeom &g) { return g.row_bytes <= 256 ? 64 : 32; 
 // d_pad elements; row_bytes is only the LDS segment when rows stream in
 // chunks), followed on the register-resident path (where a row is
 // row_bytes) by the same table in tile-major order (reg_operand).
-inline size_t operand_table_bytes(const SearchGeom &g) { return g.k_pad * g.d_pad * (size_t)op_elem_bytes(g.op); }
+inline size_t operand_table_bytes(const SearchGeom &g) {
+    return g.k_pad * g.d_pad * (size_t)op_elem_bytes(g.op) * (g.op == OP_I16 ? 3 : 1);
+}
 inline size_t operand_bytes(const SearchGeom &g) {
     return g.reg ? 2 * operand_table_bytes(g) : operand_table_bytes(g);
 }

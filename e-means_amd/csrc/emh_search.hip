@@ -38,6 +38,7 @@ template <> struct OpT<OP_F32> { static constexpr int ES = 4, LB = 4; };
 template <> struct OpT<OP_BF16> { static constexpr int ES = 2, LB = 16; };
 template <> struct OpT<OP_FP8> { static constexpr int ES = 1, LB = 32; };
 template <> struct OpT<OP_I8> { static constexpr int ES = 1, LB = 16; };
+template <> struct OpT<OP_I16> { static constexpr int ES = 1, LB = 16; };
 
 SearchGeom make_geom(int op, size_t k, size_t d) {
     SearchGeom g;
@@ -45,6 +46,21 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     g.d = d;
     g.k = k;
     g.k_pad = (k + BN - 1) / BN * BN;
+    if (op == OP_I16) {
+        // LDS kernel, d streamed in 128-byte chunks of the three planes;
+        // shift = largest s <= 7 with d * 2^(14 + s) < 2^31 (src/cpu_i8.rs:97)
+        g.reg = false;
+        g.resident = false;
+        g.d_pad = (d + CHUNK_BYTES - 1) / CHUNK_BYTES * CHUNK_BYTES;
+        g.row_bytes = CHUNK_BYTES;
+        g.shift = 0;
+        for (int s = 7; s >= 0; s--)
+            if (((uint64_t)d << (14 + s)) < (1ull << 31)) {
+                g.shift = s;
+                break;
+            }
+        return g;
+    }
     const size_t es = op_elem_bytes(op);
     // Register-resident kernel: operand rows padded to 64 bytes, at most 512
     // bytes. The LDS kernel can run on the same layout (rows resident in
@@ -111,6 +127,7 @@ struct SearchArgs {
     const unsigned *stats;
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
     int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
+    int shift;              // I16: score units are 4^-shift squared distance
     uint32_t *labels;
     float *dist;
     uint32_t *cand;
@@ -125,7 +142,7 @@ __device__ inline uint4 row_segment(const SearchArgs &a, size_t call_row, size_t
     if constexpr (OP == OP_FP8) {
         // pre-scaled e4m3 rows, already padded to d_pad
         return *(const uint4 *)(a.x8 + call_row * a.d_pad + e0);
-    } else if constexpr (OP == OP_I8) {
+    } else if constexpr (OP == OP_I8 || OP == OP_I16) {
         const int8_t *p = (const int8_t *)a.x + src_row * a.d;
         if (a.vec && e0 + 16 <= a.d) return *(const uint4 *)(p + e0);
         uint32_t w[4];
@@ -217,7 +234,13 @@ template <int OP, int ROWS, int M, bool RES>
 __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     using T = OpT<OP>;
     constexpr int ES = T::ES, LB = T::LB;
-    using Score = typename std::conditional<OP == OP_I8, int64_t, float>::type;
+    constexpr bool INT = OP == OP_I8 || OP == OP_I16;
+    // I16: three i8 planes per centroid, c16 = p0 + 2^7 p1 + 2^14 p2 with
+    // p0, p1 in [0, 127] and p2 in [-2, 1]; x.c16 = x.p0 + 2^7 x.p1 + 2^14 x.p2
+    // computed with wrapping int32 adds, which is the reference's wrapping
+    // i32 dot product (vpdpwssd lanes) exactly.
+    constexpr int NPL = OP == OP_I16 ? 3 : 1;
+    using Score = typename std::conditional<INT, int64_t, float>::type;
     extern __shared__ __align__(16) uint8_t smem[];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -226,8 +249,8 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     const int RB = a.row_bytes, RS = RB + 16;
     uint8_t *sR = smem;
     uint8_t *sC = smem + BM * RS;
-    float *sHn = (float *)(sC + BN * RS);
-    int64_t *sCn = (int64_t *)(sC + BN * RS);
+    float *sHn = (float *)(sC + NPL * BN * RS);
+    int64_t *sCn = (int64_t *)(sC + NPL * BN * RS);
     const int segs = RB / 16;
     const int nchunks = RES ? 1 : (int)((a.d_pad * ES) / RB);
     const size_t my_row = row0 + wave * 32 + j;
@@ -243,14 +266,15 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
         }
     };
     auto load_cents = [&](size_t c0, int chunk) {
-        const size_t stride = a.d_pad * ES;
-        for (int q = tid; q < BN * segs; q += 256) {
-            int r = q / segs, sg = q - r * segs;
-            const uint8_t *src = a.cen_op + (c0 + r) * stride + (size_t)chunk * RB + sg * 16;
-            *(uint4 *)(sC + r * RS + sg * 16) = *(const uint4 *)src;
+        const size_t stride = a.d_pad * ES, plane = a.k_pad * stride;
+        for (int q = tid; q < NPL * BN * segs; q += 256) {
+            const int pl = q / (BN * segs), qq = q - pl * BN * segs;
+            const int r = qq / segs, sg = qq - r * segs;
+            const uint8_t *src = a.cen_op + pl * plane + (c0 + r) * stride + (size_t)chunk * RB + sg * 16;
+            *(uint4 *)(sC + (pl * BN + r) * RS + sg * 16) = *(const uint4 *)src;
         }
         if (tid < BN) {
-            if constexpr (OP == OP_I8) sCn[tid] = a.cn[c0 + tid];
+            if constexpr (INT) sCn[tid] = a.cn[c0 + tid];
             else sHn[tid] = a.hn[c0 + tid];
         }
     };
@@ -260,7 +284,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     uint32_t bi[M];
 #pragma unroll
     for (int t = 0; t < M; t++) {
-        if constexpr (OP == OP_I8) bs[t] = INT64_MAX;
+        if constexpr (INT) bs[t] = INT64_MAX;
         else bs[t] = INFINITY;
         bi[t] = NO_LABEL;
     }
@@ -280,7 +304,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                 float f[4] = {__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w)};
 #pragma unroll
                 for (int q = 0; q < 4; q++) xn32 = __fadd_rn(xn32, __fmul_rn(f[q], f[q]));
-            } else if constexpr (OP == OP_I8) {
+            } else if constexpr (INT) {
                 uint32_t w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
                 for (int q = 0; q < 4; q++)
@@ -296,19 +320,20 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     if constexpr (RES) {
         load_rows(0);
         __syncthreads();
-        if constexpr (OP == OP_F32 || OP == OP_I8) norm_part();
+        if constexpr (OP == OP_F32 || INT) norm_part();
     }
 
     for (size_t c0 = 0; c0 < a.k_pad; c0 += BN) {
-        using Acc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
+        using Acc = typename std::conditional<INT, i32x16, f32x16>::type;
         Acc acc0 = {}, acc1 = {};
+        i32x16 pacc0[NPL] = {}, pacc1[NPL] = {};  // I16: per plane
         for (int ch = 0; ch < nchunks; ch++) {
             if constexpr (!RES) load_rows(ch);
             load_cents(c0, ch);
             __syncthreads();
             if constexpr (!RES) {
                 if (c0 == 0) {
-                    if constexpr (OP == OP_F32 || OP == OP_I8) norm_part();
+                    if constexpr (OP == OP_F32 || INT) norm_part();
                 }
             }
             const uint8_t *pb = sR + (wave * 32 + j) * RS + h * (RB / 2);
@@ -344,6 +369,18 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                     acc0 = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca, xb, acc0, 0, 0, 0);
                     acc1 = __builtin_amdgcn_mfma_i32_32x32x32_i8(cb, xb, acc1, 0, 0, 0);
                 }
+            } else if constexpr (OP == OP_I16) {
+#pragma unroll
+                for (int pl = 0; pl < NPL; pl++) {
+                    const uint8_t *qa0 = pa0 + pl * BN * RS, *qa1 = pa1 + pl * BN * RS;
+                    for (int b = 0; b < RB / 2; b += LB) {
+                        i32x4 xb = __builtin_bit_cast(i32x4, *(const uint4 *)(pb + b));
+                        i32x4 ca = __builtin_bit_cast(i32x4, *(const uint4 *)(qa0 + b));
+                        i32x4 cb = __builtin_bit_cast(i32x4, *(const uint4 *)(qa1 + b));
+                        pacc0[pl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ca, xb, pacc0[pl], 0, 0, 0);
+                        pacc1[pl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(cb, xb, pacc1[pl], 0, 0, 0);
+                    }
+                }
             } else {  // OP_FP8
                 for (int b = 0; b < RB / 2; b += LB) {
                     uint4 x0 = *(const uint4 *)(pb + b), x1 = *(const uint4 *)(pb + b + 16);
@@ -369,6 +406,12 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                         if constexpr (OP == OP_I8) {
                             int32_t dv = nb == 0 ? acc0[r] : acc1[r];
                             s = sCn[il] - 2 * (int64_t)dv;
+                        } else if constexpr (OP == OP_I16) {
+                            const i32x16 *pp = nb == 0 ? pacc0 : pacc1;
+                            // x.c16 (mod 2^32, as the reference's i32 sum)
+                            const uint32_t dv = (uint32_t)pp[0][r] + ((uint32_t)pp[1][r] << 7) +
+                                                ((uint32_t)pp[2 % NPL][r] << 14);
+                            s = sCn[il] - ((int64_t)2 << a.shift) * (int64_t)(int32_t)dv;
                         } else if constexpr (OP == OP_FP8) {
                             // operands are -c * s_C and x * s_row: acc ~ -s_row s_C x.c
                             float dv = nb == 0 ? acc0[r] : acc1[r];
@@ -397,13 +440,13 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 #pragma unroll
     for (int t = 0; t < M; t++) {
         Score ps;
-        if constexpr (OP == OP_I8) ps = shfl_xor_i64(bs[t], 32);
+        if constexpr (INT) ps = shfl_xor_i64(bs[t], 32);
         else ps = __shfl_xor(bs[t], 32, 64);
         uint32_t pi = (uint32_t)__shfl_xor((int)bi[t], 32, 64);
         if (h == 0) topm_insert_lex<M, Score>(bs, bi, ps, pi);
     }
     if constexpr (OP == OP_F32) xn32 = __fadd_rn(xn32, __shfl_xor(xn32, 32, 64));
-    if constexpr (OP == OP_I8) xn_i += __shfl_xor(xn_i, 32, 64);
+    if constexpr (INT) xn_i += __shfl_xor(xn_i, 32, 64);
 
     if (h == 0 && my_row < a.m) {
         if constexpr (OP == OP_F32 && M == 1) {
@@ -411,9 +454,10 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
             float v = __fadd_rn(__fmul_rn(2.0f, bs[0]), xn32);
             a.labels[my_row] = bi[0] == NO_LABEL ? 0u : bi[0];
             a.dist[my_row] = v > 0.0f ? v : 0.0f;
-        } else if constexpr (OP == OP_I8) {
-            // dist = ((|x|^2 + best) as f64).max(0) as f32 (src/cpu_i8.rs:183-186)
-            double v = __dadd_rn((double)(int64_t)xn_i, (double)bs[0]);
+        } else if constexpr (INT) {
+            // dist = ((|x|^2 + best * 4^-shift) as f64).max(0) as f32
+            // (src/cpu_i8.rs:183-186; the power-of-two scaling is exact)
+            double v = __dadd_rn((double)(int64_t)xn_i, ldexp((double)bs[0], -2 * a.shift));
             a.labels[my_row] = bi[0] == NO_LABEL ? 0u : bi[0];
             a.dist[my_row] = (float)(v > 0.0 ? v : 0.0);
         } else {
@@ -1094,6 +1138,36 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
     }
 }
 
+// I16 centroids (src/cpu_i8.rs:95-107): c16 = round(c * 2^shift) clamped to
+// i16 (NaN -> 0, as Rust's `as`), stored as three i8 planes of k_pad x d_pad
+// bytes each (low 7 bits, next 7 bits, the signed top), and sum c16^2 in i64.
+// One wave per centroid.
+__global__ __launch_bounds__(256) void prep_i16_kernel(const float *c, size_t k, size_t d, size_t d_pad,
+                                                       size_t k_pad, int shift, uint8_t *cen_op, int64_t *norm64) {
+    const int lane = threadIdx.x & 63;
+    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (jc >= k_pad) return;
+    const bool live = jc < k;
+    const float scale = (float)(1u << shift);
+    const size_t plane = k_pad * d_pad;
+    int8_t *o = (int8_t *)cen_op + jc * d_pad;
+    int64_t part = 0;
+    for (size_t e = lane; e < d_pad; e += 64) {
+        int32_t q = 0;
+        if (live && e < d) {
+            float r = roundf(c[jc * d + e] * scale);
+            r = r != r ? 0.0f : fminf(fmaxf(r, -32768.0f), 32767.0f);
+            q = (int32_t)r;
+        }
+        o[e] = (int8_t)(q & 127);
+        o[plane + e] = (int8_t)((q >> 7) & 127);
+        o[2 * plane + e] = (int8_t)(q >> 14);
+        part += (int64_t)q * q;
+    }
+    for (int off = 32; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
+    if (lane == 0 && live) norm64[jc] = part;
+}
+
 __global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) {
     float mx = 0.0f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
@@ -1108,10 +1182,26 @@ __global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) 
 // cn_j = sum c8^2 + round(bias_j) (src/cpu_i8.rs:112-117, unit 1). Pad
 // centroids get a score that never wins.
 __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *norm32, const int64_t *norm64,
-                                const float *bias, float *hn, int64_t *cn, unsigned *stats, int64_t cn_limit) {
+                                const float *bias, float *hn, int64_t *cn, unsigned *stats, int64_t cn_limit,
+                                int shift) {
     size_t jc = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (jc >= k_pad) return;
     if (bias && jc < k && bias[jc] < 0.0f) atomicMax(&stats[1], __float_as_uint(-bias[jc]));
+    if (op == OP_I16) {
+        // offset = sum c16^2 + round(bias * 4^shift) (src/cpu_i8.rs:112-117);
+        // the search skips indices >= k
+        int64_t v = 0;
+        if (jc < k) {
+            v = norm64[jc];
+            if (bias) {
+                double r = round(ldexp((double)bias[jc], 2 * shift));
+                r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
+                v += (int64_t)r;
+            }
+        }
+        cn[jc] = v;
+        return;
+    }
     if (op == OP_I8) {
         // pad centroids: 2^26 puts their packed score 2 x.0 - 2^26 below every
         // real one (> -2^26) and still inside int32 once shifted; the i64
@@ -1173,7 +1263,10 @@ __global__ __launch_bounds__(256) void rows_fp8_kernel(const void *x, size_t m, 
 
 // ------------------------------------------------------------- launchers ---
 
-static size_t search_lds(const SearchGeom &g) { return (size_t)(BM + BN) * (g.row_bytes + 16) + BN * 8; }
+static size_t search_lds(const SearchGeom &g) {
+    const size_t planes = g.op == OP_I16 ? 3 : 1;
+    return (size_t)(BM + planes * BN) * (g.row_bytes + 16) + BN * 8;
+}
 
 template <int OP, int ROWS, int M, bool RES>
 static hipError_t launch_one(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
@@ -1267,6 +1360,7 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
     a.labels = labels;
     a.dist = dist;
     a.cand = cand;
+    a.shift = g.shift;
     if (reg_path(g)) {
         // the register-resident kernels read the tile-major copy
         SearchArgs r = a;
@@ -1306,6 +1400,9 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
         case OP_I8:
             if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
             return launch_res<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
+        case OP_I16:
+            if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
+            return launch_one<OP_I16, EMH_ROWS_I8, 1, false>(g, a, s);
     }
     return hipErrorInvalidValue;
 }
@@ -1343,6 +1440,7 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
         case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
+        case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
@@ -1452,7 +1550,7 @@ hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64
     // packed I8 key: |2 x.c - cn| < 2^26 needs |cn| < 2^26 - 2 * d * 2^14
     const int64_t cn_limit = ((int64_t)1 << 26) - (int64_t)g.d * 32768;
     hipLaunchKernelGGL(set_bias_kernel, grid, dim3(256), 0, s, g.op, g.k, g.k_pad, norm32, norm64, bias, hn, cn, stats,
-                       cn_limit);
+                       cn_limit, g.shift);
     return hipGetLastError();
 }
 

@@ -18,7 +18,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     JobRun J;
     int rc;
     if ((rc = J.init(x, rows, n, d, k, cfg, centroids_out, labels_out, out))) return rc;
-    if (h->op == OP_I8 && rows != EMH_ROWS_I8)
+    if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
     const size_t batch = J.R.batch_size;
     const size_t rb = rows == EMH_ROWS_F32 ? 4 : 1;

@@ -608,9 +608,10 @@ static void dots8_portable(const int8_t *const q[4], const int8_t *const c[4], s
     /* src/cpu_i8.rs:193-202: sum (x + 128) * c in i32 */
     for (int qi = 0; qi < 4; qi++)
         for (int ci = 0; ci < 4; ci++) {
-            int32_t s = 0;
-            for (size_t i = 0; i < d; i++) s += ((int32_t)q[qi][i] + 128) * (int32_t)c[ci][i];
-            out[4 * qi + ci] = s;
+            /* i32 sum wrapping as the VNNI lanes do (each product fits) */
+            uint32_t s = 0;
+            for (size_t i = 0; i < d; i++) s += (uint32_t)(((int32_t)q[qi][i] + 128) * (int32_t)c[ci][i]);
+            out[4 * qi + ci] = (int32_t)s;
         }
 }
 
@@ -618,9 +619,9 @@ static void dots16_portable(const int8_t *const q[4], const int16_t *const c[4],
     /* src/cpu_i8.rs:204-212 */
     for (int qi = 0; qi < 4; qi++)
         for (int ci = 0; ci < 4; ci++) {
-            int32_t s = 0;
-            for (size_t i = 0; i < d; i++) s += (int32_t)q[qi][i] * (int32_t)c[ci][i];
-            out[4 * qi + ci] = s;
+            uint32_t s = 0;
+            for (size_t i = 0; i < d; i++) s += (uint32_t)((int32_t)q[qi][i] * (int32_t)c[ci][i]);
+            out[4 * qi + ci] = (int32_t)s;
         }
 }
 

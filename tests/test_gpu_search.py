@@ -136,11 +136,73 @@ def test_i8_large_bias_takes_the_i64_path(hip, d):
         assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias scale {scale}"
 
 
+@pytest.mark.parametrize("n,k,d", I8_SHAPES)
+def test_i16_bit_exact(hip, n, k, d):
+    """src/cpu_i8.rs:298-324 with CentroidBits::I16: integer centroids give
+    the exact brute-force labels and distances."""
+    print(f"i16 n={n} k={k} d={d}", flush=True)
+    x = O.lcg_i8(n * d, 1).reshape(n, d)
+    ci = O.lcg_i8(k * d, 2).reshape(k, d).astype(np.float32)
+    el, ed = O.brute_i64(x, ci.astype(np.int8))
+    a = E.HipAssigner(E.I16)
+    a.set_centroids(ci)
+    labels, dist = a.assign(x)
+    assert np.array_equal(labels, el)
+    assert np.array_equal(dist, ed.astype(np.float32))
+
+
+def test_i16_resolves_fractional_centroids(hip):
+    # src/cpu_i8.rs:326-336
+    a = E.HipAssigner(E.I16)
+    a.set_centroids(np.array([[0.26, 0.0], [-0.26, 0.0]], np.float32))
+    labels, dist = a.assign(np.array([[1, 0]], np.int8))
+    assert labels[0] == 0
+    assert abs(dist[0] - 0.74 * 0.74) < 0.01
+
+
+@pytest.mark.parametrize("d", [7, 96, 300])
+def test_i16_fractional_centroids_and_bias_match_cpu(hip, d):
+    rng = np.random.default_rng(12)
+    x = rng.integers(-128, 128, size=(1500, d)).astype(np.int8)
+    c = rng.normal(scale=50, size=(261, d)).astype(np.float32)
+    c[:3] = np.round(c[:3] * 4) / 4 + 1 / 256  # near rounding boundaries of the 1/2^s grid
+    bias = rng.uniform(-40, 40, size=261).astype(np.float32)
+    a = E.HipAssigner(E.I16)
+    for b in (None, bias):
+        a.set_centroids(c)
+        if b is not None:
+            a.set_bias(b)
+        gl, gd = a.assign(x)
+        cl, cd = O.i8_assign(x, c, bits=O.BITS_I16, bias=b)
+        assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias={b is not None}"
+
+
+def test_i16_dot_wraps_like_the_cpu(hip):
+    """Centroids past the i8 range at d = 1024: the i32 dot product wraps on
+    the CPU (vpdpwssd) and must wrap the same way on the GPU."""
+    rng = np.random.default_rng(11)
+    n, k, d = 40, 9, 1024
+    x = rng.integers(100, 128, size=(n, d)).astype(np.int8)
+    c = np.full((k, d), 600.0, np.float32)
+    c[1::2] *= -1
+    c[:, :5] = rng.normal(size=(k, 5))
+    a = E.HipAssigner(E.I16)
+    a.set_centroids(c)
+    gl, gd = a.assign(x)
+    cl, cd = O.i8_assign(x, c, bits=O.BITS_I16)
+    assert np.array_equal(gl, cl) and np.array_equal(gd, cd)
+
+
 def test_i8_rejects_float_rows(hip):
     a = E.HipAssigner(E.I8)
     a.set_centroids(np.zeros((3, 4), np.float32))
     with pytest.raises(E.EmeansError) as ei:
         a.assign(np.zeros((2, 4), np.float32))
+    assert ei.value.code == E.E_UNSUPPORTED
+    b = E.HipAssigner(E.I16)
+    b.set_centroids(np.zeros((3, 4), np.float32))
+    with pytest.raises(E.EmeansError) as ei:
+        b.assign(np.zeros((2, 4), np.uint8))
     assert ei.value.code == E.E_UNSUPPORTED
 
 

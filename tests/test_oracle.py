@@ -300,6 +300,29 @@ def test_i8_offsets_are_the_vnni_identity():
     assert (off2 - off).tolist() == [1, -1, 1, 3, -3, 0, 0, 0, 0]
 
 
+def test_i16_dot_wraps_like_vpdpwssd():
+    """Centroids past the i8 range clamp to 32767 and the i32 dot product
+    wraps (vpdpwssd lanes and a release-build i32 sum both wrap): labels and
+    distances equal an exact i64 brute force whose dot is wrapped to i32."""
+    rng = np.random.default_rng(11)
+    n, k, d = 40, 9, 1024
+    x = rng.integers(100, 128, size=(n, d)).astype(np.int8)
+    c = np.full((k, d), 600.0, np.float32)
+    c[1::2] *= -1
+    c[:, :5] = rng.normal(size=(k, 5))
+    _, c16, off, mult, shift = O.i8_prepare(c, bits=O.BITS_I16)
+    dp = x.astype(np.int64) @ c16.astype(np.int64).T
+    dpw = (dp + 2**31) % 2**32 - 2**31
+    assert (dp != dpw).any()
+    score = off[None, :] - mult * dpw
+    want_l = score.argmin(1)
+    norm = (x.astype(np.int64) ** 2).sum(1)
+    want_d = np.maximum(norm + score[np.arange(n), want_l] / 4.0**shift, 0.0).astype(np.float32)
+    for vnni in sorted({False, O.has_vnni()}):
+        lab, dist = O.i8_assign(x, c, bits=O.BITS_I16, vnni=vnni)
+        assert np.array_equal(lab, want_l) and np.array_equal(dist, want_d), f"vnni={vnni}"
+
+
 def test_i16_shift_rule():
     # largest s <= 7 with d * 2^(14+s) < 2^31 (src/cpu_i8.rs:95-107)
     for d, s in ((1, 7), (1023, 7), (1024, 6), (16384, 2), (8192, 3), (32768, 1)):
