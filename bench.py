@@ -8,15 +8,21 @@ timed region. One step = one pass of the search over all n rows: for the
 BF16/FP8 searches the MFMA search kernel plus the FP32 rescoring of the top-m
 candidates (plus the per-row FP8 encoding), for FP32 the fused argmin kernel.
 
-Other workloads (reported in DESIGN.md, not the headline line):
+Other workloads (separate lines, not the headline):
   search-i8          1,000,000 x 512 int8 codes (rank-64 signal + 0.1 noise,
-                     L2-normalised, faiss calibration), k = 4096, exact I8 search
+                     L2-normalised, faiss calibration), k = 4096, exact I8
+                     search (--precision i16: CentroidBits::I16)
   search-fp8-large-k 2,000,000 x 256 heavy-tailed rows (Student-t nu=3, random
                      rotation), k = 65536, FP8 top-8 + FP32 rescore vs FP32
   train-c1           e-means training, 200,000 x 96 mixture of 4096 blobs,
                      k = 1024, 5 epochs, Config::default(); GPU vs CPU port
   train-c3           int8 training, 1,000,000 x 512 codes, k = 4096, 5 epochs,
                      I8 search; GPU vs CPU port (bit-identical results)
+  train-c4           10,000,000 x 256 heavy-tailed rows, k = 65536, 2 epochs,
+                     FP8 top-8 + FP32 rescore vs the FP32 search on this GPU
+  train-c5           64 uneven jobs (n 5e3..5e5, d 64..256, k 64..4096),
+                     3 epochs, one graph replay per lockstep step (BF16 top-4)
+                     vs a loop of single-job FP32 trainings
 
 Prints one JSON line.
 """
@@ -34,7 +40,9 @@ for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
         sys.path.insert(0, _p)
 
 # MI355X dense peaks (AMD published): TFLOP/s (TOP/s for int8)
-PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2}
+PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2, "i16": 5033.2}
+# i8 MFMA operations per algorithmic multiply-add pair (I16 runs three i8 planes)
+OPS_PER_PRODUCT = {"i16": 3}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
 WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5"]
 
@@ -45,7 +53,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="search", choices=WORKLOADS)
-    ap.add_argument("--precision", default=None, choices=["bf16", "fp8", "fp32", "i8"],
+    ap.add_argument("--precision", default=None, choices=["bf16", "fp8", "fp32", "i8", "i16"],
                     help="search precision of the headline line (workload default if omitted)")
     ap.add_argument("--rescore-m", type=int, default=None)
     ap.add_argument("--n", type=int, default=None)
@@ -203,7 +211,7 @@ def judge(x, c, res, rows):
     }
 
 
-def cpu_search_baseline(x, c, seconds, i8=False):
+def cpu_search_baseline(x, c, seconds, i8=False, i16=False):
     """The oracle's restatement of the reference CPU search (src/cpu.rs:
     AVX-512 4x4 micro-kernel, 256 KiB blocks; src/cpu_i8.rs: VNNI) on the
     host cores, on a bounded sample of the same rows."""
@@ -215,7 +223,7 @@ def cpu_search_baseline(x, c, seconds, i8=False):
 
     def run(rows):
         if i8:
-            return O.i8_assign(xh[:rows], ch, nthreads=cores)
+            return O.i8_assign(xh[:rows], ch, bits=O.BITS_I16 if i16 else O.BITS_I8, nthreads=cores)
         return O.cpu_assign(xh[:rows], ch, nthreads=cores)
 
     # grow the sample until one timed run takes at least half the target
@@ -257,7 +265,8 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         steps = max(2, args.steps // 4) if cprec == "fp32" else args.steps
         variants[cprec] = run_search(cprec, cm, x, c_host, steps, 1)
     quality = {}
-    if prec != "i8":
+    integer = prec in ("i8", "i16")
+    if not integer:
         for name, v in variants.items():
             q = judge(x, c, v, min(args.judge_rows, n))
             q["mse"] = float(v["dist"].double().mean())
@@ -266,12 +275,12 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
             for name, q in quality.items():
                 q["mse_rel_vs_fp32"] = q["mse"] / quality["fp32"]["mse"] - 1.0
     flops = 2.0 * n * k * d
-    achieved = flops / (head["search_ms"] * 1e-3) / 1e12
+    achieved = OPS_PER_PRODUCT.get(prec, 1) * flops / (head["search_ms"] * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 2),
         "peak": PEAK[prec],
-        "unit": "TFLOP/s" if prec != "i8" else "TOP/s",
+        "unit": "TOP/s" if integer else "TFLOP/s",
         "frac": round(achieved / PEAK[prec], 4),
         "traffic": traffic_from_profile(args.workload, prec),
         "kernel": f"search_reg_kernel ({prec})",
@@ -280,7 +289,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
-        cpu = cpu_search_baseline(xq if prec == "i8" else x, c, args.cpu_seconds, i8=prec == "i8")
+        cpu = cpu_search_baseline(xq if integer else x, c, args.cpu_seconds, i8=integer, i16=prec == "i16")
     out = {
         "metric": METRIC,
         "value": round(head["rows_per_s"], 1),
@@ -292,7 +301,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32", "i8": "int8"}[prec],
+        "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32", "i8": "int8", "i16": "int8 (3 planes)"}[prec],
         "data": "synthetic (generated on the GPU with torch, seed 0)",
         "config": {"workload": args.workload_desc, "n": n, "d": d, "k": k,
                    "precision": prec + ("+fp32 rescore" if prec in ("bf16", "fp8") else ""),
@@ -543,8 +552,8 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
 def main():
     args = parse()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rows would shard by "
-              "range with no exchange; see the sharding note in DESIGN.md). Run with --gpus 1.")
+        print("bench.py measures one GPU: there is no multi-GPU path (an assign would shard rows by "
+              "range with no exchange; training would all-reduce per-cluster sums). Run with --gpus 1.")
         sys.exit(2)
     import torch
 
@@ -570,10 +579,14 @@ def main():
     elif w == "search-i8":
         n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
         args.workload_desc = (f"int8 search: {n}x{d} int8 codes (rank-64 signal + 0.1 noise, L2-normalised, "
-                              f"faiss calibration), k={k} centroids drawn from the codes, exact I8 search")
+                              f"faiss calibration), k={k} centroids drawn from the codes, exact "
+                              f"{(args.precision or 'i8').upper()} search")
         codes, _, _ = int8_codes(n, d, dev)
         c = pick_rows(codes.float(), k)
-        out = search_line(args, codes.float(), c, "i8", 1, [], i8_rows=codes)
+        prec = args.precision or "i8"
+        if prec not in ("i8", "i16"):
+            raise SystemExit("search-i8 runs the integer searches: --precision i8 or i16")
+        out = search_line(args, codes.float(), c, prec, 1, [], i8_rows=codes)
     elif w == "search-fp8-large-k":
         n, d, k = args.n or 2_000_000, args.d or 256, args.k or 65536
         args.workload_desc = (f"large-k search: {n}x{d} heavy-tailed rows (Student-t nu=3, random rotation), "

@@ -7,10 +7,10 @@
  *     trait AssignRows<T>       src/lib.rs:87-91
  *
  * so a Rust `impl Assigner for HipAssigner` wraps exactly these functions
- * (the binding is shown in INTEGRATION.md). The second block holds
- * extensions the reference has no counterpart for: device-resident rows,
- * device-side per-cluster sums/counts and the training driver of
- * src/lib.rs:348-491 running over them.
+ * (an extern "C" block plus the impl). The second block holds extensions
+ * the reference has no counterpart for: device-resident rows, device-side
+ * per-cluster sums/counts, the training driver of src/lib.rs:348-491
+ * running over them, and many trainings in lockstep.
  *
  * Conventions: every function returns 0 on success and a nonzero EMH_E_*
  * code on failure; emh_last_error() then holds a message. Nothing panics,
