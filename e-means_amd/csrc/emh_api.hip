@@ -12,6 +12,10 @@
 #include <algorithm>
 #include <new>
 
+#include <map>
+#include <mutex>
+#include <tuple>
+
 #include "emh_assigner.h"
 
 namespace emh {
@@ -53,6 +57,114 @@ static int mark(emh_assigner *h, hipEvent_t *slot) {
     return check(hipEventRecord(*slot, h->stream), "hipEventRecord");
 }
 
+// ---- split autotuner ----
+// A launch with few row blocks leaves most of the chip idle; splitting the
+// centroid tiles of each row block over several workgroups fills it. The
+// split changes nothing in the results (merge_slices_kernel), so the tuner
+// times the candidate splits on the call itself, the first time a shape is
+// seen, and keeps the fastest for the process. EMH_AUTOTUNE=0 in the
+// environment replaces the timing by a rule: the smallest split that puts
+// two workgroups on every CU.
+namespace {
+struct TuneKey {
+    int device, op, rows, M, row_bytes, ntiles;
+    size_t rowblocks;
+    bool operator<(const TuneKey &o) const {
+        return std::tie(device, op, rows, M, row_bytes, ntiles, rowblocks) <
+               std::tie(o.device, o.op, o.rows, o.M, o.row_bytes, o.ntiles, o.rowblocks);
+    }
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, int> g_tune;
+
+bool autotune_enabled() {
+    const char *e = getenv("EMH_AUTOTUNE");
+    return !(e && e[0] == '0');
+}
+}  // namespace
+
+// Search launch with `nslice` centroid slices (1: unsplit).
+static int search_launch(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                         const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
+                         int nslice) {
+    const SearchGeom &g = h->g;
+    uint32_t *part = nullptr;
+    if (nslice > 1) {
+        int rc = check(h->part.ensure(slice_part_bytes(M, m, nslice, h->op)), "split buffer");
+        if (rc) return rc;
+        part = h->part.as<uint32_t>();
+    }
+    h->last_slices = nslice;
+    return check(launch_search(g, M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(), h->cn.as<int64_t>(), x8,
+                               rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(), labels_dev, dist_dev,
+                               h->cand.as<uint32_t>(), h->stream, nslice, part),
+                 "search kernel");
+}
+
+static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                         const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
+                         int *out) {
+    *out = 1;
+    const SearchGeom &g = h->g;
+    if (h->forced_slices > 0) {
+        *out = h->forced_slices;
+        return EMH_OK;
+    }
+    if (!g.reg || h->timing) return EMH_OK;
+    const size_t rowblocks = (m + 255) / 256;
+    const int nt = reg_tiles(g);
+    const size_t target = 2 * (size_t)h->cu_count;  // the kernel runs two workgroups per CU
+    if (rowblocks >= target || nt < 4) return EMH_OK;
+    std::vector<int> cand;
+    for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) cand.push_back(s);
+    if (cand.size() < 2) return EMH_OK;
+    const TuneKey key{h->device, h->op, rows, M, g.row_bytes, nt, rowblocks};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            *out = it->second;
+            return EMH_OK;
+        }
+    }
+    int best = cand.back();
+    if (!autotune_enabled()) {
+        for (int s : cand)
+            if (rowblocks * (size_t)s >= target) {
+                best = s;
+                break;
+            }
+    } else {
+        hipEvent_t e0, e1;
+        int rc;
+        if ((rc = check(hipEventCreate(&e0), "tune event")) || (rc = check(hipEventCreate(&e1), "tune event")))
+            return rc;
+        float best_ms = 0.0f;
+        for (int s : cand) {
+            // one warm-up launch, then the timed one
+            if ((rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, s)) ||
+                (rc = check(hipEventRecord(e0, h->stream), "tune event")) ||
+                (rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, s)) ||
+                (rc = check(hipEventRecord(e1, h->stream), "tune event")) ||
+                (rc = check(hipEventSynchronize(e1), "tune sync")))
+                break;
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (s == cand.front() || ms < best_ms) {
+                best_ms = ms;
+                best = s;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (rc) return rc;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = best;
+    *out = best;
+    return EMH_OK;
+}
+
 int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev, uint32_t *labels_dev,
                float *dist_dev) {
     if (m == 0) return EMH_OK;
@@ -61,11 +173,11 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if ((rc = mark(h, &ev[0]))) return rc;
     if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
+        int ns = 1;
+        if ((rc = choose_slices(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev, &ns)))
+            return rc;
         if ((rc = mark(h, &ev[1]))) return rc;
-        if ((rc = check(launch_search(g, 1, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
-                                      h->cn.as<int64_t>(), nullptr, nullptr, nullptr, nullptr,
-                                      h->stats.as<unsigned>(), labels_dev, dist_dev, nullptr, h->stream),
-                        "search kernel")))
+        if ((rc = search_launch(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev, ns)))
             return rc;
         if ((rc = mark(h, &ev[2]))) return rc;
         return mark(h, &ev[3]);
@@ -85,12 +197,10 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
         rs = h->row_scale.as<float>();
         rn = h->row_norm.as<float>();
     }
+    int ns = 1;
+    if ((rc = choose_slices(h, h->M, x_dev, rows, m, ids_dev, x8, rs, rn, nullptr, nullptr, &ns))) return rc;
     if ((rc = mark(h, &ev[1]))) return rc;
-    if ((rc = check(launch_search(g, h->M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(),
-                                  h->cn.as<int64_t>(), x8, rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
-                                  nullptr, nullptr, h->cand.as<uint32_t>(), h->stream),
-                    "search kernel")))
-        return rc;
+    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, x8, rs, rn, nullptr, nullptr, ns))) return rc;
     if ((rc = mark(h, &ev[2]))) return rc;
     if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen_f32.as<float>(),
                                    h->has_bias ? h->bias.as<float>() : nullptr, g.k, h->cand.as<uint32_t>(),
@@ -115,7 +225,7 @@ hipEvent_t emh_assigner::timing_event() {
 
 emh_assigner::~emh_assigner() {
     for (DevBuf *b : {&cen_f32, &cen_op, &norm32, &norm64, &hn, &cn, &bias, &cmax, &stats, &x_stage, &labels_stage,
-                      &dist_stage, &cand, &x8, &row_scale, &row_norm})
+                      &dist_stage, &cand, &x8, &row_scale, &row_norm, &part})
         b->release();
     accum_free(acc);
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
@@ -164,6 +274,7 @@ int emh_create(int device, int precision, int rescore_m, emh_assigner **out) {
         h->M = p;
     }
     h->device_name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if ((rc = check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate"))) {
         delete h;
         return rc;
@@ -288,6 +399,13 @@ int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
     return EMH_OK;
 }
 
+int emh_set_split(emh_assigner *h, int nslice) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_split: NULL handle");
+    if (nslice < 0 || nslice > 4096) return fail(EMH_E_ARG, "emh_set_split: nslice must be in [0, 4096]");
+    h->forced_slices = nslice;
+    return EMH_OK;
+}
+
 int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     if (!h || !buf || cap == 0) return fail(EMH_E_ARG, "emh_name: bad arguments");
     static const char *kind[] = {"FP32 fused MFMA argmin", "BF16 MFMA + FP32 rescore", "FP8-e4m3 MFMA + FP32 rescore",
@@ -299,8 +417,8 @@ int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     else
         n = snprintf(tmp, sizeof tmp, "HIP %s, %s", kind[h->precision], h->device_name.c_str());
     if (h->has_centroids)
-        snprintf(tmp + n, sizeof tmp - n, ", k=%zu d=%zu (%s rows)", h->g.k, h->g.d,
-                 h->g.resident ? "LDS-resident" : "chunked");
+        snprintf(tmp + n, sizeof tmp - n, ", k=%zu d=%zu (%s rows%s)", h->g.k, h->g.d,
+                 h->g.resident ? "LDS-resident" : "chunked", h->last_slices > 1 ? ", centroids split" : "");
     strncpy(buf, tmp, cap - 1);
     buf[cap - 1] = 0;
     return EMH_OK;

@@ -57,6 +57,11 @@ struct emh_assigner {
     emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax, stats;
     // row side (per assign call)
     emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale, row_norm;
+    // centroid-range split of the register-resident search (autotuned)
+    emh::DevBuf part;
+    int cu_count = 256;
+    int last_slices = 1;
+    int forced_slices = 0;  // 0: the autotuner decides
     // per-cluster accumulation
     emh::AccumScratch acc;
 

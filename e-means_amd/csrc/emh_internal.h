@@ -72,10 +72,17 @@ hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
                            uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s);
 bool reg_path(const SearchGeom &g);
+// nslice > 1 (register-resident path only): the centroid tiles are split
+// over nslice workgroups per row block (fewer when tiles run out), each
+// writing partial results to `part` (slice_part_bytes), then merged into
+// labels/dist or cand. The results do not depend on nslice.
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
                          const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
-                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s);
+                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
+                         int nslice = 1, uint32_t *part = nullptr);
+int reg_tiles(const SearchGeom &g);
+size_t slice_part_bytes(int M, size_t m, int nslice, int op);
 hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,
                           const float *bias, size_t k, const uint32_t *cand, uint32_t *labels, float *dist,
                           hipStream_t s);

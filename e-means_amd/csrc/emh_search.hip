@@ -128,6 +128,12 @@ struct SearchArgs {
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
     int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
     int shift;              // I16: score units are 4^-shift squared distance
+    // Centroid-range split (register-resident search): workgroup (b, s)
+    // sweeps tiles [s * tiles_per_slice, +tiles_per_slice) and, with more than
+    // one slice, writes its per-row result to part[row * nslice + s] for
+    // merge_slices_kernel.
+    int nslice, tiles_per_slice;
+    uint32_t *part;
     uint32_t *labels;
     float *dist;
     uint32_t *cand;
@@ -653,7 +659,8 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
 // One workgroup's sweep: rows [row0, row0 + 128 FR) of a.x against every
 // centroid tile.
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
-__device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_t row0) {
+__device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_t row0, const int t0, const int t1,
+                                                const int slice) {
     constexpr int ES = OpT<OP>::ES;
     constexpr int RB = 32 * NS;           // operand bytes per row
     constexpr int HB = RB / 2;            // bytes per lane half
@@ -748,7 +755,6 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     // Tile DMA: wave-load q of wave w copies tile bytes [(4q + w) KiB, +1 KiB)
     // to the same place in LDS; lane L moves 16 bytes at L * 16. The buffer
     // resource spans the tile-major table (its size is checked on the host).
-    const int ntiles = (int)(a.k_pad / TB);
     const __amdgpu_buffer_rsrc_t rsrc =
         __builtin_amdgcn_make_buffer_rsrc((void *)a.cen_op, 0, (int)(a.k_pad * (size_t)RB), 0x00020000);
     auto dma = [&](uint8_t *dst, int t) {
@@ -816,10 +822,10 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
     };
-    HRaw hv0 = hload(0), hv1 = 0, hv2 = 0;
-    dma(sC0, 0);
-    dma(sC1, 1 < ntiles ? 1 : 0);
-    hv1 = hload(1 < ntiles ? 1 : 0);
+    HRaw hv0 = hload(t0), hv1 = 0, hv2 = 0;
+    dma(sC0, t0);
+    dma(sC1, t0 + 1 < t1 ? t0 + 1 : t0);
+    hv1 = hload(t0 + 1 < t1 ? t0 + 1 : t0);
     __builtin_amdgcn_s_waitcnt(WAIT_ONE);
     hstore(sH0, hv0);
     __syncthreads();
@@ -830,7 +836,7 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     // tile into the free buffer), so the wait counts are the same each time.
 #define EMH_STAGE(T, BC, BH, BN2, BH1, HV1, HV2)                                   \
     {                                                                             \
-        const int tn = (T) + 2 < ntiles ? (T) + 2 : ntiles - 1;                   \
+        const int tn = (T) + 2 < t1 ? (T) + 2 : t1 - 1;                           \
         dma(BN2, tn);                                                             \
         HV2 = hload(tn);                                                          \
         reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);                  \
@@ -840,11 +846,11 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
         hstore(BH1, HV1);                                                         \
         tile_barrier();                                                           \
     }
-    for (int t = 0; t < ntiles; t += 3) {
+    for (int t = t0; t < t1; t += 3) {
         EMH_STAGE(t, sC0, sH0, sC2, sH1, hv1, hv2);
-        if (t + 1 >= ntiles) break;
+        if (t + 1 >= t1) break;
         EMH_STAGE(t + 1, sC1, sH1, sC0, sH2, hv2, hv0);
-        if (t + 2 >= ntiles) break;
+        if (t + 2 >= t1) break;
         EMH_STAGE(t + 2, sC2, sH2, sC1, sH0, hv0, hv1);
     }
 #undef EMH_STAGE
@@ -860,8 +866,18 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
                 if (h == 0) topm_insert_lex<M, uint32_t>(bs[fr], bi[fr], ps, pi);
             }
             if (h == 0 && r < a.m) {
+                if (a.nslice > 1) {
+                    // keys then ids of this slice's top-m
+                    uint32_t *pp = a.part + (r * a.nslice + slice) * (2 * M);
 #pragma unroll
-                for (int t = 0; t < M; t++) a.cand[r * M + t] = bi[fr][t];
+                    for (int t = 0; t < M; t++) {
+                        pp[t] = bs[fr][t];
+                        pp[M + t] = bi[fr][t];
+                    }
+                } else {
+#pragma unroll
+                    for (int t = 0; t < M; t++) a.cand[r * M + t] = bi[fr][t];
+                }
             }
         } else {
             const Score ps = __shfl_xor(bs[fr][0], 32, 64);
@@ -873,15 +889,23 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
                 bi[fr][0] = pi;
             }
             if (h == 0 && r < a.m) {
-                a.labels[r] = bi[fr][0] == NO_LABEL ? 0u : bi[fr][0];
+                float dv;
                 if constexpr (OP == OP_F32) {
                     // dist = |x|^2 + |c|^2 + bias - 2 x.c, clamped at 0 (src/cpu.rs:160-162)
                     const float v = __fadd_rn(__fmul_rn(2.0f, (float)bs[fr][0]), xn[fr]);
-                    a.dist[r] = v > 0.0f ? v : 0.0f;
+                    dv = v > 0.0f ? v : 0.0f;
                 } else {
                     // dist = ((|x|^2 + best) as f64).max(0) as f32 (src/cpu_i8.rs:183-186)
                     const double v = __dadd_rn((double)xni[fr], -(double)(int32_t)bs[fr][0]);
-                    a.dist[r] = (float)(v > 0.0 ? v : 0.0);
+                    dv = (float)(v > 0.0 ? v : 0.0);
+                }
+                if (a.nslice > 1) {
+                    // (score, index, distance) of this slice's best
+                    uint32_t *pp = a.part + (r * a.nslice + slice) * 4;
+                    *(uint4 *)pp = make_uint4(__builtin_bit_cast(uint32_t, bs[fr][0]), bi[fr][0], __float_as_uint(dv), 0u);
+                } else {
+                    a.labels[r] = bi[fr][0] == NO_LABEL ? 0u : bi[fr][0];
+                    a.dist[r] = dv;
                 }
             }
         }
@@ -890,7 +914,51 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
 
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
-    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR));
+    const int ntiles = (int)(a.k_pad / (32 * NB));
+    const int s = blockIdx.y, t0 = s * a.tiles_per_slice;
+    const int t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR), t0, t1, s);
+}
+
+// Merges the per-slice results of a split search, one row per thread: the
+// lexicographic (score, index) order of the unsplit sweep, so the merged
+// labels, distances and candidate lists are the unsplit ones bit for bit.
+template <int OP, int M>
+__global__ void merge_slices_kernel(const uint32_t *part, size_t m, int nslice, uint32_t *labels, float *dist,
+                                    uint32_t *cand) {
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= m) return;
+    if constexpr (OP == OP_BF16 || OP == OP_FP8) {
+        uint32_t bs[M], bi[M];
+#pragma unroll
+        for (int t = 0; t < M; t++) {
+            bs[t] = 0xFFFFFFFFu;
+            bi[t] = NO_LABEL;
+        }
+        for (int s = 0; s < nslice; s++) {
+            const uint32_t *pp = part + (r * nslice + s) * (2 * M);
+#pragma unroll
+            for (int t = 0; t < M; t++) topm_insert_lex<M, uint32_t>(bs, bi, pp[t], pp[M + t]);
+        }
+#pragma unroll
+        for (int t = 0; t < M; t++) cand[r * M + t] = bi[t];
+    } else {
+        uint4 best = *(const uint4 *)(part + r * nslice * 4);
+        for (int s = 1; s < nslice; s++) {
+            const uint4 p = *(const uint4 *)(part + (r * nslice + s) * 4);
+            bool better;
+            if constexpr (OP == OP_F32) {
+                const float ps = __uint_as_float(p.x), bsv = __uint_as_float(best.x);
+                better = ps < bsv || (ps == bsv && p.y < best.y);
+            } else {
+                const int32_t ps = (int32_t)p.x, bsv = (int32_t)best.x;
+                better = ps > bsv || (ps == bsv && p.y < best.y);
+            }
+            if (better) best = p;
+        }
+        labels[r] = best.y == NO_LABEL ? 0u : best.y;
+        dist[r] = __uint_as_float(best.z);
+    }
 }
 
 // Many independent problems in one launch: workgroup b sweeps rows
@@ -902,7 +970,7 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
     const uint2 w = wg[blockIdx.x];
     const SearchArgs a = jobs[w.x];
     if ((size_t)w.y >= a.m) return;  // whole workgroup: before any barrier
-    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)w.y);
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)w.y, 0, (int)(a.k_pad / (32 * NB)), 0);
 }
 
 // Copies the row-major operand table into the tile-major order of the
@@ -1304,8 +1372,12 @@ bool reg_path(const SearchGeom &g) { return g.reg; }
 template <int OP, int ROWS, int M, int NS>
 static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
     constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
-    dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)));
+    dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)), (unsigned)a.nslice);
     hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || a.nslice == 1) return e;
+    hipLaunchKernelGGL((merge_slices_kernel<OP, M>), dim3((unsigned)((a.m + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t *)a.part, a.m, a.nslice, a.labels, a.dist, a.cand);
     return hipGetLastError();
 }
 
@@ -1333,12 +1405,30 @@ static hipError_t launch_reg_m(const SearchGeom &g, int M, const SearchArgs &a, 
     }
 }
 
+int reg_tiles(const SearchGeom &g) { return g.reg ? (int)(g.k_pad / reg_tile(g)) : 0; }
+
+size_t slice_part_bytes(int M, size_t m, int nslice, int op) {
+    return m * (size_t)nslice * 4 * ((op == OP_BF16 || op == OP_FP8) ? 2 * (size_t)M : 4);
+}
+
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
                          const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
-                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s) {
+                         const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
+                         int nslice, uint32_t *part) {
     if (m == 0) return hipSuccess;
     SearchArgs a;
+    memset(&a, 0, sizeof a);
+    // slices of whole tiles, none empty
+    const int nt = reg_tiles(g);
+    a.nslice = 1;
+    a.tiles_per_slice = nt;
+    if (g.reg && nslice > 1 && part) {
+        const int per = (nt + nslice - 1) / nslice;
+        a.tiles_per_slice = per;
+        a.nslice = (nt + per - 1) / per;
+        a.part = part;
+    }
     a.stats = stats;
     a.row_norm = row_norm;
     const size_t rb = rows == EMH_ROWS_F32 ? 4 : 1;
@@ -1474,6 +1564,8 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     a.labels = j.labels;
     a.dist = j.dist;
     a.cand = j.cand;
+    a.nslice = 1;
+    a.tiles_per_slice = (int)(g.k_pad / reg_tile(g));
     memcpy(dst, &a, sizeof a);
 }
 

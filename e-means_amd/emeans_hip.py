@@ -28,7 +28,7 @@ EXPORTS = [
     "emh_create", "emh_destroy", "emh_set_centroids", "emh_set_bias", "emh_name", "emh_assign_f32",
     "emh_assign_i8", "emh_assign_u8", "emh_last_error", "emh_set_stream", "emh_assign_device",
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
-    "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch",
+    "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split",
 ]
 
 
@@ -135,6 +135,7 @@ def lib():
     L.emh_train.argtypes = [vp, vp, i32, sz, sz, sz, C.POINTER(Config), vp, vp, C.POINTER(TrainOut)]
     L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
     L.emh_train_batch.argtypes = [vp, C.POINTER(Job), sz, C.POINTER(BatchStats)]
+    L.emh_set_split.argtypes = [vp, i32]
     _lib = L
     return L
 
@@ -234,6 +235,10 @@ class HipAssigner:
 
     def synchronize(self):
         _check(lib().emh_synchronize(self._h))
+
+    def set_split(self, nslice: int):
+        """Centroid-range split of the search: 0 = autotuned, n >= 1 forced."""
+        _check(lib().emh_set_split(self._h, int(nslice)))
 
     def set_timing(self, on: bool):
         _check(lib().emh_set_timing(self._h, int(on)))

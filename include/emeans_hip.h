@@ -157,6 +157,16 @@ int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size
 int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,
                    uint32_t *labels, float *dist);
 
+/* Centroid-range split of the register-resident search (no counterpart in
+ * the reference). A launch with few rows fills the chip by splitting every
+ * row block's centroid tiles over several workgroups and merging their
+ * per-row results; the results are the same for every split. nslice = 0
+ * (the default) lets the built-in autotuner time the candidate splits the
+ * first time a shape is seen and keep the fastest for the process
+ * (EMH_AUTOTUNE=0 in the environment replaces the timing by a rule);
+ * nslice >= 1 forces that many slices (fewer when the tiles run out). */
+int emh_set_split(emh_assigner *h, int nslice);
+
 /* Many independent problems, trained in lockstep (no counterpart in the
  * reference, which trains one problem per train() call, src/lib.rs:348).
  * Each job is one train() call's arguments; every job ends with the results

@@ -270,3 +270,32 @@ def test_name_and_state_errors(hip):
         a.set_bias(np.zeros(5, np.float32))
     with pytest.raises(E.EmeansError):
         E.HipAssigner(E.I8).set_centroids(np.zeros((1, 40000), np.float32))
+
+
+@pytest.mark.parametrize("prec,rows", [("fp32", "f32"), ("i8", "i8"), ("bf16", "f32"), ("fp8", "f32")])
+def test_centroid_split_does_not_change_results(hip, prec, rows):
+    """The search split over centroid slices (forced 1..8 and autotuned,
+    uneven tile counts) returns the same labels and distances bit for bit."""
+    rng = np.random.default_rng(13)
+    n, k, d = 3000, 1300, 64
+    if rows == "i8":
+        x = rng.integers(-128, 128, size=(n, d)).astype(np.int8)
+        c = rng.normal(scale=40, size=(k, d)).astype(np.float32)
+    else:
+        x = _mixture(n, d, 600, 14)
+        c = x[O.permutation(99, n)[:k]]
+    bias = rng.uniform(0, 2, size=k).astype(np.float32)
+    a = E.HipAssigner(prec, rescore_m=4)
+    ref = None
+    for ns in (1, 2, 3, 5, 8, 0):
+        print(f"{prec} split {ns}", flush=True)
+        a.set_split(ns)
+        a.set_centroids(c)
+        a.set_bias(bias)
+        got = a.assign(x)
+        if ref is None:
+            ref = got
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), f"split {ns}"
+    if prec == "i8":
+        cl, cd = O.i8_assign(x, c, bias=bias)
+        assert np.array_equal(ref[0], cl) and np.array_equal(ref[1], cd)
