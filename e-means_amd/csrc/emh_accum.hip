@@ -367,6 +367,92 @@ hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uin
     return hipGetLastError();
 }
 
+// ---- the EMA step and the revival split on device-resident centroids ----
+
+// N' = (1 - a) N + a n; c' = c + (a n / N') (s / n - c), each f32 operation
+// rounded to nearest in the order of src/lib.rs:287-309, so the centroids are
+// bit-identical to the host loop. One wave per cluster.
+__device__ __forceinline__ void ema_cluster(float *cen, float *usage, const float *sums, const uint32_t *counts,
+                                            size_t j, size_t d, float alpha, float decay, int replace, int lane) {
+    float u = usage[j];
+    if (!replace) u = __fmul_rn(u, decay);
+    const uint32_t cnt = counts[j];
+    if (cnt) {
+        const float n = (float)cnt;
+        const float retained = replace ? 0.0f : u;
+        const float an = __fmul_rn(alpha, n);
+        const float mass = __fadd_rn(retained, an);
+        const float weight = __fdiv_rn(an, mass);
+        float *c = cen + j * d;
+        const float *sm = sums + j * d;
+        for (size_t t = lane; t < d; t += 64) {
+            const float mean = __fdiv_rn(sm[t], n);
+            c[t] = weight == 1.0f ? mean : __fadd_rn(c[t], __fmul_rn(weight, __fsub_rn(mean, c[t])));
+        }
+        u = mass;
+    }
+    if (lane == 0) usage[j] = u;
+}
+
+__global__ __launch_bounds__(256) void ema_apply_kernel(float *cen, float *usage, const float *sums,
+                                                        const uint32_t *counts, size_t k, size_t d, float alpha,
+                                                        float decay, int replace) {
+    const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= k) return;
+    ema_cluster(cen, usage, sums, counts, j, d, alpha, decay, replace, threadIdx.x & 63);
+}
+
+__global__ __launch_bounds__(256) void jobs_ema_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters) {
+    const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= nclusters) return;
+    const JobDev &j = jobs[cjob[g]];
+    if (j.m == 0) return;
+    ema_cluster(j.cen, j.usage, j.sums, j.counts, g - j.cluster_base, j.d, j.alpha, j.decay, (int)j.replace,
+                threadIdx.x & 63);
+}
+
+hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s) {
+    hipLaunchKernelGGL(jobs_ema_kernel, dim3((unsigned)((nclusters + 3) / 4)), dim3(256), 0, s, jobs, cjob, nclusters);
+    return hipGetLastError();
+}
+
+// Revival (src/lib.rs:330-340): donor row c -> c (1 - eps_i), copy ->
+// c (1 + eps_i), eps_i = +1e-4 on even and -1e-4 on odd dimensions; if the
+// two rows came out equal the copy's first value moves up one ulp. One wave
+// per (donor, unused) pair; pairs never share a row.
+__global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint32_t *pairs, size_t npairs, size_t d) {
+    const size_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (t >= npairs) return;
+    float *cs = cen + (size_t)pairs[2 * t] * d, *cd = cen + (size_t)pairs[2 * t + 1] * d;
+    const float lo = 1.0f - 1e-4f, hi = 1.0f + 1e-4f;
+    bool same = true;
+    for (size_t i = lane; i < d; i += 64) {
+        const float c = cs[i];
+        const float a = __fmul_rn(c, (i % 2 == 0) ? lo : hi), b = __fmul_rn(c, (i % 2 == 0) ? hi : lo);
+        cs[i] = a;
+        cd[i] = b;
+        same = same && a == b;
+    }
+    const bool all_same = __ballot(!same) == 0;
+    if (all_same && lane == 0) cd[0] = nextafterf(cd[0], INFINITY);
+}
+
+hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
+                            float alpha, hipStream_t s) {
+    const int replace = alpha == 1.0f;
+    const float decay = 1.0f - alpha;
+    hipLaunchKernelGGL(ema_apply_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
+                       alpha, decay, replace);
+    return hipGetLastError();
+}
+
+hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s) {
+    if (!npairs) return hipSuccess;
+    hipLaunchKernelGGL(revive_split_kernel, dim3((unsigned)npairs), dim3(64), 0, s, cen, pairs, npairs, d);
+    return hipGetLastError();
+}
+
 hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, size_t k, size_t d, double *pass_sums,
                                   uint64_t *pass_counts, hipStream_t s) {
     dim3 grid((unsigned)((k + 3) / 4));

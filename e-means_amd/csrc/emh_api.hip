@@ -244,6 +244,56 @@ static int refresh_i8_range(emh_assigner *h) {
     return EMH_OK;
 }
 
+namespace emh {
+// set_centroids from host memory (synchronous: the caller may reuse the
+// table) or from device memory (stream-ordered).
+int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_centroids: NULL handle");
+    if (k == 0 || d == 0 || !centroids) return fail(EMH_E_ARG, "set_centroids: need k > 0, d > 0 and a k x d table");
+    if ((h->op == OP_I8 || h->op == OP_I16) && d > 32768)
+        return fail(EMH_E_ARG, "integer assignment supports d <= 32768");
+    if (k > 0xFFFFFFFEull) return fail(EMH_E_ARG, "set_centroids: k must fit in 32 bits");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    SearchGeom g = make_geom(h->op, k, d);
+    if ((rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
+    if ((rc = check(h->cen_op.ensure(operand_bytes(g)), "operand buffer"))) return rc;
+    if ((rc = check(h->norm32.ensure(g.k_pad * 4), "norm buffer"))) return rc;
+    if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
+    if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
+    if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
+    if ((rc = check(h->cmax.ensure(4), "scale buffer"))) return rc;
+    if ((rc = check(h->stats.ensure(12), "stats buffer"))) return rc;
+    h->g = g;
+    h->has_centroids = false;
+    h->has_bias = false;
+    if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4,
+                                   device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
+                    "copy centroids")))
+        return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
+    if (h->op == OP_FP8) {
+        if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
+        if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
+            return rc;
+    }
+    if ((rc = check(launch_prep_centroids(g, h->cen_f32.as<float>(), h->cen_op.p, h->norm32.as<float>(),
+                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
+                                          h->stream),
+                    "centroid prep")))
+        return rc;
+    if ((rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr, h->hn.as<float>(),
+                                    h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
+                    "score offsets")))
+        return rc;
+    // The caller may reuse `centroids` on return (pageable source).
+    if (!device_src && (rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;
+    if ((rc = refresh_i8_range(h))) return rc;
+    h->has_centroids = true;
+    return EMH_OK;
+}
+}  // namespace emh
+
 extern "C" {
 
 const char *emh_last_error(void) { return g_last_error.c_str(); }
@@ -336,48 +386,9 @@ int emh_get_timing(emh_assigner *h, double *ms_out, uint64_t *calls) {
     return EMH_OK;
 }
 
+
 int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t d) {
-    if (!h) return fail(EMH_E_ARG, "emh_set_centroids: NULL handle");
-    if (k == 0 || d == 0 || !centroids) return fail(EMH_E_ARG, "set_centroids: need k > 0, d > 0 and a k x d table");
-    if ((h->op == OP_I8 || h->op == OP_I16) && d > 32768)
-        return fail(EMH_E_ARG, "integer assignment supports d <= 32768");
-    if (k > 0xFFFFFFFEull) return fail(EMH_E_ARG, "set_centroids: k must fit in 32 bits");
-    int rc;
-    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
-    SearchGeom g = make_geom(h->op, k, d);
-    if ((rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
-    if ((rc = check(h->cen_op.ensure(operand_bytes(g)), "operand buffer"))) return rc;
-    if ((rc = check(h->norm32.ensure(g.k_pad * 4), "norm buffer"))) return rc;
-    if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
-    if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
-    if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
-    if ((rc = check(h->cmax.ensure(4), "scale buffer"))) return rc;
-    if ((rc = check(h->stats.ensure(12), "stats buffer"))) return rc;
-    h->g = g;
-    h->has_centroids = false;
-    h->has_bias = false;
-    if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4, hipMemcpyHostToDevice, h->stream), "copy centroids")))
-        return rc;
-    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
-    if (h->op == OP_FP8) {
-        if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
-        if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
-            return rc;
-    }
-    if ((rc = check(launch_prep_centroids(g, h->cen_f32.as<float>(), h->cen_op.p, h->norm32.as<float>(),
-                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
-                                          h->stream),
-                    "centroid prep")))
-        return rc;
-    if ((rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr, h->hn.as<float>(),
-                                    h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
-                    "score offsets")))
-        return rc;
-    // The caller may reuse `centroids` on return (pageable source).
-    if ((rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;
-    if ((rc = refresh_i8_range(h))) return rc;
-    h->has_centroids = true;
-    return EMH_OK;
+    return set_centroids_impl(h, centroids, k, d, false);
 }
 
 int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {

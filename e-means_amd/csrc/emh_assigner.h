@@ -79,5 +79,7 @@ namespace emh {
 int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev, uint32_t *labels_dev,
                float *dist_dev);
 int fail(int code, const std::string &msg);
+// set_centroids from host (synchronous) or device (stream-ordered) memory.
+int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src);
 int check(hipError_t e, const char *what);
 }  // namespace emh

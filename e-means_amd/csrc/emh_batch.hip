@@ -1,15 +1,17 @@
 // Batched training: many independent e-means problems in lockstep.
 //
 // Each job keeps the host recipe of src/lib.rs:348-491 exactly (JobRun);
-// what is shared is the device step. Every job's batch of one training step
-// is a slot range of one concatenated batch, every job's clusters a range of
-// one concatenated cluster list, and the step (upload of the staged
-// snapshots, prep of every job's centroid operand, the search per row-width
-// class, the FP32 rescore, the per-cluster sums with one stable sort, the
-// download) is captured once as a hipGraph and replayed once per step. The
-// kernels read the step's row counts and flags from a device job table that
-// the replay's first copy refreshes, so the graph never changes; jobs that
-// have finished keep their slots and idle.
+// what is shared is the device step. Every job's model (centroids, cluster
+// masses) stays on the device, every job's batch of one training step is a
+// slot range of one concatenated batch and its clusters a range of one
+// concatenated cluster list, and the step (upload of the job table, batch
+// ids and biases; prep of every job's centroid operand; the search per
+// row-width class; the FP32 rescore; the per-cluster sums with one stable
+// sort; the EMA step; download of labels, distances and masses) is captured
+// once as a hipGraph and replayed once per step. The kernels read the step's
+// row counts, rates and flags from the device job table that the replay's
+// first copy refreshes, so the graph never changes; jobs that have finished
+// keep their slots and idle. Revival splits run between replays.
 #include <chrono>
 
 #include "emh_recipe.h"
@@ -89,19 +91,24 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     }
 
     // Staged per-step input (host pinned -> device, one copy) and output
-    // (device -> host pinned, one copy), same layout on both sides.
+    // (device -> host pinned, one copy), same layout on both sides. The
+    // output block's mass region is also the device-resident masses.
     const size_t sa_bytes = search_args_bytes();
     Layout in;
     const size_t in_job = in.add(njobs * sizeof(JobDev)), in_sa = in.add(njobs * sa_bytes),
-                 in_cen = in.add(cen_floats * 4), in_bias = in.add(nclusters * 4), in_ids = in.add(slots * 4);
+                 in_bias = in.add(nclusters * 4), in_ids = in.add(slots * 4);
     Layout outl;
-    const size_t out_lab = outl.add(slots * 4), out_dist = outl.add(slots * 4), out_cnt = outl.add(nclusters * 4),
-                 out_sums = outl.add(cen_floats * 4);
-    DevBuf in_d, out_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts, maps;
-    HostBuf in_h, out_h;
+    const size_t out_lab = outl.add(slots * 4), out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
+    DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts,
+        maps, pairs_d;
+    HostBuf in_h, out_h, pairs_h;
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
         (rc = check(out_h.alloc(outl.bytes), "pinned step output")) ||
+        (rc = check(pairs_h.alloc(nclusters * 8), "pinned revival pairs")) ||
+        (rc = check(pairs_d.ensure(nclusters * 8), "revival pairs")) ||
+        (rc = check(cen_model.ensure(cen_floats * 4), "centroids")) ||
+        (rc = check(sums_d.ensure(cen_floats * 4), "sums")) || (rc = check(counts_d.ensure(nclusters * 4), "counts")) ||
         (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
         (rc = check(hn.ensure(cpad * 4), "offsets")) || (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
         (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
@@ -111,6 +118,18 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     memset(in_h.p, 0, in.bytes);
     uint8_t *in_dp = (uint8_t *)in_d.p, *out_dp = (uint8_t *)out_d.p;
     uint8_t *in_hp = (uint8_t *)in_h.p, *out_hp = (uint8_t *)out_h.p;
+    float *use_dev = (float *)(out_dp + out_use);
+    // initial models: random data rows (JobRun::init), masses 1/65536
+    for (size_t j = 0; j < njobs; j++) {
+        memcpy(out_hp + out_use + cl_base[j] * 4, run[j].st.usage.data(), jobs[j].k * 4);
+        if ((rc = check(hipMemcpyAsync(cen_model.as<float>() + cen_base[j], run[j].st.centroids.data(),
+                                       jobs[j].k * jobs[j].d * 4, hipMemcpyHostToDevice, h->stream),
+                        "upload centroids")))
+            return rc;
+    }
+    if ((rc = check(hipMemcpyAsync(use_dev, out_hp + out_use, nclusters * 4, hipMemcpyHostToDevice, h->stream),
+                    "upload masses")))
+        return rc;
 
     // Static maps: search workgroups per row-width class, prep entries, batch
     // slots, clusters.
@@ -154,8 +173,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         memset(&t, 0, sizeof t);
         t.x = rows_d[j].as<float>();
         t.ids = (const uint32_t *)(in_dp + in_ids) + row_base[j];
-        t.cen = (const float *)(in_dp + in_cen) + cen_base[j];
+        t.cen = cen_model.as<float>() + cen_base[j];
         t.bias = (const float *)(in_dp + in_bias) + cl_base[j];
+        t.usage = use_dev + cl_base[j];
         t.cen_op = (uint8_t *)cen_op.p + op_base[j];
         t.norm32 = norm32.as<float>() + cpad_base[j];
         t.hn = hn.as<float>() + cpad_base[j];
@@ -163,8 +183,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.labels = (uint32_t *)(out_dp + out_lab) + row_base[j];
         t.dist = (float *)(out_dp + out_dist) + row_base[j];
         t.cand = cand.as<uint32_t>() + row_base[j] * M;
-        t.sums = (float *)(out_dp + out_sums) + cen_base[j];
-        t.counts = (uint32_t *)(out_dp + out_cnt) + cl_base[j];
+        t.sums = sums_d.as<float>() + cen_base[j];
+        t.counts = counts_d.as<uint32_t>() + cl_base[j];
         t.pass_sums = pass_sums.as<double>() + cen_base[j];
         t.pass_counts = pass_counts.as<uint64_t>() + cl_base[j];
         t.d = (uint32_t)jobs[j].d;
@@ -191,7 +211,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess)
         ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
-                                    nclusters, keys.as<uint32_t>(), (uint32_t *)(out_dp + out_cnt), h->stream);
+                                    nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+    if (ce == hipSuccess) ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, h->stream);
     if (ce == hipSuccess) ce = hipMemcpyAsync(out_hp, out_dp, outl.bytes, hipMemcpyDeviceToHost, h->stream);
     hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);
     if ((rc = check(ce, "capture step")) || (rc = check(ee, "end capture"))) return rc;
@@ -201,6 +222,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
 
     // Lockstep steps.
     uint64_t steps = 0;
+    std::vector<std::pair<size_t, size_t>> pairs;
     std::vector<char> live(njobs);
     JobDev *jd_h = (JobDev *)(in_hp + in_job);
     for (;;) {
@@ -211,13 +233,15 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
             live[j] = J.next();
             if (live[j]) {
                 any = true;
-                memcpy(in_hp + in_cen + cen_base[j] * 4, J.st.centroids.data(), J.k * J.d * 4);
                 if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
                 memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
                 t.m = (uint32_t)J.nb;
                 t.penalized = J.penalized;
                 t.last_epoch = J.is_last;
                 t.clear_pass = J.last_epoch_start;
+                t.alpha = J.alpha();
+                t.decay = 1.0f - t.alpha;
+                t.replace = t.alpha == 1.0f;
             }
             jd_h[j] = t;
             fill_search_args(in_hp + in_sa + j * sa_bytes, geo[j], t);
@@ -227,22 +251,50 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
             (rc = check(hipStreamSynchronize(h->stream), "step")))
             return rc;
         steps++;
+        // host half per job; revival planned on the downloaded masses, rows
+        // split on the device before the next replay
+        uint32_t *pp = pairs_h.as<uint32_t>();
+        size_t pbase = 0;
+        bool revived = false;
         for (size_t j = 0; j < njobs; j++) {
             if (!live[j]) continue;
-            run[j].finish_batch((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j],
-                                (const uint32_t *)(out_hp + out_cnt) + cl_base[j], (const float *)(out_hp + out_sums) + cen_base[j]);
+            JobRun &J = run[j];
+            memcpy(J.st.usage.data(), out_hp + out_use + cl_base[j] * 4, J.k * 4);
+            J.finish_stats((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j]);
+            if (J.revive_due()) {
+                J.st.revive_plan(J.R.unused_threshold, J.split_rng, pairs);
+                if (!pairs.empty()) {
+                    for (size_t t = 0; t < pairs.size(); t++) {
+                        pp[2 * (pbase + t)] = (uint32_t)pairs[t].first;
+                        pp[2 * (pbase + t) + 1] = (uint32_t)pairs[t].second;
+                    }
+                    memcpy(out_hp + out_use + cl_base[j] * 4, J.st.usage.data(), J.k * 4);
+                    if ((rc = check(hipMemcpyAsync(pairs_d.as<uint32_t>() + 2 * pbase, pp + 2 * pbase, pairs.size() * 8,
+                                                   hipMemcpyHostToDevice, h->stream),
+                                    "revival pairs")) ||
+                        (rc = check(hipMemcpyAsync(use_dev + cl_base[j], out_hp + out_use + cl_base[j] * 4, J.k * 4,
+                                                   hipMemcpyHostToDevice, h->stream),
+                                    "masses")) ||
+                        (rc = check(launch_revive_split(cen_model.as<float>() + cen_base[j], pairs_d.as<uint32_t>() + 2 * pbase,
+                                                        pairs.size(), J.d, h->stream),
+                                    "revival")))
+                        return rc;
+                    pbase += pairs.size();
+                    revived = true;
+                }
+            }
+            J.advance();
         }
+        // the staging of the revival copies must not be reused before they ran
+        if (revived && (rc = check(hipStreamSynchronize(h->stream), "revival"))) return rc;
     }
 
-    // Closing means (src/lib.rs:481-489) from the FP64 pass totals, per job;
-    // the staged centroid area holds each job's final EMA centroids.
-    float *cen_dev = (float *)(in_dp + in_cen);
+    // Closing means (src/lib.rs:481-489) from the FP64 pass totals, per job,
+    // over the final EMA centroids.
+    float *cen_dev = cen_model.as<float>();
     for (size_t j = 0; j < njobs; j++) {
         const size_t k = jobs[j].k, d = jobs[j].d;
-        if ((rc = check(hipMemcpyAsync(cen_dev + cen_base[j], run[j].st.centroids.data(), k * d * 4,
-                                       hipMemcpyHostToDevice, h->stream),
-                        "copy centroids")) ||
-            (rc = check(launch_closing_mean(jd[j].pass_sums, jd[j].pass_counts, k, d, cen_dev + cen_base[j], h->stream),
+        if ((rc = check(launch_closing_mean(jd[j].pass_sums, jd[j].pass_counts, k, d, cen_dev + cen_base[j], h->stream),
                         "closing mean")) ||
             (rc = check(hipMemcpyAsync(jobs[j].centroids_out, cen_dev + cen_base[j], k * d * 4, hipMemcpyDeviceToHost,
                                        h->stream),

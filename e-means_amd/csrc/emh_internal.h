@@ -94,8 +94,9 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 struct JobDev {
     const float *x;          // resident rows, n x d f32
     const uint32_t *ids;     // this step's batch row positions
-    const float *cen;        // this step's centroid snapshot, k x d
+    float *cen;              // the job's device-resident centroids, k x d
     const float *bias;       // this step's balance bias, k (read when penalized)
+    float *usage;            // the job's device-resident cluster masses, k
     uint8_t *cen_op;         // tile-major operand table (register-resident search)
     float *norm32, *hn;      // k_pad each
     unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias (f32 bits)
@@ -113,6 +114,8 @@ struct JobDev {
     uint32_t penalized;      // balance bias applies this step
     uint32_t last_epoch;     // fold this step's sums into the FP64 pass totals
     uint32_t clear_pass;     // first batch of the last epoch: zero the pass totals first
+    float alpha, decay;      // this step's EMA rate and 1 - rate
+    uint32_t replace;        // alpha == 1
 };
 
 // prep: one wave per (job, centroid < k_pad) of cmap; operand, norms,
@@ -145,9 +148,16 @@ hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, siz
 hipError_t launch_closing_mean(const double *pass_sums, const uint64_t *pass_counts, size_t k, size_t d,
                                float *centroids, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s);
+// The EMA step of src/lib.rs:287-309 on device centroids/usage (k x d, k).
+hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
+                            float alpha, hipStream_t s);
+// The revival row split of src/lib.rs:330-340 for (donor, unused) pairs.
+hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s);
 // Batched: keys = cluster_base + label for live slots (nclusters for spare
 // slots), counts per global cluster, stable sort, then per-job f32 sums and
 // FP64 pass totals. cjob maps a global cluster to its job.
+// The EMA step of every job still running, one wave per global cluster.
+hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
 hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
                                   const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
                                   hipStream_t s);

@@ -157,8 +157,11 @@ struct State {
         }
     }
 
-    // Gumbel-top-k donor split of clusters below tau * mean mass (src/lib.rs:314-344)
-    void revive(float tau, Rng &rng) {
+    // Gumbel-top-k donor split of clusters below tau * mean mass
+    // (src/lib.rs:314-344), planned on `usage` (updated here): the (donor,
+    // unused) pairs whose centroid rows split_rows then splits.
+    void revive_plan(float tau, Rng &rng, std::vector<std::pair<size_t, size_t>> &pairs) {
+        pairs.clear();
         float total = 0.0f;
         for (float u : usage) total += u;
         const float threshold = total * tau / (float)k;
@@ -176,6 +179,18 @@ struct State {
         for (size_t t = 0; t < nsplit; t++) {
             const size_t src = donors[t].second, dst = unused[t];
             const float half = usage[src] / 2.0f;
+            usage[src] = half;
+            usage[dst] = half;
+            pairs.emplace_back(src, dst);
+        }
+        n_splits += nsplit;
+    }
+
+    // donor row c -> c (1 - eps_i) and c (1 + eps_i) with eps_i = +-1e-4 by
+    // parity, nudging the copy up one ulp if both came out equal
+    void split_rows(const std::vector<std::pair<size_t, size_t>> &pairs) {
+        for (const auto &pr : pairs) {
+            const size_t src = pr.first, dst = pr.second;
             for (size_t i = 0; i < d; i++) {
                 const float offset = (i % 2 == 0) ? SPLIT_EPSILON : -SPLIT_EPSILON;
                 const float c = centroids[src * d + i];
@@ -185,10 +200,13 @@ struct State {
             bool same = true;
             for (size_t i = 0; i < d; i++) same = same && centroids[src * d + i] == centroids[dst * d + i];
             if (same) centroids[dst * d] = nextafterf(centroids[dst * d], INFINITY);
-            usage[src] = half;
-            usage[dst] = half;
         }
-        n_splits += nsplit;
+    }
+
+    void revive(float tau, Rng &rng) {
+        std::vector<std::pair<size_t, size_t>> pairs;
+        revive_plan(tau, rng, pairs);
+        split_rows(pairs);
     }
 };
 
@@ -302,9 +320,8 @@ struct JobRun {
         return true;
     }
 
-    // The host half of the batch from its labels, distances (biased when
-    // penalized; corrected here), per-cluster counts and f32 sums.
-    void finish_batch(const uint32_t *lab, float *dist, const uint32_t *counts, const float *sums) {
+    // Objective, true distances, last-pass labels (src/lib.rs:425-440).
+    void finish_stats(const uint32_t *lab, float *dist) {
         double batch_obj = 0.0;
         for (size_t p0 = 0; p0 < nb; p0 += chunk) {
             const size_t m = std::min(chunk, nb - p0);
@@ -323,12 +340,15 @@ struct JobRun {
             for (size_t r = 0; r < nb; r++) labels_out[ids[r]] = lab[r];
         epoch_obj += batch_obj;
         mean_sqdist = batch_obj / (double)nb;
-        touched.clear();
-        for (size_t j = 0; j < k; j++)
-            if (counts[j]) touched.push_back((uint32_t)j);
-        const float alpha = (float)scheduled_alpha(R.alpha0, step, R.total_steps);
-        st.apply(touched, counts, sums, alpha);
-        if ((step + 1) % CHECK_UNUSED_EVERY == 0 && R.unused_threshold > 0.0f) st.revive(R.unused_threshold, split_rng);
+    }
+
+    // This step's EMA rate (src/lib.rs:455) and whether revival follows it
+    // (src/lib.rs:463-465).
+    float alpha() const { return (float)scheduled_alpha(R.alpha0, step, R.total_steps); }
+    bool revive_due() const { return (step + 1) % CHECK_UNUSED_EVERY == 0 && R.unused_threshold > 0.0f; }
+
+    // Step, batch position and epoch bookkeeping after a batch.
+    void advance() {
         step++;
         b0 += nb;
         if (b0 >= R.epoch_rows[epoch]) {
@@ -337,6 +357,19 @@ struct JobRun {
             b0 = 0;
             if (epoch == R.n_epochs) done = true;
         }
+    }
+
+    // The host half of the batch with the EMA step on the host copy of the
+    // model: labels, distances (biased when penalized; corrected here),
+    // per-cluster counts and f32 sums.
+    void finish_batch(const uint32_t *lab, float *dist, const uint32_t *counts, const float *sums) {
+        finish_stats(lab, dist);
+        touched.clear();
+        for (size_t j = 0; j < k; j++)
+            if (counts[j]) touched.push_back((uint32_t)j);
+        st.apply(touched, counts, sums, alpha());
+        if (revive_due()) st.revive(R.unused_threshold, split_rng);
+        advance();
     }
 };
 

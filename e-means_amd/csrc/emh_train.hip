@@ -1,12 +1,16 @@
 // The e-means training driver over the HIP assigner.
 //
 // The recipe stays on the host exactly as the reference crate has it
-// (src/lib.rs:165-491): recipe laws, xoshiro256++ batch order, EMA update,
-// revival and the closing mean. What moves to the device is the data path of
-// each batch: the rows are uploaded once, and per batch only the ids (4 bytes
-// per row) go up; the gather, the search and the per-cluster sums/counts run
-// on the GPU, and the per-cluster totals of the last pass accumulate in FP64
-// on the GPU.
+// (src/lib.rs:165-491): recipe laws, xoshiro256++ batch order, step
+// schedule, balance bias and the revival plan. What moves to the device is
+// the data path of each batch and the model it updates: the rows are
+// uploaded once and per batch only the ids (4 bytes per row) go up; the
+// gather, the search, the per-cluster sums/counts and the EMA step (f32
+// operations in the host loop's order, so bit-identical) run on the GPU
+// against device-resident centroids; the last pass's totals accumulate in
+// FP64 on the GPU for the closing mean. Labels and distances come down per
+// batch (objective, last-pass labels), the usage vector only when the bias
+// or a revival needs it.
 #include <chrono>
 
 #include "emh_recipe.h"
@@ -25,8 +29,11 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     const auto t0 = std::chrono::steady_clock::now();
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
 
-    // Device-resident rows and per-batch buffers.
-    DevBuf data, ids_d, lab_d, dist_d, sums_d, counts_d, pass_d, passn_d, cent_d;
+    // Device-resident rows, model (centroids k x d, usage k) and per-batch
+    // buffers. The EMA step runs on the device; the host keeps the recipe
+    // and pulls the usage vector down when the balance bias or a revival
+    // needs it.
+    DevBuf data, ids_d, lab_d, dist_d, sums_d, counts_d, pass_d, passn_d, cent_d, usage_d, pairs_d;
     if ((rc = check(data.ensure(n * d * rb), "resident rows"))) return rc;
     if ((rc = check(hipMemcpyAsync(data.p, x, n * d * rb, hipMemcpyHostToDevice, h->stream), "upload rows"))) return rc;
     if ((rc = check(ids_d.ensure(batch * 4), "ids"))) return rc;
@@ -37,12 +44,20 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     if ((rc = check(pass_d.ensure(k * d * 8), "pass sums"))) return rc;
     if ((rc = check(passn_d.ensure(k * 8), "pass counts"))) return rc;
     if ((rc = check(cent_d.ensure(k * d * 4), "centroids"))) return rc;
-    HostBuf ids_h, lab_h, dist_h, sums_h, counts_h;
+    if ((rc = check(usage_d.ensure(k * 4), "usage"))) return rc;
+    if ((rc = check(pairs_d.ensure(k * 8), "revival pairs"))) return rc;
+    HostBuf ids_h, lab_h, dist_h, usage_h, pairs_h;
     if ((rc = check(ids_h.alloc(batch * 4), "pinned ids"))) return rc;
     if ((rc = check(lab_h.alloc(batch * 4), "pinned labels"))) return rc;
     if ((rc = check(dist_h.alloc(batch * 4), "pinned distances"))) return rc;
-    if ((rc = check(sums_h.alloc(k * d * 4), "pinned sums"))) return rc;
-    if ((rc = check(counts_h.alloc(k * 4), "pinned counts"))) return rc;
+    if ((rc = check(usage_h.alloc(k * 4), "pinned usage"))) return rc;
+    if ((rc = check(pairs_h.alloc(k * 8), "pinned pairs"))) return rc;
+    memcpy(usage_h.p, J.st.usage.data(), k * 4);
+    if ((rc = check(hipMemcpyAsync(cent_d.p, J.st.centroids.data(), k * d * 4, hipMemcpyHostToDevice, h->stream),
+                    "upload centroids")) ||
+        (rc = check(hipMemcpyAsync(usage_d.p, usage_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "upload usage")))
+        return rc;
+    std::vector<std::pair<size_t, size_t>> pairs;
 
     while (J.next()) {
         const size_t nb = J.nb;
@@ -51,7 +66,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
             if ((rc = check(launch_fill_f64((double *)passn_d.p, k, h->stream), "clear pass counts"))) return rc;
         }
         // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
-        if ((rc = emh_set_centroids(h, J.st.centroids.data(), k, d))) return rc;
+        if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true))) return rc;
         if (J.penalized && (rc = emh_set_bias(h, J.bias.data(), k))) return rc;
         memcpy(ids_h.p, J.ids, nb * 4);
         if ((rc = check(hipMemcpyAsync(ids_d.p, ids_h.p, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
@@ -69,19 +84,45 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                                                passn_d.as<uint64_t>(), h->stream),
                         "pass accumulate")))
             return rc;
+        // the EMA step (src/lib.rs:455-456)
+        if ((rc = check(launch_ema_apply(cent_d.as<float>(), usage_d.as<float>(), sums_d.as<float>(),
+                                         counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream),
+                        "EMA step")))
+            return rc;
+        const bool revive = J.revive_due();
+        const bool need_usage = revive || cfg.balance > 0.0f;
         if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
             (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
-            (rc = check(hipMemcpyAsync(counts_h.p, counts_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "counts")) ||
-            (rc = check(hipMemcpyAsync(sums_h.p, sums_d.p, k * d * 4, hipMemcpyDeviceToHost, h->stream), "sums")) ||
+            (need_usage &&
+             (rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage"))) ||
             (rc = check(hipStreamSynchronize(h->stream), "batch")))
             return rc;
-        J.finish_batch(lab_h.as<uint32_t>(), dist_h.as<float>(), counts_h.as<uint32_t>(), sums_h.as<float>());
+        if (need_usage) memcpy(J.st.usage.data(), usage_h.p, k * 4);
+        J.finish_stats(lab_h.as<uint32_t>(), dist_h.as<float>());
+        if (revive) {
+            // src/lib.rs:463-465: planned on the host (its RNG), rows split on the device
+            J.st.revive_plan(J.R.unused_threshold, J.split_rng, pairs);
+            if (!pairs.empty()) {
+                uint32_t *pp = pairs_h.as<uint32_t>();
+                for (size_t t = 0; t < pairs.size(); t++) {
+                    pp[2 * t] = (uint32_t)pairs[t].first;
+                    pp[2 * t + 1] = (uint32_t)pairs[t].second;
+                }
+                memcpy(usage_h.p, J.st.usage.data(), k * 4);
+                if ((rc = check(hipMemcpyAsync(pairs_d.p, pairs_h.p, pairs.size() * 8, hipMemcpyHostToDevice, h->stream),
+                                "revival pairs")) ||
+                    (rc = check(hipMemcpyAsync(usage_d.p, usage_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "usage")) ||
+                    (rc = check(launch_revive_split(cent_d.as<float>(), pairs_d.as<uint32_t>(), pairs.size(), d, h->stream),
+                                "revival")) ||
+                    (rc = check(hipStreamSynchronize(h->stream), "revival")))
+                    return rc;
+            }
+        }
+        J.advance();
     }
 
     // Closing mean on the device from the FP64 pass totals (src/lib.rs:481-489).
-    if ((rc = check(hipMemcpyAsync(cent_d.p, J.st.centroids.data(), k * d * 4, hipMemcpyHostToDevice, h->stream),
-                    "copy centroids")) ||
-        (rc = check(launch_closing_mean(pass_d.as<double>(), passn_d.as<uint64_t>(), k, d, cent_d.as<float>(),
+    if ((rc = check(launch_closing_mean(pass_d.as<double>(), passn_d.as<uint64_t>(), k, d, cent_d.as<float>(),
                                         h->stream),
                     "closing mean")) ||
         (rc = check(hipMemcpyAsync(centroids_out, cent_d.p, k * d * 4, hipMemcpyDeviceToHost, h->stream),
@@ -91,7 +132,6 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     out->n_splits = J.st.n_splits;
     out->recipe = J.R;
     out->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (DevBuf *b : {&data, &ids_d, &lab_d, &dist_d, &sums_d, &counts_d, &pass_d, &passn_d, &cent_d}) b->release();
     return EMH_OK;
 }
 
