@@ -377,17 +377,19 @@ def train_batch_line(args, njobs, prec, m):
     t = time.perf_counter()
     got, st = E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
     bs = time.perf_counter() - t
-    log("per-job fp32 loop")
-    t = time.perf_counter()
-    ref = [E.train(x, k, cfg, E.HipAssigner("fp32")) for x, k, cfg in probs]
-    ls = time.perf_counter() - t
-    log("exact re-assignment MSE")
-    rel = []
-    for (x, k, cfg), g, r in zip(probs, got, ref):
-        a = E.assign(x, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
-        b = E.assign(x, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
-        rel.append(a / b - 1.0)
-    loop_steps = sum(int(r.recipe.total_steps) for r in ref)
+    rel, ls, loop_steps = [0.0], float("nan"), 0
+    if not args.no_compare:
+        log("per-job fp32 loop")
+        t = time.perf_counter()
+        ref = [E.train(x, k, cfg, E.HipAssigner("fp32")) for x, k, cfg in probs]
+        ls = time.perf_counter() - t
+        log("exact re-assignment MSE")
+        rel = []
+        for (x, k, cfg), g, r in zip(probs, got, ref):
+            a = E.assign(x, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+            b = E.assign(x, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+            rel.append(a / b - 1.0)
+        loop_steps = sum(int(r.recipe.total_steps) for r in ref)
     res = {
         "metric": "train rows/s (sum over jobs of epochs x n / wall time)",
         "value": round(rows / bs, 1),

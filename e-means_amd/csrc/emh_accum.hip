@@ -59,14 +59,41 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t 
     if (jc >= k) return;
     const uint32_t n = counts[jc];
     const uint32_t s0 = n ? start[jc] : 0;
-    for (size_t t = lane; t < d; t += 64) {
-        float acc = 0.0f;
-        for (uint32_t q = 0; q < n; q++) {
-            size_t r = sorted_pos[s0 + q];
-            size_t src = ids ? (size_t)ids[r] : r;
-            acc = __fadd_rn(acc, row_elem<ROWS>(x, src * d + t));
+    // one pass over the members per 256 dimensions (4 per lane); member rows
+    // fetched 64 at a time (one per lane), 64 values in flight ahead of the
+    // in-order adds
+    for (size_t tg = 0; tg < d; tg += 256) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t q0 = 0; q0 < n; q0 += 64) {
+            const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
+            size_t my_src = 0;
+            if ((uint32_t)lane < cnt) {
+                const size_t r = sorted_pos[s0 + q0 + lane];
+                my_src = ids ? (size_t)ids[r] : r;
+            }
+            for (uint32_t u0 = 0; u0 < cnt; u0 += 16) {
+                float v[4][16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
+#pragma unroll
+                    for (int c = 0; c < 4; c++) {
+                        const size_t t = tg + 64 * c + lane;
+                        v[c][u] = (t < d && u0 + u < cnt) ? row_elem<ROWS>(x, src * d + t) : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                    if (u0 + u < cnt)
+#pragma unroll
+                        for (int c = 0; c < 4; c++) acc[c] = __fadd_rn(acc[c], v[c][u]);
+            }
         }
-        sums[jc * d + t] = acc;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const size_t t = tg + 64 * c + lane;
+            if (t < d) sums[jc * d + t] = acc[c];
+        }
     }
 }
 
@@ -118,29 +145,48 @@ __global__ __launch_bounds__(256) void digit_hist_kernel(const uint32_t *keys, s
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// In-place exclusive scan of count entries by one workgroup of 1024 threads:
-// each thread scans a contiguous slice, then the slice totals are scanned.
-__global__ __launch_bounds__(1024) void exclusive_scan_kernel(uint32_t *v, size_t count) {
-    __shared__ uint32_t tot[1024];
-    const size_t per = (count + 1023) / 1024;
-    const size_t a = (size_t)threadIdx.x * per, b = std::min(count, a + per);
-    uint32_t s = 0;
-    for (size_t i = a; i < b; i++) s += v[i];
-    tot[threadIdx.x] = s;
+// Exclusive scan of the digit-major histogram (hist[digit * nblocks + block])
+// in two launches of 256 workgroups: each digit's row over the blocks, then
+// each row plus the total of the smaller digits.
+__device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sm, uint32_t *total) {
+    // 256 threads: wave prefix by shuffles, then the 4 wave totals
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_up((int)inc, off, 64);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) sm[w] = inc;
     __syncthreads();
-    // Hillis-Steele over the 1024 slice totals
-    for (int off = 1; off < 1024; off <<= 1) {
-        const uint32_t add = threadIdx.x >= (unsigned)off ? tot[threadIdx.x - off] : 0u;
-        __syncthreads();
-        tot[threadIdx.x] += add;
-        __syncthreads();
+    uint32_t base = 0;
+    for (int q = 0; q < w; q++) base += sm[q];
+    *total = sm[0] + sm[1] + sm[2] + sm[3];
+    __syncthreads();
+    return base + inc - v;
+}
+
+__global__ __launch_bounds__(256) void digit_row_scan_kernel(uint32_t *hist, size_t nblocks, uint32_t *totals) {
+    __shared__ uint32_t sm[4];
+    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+    uint32_t carry = 0;
+    for (size_t b0 = 0; b0 < nblocks; b0 += 256) {
+        const size_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nblocks ? row[b] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, sm, &tot);
+        if (b < nblocks) row[b] = carry + ex;
+        carry += tot;
     }
-    uint32_t run = tot[threadIdx.x] - s;  // exclusive prefix of this slice
-    for (size_t i = a; i < b; i++) {
-        const uint32_t c = v[i];
-        v[i] = run;
-        run += c;
-    }
+    if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
+__global__ __launch_bounds__(256) void digit_base_kernel(uint32_t *hist, size_t nblocks, const uint32_t *totals) {
+    __shared__ uint32_t sm[4];
+    const uint32_t v = threadIdx.x < blockIdx.x ? totals[threadIdx.x] : 0u;
+    uint32_t base;
+    (void)block_exclusive_scan(v, sm, &base);  // base = sum of the smaller digits' totals
+    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
+    for (size_t b = threadIdx.x; b < nblocks; b += 256) row[b] += base;
 }
 
 // Stable scatter: an element's destination is the start of its digit in its
@@ -198,7 +244,7 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         if ((e = hipMalloc(&a.vals, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.vals_alt, m * 4)) != hipSuccess) return e;
         const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
-        a.sort_tmp_bytes = 256 * nblocks * 4;
+        a.sort_tmp_bytes = 256 * nblocks * 4 + 256 * 4;  // histogram + digit totals
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
         a.cap_rows = m;
     }
@@ -231,7 +277,9 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, size_t m,
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, shift, hist, nblocks);
-        hipLaunchKernelGGL(exclusive_scan_kernel, dim3(1), dim3(1024), 0, s, hist, 256 * nblocks);
+        hipLaunchKernelGGL(digit_row_scan_kernel, dim3(256), dim3(256), 0, s, hist, nblocks, hist + 256 * nblocks);
+        hipLaunchKernelGGL(digit_base_kernel, dim3(256), dim3(256), 0, s, hist, nblocks,
+                           (const uint32_t *)(hist + 256 * nblocks));
         hipLaunchKernelGGL(digit_scatter_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m, shift,
                            (const uint32_t *)hist, nblocks, kb[cur], vb[cur]);
         kin = kb[cur];
@@ -328,14 +376,38 @@ __global__ __launch_bounds__(256) void jobs_segment_sum_kernel(const JobDev *job
     const size_t c = g - j.cluster_base, d = j.d;
     const uint32_t n = counts_all[g];
     const uint32_t s0 = n ? start[g] : 0;
-    for (size_t t = lane; t < d; t += 64) {
-        float acc = 0.0f;
-        for (uint32_t q = 0; q < n; q++) {
-            const size_t r = sorted_slot[s0 + q] - j.row_base;
-            acc = __fadd_rn(acc, j.x[(size_t)j.ids[r] * d + t]);
+    for (size_t tg = 0; tg < d; tg += 256) {
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        for (uint32_t q0 = 0; q0 < n; q0 += 64) {
+            const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
+            size_t my_src = 0;
+            if ((uint32_t)lane < cnt) my_src = (size_t)j.ids[sorted_slot[s0 + q0 + lane] - j.row_base];
+            for (uint32_t u0 = 0; u0 < cnt; u0 += 16) {
+                float v[4][16];
+#pragma unroll
+                for (int u = 0; u < 16; u++) {
+                    const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
+#pragma unroll
+                    for (int cc = 0; cc < 4; cc++) {
+                        const size_t t = tg + 64 * cc + lane;
+                        v[cc][u] = (t < d && u0 + u < cnt) ? j.x[src * d + t] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                    if (u0 + u < cnt)
+#pragma unroll
+                        for (int cc = 0; cc < 4; cc++) acc[cc] = __fadd_rn(acc[cc], v[cc][u]);
+            }
         }
-        j.sums[c * d + t] = acc;
-        if (j.last_epoch && n) j.pass_sums[c * d + t] += (double)acc;
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            const size_t t = tg + 64 * cc + lane;
+            if (t < d) {
+                j.sums[c * d + t] = acc[cc];
+                if (j.last_epoch && n) j.pass_sums[c * d + t] += (double)acc[cc];
+            }
+        }
     }
     if (lane == 0 && j.last_epoch && n) j.pass_counts[c] += n;
 }
@@ -444,6 +516,34 @@ hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const u
     const float decay = 1.0f - alpha;
     hipLaunchKernelGGL(ema_apply_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
                        alpha, decay, replace);
+    return hipGetLastError();
+}
+
+// Revival splits of many jobs: triple t = (job, donor, unused).
+__global__ __launch_bounds__(64) void jobs_revive_split_kernel(const JobDev *jobs, const uint32_t *triples,
+                                                              size_t ntriples) {
+    const size_t t = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (t >= ntriples) return;
+    const JobDev &j = jobs[triples[3 * t]];
+    const size_t d = j.d;
+    float *cs = j.cen + (size_t)triples[3 * t + 1] * d, *cd = j.cen + (size_t)triples[3 * t + 2] * d;
+    const float lo = 1.0f - 1e-4f, hi = 1.0f + 1e-4f;
+    bool same = true;
+    for (size_t i = lane; i < d; i += 64) {
+        const float c = cs[i];
+        const float a = __fmul_rn(c, (i % 2 == 0) ? lo : hi), b = __fmul_rn(c, (i % 2 == 0) ? hi : lo);
+        cs[i] = a;
+        cd[i] = b;
+        same = same && a == b;
+    }
+    const bool all_same = __ballot(!same) == 0;
+    if (all_same && lane == 0) cd[0] = nextafterf(cd[0], INFINITY);
+}
+
+hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_t ntriples, hipStream_t s) {
+    if (!ntriples) return hipSuccess;
+    hipLaunchKernelGGL(jobs_revive_split_kernel, dim3((unsigned)ntriples), dim3(64), 0, s, jobs, triples, ntriples);
     return hipGetLastError();
 }
 

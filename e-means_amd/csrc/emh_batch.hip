@@ -12,7 +12,12 @@
 // row counts, rates and flags from the device job table that the replay's
 // first copy refreshes, so the graph never changes; jobs that have finished
 // keep their slots and idle. Revival splits run between replays.
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
+#include <functional>
+#include <mutex>
+#include <thread>
 
 #include "emh_recipe.h"
 
@@ -30,6 +35,71 @@ struct Layout {
         bytes = at + (n ? n : 1);
         return at;
     }
+};
+
+// Runs fn(0..n-1) on a few persistent host threads plus the caller; each
+// index runs exactly once, and run() returns when all have.
+class HostPool {
+  public:
+    explicit HostPool(unsigned nthreads) {
+        for (unsigned t = 0; t < nthreads; t++) th_.emplace_back([this] { work(); });
+    }
+    ~HostPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            gen_++;
+        }
+        cv_.notify_all();
+        for (auto &t : th_) t.join();
+    }
+    void run(size_t n, const std::function<void(size_t)> &fn) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            fn_ = &fn;
+            n_ = n;
+            next_ = 0;
+            done_ = 0;
+            gen_++;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return done_ == n_; });
+        fn_ = nullptr;
+    }
+
+  private:
+    void drain() {
+        for (;;) {
+            const size_t i = next_.fetch_add(1);
+            if (i >= n_) return;
+            (*fn_)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (++done_ == n_) done_cv_.notify_all();
+        }
+    }
+    void work() {
+        uint64_t seen = 0;
+        for (;;) {
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                cv_.wait(lk, [&] { return gen_ != seen; });
+                seen = gen_;
+                if (stop_) return;
+                if (!fn_) continue;
+            }
+            drain();
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(size_t)> *fn_ = nullptr;
+    size_t n_ = 0, done_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
 };
 
 struct GraphGuard {
@@ -105,8 +175,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
         (rc = check(out_h.alloc(outl.bytes), "pinned step output")) ||
-        (rc = check(pairs_h.alloc(nclusters * 8), "pinned revival pairs")) ||
-        (rc = check(pairs_d.ensure(nclusters * 8), "revival pairs")) ||
+        (rc = check(pairs_h.alloc(nclusters * 12), "pinned revival triples")) ||
+        (rc = check(pairs_d.ensure(nclusters * 12), "revival triples")) ||
         (rc = check(cen_model.ensure(cen_floats * 4), "centroids")) ||
         (rc = check(sums_d.ensure(cen_floats * 4), "sums")) || (rc = check(counts_d.ensure(nclusters * 4), "counts")) ||
         (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
@@ -220,72 +290,72 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     size_t nodes = 0;
     (void)hipGraphGetNodes(gg.graph, nullptr, &nodes);
 
-    // Lockstep steps.
+    // Lockstep steps. The host halves of the jobs are independent (own RNG,
+    // own staging slots) and run on a few host threads; revival triples are
+    // gathered in job order.
     uint64_t steps = 0;
-    std::vector<std::pair<size_t, size_t>> pairs;
     std::vector<char> live(njobs);
+    std::vector<std::vector<std::pair<size_t, size_t>>> jpairs(njobs);
     JobDev *jd_h = (JobDev *)(in_hp + in_job);
-    for (;;) {
-        bool any = false;
-        for (size_t j = 0; j < njobs; j++) {
-            JobRun &J = run[j];
-            JobDev t = jd[j];
-            live[j] = J.next();
-            if (live[j]) {
-                any = true;
-                if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
-                memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
-                t.m = (uint32_t)J.nb;
-                t.penalized = J.penalized;
-                t.last_epoch = J.is_last;
-                t.clear_pass = J.last_epoch_start;
-                t.alpha = J.alpha();
-                t.decay = 1.0f - t.alpha;
-                t.replace = t.alpha == 1.0f;
-            }
-            jd_h[j] = t;
-            fill_search_args(in_hp + in_sa + j * sa_bytes, geo[j], t);
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    HostPool pool(std::min<unsigned>({hw, 16u, (unsigned)njobs}) - 1);
+    const std::function<void(size_t)> stage = [&](size_t j) {
+        JobRun &J = run[j];
+        JobDev t = jd[j];
+        live[j] = J.next();
+        if (live[j]) {
+            if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
+            memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
+            t.m = (uint32_t)J.nb;
+            t.penalized = J.penalized;
+            t.last_epoch = J.is_last;
+            t.clear_pass = J.last_epoch_start;
+            t.alpha = J.alpha();
+            t.decay = 1.0f - t.alpha;
+            t.replace = t.alpha == 1.0f;
         }
-        if (!any) break;
+        jd_h[j] = t;
+        fill_search_args(in_hp + in_sa + j * sa_bytes, geo[j], t);
+    };
+    const std::function<void(size_t)> finish = [&](size_t j) {
+        jpairs[j].clear();
+        if (!live[j]) return;
+        JobRun &J = run[j];
+        memcpy(J.st.usage.data(), out_hp + out_use + cl_base[j] * 4, J.k * 4);
+        J.finish_stats((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j]);
+        if (J.revive_due()) {
+            J.st.revive_plan(J.R.unused_threshold, J.split_rng, jpairs[j]);
+            if (!jpairs[j].empty()) memcpy(out_hp + out_use + cl_base[j] * 4, J.st.usage.data(), J.k * 4);
+        }
+        J.advance();
+    };
+    for (;;) {
+        pool.run(njobs, stage);
+        if (std::find(live.begin(), live.end(), (char)1) == live.end()) break;
         if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch")) ||
             (rc = check(hipStreamSynchronize(h->stream), "step")))
             return rc;
         steps++;
         // host half per job; revival planned on the downloaded masses, rows
-        // split on the device before the next replay
-        uint32_t *pp = pairs_h.as<uint32_t>();
-        size_t pbase = 0;
-        bool revived = false;
-        for (size_t j = 0; j < njobs; j++) {
-            if (!live[j]) continue;
-            JobRun &J = run[j];
-            memcpy(J.st.usage.data(), out_hp + out_use + cl_base[j] * 4, J.k * 4);
-            J.finish_stats((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j]);
-            if (J.revive_due()) {
-                J.st.revive_plan(J.R.unused_threshold, J.split_rng, pairs);
-                if (!pairs.empty()) {
-                    for (size_t t = 0; t < pairs.size(); t++) {
-                        pp[2 * (pbase + t)] = (uint32_t)pairs[t].first;
-                        pp[2 * (pbase + t) + 1] = (uint32_t)pairs[t].second;
-                    }
-                    memcpy(out_hp + out_use + cl_base[j] * 4, J.st.usage.data(), J.k * 4);
-                    if ((rc = check(hipMemcpyAsync(pairs_d.as<uint32_t>() + 2 * pbase, pp + 2 * pbase, pairs.size() * 8,
-                                                   hipMemcpyHostToDevice, h->stream),
-                                    "revival pairs")) ||
-                        (rc = check(hipMemcpyAsync(use_dev + cl_base[j], out_hp + out_use + cl_base[j] * 4, J.k * 4,
-                                                   hipMemcpyHostToDevice, h->stream),
-                                    "masses")) ||
-                        (rc = check(launch_revive_split(cen_model.as<float>() + cen_base[j], pairs_d.as<uint32_t>() + 2 * pbase,
-                                                        pairs.size(), J.d, h->stream),
-                                    "revival")))
-                        return rc;
-                    pbase += pairs.size();
-                    revived = true;
-                }
+        // split on the device before the next replay (one upload of all
+        // triples and masses, one kernel)
+        pool.run(njobs, finish);
+        uint32_t *tp = pairs_h.as<uint32_t>();
+        size_t ntrip = 0;
+        for (size_t j = 0; j < njobs; j++)
+            for (const auto &pr : jpairs[j]) {
+                tp[3 * ntrip] = (uint32_t)j;
+                tp[3 * ntrip + 1] = (uint32_t)pr.first;
+                tp[3 * ntrip + 2] = (uint32_t)pr.second;
+                ntrip++;
             }
-            J.advance();
-        }
-        // the staging of the revival copies must not be reused before they ran
+        const bool revived = ntrip > 0;
+        if (revived &&
+            ((rc = check(hipMemcpyAsync(pairs_d.p, tp, ntrip * 12, hipMemcpyHostToDevice, h->stream), "revival")) ||
+             (rc = check(hipMemcpyAsync(use_dev, out_hp + out_use, nclusters * 4, hipMemcpyHostToDevice, h->stream),
+                         "masses")) ||
+             (rc = check(launch_jobs_revive(jobs_dev, pairs_d.as<uint32_t>(), ntrip, h->stream), "revival"))))
+            return rc;
         if (revived && (rc = check(hipStreamSynchronize(h->stream), "revival"))) return rc;
     }
 

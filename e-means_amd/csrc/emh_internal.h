@@ -156,6 +156,8 @@ hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs,
 // Batched: keys = cluster_base + label for live slots (nclusters for spare
 // slots), counts per global cluster, stable sort, then per-job f32 sums and
 // FP64 pass totals. cjob maps a global cluster to its job.
+// Revival row splits of many jobs: triples (job, donor, unused).
+hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_t ntriples, hipStream_t s);
 // The EMA step of every job still running, one wave per global cluster.
 hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
 hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
