@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemeans_hip.so")
-SOURCES = ["emh_search.hip", "emh_accum.hip", "emh_api.hip", "emh_train.hip", "emh_batch.hip"]
+SOURCES = ["emh_search.hip", "emh_accum.hip", "emh_api.hip", "emh_train.hip", "emh_batch.hip", "emh_quant.hip"]
 ARCH = os.environ.get("EMH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

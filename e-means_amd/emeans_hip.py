@@ -29,6 +29,8 @@ EXPORTS = [
     "emh_assign_i8", "emh_assign_u8", "emh_last_error", "emh_set_stream", "emh_assign_device",
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
     "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split",
+    "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
+    "emh_quant_encode_device",
 ]
 
 
@@ -136,6 +138,13 @@ def lib():
     L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
     L.emh_train_batch.argtypes = [vp, C.POINTER(Job), sz, C.POINTER(BatchStats)]
     L.emh_set_split.argtypes = [vp, i32]
+    f32 = C.c_float
+    L.emh_quant_fit.argtypes = [i32, vp, u64, u64, f32, C.c_double, i32, vp, C.POINTER(f32)]
+    L.emh_quant_encode.argtypes = [i32, vp, u64, u64, vp, f32, vp]
+    L.emh_quant_decode.argtypes = [i32, vp, u64, u64, vp, f32, vp]
+    L.emh_quant_clipped_fraction.argtypes = [i32, vp, u64, u64, vp, f32, C.POINTER(C.c_double)]
+    L.emh_quant_fit_device.argtypes = [vp, u64, u64, f32, C.c_double, i32, vp, C.POINTER(f32), vp]
+    L.emh_quant_encode_device.argtypes = [vp, u64, u64, vp, f32, vp, vp]
     _lib = L
     return L
 
@@ -319,6 +328,66 @@ def train_batch(problems, assigner: HipAssigner | None = None):
         obj = [out.objective_[i] for i in range(out.n_objective)]
         res.append(Output(cent, labels, obj, int(out.n_splits), out.recipe, float(out.seconds)))
     return res, st
+
+
+class Quantizer:
+    """Int8Quantizer (src/quantize.rs:43-96) computed on the GPU.
+
+    Calibration defaults to faiss's (target peak 112, quantile 1, power-of-two
+    scale); ``Quantizer.clipped(q)`` is Calibration::clipped."""
+
+    def __init__(self, shift, scale: float):
+        self.shift = np.ascontiguousarray(shift, np.float32)
+        self.scale = float(scale)
+        self.d = self.shift.shape[0]
+
+    @classmethod
+    def fit(cls, sample, target_peak=112.0, quantile=1.0, power_of_two=True, device=0):
+        x = np.ascontiguousarray(sample, np.float32)
+        n, d = x.shape
+        shift = np.empty(d, np.float32)
+        scale = C.c_float()
+        _check(lib().emh_quant_fit(device, _p(x), n, d, target_peak, quantile, int(power_of_two), _p(shift),
+                                   C.byref(scale)))
+        return cls(shift, scale.value)
+
+    @classmethod
+    def fit_clipped(cls, sample, quantile, device=0):
+        return cls.fit(sample, 127.0, quantile, False, device)
+
+    def encode(self, x, device=0):
+        x = np.ascontiguousarray(x, np.float32)
+        codes = np.empty(x.shape, np.int8)
+        _check(lib().emh_quant_encode(device, _p(x), x.shape[0], self.d, _p(self.shift), self.scale, _p(codes)))
+        return codes
+
+    def decode(self, codes, device=0):
+        c = np.ascontiguousarray(codes, np.float32)
+        out = np.empty(c.shape, np.float32)
+        _check(lib().emh_quant_decode(device, _p(c), c.shape[0], self.d, _p(self.shift), self.scale, _p(out)))
+        return out
+
+    def clipped_fraction(self, x, device=0):
+        x = np.ascontiguousarray(x, np.float32)
+        out = C.c_double()
+        _check(lib().emh_quant_clipped_fraction(device, _p(x), x.shape[0], self.d, _p(self.shift), self.scale,
+                                                C.byref(out)))
+        return out.value
+
+
+def quant_fit_device(x, shift, target_peak=112.0, quantile=1.0, power_of_two=True, stream=0):
+    """Fit on a torch CUDA tensor (n x d f32); shift: a d-float CUDA tensor."""
+    scale = C.c_float()
+    _check(lib().emh_quant_fit_device(C.c_void_p(x.data_ptr()), x.shape[0], x.shape[1], target_peak, quantile,
+                                      int(power_of_two), C.c_void_p(shift.data_ptr()), C.byref(scale),
+                                      C.c_void_p(stream)))
+    return scale.value
+
+
+def quant_encode_device(x, shift, scale, codes, stream=0):
+    _check(lib().emh_quant_encode_device(C.c_void_p(x.data_ptr()), x.shape[0], x.shape[1],
+                                         C.c_void_p(shift.data_ptr()), scale, C.c_void_p(codes.data_ptr()),
+                                         C.c_void_p(stream)))
 
 
 def assign(x, centroids, assigner: HipAssigner | None = None):

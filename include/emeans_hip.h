@@ -157,6 +157,27 @@ int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size
 int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,
                    uint32_t *labels, float *dist);
 
+/* Int8Quantizer (src/quantize.rs:43-96) on the device: the calibrated
+ * f32 -> i8 codec that produces the int8 rows. fit = Int8Quantizer::fit with
+ * Calibration {target_peak, quantile, power_of_two} (faiss: 112, 1.0, 1):
+ * shift_out gets d per-dimension means (f64 sums in row order), scale_out
+ * the global scale. encode/decode/clipped_fraction = the methods of the same
+ * names. Results equal the reference's bit for bit. The host-pointer forms
+ * upload, compute and download on `device`; the _device forms take device
+ * pointers and a hipStream_t (as void *), leaving everything on the GPU. */
+int emh_quant_fit(int device, const float *sample, uint64_t n, uint64_t d, float target_peak, double quantile,
+                  int power_of_two, float *shift_out, float *scale_out);
+int emh_quant_encode(int device, const float *x, uint64_t n, uint64_t d, const float *shift, float scale,
+                     int8_t *codes);
+int emh_quant_decode(int device, const float *codes, uint64_t n, uint64_t d, const float *shift, float scale,
+                     float *out);
+int emh_quant_clipped_fraction(int device, const float *x, uint64_t n, uint64_t d, const float *shift, float scale,
+                               double *out);
+int emh_quant_fit_device(const float *sample_dev, uint64_t n, uint64_t d, float target_peak, double quantile,
+                         int power_of_two, float *shift_dev, float *scale_out, void *stream);
+int emh_quant_encode_device(const float *x_dev, uint64_t n, uint64_t d, const float *shift_dev, float scale,
+                            int8_t *codes_dev, void *stream);
+
 /* Centroid-range split of the register-resident search (no counterpart in
  * the reference). A launch with few rows fills the chip by splitting every
  * row block's centroid tiles over several workgroups and merging their
