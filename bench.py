@@ -44,7 +44,8 @@ PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2, "i16": 5033.
 # i8 MFMA operations per algorithmic multiply-add pair (I16 runs three i8 planes)
 OPS_PER_PRODUCT = {"i16": 3}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
-WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5"]
+WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5",
+             "quantize"]
 
 
 def parse():
@@ -107,10 +108,8 @@ def pick_rows(x, k, seed=1234):
 
 def int8_codes(n, d, device, seed=0):
     """Rank-64 Gaussian signal + 0.1 isotropic noise, L2-normalised, then
-    int8 codes with the faiss calibration (per-dimension mean shift, one
-    power-of-two scale putting the largest centred magnitude <= 112, round
-    half away from zero, clamp), as Int8Quantizer::fit/encode do
-    (src/quantize.rs:45-73)."""
+    int8 codes with the faiss calibration through the library's device
+    Int8Quantizer (src/quantize.rs:45-73)."""
     import torch
 
     g = torch.Generator(device=device).manual_seed(seed)
@@ -121,12 +120,15 @@ def int8_codes(n, d, device, seed=0):
         z = torch.randn(r1 - r0, 64, generator=g, device=device)
         v = z @ w + 0.1 * torch.randn(r1 - r0, d, generator=g, device=device)
         x[r0:r1] = v / v.norm(dim=1, keepdim=True)
-    shift = x.double().mean(0).float()
-    peak = (x - shift).abs().max().item()
-    scale = 2.0 ** __import__("math").floor(__import__("math").log2(112.0 / peak))
-    v = (x - shift) * scale
-    codes = (torch.sign(v) * torch.floor(v.abs() + 0.5)).clamp(-128, 127).to(torch.int8)
-    return codes.contiguous(), shift, scale
+    import emeans_hip as E
+
+    shift = torch.empty(d, dtype=torch.float32, device=device)
+    codes = torch.empty(n, d, dtype=torch.int8, device=device)
+    torch.cuda.synchronize()
+    scale = E.quant_fit_device(x, shift)
+    E.quant_encode_device(x, shift, scale, codes)
+    torch.cuda.synchronize()
+    return codes, shift, scale
 
 
 def heavy_tailed(n, d, device, seed=0):
@@ -336,6 +338,81 @@ def traffic_from_profile(workload, prec):
             return json.load(f).get(f"{workload}/{prec}")
     except (OSError, ValueError):
         return None
+
+
+# -------------------------------------------------------------- quantize ----
+def quantize_line(args, n, d):
+    """Int8Quantizer fit + encode on rows resident in HBM (the int8 data
+    path's codec): encode is elementwise, 4 bytes read and 1 written per
+    value; fit is one ordered f64 pass per dimension plus one max pass."""
+    import numpy as np
+    import torch
+
+    import emeans_hip as E
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0)
+    w = torch.randn(64, d, generator=g, device=dev)
+    x = torch.empty(n, d, device=dev)
+    for r0 in range(0, n, 262144):
+        r1 = min(n, r0 + 262144)
+        v = torch.randn(r1 - r0, 64, generator=g, device=dev) @ w + 0.1 * torch.randn(r1 - r0, d, generator=g, device=dev)
+        x[r0:r1] = v / v.norm(dim=1, keepdim=True)
+    shift = torch.empty(d, dtype=torch.float32, device=dev)
+    codes = torch.empty(n, d, dtype=torch.int8, device=dev)
+    scale = E.quant_fit_device(x, shift)
+    for _ in range(args.warmup):
+        E.quant_encode_device(x, shift, scale, codes)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        E.quant_encode_device(x, shift, scale, codes)
+    e1.record()
+    torch.cuda.synchronize()
+    enc_ms = e0.elapsed_time(e1) / args.steps
+    t = time.perf_counter()
+    for _ in range(max(1, args.steps // 5)):
+        scale = E.quant_fit_device(x, shift)
+    torch.cuda.synchronize()
+    fit_ms = 1e3 * (time.perf_counter() - t) / max(1, args.steps // 5)
+    enc_bytes = 5.0 * n * d
+    achieved = enc_bytes / (enc_ms * 1e-3) / 1e9
+    res = {
+        "metric": "encoded rows/s (Int8Quantizer::encode, rows resident in HBM)",
+        "value": round(n / (enc_ms * 1e-3), 1),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(enc_ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 -> int8",
+        "data": "synthetic (generated on the GPU with torch, seed 0)",
+        "config": {"workload": args.workload_desc, "n": n, "d": d, "calibration": "faiss"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 4), "traffic": traffic_from_profile("quantize", "encode"),
+                     "kernel": "encode4_kernel",
+                     "kernel_ms": round(enc_ms, 4)},
+        "fit_ms": round(fit_ms, 3),
+        "scale": scale,
+    }
+    if not args.no_cpu:
+        import oracle as O
+
+        log("cpu baseline (oracle quantizer on a sample)")
+        xs = x[: min(n, 200000)].cpu().numpy()
+        t = time.perf_counter()
+        q = O.Quantizer.fit(xs)
+        q.encode(xs)
+        dt = time.perf_counter() - t
+        same = O.Quantizer(d, shift.cpu().numpy(), scale).encode(xs)
+        res["cpu_baseline"] = {"value": round(xs.shape[0] / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"fit + encode of the first {xs.shape[0]} rows (scalar C, {dt:.2f} s)"}
+        res["sample_codes_equal_cpu"] = bool(np.array_equal(same, codes[: xs.shape[0]].cpu().numpy()))
+    return res
 
 
 # ----------------------------------------------------------------- train ----
@@ -619,6 +696,11 @@ def main():
                               f"{{64, 96, 128, 256}}, k log-uniform in [64, 4096], Gaussian-mixture rows, 3 epochs, "
                               f"balance 0.1 on odd jobs; one graph replay per lockstep step vs a per-job FP32 loop")
         out = train_batch_line(args, njobs, prec, m)
+    elif w == "quantize":
+        n, d = args.n or 1_000_000, args.d or 512
+        args.workload_desc = (f"Int8Quantizer encode (faiss calibration) of {n}x{d} unit-norm f32 rows "
+                              f"(rank-64 signal + 0.1 noise), rows resident in HBM")
+        out = quantize_line(args, n, d)
     else:  # train-c3
         n, d, k = args.n or 1_000_000, args.d or 512, args.k or 4096
         args.workload_desc = (f"int8 training: {n}x{d} int8 codes (faiss calibration), k={k}, 5 epochs, "

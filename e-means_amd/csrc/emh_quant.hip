@@ -39,6 +39,52 @@ __global__ __launch_bounds__(64) void col_mean_kernel(const float *x, size_t n, 
     shift[t] = (float)__ddiv_rn(acc, (double)n);
 }
 
+// The same sums with the loads taken off the add chain: a workgroup owns 32
+// dimensions, all 256 lanes stage MEAN_TR-row tiles through LDS (16-byte
+// loads, double buffered, the next tile's loads in flight during the adds),
+// and lanes 0..31 of wave 0 add the staged rows in row order, so each
+// dimension's f64 chain is the one above, bit for bit. Needs d % 4 == 0 and a
+// 16-byte aligned x.
+constexpr int MEAN_TR = 256;
+constexpr int MEAN_DIMS = 32;
+__global__ __launch_bounds__(256) void col_mean_tiled_kernel(const float4 *x, size_t n, size_t d, float *shift) {
+    __shared__ float4 tile[2][MEAN_TR][MEAN_DIMS / 4];
+    const size_t dq = d / 4, j0 = (size_t)blockIdx.x * MEAN_DIMS;
+    const unsigned quad = threadIdx.x % 8, row_in = threadIdx.x / 8;  // 32 rows per pass, 8 passes
+    const bool quad_ok = j0 + 4 * quad < d;
+    const size_t ntiles = (n + MEAN_TR - 1) / MEAN_TR;
+    float4 v[MEAN_TR / 32];
+    auto load = [&](size_t r0) {
+#pragma unroll
+        for (int u = 0; u < MEAN_TR / 32; u++) {
+            const size_t r = r0 + row_in + 32 * u;
+            v[u] = quad_ok && r < n ? x[r * dq + j0 / 4 + quad] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    auto store = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < MEAN_TR / 32; u++) tile[b][row_in + 32 * u][quad] = v[u];
+    };
+    load(0);
+    store(0);
+    __syncthreads();
+    double acc = 0.0;
+    const float *lane_col = (const float *)&tile[0][0][0] + threadIdx.x;
+    for (size_t t = 0; t < ntiles; t++) {
+        const int b = (int)(t & 1);
+        if (t + 1 < ntiles) load((t + 1) * MEAN_TR);
+        if (threadIdx.x < MEAN_DIMS) {
+            const size_t rows = std::min<size_t>(MEAN_TR, n - t * MEAN_TR);
+            const float *col = lane_col + (size_t)b * MEAN_TR * MEAN_DIMS;
+#pragma unroll 8
+            for (size_t i = 0; i < rows; i++) acc = __dadd_rn(acc, (double)col[i * MEAN_DIMS]);
+        }
+        if (t + 1 < ntiles) store(b ^ 1);
+        __syncthreads();
+    }
+    if (threadIdx.x < MEAN_DIMS && j0 + threadIdx.x < d) shift[j0 + threadIdx.x] = (float)__ddiv_rn(acc, (double)n);
+}
+
 // Elementwise kernels walk rows by workgroup and dimensions by lane.
 #define EMH_FOR_ELEMS(n, d)                                                        \
     for (size_t r = blockIdx.x; r < (n); r += gridDim.x)                          \
@@ -79,6 +125,61 @@ __global__ __launch_bounds__(256) void encode_kernel(const float *x, size_t n, s
     }
 }
 
+__device__ inline uint32_t encode_one(float x, float s, float scale) {
+    float r = roundf(__fmul_rn(__fsub_rn(x, s), scale));
+    r = r != r ? 0.0f : fminf(fmaxf(r, -128.0f), 127.0f);
+    return (uint32_t)(uint8_t)(int8_t)r;
+}
+
+// The same codes four values at a time: 16-byte loads, 4-byte stores. A
+// workgroup covers `rows_per` rows of dq = d/4 quads per pass and keeps
+// ENC_U passes of loads in flight before it stores.
+constexpr int ENC_U = 4;
+template <class F>
+__device__ inline void for_quads(const float4 *x, size_t n, unsigned dq, unsigned rows_per, F f) {
+    const unsigned span = rows_per * dq;  // <= 256 when dq <= 256, else dq
+    const size_t step = (size_t)gridDim.x * rows_per * ENC_U;
+    for (size_t r0 = (size_t)blockIdx.x * rows_per * ENC_U; r0 < n; r0 += step) {
+        for (unsigned e = threadIdx.x; e < span; e += 256) {
+            const unsigned rr = e / dq, q = e - rr * dq;
+            float4 v[ENC_U];
+#pragma unroll
+            for (int u = 0; u < ENC_U; u++) {
+                const size_t r = r0 + (size_t)u * rows_per + rr;
+                if (r < n) v[u] = x[r * dq + q];
+            }
+#pragma unroll
+            for (int u = 0; u < ENC_U; u++) {
+                const size_t r = r0 + (size_t)u * rows_per + rr;
+                if (r < n) f(r * dq + q, q, v[u]);
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void encode4_kernel(const float4 *x, size_t n, unsigned dq, unsigned rows_per,
+                                                      const float4 *shift, float scale, uint32_t *codes) {
+    for_quads(x, n, dq, rows_per, [&](size_t i, unsigned q, float4 v) {
+        const float4 sh = shift[q];
+        codes[i] = encode_one(v.x, sh.x, scale) | encode_one(v.y, sh.y, scale) << 8 |
+                   encode_one(v.z, sh.z, scale) << 16 | encode_one(v.w, sh.w, scale) << 24;
+    });
+}
+
+__device__ inline uint32_t mag_key(float x, float s) { return total_key(__float_as_uint(fabsf(__fsub_rn(x, s)))); }
+
+// mag_max_kernel four values at a time
+__global__ __launch_bounds__(256) void mag_max4_kernel(const float4 *x, size_t n, unsigned dq, unsigned rows_per,
+                                                       const float4 *shift, unsigned *out) {
+    uint32_t best = 0;
+    for_quads(x, n, dq, rows_per, [&](size_t, unsigned q, float4 v) {
+        const float4 sh = shift[q];
+        best = max(best, max(max(mag_key(v.x, sh.x), mag_key(v.y, sh.y)), max(mag_key(v.z, sh.z), mag_key(v.w, sh.w))));
+    });
+    for (int off = 32; off > 0; off >>= 1) best = max(best, (uint32_t)__shfl_xor((int)best, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
 // v * (1 / scale) + s, without contraction
 __global__ __launch_bounds__(256) void decode_kernel(const float *codes, size_t n, size_t d, const float *shift,
                                                      float inv, float *out) {
@@ -99,11 +200,44 @@ __global__ __launch_bounds__(256) void clipped_kernel(const float *x, size_t n, 
 
 static dim3 row_grid(size_t n) { return dim3((unsigned)std::max<size_t>(1, std::min<size_t>(n, 16384))); }
 
+// encode on device buffers: the 4-wide kernel when d % 4 == 0 and the
+// pointers allow 16-byte loads and 4-byte stores, else the scalar one
+struct QuadGrid {
+    unsigned dq, rows_per, grid;
+};
+static QuadGrid quad_grid(size_t n, size_t d) {
+    QuadGrid g;
+    g.dq = (unsigned)(d / 4);
+    g.rows_per = g.dq >= 256 ? 1u : 256u / g.dq;
+    const size_t groups = (n + (size_t)g.rows_per * ENC_U - 1) / ((size_t)g.rows_per * ENC_U);
+    g.grid = (unsigned)std::max<size_t>(1, std::min<size_t>(groups, 8192));
+    return g;
+}
+static bool quads_ok(size_t d, const void *x, const void *shift) {
+    return d % 4 == 0 && d / 4 <= 0xFFFFFFFFu && (uintptr_t)x % 16 == 0 && (uintptr_t)shift % 16 == 0;
+}
+
+static int launch_encode(const float *x, size_t n, size_t d, const float *shift, float scale, int8_t *codes,
+                         hipStream_t s) {
+    if (quads_ok(d, x, shift) && (uintptr_t)codes % 4 == 0) {
+        const QuadGrid g = quad_grid(n, d);
+        hipLaunchKernelGGL(encode4_kernel, dim3(g.grid), dim3(256), 0, s, (const float4 *)x, n, g.dq, g.rows_per,
+                           (const float4 *)shift, scale, (uint32_t *)codes);
+    } else {
+        hipLaunchKernelGGL(encode_kernel, row_grid(n), dim3(256), 0, s, x, n, d, shift, scale, codes);
+    }
+    return check(hipGetLastError(), "encode");
+}
+
 // fit on device rows; shift_dev (d floats) written, scale returned
 static int quant_fit_dev(const float *x, size_t n, size_t d, float target_peak, double quantile, int power_of_two,
                          float *shift_dev, float *scale_out, hipStream_t s) {
     int rc;
-    hipLaunchKernelGGL(col_mean_kernel, dim3((unsigned)((d + 63) / 64)), dim3(64), 0, s, x, n, d, shift_dev);
+    if (d % 4 == 0 && (uintptr_t)x % 16 == 0)
+        hipLaunchKernelGGL(col_mean_tiled_kernel, dim3((unsigned)((d + MEAN_DIMS - 1) / MEAN_DIMS)), dim3(256), 0, s,
+                           (const float4 *)x, n, d, shift_dev);
+    else
+        hipLaunchKernelGGL(col_mean_kernel, dim3((unsigned)((d + 63) / 64)), dim3(64), 0, s, x, n, d, shift_dev);
     if ((rc = check(hipGetLastError(), "quantizer mean"))) return rc;
     const size_t count = n * d;
     DevBuf scratch;
@@ -114,8 +248,14 @@ static int quant_fit_dev(const float *x, size_t n, size_t d, float target_peak, 
     uint32_t key = 0;
     if (idx == count - 1) {
         if ((rc = check(hipMemsetAsync(scratch.p, 0, 4, s), "quantizer scratch"))) return rc;
-        hipLaunchKernelGGL(mag_max_kernel, row_grid(n), dim3(256), 0, s, x, n, d, (const float *)shift_dev,
-                           scratch.as<unsigned>());
+        if (quads_ok(d, x, shift_dev)) {
+            const QuadGrid g = quad_grid(n, d);
+            hipLaunchKernelGGL(mag_max4_kernel, dim3(g.grid), dim3(256), 0, s, (const float4 *)x, n, g.dq, g.rows_per,
+                               (const float4 *)shift_dev, scratch.as<unsigned>());
+        } else {
+            hipLaunchKernelGGL(mag_max_kernel, row_grid(n), dim3(256), 0, s, x, n, d, (const float *)shift_dev,
+                               scratch.as<unsigned>());
+        }
         if ((rc = check(hipGetLastError(), "quantizer peak")) ||
             (rc = check(hipMemcpyAsync(&key, scratch.p, 4, hipMemcpyDeviceToHost, s), "quantizer peak")) ||
             (rc = check(hipStreamSynchronize(s), "quantizer peak")))
@@ -199,9 +339,7 @@ int emh_quant_encode(int device, const float *x, uint64_t n, uint64_t d, const f
             (rc = check(hipMemcpyAsync(xd.p, x, count * 4, hipMemcpyHostToDevice, s), "upload rows")) ||
             (rc = check(hipMemcpyAsync(sh.p, shift, d * 4, hipMemcpyHostToDevice, s), "upload shift")))
             return rc;
-        hipLaunchKernelGGL(encode_kernel, row_grid(n), dim3(256), 0, s, xd.as<float>(), (size_t)n, (size_t)d,
-                           sh.as<float>(), scale, cd.as<int8_t>());
-        if ((rc = check(hipGetLastError(), "encode")) ||
+        if ((rc = launch_encode(xd.as<float>(), n, d, sh.as<float>(), scale, cd.as<int8_t>(), s)) ||
             (rc = check(hipMemcpyAsync(codes, cd.p, count, hipMemcpyDeviceToHost, s), "download codes")) ||
             (rc = check(hipStreamSynchronize(s), "quant_encode")))
             return rc;
@@ -271,9 +409,7 @@ int emh_quant_encode_device(const float *x_dev, uint64_t n, uint64_t d, const fl
                             int8_t *codes_dev, void *stream) {
     if (!x_dev || !shift_dev || !codes_dev || d == 0) return fail(EMH_E_ARG, "quant_encode_device: bad arguments");
     if (n == 0) return EMH_OK;
-    hipLaunchKernelGGL(encode_kernel, row_grid(n), dim3(256), 0, (hipStream_t)stream, x_dev, (size_t)n, (size_t)d,
-                       shift_dev, scale, codes_dev);
-    return check(hipGetLastError(), "encode");
+    return launch_encode(x_dev, n, d, shift_dev, scale, codes_dev, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -30,19 +30,23 @@ def _heavy(n, d, seed):
     return x
 
 
+# d = 37: the scalar kernels; d = 100 (25 quads, 10 rows per workgroup pass)
+# and d = 1028 (257 quads, one row per pass): the 4-wide kernels and the
+# LDS-staged mean with a partial last dimension strip and row tile
+@pytest.mark.parametrize("d", [37, 100, 1028])
 @pytest.mark.parametrize("cal", ["faiss", "clipped99", "clipped50", "clipped0"])
-def test_quantizer_matches_cpu(hip, cal):
-    x = _heavy(20000, 37, 3)
+def test_quantizer_matches_cpu(hip, cal, d):
+    x = _heavy(20000 if d < 1000 else 5003, d, 3)
     if cal == "faiss":
         args = dict(target_peak=112.0, quantile=1.0, power_of_two=True)
     else:
         args = dict(target_peak=127.0, quantile={"clipped99": 0.99, "clipped50": 0.5, "clipped0": 0.0}[cal],
                     power_of_two=False)
-    print(f"quantizer {cal}", flush=True)
+    print(f"quantizer {cal} d={d}", flush=True)
     g = E.Quantizer.fit(x, **args)
     c = O.Quantizer.fit(x, **args)
     assert np.array_equal(g.shift, c.shift) and g.scale == c.scale
-    y = _heavy(3000, 37, 4)
+    y = _heavy(3000, d, 4)
     y[0, 0], y[1, 1], y[2, 2] = np.nan, np.inf, -np.inf
     assert np.array_equal(g.encode(y), c.encode(y))
     codes = g.encode(y).astype(np.float32)
