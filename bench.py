@@ -23,6 +23,10 @@ Other workloads (separate lines, not the headline):
   train-c5           64 uneven jobs (n 5e3..5e5, d 64..256, k 64..4096),
                      3 epochs, one graph replay per lockstep step (BF16 top-4)
                      vs a loop of single-job FP32 trainings
+  quantize           Int8Quantizer encode of 1,000,000 x 512 f32 rows (faiss
+                     calibration fitted on the device first)
+  report-nearest     the report's nearest-centroid step over k = 16384
+                     unit-norm centroids of d = 128 (bit-identical distances)
 
 Prints one JSON line.
 """
@@ -45,7 +49,7 @@ PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2, "i16": 5033.
 OPS_PER_PRODUCT = {"i16": 3}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
 WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5",
-             "quantize"]
+             "quantize", "report-nearest"]
 
 
 def parse():
@@ -415,6 +419,82 @@ def quantize_line(args, n, d):
     return res
 
 
+# -------------------------------------------------------- report-nearest ----
+def report_nearest_line(args, k, d):
+    """The report's nearest-centroid step (src/report.rs:169-183) on k
+    centroids resident in HBM: every unordered pair's sqdist once, in the
+    reference's f32 order (sub, mul, add per element: 3 flops), on the vector
+    ALUs."""
+    import numpy as np
+    import torch
+
+    import emeans_hip as E
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(0)
+    c = torch.randn(k, d, generator=g, device=dev)
+    c = (c / c.norm(dim=1, keepdim=True)).contiguous()
+    sizes = (torch.rand(k, generator=g, device=dev) > 0.01).to(torch.int32) * 7
+    keys = torch.empty(k, dtype=torch.int64, device=dev)
+    nearest = torch.empty(k, dtype=torch.int32, device=dev)
+    dist = torch.empty(k, dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(args.warmup):
+        E.report_nearest_device(c, sizes, keys, nearest, dist, stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        E.report_nearest_device(c, sizes, keys, nearest, dist, stream)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    flops = 3.0 * d * k * (k - 1) / 2
+    achieved = flops / (ms * 1e-3) / 1e12
+    res = {
+        "metric": "report nearest-centroid step: centroids/s (src/report.rs:169-183, bit-identical)",
+        "value": round(k / (ms * 1e-3), 1),
+        "unit": "centroids/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (generated on the GPU with torch, seed 0)",
+        "config": {"workload": args.workload_desc, "k": k, "d": d},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
+                     "frac": round(achieved / 157.3, 4), "traffic": None, "kernel": "nearest_pairs_kernel",
+                     "kernel_ms": round(ms, 4),
+                     "note": "3 unfused, unpacked f32 ops per element per unordered pair; that instruction "
+                             "mix caps at 1/4 of the packed-FMA peak"},
+    }
+    if not args.no_cpu:
+        import oracle as O
+
+        log("cpu baseline (oracle nearest step on a row sample)")
+        ch = c.cpu().numpy()
+        sz = sizes.cpu().numpy().view(np.uint32)
+        threads = min(16, os.cpu_count() or 1)
+        rows = 64
+        while True:
+            t = time.perf_counter()
+            on, od = O.report_nearest(ch, sz, rows=(0, rows), nthreads=threads)
+            dt = time.perf_counter() - t
+            if dt > args.cpu_seconds / 4 or rows >= k:
+                break
+            rows = min(k, rows * 4)
+        res["cpu_baseline"] = {"value": round(rows / dt, 1), "unit": "centroids/s", "cores": threads, "kind": "port",
+                               "sample": f"nearest of the first {rows} centroids against all {k} "
+                                         f"(oracle C, gcc -O3, {threads} threads, {dt:.2f} s)"}
+        gn = nearest[:rows].cpu().numpy().view(np.uint32)
+        res["sample_equal_cpu"] = bool(np.array_equal(gn, on) and
+                                       dist[:rows].cpu().numpy().tobytes() == od.tobytes())
+    return res
+
+
 # ----------------------------------------------------------------- train ----
 def c5_problems(njobs, dev, seed=5):
     """Uneven problems: n log-uniform in [5e3, 5e5], d in {64, 96, 128, 256},
@@ -696,6 +776,11 @@ def main():
                               f"{{64, 96, 128, 256}}, k log-uniform in [64, 4096], Gaussian-mixture rows, 3 epochs, "
                               f"balance 0.1 on odd jobs; one graph replay per lockstep step vs a per-job FP32 loop")
         out = train_batch_line(args, njobs, prec, m)
+    elif w == "report-nearest":
+        k, d = args.k or 16384, args.d or 128
+        args.workload_desc = (f"report nearest-centroid step over k={k} unit-norm f32 centroids of d={d} "
+                              f"(about 1% empty clusters), centroids resident in HBM")
+        out = report_nearest_line(args, k, d)
     elif w == "quantize":
         n, d = args.n or 1_000_000, args.d or 512
         args.workload_desc = (f"Int8Quantizer encode (faiss calibration) of {n}x{d} unit-norm f32 rows "

@@ -17,7 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemeans_hip.so")
-SOURCES = ["emh_search.hip", "emh_accum.hip", "emh_api.hip", "emh_train.hip", "emh_batch.hip", "emh_quant.hip"]
+SOURCES = ["emh_search.hip", "emh_accum.hip", "emh_api.hip", "emh_train.hip", "emh_batch.hip", "emh_quant.hip",
+           "emh_report.hip"]
 ARCH = os.environ.get("EMH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -27,6 +28,7 @@ CFLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=off",          # the host recipe is f32 without contraction, as Rust
+    "-fhip-fp32-correctly-rounded-divide-sqrt",  # f32 / and sqrtf round to nearest, as IEEE
     "-fvisibility=hidden",
     "-Wno-unused-result",
     "-Wno-unused-value",

@@ -82,4 +82,15 @@ int fail(int code, const std::string &msg);
 // set_centroids from host (synchronous) or device (stream-ordered) memory.
 int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src);
 int check(hipError_t e, const char *what);
+// Run body(stream) on `device` with a stream of its own, destroyed afterwards
+// (the host-pointer entry points that take a device ordinal).
+template <class F> inline int with_device(int device, F &&body) {
+    int rc;
+    if ((rc = check(hipSetDevice(device), "hipSetDevice"))) return rc;
+    hipStream_t s;
+    if ((rc = check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
+    rc = body(s);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
 }  // namespace emh

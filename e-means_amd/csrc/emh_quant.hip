@@ -293,16 +293,6 @@ static int quant_fit_dev(const float *x, size_t n, size_t d, float target_peak, 
     return EMH_OK;
 }
 
-template <class F> static int with_device(int device, F &&body) {
-    int rc;
-    if ((rc = check(hipSetDevice(device), "hipSetDevice"))) return rc;
-    hipStream_t s;
-    if ((rc = check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate"))) return rc;
-    rc = body(s);
-    (void)hipStreamDestroy(s);
-    return rc;
-}
-
 }  // namespace emh
 
 using namespace emh;

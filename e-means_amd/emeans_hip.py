@@ -30,7 +30,7 @@ EXPORTS = [
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
     "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split",
     "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
-    "emh_quant_encode_device",
+    "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device",
 ]
 
 
@@ -145,6 +145,8 @@ def lib():
     L.emh_quant_clipped_fraction.argtypes = [i32, vp, u64, u64, vp, f32, C.POINTER(C.c_double)]
     L.emh_quant_fit_device.argtypes = [vp, u64, u64, f32, C.c_double, i32, vp, C.POINTER(f32), vp]
     L.emh_quant_encode_device.argtypes = [vp, u64, u64, vp, f32, vp, vp]
+    L.emh_report_nearest.argtypes = [C.c_int, vp, u64, u64, vp, vp, vp]
+    L.emh_report_nearest_device.argtypes = [vp, u64, u64, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -388,6 +390,39 @@ def quant_encode_device(x, shift, scale, codes, stream=0):
     _check(lib().emh_quant_encode_device(C.c_void_p(x.data_ptr()), x.shape[0], x.shape[1],
                                          C.c_void_p(shift.data_ptr()), scale, C.c_void_p(codes.data_ptr()),
                                          C.c_void_p(stream)))
+
+
+def report_nearest(centroids, sizes=None, device=0):
+    """The report's nearest-centroid step (src/report.rs:169-183): for each
+    centroid the nearest live other centroid (sizes[j] > 0; all live when
+    sizes is None) and the L2 distance to it; (i, inf) when there is none."""
+    c = np.ascontiguousarray(centroids, np.float32)
+    if c.ndim != 2:
+        raise ValueError("centroids must be k x d")
+    k, d = c.shape
+    sz = None
+    if sizes is not None:
+        sz = np.ascontiguousarray(sizes, np.uint32)
+        if sz.shape != (k,):
+            raise ValueError("sizes must hold one count per centroid")
+    nearest = np.empty(k, np.uint32)
+    dist = np.empty(k, np.float32)
+    _check(lib().emh_report_nearest(device, _p(c), k, d, _p(sz), _p(nearest), _p(dist)))
+    return nearest, dist
+
+
+def report_nearest_device(centroids, sizes, keys, nearest, dist, stream=0):
+    """report_nearest on torch CUDA tensors: centroids k x d f32, sizes k u32
+    (int32 storage) or None, keys k int64 workspace, nearest k int32, dist k
+    f32; stream-ordered, no synchronisation."""
+    k, d = centroids.shape
+    for t, n in ((keys, k), (nearest, k), (dist, k)):
+        if t.numel() != n or not t.is_contiguous():
+            raise ValueError("keys, nearest and dist must be contiguous with k elements")
+    _check(lib().emh_report_nearest_device(C.c_void_p(centroids.data_ptr()), k, d,
+                                           None if sizes is None else C.c_void_p(sizes.data_ptr()),
+                                           C.c_void_p(keys.data_ptr()), C.c_void_p(nearest.data_ptr()),
+                                           C.c_void_p(dist.data_ptr()), C.c_void_p(stream)))
 
 
 def assign(x, centroids, assigner: HipAssigner | None = None):

@@ -178,6 +178,19 @@ int emh_quant_fit_device(const float *sample_dev, uint64_t n, uint64_t d, float 
 int emh_quant_encode_device(const float *x_dev, uint64_t n, uint64_t d, const float *shift_dev, float scale,
                             int8_t *codes_dev, void *stream);
 
+/* The report's nearest-centroid step (src/report.rs:169-183, the O(k^2 d) part
+ * of report::report): for each of the k centroids (k x d, row-major) the
+ * nearest other centroid among live clusters (sizes[j] > 0; sizes NULL: all
+ * live) and the plain L2 distance to it, (i, +inf) when there is none. The
+ * distance is the reference's sqdist (src/report.rs:104-114) in its own f32
+ * order, so indices and distances equal the reference's bit for bit. The
+ * _device form takes device pointers, k u64 of workspace (keys_dev) and a
+ * hipStream_t (as void *). */
+int emh_report_nearest(int device, const float *centroids, uint64_t k, uint64_t d, const uint32_t *sizes,
+                       uint32_t *nearest, float *nearest_dist);
+int emh_report_nearest_device(const float *centroids_dev, uint64_t k, uint64_t d, const uint32_t *sizes_dev,
+                              uint64_t *keys_dev, uint32_t *nearest_dev, float *dist_dev, void *stream);
+
 /* Centroid-range split of the register-resident search (no counterpart in
  * the reference). A launch with few rows fills the chip by splitting every
  * row block's centroid tiles over several workgroups and merging their

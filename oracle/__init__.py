@@ -126,6 +126,9 @@ def lib():
         L.or_quant_decode.argtypes = [vp, sz, sz, vp, C.c_float, vp]
         L.or_quant_clipped_fraction.argtypes = [vp, sz, sz, vp, C.c_float]
         L.or_quant_clipped_fraction.restype = C.c_double
+        L.or_report_sqdist.argtypes = [vp, vp, sz]
+        L.or_report_sqdist.restype = C.c_float
+        L.or_report_nearest.argtypes = [i32, vp, sz, sz, vp, sz, sz, vp, vp]
         L.or_brute_f64.argtypes = [i32, vp, i32, sz, sz, vp, sz, vp, vp, vp]
         L.or_dist_f64.argtypes = [i32, vp, i32, sz, sz, vp, vp, vp]
         L.or_brute_i64.argtypes = [vp, sz, sz, vp, sz, vp, vp]
@@ -308,6 +311,27 @@ def brute_i64(x, c8):
     dist = np.empty(n, np.int64)
     lib().or_brute_i64(_p(x), n, d, _p(c8), c8.shape[0], _p(labels), _p(dist))
     return labels, dist
+
+
+# ---- report's nearest-centroid step (src/report.rs) -------------------------
+def report_sqdist(a, b) -> np.float32:
+    """sqdist (src/report.rs:104-114) of two equal-length f32 vectors."""
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b, np.float32)
+    return np.float32(lib().or_report_sqdist(_p(a), _p(b), a.shape[0]))
+
+
+def report_nearest(centroids, sizes=None, rows=None, nthreads=None):
+    """Nearest live other centroid of each centroid (src/report.rs:169-183):
+    (index u32, L2 distance f32); rows=(r0, r1) restricts to those rows."""
+    c = np.ascontiguousarray(centroids, np.float32)
+    k, d = c.shape
+    r0, r1 = rows if rows is not None else (0, k)
+    sz = None if sizes is None else np.ascontiguousarray(sizes, np.uint32)
+    nearest = np.empty(r1 - r0, np.uint32)
+    dist = np.empty(r1 - r0, np.float32)
+    lib().or_report_nearest(nthreads or os.cpu_count() or 1, _p(c), k, d, _p(sz), r0, r1, _p(nearest), _p(dist))
+    return nearest, dist
 
 
 # ---- quantizer (src/quantize.rs) -------------------------------------------

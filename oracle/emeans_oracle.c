@@ -22,6 +22,10 @@
  *   i8/i16 search  exact i64 brute force (src/cpu_i8.rs:294-324) and the
  *                  fractional-centroid case (src/cpu_i8.rs:326-336).
  *   quantizer      known-answer test src/quantize.rs:102-116.
+ *   report nearest the report test's 1-D case (src/report.rs:283-312);
+ *                  hand-derived f32 values that only the sixteen-lane order
+ *                  with a separate remainder fold produces; exact integer
+ *                  brute force; ties, self and empty-cluster exclusion.
  *   train          closing-mean invariant, loss bound and monotone objective
  *                  (src/lib.rs:575-614).
  * Parity unpinned: the exact f32 bits of the AVX-512 horizontal reduction
@@ -1184,4 +1188,64 @@ void or_balance_stats(const uint32_t *labels, size_t n, size_t k, double *imbala
     *max_over_mean = (double)mx * (double)k / nn;
     *empty = e;
     free(counts);
+}
+
+
+/* ---- Report's nearest-centroid step -------------------------------------- */
+
+/* src/report.rs:104-114: sixteen running sums, lane i taking elements
+ * i, i+16, ..., folded left; the remainder past the last full 16-chunk is
+ * folded left on its own and added last. */
+float or_report_sqdist(const float *a, const float *b, size_t d) {
+    float acc[16] = {0};
+    const size_t full = d / 16 * 16;
+    float tail = 0.0f;
+    for (size_t e = full; e < d; e++) {
+        const float t = a[e] - b[e];
+        tail += t * t;
+    }
+    for (size_t c = 0; c < full; c += 16)
+        for (int i = 0; i < 16; i++) {
+            const float t = a[c + i] - b[c + i];
+            acc[i] += t * t;
+        }
+    float s = 0.0f;
+    for (int i = 0; i < 16; i++) s += acc[i];
+    return s + tail;
+}
+
+typedef struct {
+    const float *c;
+    size_t k, d, row0, row1;
+    const uint32_t *sizes;
+    uint32_t *nearest;
+    float *dist;
+} nearest_job;
+
+/* src/report.rs:169-183: j ascending over live clusters other than i, a
+ * strict < against the running best, which starts at (i, +inf) */
+static void nearest_block(void *vctx, size_t blk) {
+    const nearest_job *J = vctx;
+    const size_t r0 = J->row0 + blk * 64, r1 = r0 + 64 < J->row1 ? r0 + 64 : J->row1;
+    for (size_t i = r0; i < r1; i++) {
+        uint32_t bj = (uint32_t)i;
+        float bs = INFINITY;
+        for (size_t j = 0; j < J->k; j++) {
+            if (j == i || (J->sizes && J->sizes[j] == 0)) continue;
+            const float s = or_report_sqdist(J->c + i * J->d, J->c + j * J->d, J->d);
+            if (s < bs) {
+                bs = s;
+                bj = (uint32_t)j;
+            }
+        }
+        J->nearest[i - J->row0] = bj;
+        J->dist[i - J->row0] = sqrtf(bs);
+    }
+}
+
+void or_report_nearest(int nthreads, const float *c, size_t k, size_t d, const uint32_t *sizes, size_t row0,
+                       size_t row1, uint32_t *nearest, float *nearest_dist) {
+    nearest_job J = {c, k, d, row0, row1, sizes, nearest, nearest_dist};
+    if (row1 <= row0) return;
+    run_pool(nthreads, (row1 - row0 + 63) / 64, &J, nearest_block);
 }

@@ -100,6 +100,13 @@ void or_quant_encode(const float *x, size_t n, size_t d, const float *shift, flo
 void or_quant_decode(const float *codes, size_t n, size_t d, const float *shift, float scale, float *out);
 double or_quant_clipped_fraction(const float *x, size_t n, size_t d, const float *shift, float scale);
 
+/* ---- Report's nearest-centroid step (src/report.rs:104-114, 169-183) ---- */
+float or_report_sqdist(const float *a, const float *b, size_t d);
+/* rows [row0, row1): nearest live other centroid and its L2 distance; sizes
+ * NULL means every cluster is live. Outputs are indexed from row0. */
+void or_report_nearest(int nthreads, const float *c, size_t k, size_t d, const uint32_t *sizes, size_t row0,
+                       size_t row1, uint32_t *nearest, float *nearest_dist);
+
 /* ---- Brute-force judges ------------------------------------------------- */
 void or_brute_f64(int nthreads, const void *x, int rows, size_t n, size_t d, const float *c, size_t k,
                   const float *bias, uint32_t *labels, double *dist);

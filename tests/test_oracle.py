@@ -409,3 +409,69 @@ def test_train_i8_rows_integer_sums():
 def test_balance_stats():
     imb, mx, empty = O.balance_stats(np.array([0, 0, 1, 1], np.uint32), 3)
     assert abs(imb - 3 * 8 / 16) < 1e-12 and abs(mx - 2 * 3 / 4) < 1e-12 and empty == 1
+
+
+# ---- report's nearest-centroid step (src/report.rs:104-114, 169-183) -------
+def test_report_nearest_reference_case():
+    # src/report.rs:283-312: 1-D centroids 0, 1, 50 with sizes 101, 100, 3;
+    # the test's rare_prototypes == [(2, 3, 49.0)] fixes cluster 2's nearest
+    # distance to 49 (isolation 49 / median(1, 1, 49)).
+    c = np.array([[0.0], [1.0], [50.0]], np.float32)
+    nearest, dist = O.report_nearest(c, np.array([101, 100, 3], np.uint32))
+    assert nearest.tolist() == [1, 0, 1]
+    assert dist.tolist() == [1.0, 1.0, 49.0]
+
+
+def test_report_sqdist_order_known_answers():
+    # Values derived by hand in f32 (spacing 2 above 2^24, ties to even):
+    # d = 32, v0 = 4096, v2 = v16 = v18 = 1: lane 0 = 2^24 + 1 -> 2^24, lane
+    # 2 = 1 + 1 = 2, so the fold gives 2^24 + 2. A single left fold gives
+    # 2^24 (each +1 rounds away), lanes over contiguous blocks give 2^24, the
+    # exact sum 2^24 + 3 rounds to 2^24 + 4.
+    a = np.zeros(32, np.float32)
+    a[[0, 2, 16, 18]] = [4096, 1, 1, 1]
+    assert O.report_sqdist(a, np.zeros(32, np.float32)) == np.float32(16777218.0)
+    # d = 34, v0 = 4096, v1 = 1, remainder v32 = v33 = 1: lanes fold to 2^24,
+    # the remainder folds to 2 on its own, and 2^24 + 2 is exact. Adding the
+    # remainder one element at a time would give 2^24, exactly 2^24 + 4.
+    b = np.zeros(34, np.float32)
+    b[[0, 1, 32, 33]] = [4096, 1, 1, 1]
+    assert O.report_sqdist(b, np.zeros(34, np.float32)) == np.float32(16777218.0)
+    # symmetric bit for bit
+    rng = np.random.default_rng(0)
+    x, y = rng.normal(size=(2, 77)).astype(np.float32)
+    assert O.report_sqdist(x, y).tobytes() == O.report_sqdist(y, x).tobytes()
+
+
+@pytest.mark.parametrize("k,d", [(50, 3), (200, 37), (300, 64)])
+def test_report_nearest_integer_brute_force(k, d):
+    # small integers: every f32 operation is exact, so the distances are the
+    # exact integer ones, and ties (frequent here) must go to the lowest index
+    rng = np.random.default_rng(k + d)
+    c = rng.integers(-3, 4, size=(k, d)).astype(np.float32)
+    c[5] = c[9]  # an exact duplicate: distance 0
+    sizes = rng.integers(0, 3, size=k).astype(np.uint32)
+    nearest, dist = O.report_nearest(c, sizes, nthreads=2)
+    ci = c.astype(np.int64)
+    sq = ((ci[:, None, :] - ci[None, :, :]) ** 2).sum(-1)
+    sq[np.arange(k), np.arange(k)] = 1 << 62
+    sq[:, sizes == 0] = 1 << 62
+    best = sq.argmin(1)
+    has = sq[np.arange(k), best] < (1 << 62)
+    assert np.array_equal(nearest[has], best[has])
+    assert np.array_equal(dist[has], np.sqrt(sq[np.arange(k), best][has].astype(np.float32)))
+    assert np.array_equal(nearest[~has], np.arange(k)[~has]) and np.all(np.isinf(dist[~has]))
+
+
+def test_report_nearest_edge_cases():
+    one = np.ones((1, 4), np.float32)
+    n, dd = O.report_nearest(one)
+    assert n.tolist() == [0] and np.isinf(dd[0])
+    c = np.arange(12, dtype=np.float32).reshape(4, 3)
+    n, dd = O.report_nearest(c, np.zeros(4, np.uint32))  # no live cluster
+    assert n.tolist() == [0, 1, 2, 3] and np.all(np.isinf(dd))
+    c[2, 1] = np.nan  # its distances are NaN: never chosen, and it finds none
+    n, dd = O.report_nearest(c)
+    assert n.tolist() == [1, 0, 2, 1] and np.isinf(dd[2])
+    r = O.report_nearest(c, rows=(1, 3))
+    assert r[0].tolist() == [0, 2]
