@@ -466,7 +466,7 @@ def report_nearest_line(args, k, d):
         "data": "synthetic (generated on the GPU with torch, seed 0)",
         "config": {"workload": args.workload_desc, "k": k, "d": d},
         "roofline": {"bound": "valu", "achieved": round(achieved, 2), "peak": 157.3, "unit": "TFLOP/s",
-                     "frac": round(achieved / 157.3, 4), "traffic": None, "kernel": "nearest_pairs_kernel",
+                     "frac": round(achieved / 157.3, 4), "traffic": None, "kernel": "pairs_kernel<PAIR_NEAREST>",
                      "kernel_ms": round(ms, 4),
                      "note": "3 unfused, unpacked f32 ops per element per unordered pair; that instruction "
                              "mix caps at 1/4 of the packed-FMA peak"},

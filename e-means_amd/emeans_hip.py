@@ -30,7 +30,8 @@ EXPORTS = [
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
     "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split",
     "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
-    "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device",
+    "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device", "emh_report_config_default",
+    "emh_report",
 ]
 
 
@@ -147,6 +148,9 @@ def lib():
     L.emh_quant_encode_device.argtypes = [vp, u64, u64, vp, f32, vp, vp]
     L.emh_report_nearest.argtypes = [C.c_int, vp, u64, u64, vp, vp, vp]
     L.emh_report_nearest_device.argtypes = [vp, u64, u64, vp, vp, vp, vp, vp]
+    L.emh_report_config_default.argtypes = [C.POINTER(ReportConfig)]
+    L.emh_report.argtypes = [C.c_int, vp, vp, u64, vp, u64, u64, C.POINTER(ReportConfig), vp, vp, vp, vp, vp, vp,
+                             vp, vp, C.POINTER(ReportSummary)]
     _lib = L
     return L
 
@@ -423,6 +427,146 @@ def report_nearest_device(centroids, sizes, keys, nearest, dist, stream=0):
                                            None if sizes is None else C.c_void_p(sizes.data_ptr()),
                                            C.c_void_p(keys.data_ptr()), C.c_void_p(nearest.data_ptr()),
                                            C.c_void_p(dist.data_ptr()), C.c_void_p(stream)))
+
+
+class ReportConfig(C.Structure):
+    """ReportConfig (src/report.rs:20-35)."""
+    _fields_ = [("family_radius", C.c_float), ("min_local_size", C.c_uint64)]
+
+    def __init__(self, family_radius=1.25, min_local_size=30):
+        super().__init__(family_radius, min_local_size)
+
+
+class ReportSummary(C.Structure):
+    _fields_ = [("median_dist", C.c_float), ("median_rms", C.c_float), ("median_nearest_dist", C.c_float),
+                ("n_families", C.c_uint32)]
+
+
+# ClusterStats (src/report.rs:37-56), field by field
+CLUSTER_STATS = np.dtype([("size", "<u4"), ("rms", "<f4"), ("median", "<f4"), ("mad", "<f4"), ("p90", "<f4"),
+                          ("p99", "<f4"), ("max", "<f4"), ("looseness", "<f4"), ("nearest_centroid", "<u4"),
+                          ("nearest_dist", "<f4"), ("isolation", "<f4"), ("family", "<u4")])
+
+
+def _total_key(v):
+    u = np.ascontiguousarray(v, np.float32).view(np.uint32)
+    return np.where(u & np.uint32(0x80000000), ~u, u | np.uint32(0x80000000))
+
+
+def _rust_fmt(v, prec):
+    """Rust's `{:.prec}` for a float (exact decimal rounding, "NaN"/"inf")."""
+    v = float(v)
+    if v != v:
+        return "NaN"
+    if v in (float("inf"), float("-inf")):
+        return "inf" if v > 0 else "-inf"
+    return f"{v:.{prec}f}"
+
+
+class Family:
+    def __init__(self, leader, members, size, share):
+        self.leader, self.members, self.size, self.share = leader, members, size, share
+
+    def __repr__(self):
+        return f"Family(leader={self.leader}, members={self.members.tolist()}, size={self.size}, share={self.share})"
+
+
+class Report:
+    """report::Report (src/report.rs:58-81) with its methods (src/report.rs:245-291)."""
+
+    def __init__(self, clusters, families, local_z, global_z, summary):
+        self.clusters = clusters  # structured array of CLUSTER_STATS, one per cluster
+        self.families = families  # list of Family, largest first
+        self.local_z = local_z
+        self.global_z = global_z
+        self.median_dist = np.float32(summary.median_dist)
+        self.median_rms = np.float32(summary.median_rms)
+        self.median_nearest_dist = np.float32(summary.median_nearest_dist)
+
+    def score(self, row):
+        """max(local_z, global_z) with f32::max's NaN rule (src/report.rs:247-249)."""
+        return np.fmax(self.local_z[row], self.global_z[row])
+
+    def top_outliers(self, count):
+        """(row, score) of the `count` highest scores, best first
+        (src/report.rs:252-261); equal scores by row index."""
+        scores = np.fmax(self.local_z, self.global_z)
+        order = np.argsort(~_total_key(scores), kind="stable")[: min(count, scores.shape[0])]
+        return [(int(r), np.float32(scores[r])) for r in order]
+
+    def rare_prototypes(self, max_size_ratio, min_isolation):
+        """(cluster, size, isolation) of small, isolated clusters, most
+        isolated first (src/report.rs:265-277); equal isolation by index."""
+        c = self.clusters
+        live = max(int((c["size"] > 0).sum()), 1)
+        mean = np.float32(np.float32(self.local_z.shape[0]) / np.float32(live))
+        lim = np.float32(np.float32(max_size_ratio) * mean)
+        keep = np.flatnonzero((c["size"] > 0) & (c["size"].astype(np.float32) <= lim) &
+                              (c["isolation"] >= np.float32(min_isolation)))
+        keep = keep[np.argsort(~_total_key(c["isolation"][keep]), kind="stable")]
+        return [(int(j), int(c["size"][j]), np.float32(c["isolation"][j])) for j in keep]
+
+    def write_clusters_tsv(self, w):
+        """One line per cluster after a header (src/report.rs:280-290)."""
+        w.write("cluster\tsize\trms\tmedian\tmad\tp90\tp99\tmax\tlooseness\tnearest_centroid\tnearest_dist"
+                "\tisolation\tfamily\tfamily_share\n")
+        for j, c in enumerate(self.clusters):
+            share = self.families[int(c["family"])].share if c["size"] > 0 else 0.0
+            f5 = [_rust_fmt(c[f], 5) for f in ("rms", "median", "mad", "p90", "p99", "max")]
+            w.write(f"{j}\t{int(c['size'])}\t" + "\t".join(f5) + f"\t{_rust_fmt(c['looseness'], 3)}"
+                    f"\t{int(c['nearest_centroid'])}\t{_rust_fmt(c['nearest_dist'], 5)}"
+                    f"\t{_rust_fmt(c['isolation'], 3)}\t{int(c['family'])}\t{_rust_fmt(share, 6)}\n")
+
+
+def report(labels, sqdist, centroids, cfg: ReportConfig | None = None, device=0) -> Report:
+    """report::report (src/report.rs:119-243): the nearest-centroid and family
+    radius steps on the GPU, the rest on the host in the reference's
+    arithmetic."""
+    labels = np.ascontiguousarray(labels, np.uint32)
+    sq = np.ascontiguousarray(sqdist, np.float32)
+    c = np.ascontiguousarray(centroids, np.float32)
+    if c.ndim != 2 or labels.shape != sq.shape or labels.ndim != 1:
+        raise ValueError("report: labels and sqdist are n-vectors, centroids k x d")
+    n = labels.shape[0]
+    k, d = c.shape
+    cfg = cfg or ReportConfig()
+    clusters = np.zeros(k, CLUSTER_STATS)
+    local_z = np.empty(n, np.float32)
+    global_z = np.empty(n, np.float32)
+    leader = np.empty(k, np.uint32)
+    fsize = np.empty(k, np.uint64)
+    share = np.empty(k, np.float64)
+    start = np.empty(k + 1, np.uint32)
+    members = np.empty(k, np.uint32)
+    summ = ReportSummary()
+    _check(lib().emh_report(device, _p(labels), _p(sq), n, _p(c), k, d, C.byref(cfg), _p(clusters), _p(local_z),
+                            _p(global_z), _p(leader), _p(fsize), _p(share), _p(start), _p(members), C.byref(summ)))
+    fams = [Family(int(leader[f]), members[start[f]:start[f + 1]].copy(), int(fsize[f]), float(share[f]))
+            for f in range(summ.n_families)]
+    return Report(clusters, fams, local_z, global_z, summ)
+
+
+def read_f32(path, d):
+    """A raw little-endian f32 file of n x d rows (examples/cluster.rs:11-14),
+    memory-mapped: n = file size / (4 d)."""
+    a = np.memmap(path, dtype="<f4", mode="r")
+    if a.shape[0] % d:
+        raise ValueError(f"{path}: {a.shape[0]} floats is not a multiple of d = {d}")
+    return a.reshape(-1, d)
+
+
+def write_f32(path, a):
+    """Rows or centroids as raw little-endian f32 (examples/cluster.rs:22-23)."""
+    np.ascontiguousarray(a, "<f4").tofile(path)
+
+
+def read_u32(path):
+    """Labels as raw little-endian u32 (examples/outliers.rs:70-71)."""
+    return np.fromfile(path, dtype="<u4")
+
+
+def write_u32(path, labels):
+    np.ascontiguousarray(labels, "<u4").tofile(path)
 
 
 def assign(x, centroids, assigner: HipAssigner | None = None):

@@ -191,6 +191,38 @@ int emh_report_nearest(int device, const float *centroids, uint64_t k, uint64_t 
 int emh_report_nearest_device(const float *centroids_dev, uint64_t k, uint64_t d, const uint32_t *sizes_dev,
                               uint64_t *keys_dev, uint32_t *nearest_dev, float *dist_dev, void *stream);
 
+/* report::report (src/report.rs:119-250) with its two O(k^2 d) steps on the
+ * device (the nearest-centroid step above, and the family step's "within
+ * radius" test, src/report.rs:208) and the per-cluster statistics, families
+ * and z-scores on the host in the reference's arithmetic. Inputs are what
+ * assign returns (labels < k, squared distances) and the k x d centroids.
+ * Outputs: clusters[k] (ClusterStats field by field), local_z[n],
+ * global_z[n], and the families sorted by decreasing size as parallel arrays
+ * of capacity k: family_leader, family_size, family_share, and member lists
+ * family_members[family_start[f] .. family_start[f + 1]) (family_start has
+ * k + 1 entries). Clusters of equal size are taken in index order where the
+ * reference's unstable sort leaves the order open. k <= 131072. */
+typedef struct {
+    float family_radius;     /* ReportConfig::family_radius (1.25) */
+    uint64_t min_local_size; /* ReportConfig::min_local_size (30) */
+} emh_report_config;
+typedef struct {
+    uint32_t size;
+    float rms, median, mad, p90, p99, max, looseness;
+    uint32_t nearest_centroid;
+    float nearest_dist, isolation;
+    uint32_t family;
+} emh_cluster_stats;
+typedef struct {
+    float median_dist, median_rms, median_nearest_dist;
+    uint32_t n_families;
+} emh_report_summary;
+void emh_report_config_default(emh_report_config *cfg);
+int emh_report(int device, const uint32_t *labels, const float *sqdist, uint64_t n, const float *centroids,
+               uint64_t k, uint64_t d, const emh_report_config *cfg, emh_cluster_stats *clusters, float *local_z,
+               float *global_z, uint32_t *family_leader, uint64_t *family_size, double *family_share,
+               uint32_t *family_start, uint32_t *family_members, emh_report_summary *summary);
+
 /* Centroid-range split of the register-resident search (no counterpart in
  * the reference). A launch with few rows fills the chip by splitting every
  * row block's centroid tiles over several workgroups and merging their

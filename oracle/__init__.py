@@ -7,6 +7,7 @@ the CPU baseline of bench.py. The product (e-means_amd/) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 from dataclasses import dataclass, field
@@ -332,6 +333,98 @@ def report_nearest(centroids, sizes=None, rows=None, nthreads=None):
     dist = np.empty(r1 - r0, np.float32)
     lib().or_report_nearest(nthreads or os.cpu_count() or 1, _p(c), k, d, _p(sz), r0, r1, _p(nearest), _p(dist))
     return nearest, dist
+
+
+def _total_sorted(v):
+    """Ascending f32::total_cmp order."""
+    v = np.ascontiguousarray(v, np.float32)
+    u = v.view(np.uint32)
+    key = np.where(u & np.uint32(0x80000000), ~u, u | np.uint32(0x80000000))
+    return v[np.argsort(key, kind="stable")]
+
+
+def _quantile(sorted_v, q):
+    """src/report.rs:83-88 (f64::round: half away from zero)."""
+    if len(sorted_v) == 0:
+        return np.float32(0.0)
+    x = (len(sorted_v) - 1) * q
+    i = math.floor(x)
+    if x - i >= 0.5:
+        i += 1
+    return np.float32(sorted_v[i])
+
+
+def _median_mad(sorted_v):
+    """src/report.rs:91-96."""
+    med = _quantile(sorted_v, 0.5)
+    dev = _total_sorted(np.abs(sorted_v - med).astype(np.float32))
+    return med, _quantile(dev, 0.5)
+
+
+def report(labels, sqdist, centroids, family_radius=1.25, min_local_size=30):
+    """report::report (src/report.rs:119-243) for small inputs: a dict of the
+    per-cluster fields (arrays over k), families (leader, members, size,
+    share; largest first), local_z, global_z and the three medians. Clusters
+    of equal size are taken in index order (the reference's unstable sorts
+    leave that order open: parity unpinned on size ties)."""
+    labels = np.ascontiguousarray(labels, np.uint32)
+    sq = np.ascontiguousarray(sqdist, np.float32)
+    c = np.ascontiguousarray(centroids, np.float32)
+    k, d = c.shape
+    n = labels.shape[0]
+    f32 = np.float32
+    dist = np.sqrt(np.fmax(sq, f32(0.0))).astype(np.float32)
+    g_med, g_mad = _median_mad(_total_sorted(dist))
+    g_scale = max(f32(f32(1.4826) * g_mad), f32(np.finfo(np.float32).tiny))
+    cols = ("size", "rms", "median", "mad", "p90", "p99", "max", "looseness", "nearest_centroid", "nearest_dist",
+            "isolation", "family")
+    cl = {f: np.zeros(k, np.uint32 if f in ("size", "nearest_centroid", "family") else np.float32) for f in cols}
+    for j in range(k):
+        m = _total_sorted(dist[labels == j])
+        if len(m) == 0:
+            continue
+        cl["size"][j] = len(m)
+        cl["median"][j], cl["mad"][j] = _median_mad(m)
+        m64 = m.astype(np.float64)
+        cl["rms"][j] = f32(math.sqrt(np.cumsum(m64 * m64)[-1] / len(m)))
+        cl["p90"][j] = _quantile(m, 0.9)
+        cl["p99"][j] = _quantile(m, 0.99)
+        cl["max"][j] = m[-1]
+    live = cl["size"] > 0
+    nearest, nd = report_nearest(c, cl["size"])
+    median_rms = _quantile(_total_sorted(cl["rms"][live]), 0.5)
+    median_nearest = _quantile(_total_sorted(nd[live]), 0.5)
+    tiny = f32(np.finfo(np.float32).tiny)
+    cl["nearest_centroid"][:] = nearest
+    cl["nearest_dist"][:] = nd
+    with np.errstate(invalid="ignore", divide="ignore"):  # inf / inf is NaN here as in the reference
+        cl["looseness"][:] = cl["rms"] / max(median_rms, tiny)
+        cl["isolation"][:] = nd / max(median_nearest, tiny)
+    radius = f32(f32(family_radius) * median_nearest)
+    order = sorted(np.flatnonzero(live).tolist(), key=lambda j: -int(cl["size"][j]))
+    family_of = [None] * k
+    fams = []
+    for lead in order:
+        if family_of[lead] is not None:
+            continue
+        joins = [j for j in order if family_of[j] is None and
+                 (j == lead or np.sqrt(report_sqdist(c[lead], c[j])) <= radius)]
+        for j in joins:
+            family_of[j] = len(fams)
+        size = sum(int(cl["size"][j]) for j in joins)
+        fams.append({"leader": lead, "members": joins, "size": size, "share": size / n})
+    fams = sorted(fams, key=lambda f: -f["size"])
+    for f, fam in enumerate(fams):
+        for j in fam["members"]:
+            cl["family"][j] = f
+    global_z = ((dist - g_med) / g_scale).astype(np.float32)
+    local_z = global_z.copy()
+    for r in range(n):
+        j = labels[r]
+        if not (int(cl["size"][j]) < min_local_size or cl["mad"][j] <= 0.0):
+            local_z[r] = (dist[r] - cl["median"][j]) / (f32(1.4826) * cl["mad"][j])
+    return {"clusters": cl, "families": fams, "local_z": local_z, "global_z": global_z, "median_dist": g_med,
+            "median_rms": median_rms, "median_nearest_dist": median_nearest}
 
 
 # ---- quantizer (src/quantize.rs) -------------------------------------------

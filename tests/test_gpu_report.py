@@ -106,3 +106,72 @@ def test_rejects_bad_shapes(hip):
         E.report_nearest(np.zeros(5, np.float32))
     with pytest.raises(E.EmeansError):
         E.report_nearest(np.zeros((0, 4), np.float32))
+
+
+# ---- report::report around the two device steps ------------------------------
+def _reference_report_case():
+    labels, sq = [], []
+    for i in range(200):
+        off = np.float32(0.001) * np.float32(i % 100)
+        labels.append(i % 2)
+        sq.append(off * off)
+    labels.append(0)
+    sq.append(np.float32(0.45) * np.float32(0.45))
+    for _ in range(3):
+        labels.append(2)
+        sq.append(np.float32(1e-4))
+    return np.array(labels, np.uint32), np.array(sq, np.float32), np.array([[0.0], [1.0], [50.0]], np.float32)
+
+
+def test_report_reference_test(hip):
+    # src/report.rs:283-312, assertion by assertion
+    import io
+
+    labels, sq, c = _reference_report_case()
+    r = E.report(labels, sq, c)
+    assert r.clusters["size"].tolist() == [101, 100, 3]
+    assert r.top_outliers(1)[0][0] == 200
+    assert r.local_z[200] > 10.0
+    assert len(r.families) == 2
+    assert r.families[0].size == 201
+    assert r.clusters["family"][0] == r.clusters["family"][1]
+    assert r.rare_prototypes(0.25, 2.0) == [(2, 3, np.float32(49.0))]
+    buf = io.StringIO()
+    r.write_clusters_tsv(buf)
+    assert buf.getvalue().count("\n") == 4
+
+
+def _compare_report(r, o):
+    for f in r.clusters.dtype.names:
+        assert r.clusters[f].tobytes() == o["clusters"][f].tobytes(), f
+    assert r.local_z.tobytes() == o["local_z"].tobytes()
+    assert r.global_z.tobytes() == o["global_z"].tobytes()
+    for a, b in (("median_dist", "median_dist"), ("median_rms", "median_rms"),
+                 ("median_nearest_dist", "median_nearest_dist")):
+        assert np.float32(getattr(r, a)).tobytes() == np.float32(o[b]).tobytes(), a
+    assert len(r.families) == len(o["families"])
+    for f, g in zip(r.families, o["families"]):
+        assert (f.leader, f.members.tolist(), f.size, f.share) == (g["leader"], g["members"], g["size"], g["share"])
+
+
+@pytest.mark.parametrize("radius", [0.5, 1.25, 3.0])
+@pytest.mark.parametrize("n,k,d", [(3000, 40, 8), (20000, 150, 37)])
+def test_report_matches_cpu(hip, n, k, d, radius):
+    print(f"report n={n} k={k} d={d} radius={radius}", flush=True)
+    rng = np.random.default_rng(n + k + d)
+    c = rng.normal(size=(k, d)).astype(np.float32)
+    # uneven cluster sizes, some clusters empty, a NaN and a negative distance
+    w = rng.pareto(1.5, size=k)
+    w[rng.choice(k, size=k // 10, replace=False)] = 0
+    labels = rng.choice(k, size=n, p=w / w.sum()).astype(np.uint32)
+    sq = (rng.gamma(2.0, 1.0, size=n) * rng.uniform(0.5, 2.0, size=k)[labels]).astype(np.float32)
+    sq[0], sq[1] = np.nan, -0.5
+    cfg = E.ReportConfig(family_radius=radius, min_local_size=30)
+    r = E.report(labels, sq, c, cfg)
+    o = O.report(labels, sq, c, family_radius=radius, min_local_size=30)
+    _compare_report(r, o)
+
+
+def test_report_rejects_bad_labels(hip):
+    with pytest.raises(E.EmeansError):
+        E.report(np.array([0, 3], np.uint32), np.ones(2, np.float32), np.zeros((3, 2), np.float32))

@@ -475,3 +475,50 @@ def test_report_nearest_edge_cases():
     assert n.tolist() == [1, 0, 2, 1] and np.isinf(dd[2])
     r = O.report_nearest(c, rows=(1, 3))
     assert r[0].tolist() == [0, 2]
+
+
+def _reference_report_case():
+    # the inputs of src/report.rs:285-299, in f32 as the test builds them
+    labels, sq = [], []
+    for i in range(200):
+        off = np.float32(0.001) * np.float32(i % 100)
+        labels.append(i % 2)
+        sq.append(off * off)
+    labels.append(0)
+    sq.append(np.float32(0.45) * np.float32(0.45))
+    for _ in range(3):
+        labels.append(2)
+        sq.append(np.float32(1e-4))
+    return np.array(labels, np.uint32), np.array(sq, np.float32), np.array([[0.0], [1.0], [50.0]], np.float32)
+
+
+def test_report_reference_case():
+    # src/report.rs:283-312
+    labels, sq, c = _reference_report_case()
+    r = O.report(labels, sq, c)
+    cl = r["clusters"]
+    assert cl["size"].tolist() == [101, 100, 3]
+    score = np.fmax(r["local_z"], r["global_z"])
+    assert int(np.argmax(score)) == 200
+    assert r["local_z"][200] > 10.0
+    assert len(r["families"]) == 2 and r["families"][0]["size"] == 201
+    assert cl["family"][0] == cl["family"][1]
+    assert cl["isolation"].tolist() == [1.0, 1.0, 49.0]
+
+
+def test_report_quantiles_and_spread_closed_forms():
+    # quantile index = round half away from zero of (len - 1) q
+    # (src/report.rs:83-88): len 6 -> index 3 for the median (2.5 -> 3)
+    v = np.array([1, 2, 3, 4, 5, 6], np.float32)
+    assert O._quantile(v, 0.5) == 4.0 and O._quantile(v, 0.9) == 6.0 and O._quantile(v, 0.0) == 1.0
+    # MAD of 1..7 around the median 4: |dev| = 3,2,1,0,1,2,3 -> sorted median 2
+    assert O._median_mad(np.arange(1, 8, dtype=np.float32)) == (4.0, 2.0)
+    # one cluster of distances 3 and 4 (squared 9, 16): rms = sqrt(12.5)
+    r = O.report(np.zeros(2, np.uint32), np.array([9.0, 16.0], np.float32), np.zeros((1, 2), np.float32))
+    c = r["clusters"]
+    assert c["rms"][0] == np.float32(np.sqrt(12.5)) and c["max"][0] == 4.0 and c["median"][0] == 4.0
+    # no other live centroid: nearest is itself at +inf; the median nearest
+    # distance is then +inf too, and isolation = inf / inf = NaN
+    assert c["nearest_centroid"][0] == 0 and np.isinf(c["nearest_dist"][0]) and np.isnan(c["isolation"][0])
+    # the only cluster is its own family holding every row
+    assert len(r["families"]) == 1 and r["families"][0]["share"] == 1.0
