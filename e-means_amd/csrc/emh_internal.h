@@ -45,7 +45,8 @@ SearchGeom make_geom(int op, size_t k, size_t d);
 
 // Centroids per tile of the register-resident search (one or two 32-wide
 // MFMA blocks).
-inline int reg_tile(const SearchGeom &g) { return g.row_bytes <= 256 ? 64 : 32; }
+// (I16: 32 centroids, whose three planes are the tile's three MFMA blocks.)
+inline int reg_tile(const SearchGeom &g) { return g.op == OP_I16 ? 32 : g.row_bytes <= 256 ? 64 : 32; }
 // Bytes of the operand centroid buffer: the row-major table (k_pad rows of
 // d_pad elements; row_bytes is only the LDS segment when rows stream in
 // chunks), followed on the register-resident path (where a row is

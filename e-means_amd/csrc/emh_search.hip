@@ -47,18 +47,20 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     g.k = k;
     g.k_pad = (k + BN - 1) / BN * BN;
     if (op == OP_I16) {
-        // LDS kernel, d streamed in 128-byte chunks of the three planes;
-        // shift = largest s <= 7 with d * 2^(14 + s) < 2^31 (src/cpu_i8.rs:97)
-        g.reg = false;
-        g.resident = false;
+        // three i8 planes, rows padded to 128 bytes; shift = largest s <= 7
+        // with d * 2^(14 + s) < 2^31 (src/cpu_i8.rs:97). Rows of at most 512
+        // bytes take the register-resident kernel (tiles of 32 centroids x 3
+        // planes), wider ones the LDS kernel, d streamed in 128-byte chunks.
         g.d_pad = (d + CHUNK_BYTES - 1) / CHUNK_BYTES * CHUNK_BYTES;
-        g.row_bytes = CHUNK_BYTES;
         g.shift = 0;
         for (int s = 7; s >= 0; s--)
             if (((uint64_t)d << (14 + s)) < (1ull << 31)) {
                 g.shift = s;
                 break;
             }
+        g.resident = false;
+        g.reg = g.d_pad <= 512u && 3 * g.k_pad * g.d_pad <= 0x7FFFFFFFu;
+        g.row_bytes = g.reg ? (int)g.d_pad : CHUNK_BYTES;
         return g;
     }
     const size_t es = op_elem_bytes(op);
@@ -510,11 +512,17 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 template <int OP> struct RegScore { using T = uint32_t; };
 template <> struct RegScore<OP_F32> { using T = float; };
 template <> struct RegScore<OP_I8> { using T = int32_t; };
+template <> struct RegScore<OP_I16> { using T = int64_t; };
 
-template <int OP> using RegAcc = typename std::conditional<OP == OP_I8, i32x16, f32x16>::type;
+template <int OP> using RegAcc = typename std::conditional<OP == OP_I8 || OP == OP_I16, i32x16, f32x16>::type;
+
+// Centroids per tile of the register-resident search: the TB = 32 NB MFMA
+// rows, except for I16, whose tile holds the three planes of 32 centroids
+// as its three 32-row blocks (block b = plane b).
+template <int OP, int NB> constexpr int reg_centroids() { return OP == OP_I16 ? 32 : 32 * NB; }
 
 // Operand quads a lane reads ahead of the MFMAs that use them.
-template <int OP> struct RegPrefetch { static constexpr int D = (OP == OP_BF16 || OP == OP_I8) ? 2 : 1; };
+template <int OP> struct RegPrefetch { static constexpr int D = (OP == OP_BF16 || OP == OP_I8 || OP == OP_I16) ? 2 : 1; };
 
 // The MFMA half of a tile: acc = (score offsets) + centroid tile . row
 // fragments. sT is the tile in LDS, sH its per-centroid offsets.
@@ -529,7 +537,7 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, con
     Acc init[NB];
 #pragma unroll
     for (int nb = 0; nb < NB; nb++) {
-        if constexpr (OP == OP_I8) {
+        if constexpr (OP == OP_I8 || OP == OP_I16) {
             init[nb] = Acc{};
         } else {
 #pragma unroll
@@ -557,7 +565,7 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, con
                 }
             }
         }
-    } else if constexpr (OP == OP_BF16 || OP == OP_I8) {
+    } else if constexpr (OP == OP_BF16 || OP == OP_I8 || OP == OP_I16) {
         // a ring of PD + 1 operand slots: step s reads step s + PD's quads
         constexpr int PD = RegPrefetch<OP>::D;
         uint4 ring[PD + 1][NB];
@@ -612,7 +620,7 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, con
 // The epilogue half: fold a tile's scores into each row's running result.
 template <int OP, int M, int FR, int NB>
 __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const float *sN, int h, uint32_t c0,
-                                         typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
+                                         int shift, typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
         if constexpr (OP == OP_BF16 || OP == OP_FP8) {
@@ -635,6 +643,20 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
                     bs[fr][0] = u ? s : bs[fr][0];
                     bi[fr][0] = u ? c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3) : bi[fr][0];
                 }
+        } else if constexpr (OP == OP_I16) {
+            // score = cn - 2^(shift+1) x.c16, x.c16 = p0 + 2^7 p1 + 2^14 p2 taken
+            // mod 2^32 as the reference's i32 sum; strict '<' in ascending index
+            const int64_t *sC = (const int64_t *)sN;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int cl = 8 * (r >> 2) + 4 * h + (r & 3);
+                const uint32_t dv = (uint32_t)acc[fr][0][r] + ((uint32_t)acc[fr][1][r] << 7) +
+                                    ((uint32_t)acc[fr][2][r] << 14);
+                const int64_t sc = sC[cl] - ((int64_t)2 << shift) * (int64_t)(int32_t)dv;
+                const bool u = sc < bs[fr][0];
+                bs[fr][0] = u ? sc : bs[fr][0];
+                bi[fr][0] = u ? c0 + (uint32_t)cl : bi[fr][0];
+            }
         } else {  // OP_I8: maximise key = (dot << 6) + ((-cn) << 5) + 31 - position
             const int32_t *sK = (const int32_t *)sN;
             int32_t g = INT32_MIN;
@@ -664,16 +686,18 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     constexpr int ES = OpT<OP>::ES;
     constexpr int RB = 32 * NS;           // operand bytes per row
     constexpr int HB = RB / 2;            // bytes per lane half
-    constexpr int TB = 32 * NB;           // centroids per tile
+    constexpr int TB = 32 * NB;           // MFMA rows per tile
+    constexpr int CT = reg_centroids<OP, NB>();  // centroids per tile
     constexpr int TILE = TB * RB;         // bytes per tile
     constexpr int PER_T = TILE / 4096;    // 1 KiB wave-loads per wave per tile
     constexpr bool CAND = OP == OP_BF16 || OP == OP_FP8;
     using Score = typename RegScore<OP>::T;
     static_assert(NS % 2 == 0 && PER_T >= 1 && TILE % 4096 == 0, "shape");
+    static_assert(OP != OP_I16 || NB == 3, "an I16 tile is three planes");
     __shared__ __align__(16) uint8_t sC0[TILE];
     __shared__ __align__(16) uint8_t sC1[TILE];
     __shared__ __align__(16) uint8_t sC2[TILE];
-    __shared__ float sH0[256], sH1[256], sH2[256];
+    __shared__ __align__(16) float sH0[256], sH1[256], sH2[256];
     __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -755,8 +779,8 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     // Tile DMA: wave-load q of wave w copies tile bytes [(4q + w) KiB, +1 KiB)
     // to the same place in LDS; lane L moves 16 bytes at L * 16. The buffer
     // resource spans the tile-major table (its size is checked on the host).
-    const __amdgpu_buffer_rsrc_t rsrc =
-        __builtin_amdgcn_make_buffer_rsrc((void *)a.cen_op, 0, (int)(a.k_pad * (size_t)RB), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)a.cen_op, 0, (int)(a.k_pad / CT * (size_t)TILE), 0x00020000);
     auto dma = [&](uint8_t *dst, int t) {
         const uint32_t so = (uint32_t)t * TILE + (uint32_t)wv * 1024u;
 #pragma unroll
@@ -771,9 +795,11 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     // I8 loads the low word of cn only (|cn| <= 2^26 on this path): a dead
     // high word still being written by a load would make the compiler wait
     // for it, and with it for the tile DMA, before reusing its register.
-    using HRaw = typename std::conditional<OP == OP_I8, int32_t, float>::type;
+    // I16 loads both words of its 32 i64 offsets, one word per thread.
+    using HRaw = typename std::conditional<OP == OP_I8 || OP == OP_I16, int32_t, float>::type;
     auto hload = [&](int t) -> HRaw {
         // every thread loads (threads past TB repeat an entry): no branch
+        if constexpr (OP == OP_I16) return ((const int32_t *)a.cn)[(size_t)t * 64 + (tid & 63)];
         const size_t ci = (size_t)t * TB + (tid & (TB - 1));
         if constexpr (OP == OP_I8) return ((const int32_t *)a.cn)[2 * ci];  // little-endian low word
         else return a.hn[ci];
@@ -781,7 +807,9 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     // every thread stores (slots past TB are never read): no branch, so the
     // compiler's wait tracking stays exact across the stage
     auto hstore = [&](float *dst, HRaw v) {
-        if constexpr (OP == OP_I8) {
+        if constexpr (OP == OP_I16) {
+            dst[tid] = __int_as_float(v);  // words 0..63: the tile's offsets, as int64
+        } else if constexpr (OP == OP_I8) {
             // centroid c of the tile is value r of block nb for lanes of half
             // (c >> 2) & 1, with position nb * 16 + r in the fold
             const int c = tid & (TB - 1), cl = c & 31;
@@ -801,6 +829,7 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
 #pragma unroll
         for (int t = 0; t < M; t++) {
             if constexpr (OP == OP_I8) bs[fr][t] = INT32_MIN;
+            else if constexpr (OP == OP_I16) bs[fr][t] = INT64_MAX;
             else if constexpr (OP == OP_F32) bs[fr][t] = INFINITY;
             else bs[fr][t] = 0xFFFFFFFFu;
             bi[fr][t] = NO_LABEL;
@@ -840,7 +869,7 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
         dma(BN2, tn);                                                             \
         HV2 = hload(tn);                                                          \
         reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);                  \
-        reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * TB), bs, bi);        \
+        reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * CT), a.shift, bs, bi); \
         __builtin_amdgcn_sched_barrier(0); /* keep the wait after the compute */  \
         __builtin_amdgcn_s_waitcnt(WAIT_ONE);                                     \
         hstore(BH1, HV1);                                                         \
@@ -880,7 +909,9 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
                 }
             }
         } else {
-            const Score ps = __shfl_xor(bs[fr][0], 32, 64);
+            Score ps;
+            if constexpr (OP == OP_I16) ps = shfl_xor_i64(bs[fr][0], 32);
+            else ps = __shfl_xor(bs[fr][0], 32, 64);
             const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][0], 32, 64);
             const bool better = OP == OP_I8 ? (ps > bs[fr][0] || (ps == bs[fr][0] && pi < bi[fr][0]))
                                             : (ps < bs[fr][0] || (ps == bs[fr][0] && pi < bi[fr][0]));
@@ -894,15 +925,27 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
                     // dist = |x|^2 + |c|^2 + bias - 2 x.c, clamped at 0 (src/cpu.rs:160-162)
                     const float v = __fadd_rn(__fmul_rn(2.0f, (float)bs[fr][0]), xn[fr]);
                     dv = v > 0.0f ? v : 0.0f;
+                } else if constexpr (OP == OP_I16) {
+                    // dist = ((|x|^2 + best * 4^-shift) as f64).max(0) as f32
+                    // (src/cpu_i8.rs:183-186; the power-of-two scaling is exact)
+                    const double v = __dadd_rn((double)(int64_t)xni[fr], ldexp((double)bs[fr][0], -2 * a.shift));
+                    dv = (float)(v > 0.0 ? v : 0.0);
                 } else {
                     // dist = ((|x|^2 + best) as f64).max(0) as f32 (src/cpu_i8.rs:183-186)
                     const double v = __dadd_rn((double)xni[fr], -(double)(int32_t)bs[fr][0]);
                     dv = (float)(v > 0.0 ? v : 0.0);
                 }
                 if (a.nslice > 1) {
-                    // (score, index, distance) of this slice's best
+                    // (score low word, index, distance, score high word) of this slice's best
                     uint32_t *pp = a.part + (r * a.nslice + slice) * 4;
-                    *(uint4 *)pp = make_uint4(__builtin_bit_cast(uint32_t, bs[fr][0]), bi[fr][0], __float_as_uint(dv), 0u);
+                    uint32_t lo, hi = 0u;
+                    if constexpr (OP == OP_I16) {
+                        lo = (uint32_t)(uint64_t)bs[fr][0];
+                        hi = (uint32_t)((uint64_t)bs[fr][0] >> 32);
+                    } else {
+                        lo = __builtin_bit_cast(uint32_t, bs[fr][0]);  // f32 or int32 score bits
+                    }
+                    *(uint4 *)pp = make_uint4(lo, bi[fr][0], __float_as_uint(dv), hi);
                 } else {
                     a.labels[r] = bi[fr][0] == NO_LABEL ? 0u : bi[fr][0];
                     a.dist[r] = dv;
@@ -914,7 +957,7 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
 
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
-    const int ntiles = (int)(a.k_pad / (32 * NB));
+    const int ntiles = (int)(a.k_pad / reg_centroids<OP, NB>());
     const int s = blockIdx.y, t0 = s * a.tiles_per_slice;
     const int t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
     search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR), t0, t1, s);
@@ -950,6 +993,10 @@ __global__ void merge_slices_kernel(const uint32_t *part, size_t m, int nslice, 
             if constexpr (OP == OP_F32) {
                 const float ps = __uint_as_float(p.x), bsv = __uint_as_float(best.x);
                 better = ps < bsv || (ps == bsv && p.y < best.y);
+            } else if constexpr (OP == OP_I16) {
+                const int64_t ps = (int64_t)(((uint64_t)p.w << 32) | p.x);
+                const int64_t bsv = (int64_t)(((uint64_t)best.w << 32) | best.x);
+                better = ps < bsv || (ps == bsv && p.y < best.y);
             } else {
                 const int32_t ps = (int32_t)p.x, bsv = (int32_t)best.x;
                 better = ps > bsv || (ps == bsv && p.y < best.y);
@@ -976,13 +1023,17 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
 // Copies the row-major operand table into the tile-major order of the
 // register-resident search: quad q of centroid c of tile t goes to quad
 // (t * nq + q) * tb + c. One thread per 16-byte quad.
-__global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, int nq, int tb) {
+// With planes > 1 (I16) the source holds `planes` such tables one after the
+// other and tile t's MFMA row p * tb + c is plane p of centroid t * tb + c.
+__global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, int nq, int tb, int planes) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= k_pad * (size_t)nq) return;
-    const size_t per_tile = (size_t)nq * tb;
+    if (i >= (size_t)planes * k_pad * (size_t)nq) return;
+    const size_t vt = (size_t)tb * planes;  // MFMA rows per tile
+    const size_t per_tile = (size_t)nq * vt;
     const size_t t = i / per_tile, rem = i - t * per_tile;
-    const size_t q = rem / tb, c = rem - q * tb;
-    dst[i] = src[(t * tb + c) * nq + q];
+    const size_t q = rem / vt, v = rem - q * vt;
+    const size_t p = v / tb, c = v - p * tb;
+    dst[i] = src[(p * k_pad + t * tb + c) * nq + q];
 }
 
 // -------------------------------------------------------------- rescore ----
@@ -1257,8 +1308,9 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
     if (bias && jc < k && bias[jc] < 0.0f) atomicMax(&stats[1], __float_as_uint(-bias[jc]));
     if (op == OP_I16) {
         // offset = sum c16^2 + round(bias * 4^shift) (src/cpu_i8.rs:112-117);
-        // the search skips indices >= k
-        int64_t v = 0;
+        // pad centroids (zero planes) score INT64_MAX, which never wins the
+        // register kernel's strict '<'; the LDS kernel skips indices >= k
+        int64_t v = INT64_MAX;
         if (jc < k) {
             v = norm64[jc];
             if (bias) {
@@ -1371,7 +1423,7 @@ bool reg_path(const SearchGeom &g) { return g.reg; }
 
 template <int OP, int ROWS, int M, int NS>
 static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
-    constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
+    constexpr int FR = 2, NB = OP == OP_I16 ? 3 : (NS <= 8 ? 2 : 1);
     dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)), (unsigned)a.nslice);
     hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
@@ -1390,6 +1442,18 @@ static hipError_t launch_reg_ns(const SearchGeom &g, const SearchArgs &a, hipStr
         case 8: return launch_reg<OP, ROWS, M, 8>(a, s);
         case 12: return launch_reg<OP, ROWS, M, 12>(a, s);
         case 16: return launch_reg<OP, ROWS, M, 16>(a, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// I16 rows are multiples of 128 bytes (a tile of three planes is then a
+// whole number of 4 KiB workgroup loads)
+static hipError_t launch_reg_i16(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
+    switch (g.row_bytes / 32) {
+        case 4: return launch_reg<OP_I16, EMH_ROWS_I8, 1, 4>(a, s);
+        case 8: return launch_reg<OP_I16, EMH_ROWS_I8, 1, 8>(a, s);
+        case 12: return launch_reg<OP_I16, EMH_ROWS_I8, 1, 12>(a, s);
+        case 16: return launch_reg<OP_I16, EMH_ROWS_I8, 1, 16>(a, s);
         default: return hipErrorInvalidValue;
     }
 }
@@ -1472,6 +1536,9 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
                 // scores past the packed key's range take the LDS kernel's i64 path
                 if (g.i8_packed) return launch_reg_ns<OP_I8, EMH_ROWS_I8, 1>(g, r, s);
                 break;
+            case OP_I16:
+                if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
+                return launch_reg_i16(g, r, s);
         }
     }
     // LDS kernel (rows resident in LDS, or streamed in d-chunks)
@@ -1535,10 +1602,10 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !g.reg) return e;
-    const int nq = g.row_bytes / 16, tb = reg_tile(g);
-    const size_t quads = g.k_pad * (size_t)nq;
+    const int nq = g.row_bytes / 16, tb = reg_tile(g), planes = g.op == OP_I16 ? 3 : 1;
+    const size_t quads = (size_t)planes * g.k_pad * (size_t)nq;
     hipLaunchKernelGGL(tile_layout_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s,
-                       (const uint4 *)cen_op, (uint4 *)reg_operand(g, cen_op), g.k_pad, nq, tb);
+                       (const uint4 *)cen_op, (uint4 *)reg_operand(g, cen_op), g.k_pad, nq, tb, planes);
     return hipGetLastError();
 }
 

@@ -177,15 +177,20 @@ def test_i16_fractional_centroids_and_bias_match_cpu(hip, d):
         assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias={b is not None}"
 
 
-def test_i16_dot_wraps_like_the_cpu(hip):
-    """Centroids past the i8 range at d = 1024: the i32 dot product wraps on
-    the CPU (vpdpwssd) and must wrap the same way on the GPU."""
+@pytest.mark.parametrize("d", [512, 1024])
+def test_i16_dot_wraps_like_the_cpu(hip, d):
+    """Centroids past the i8 range: the i32 dot product wraps on the CPU
+    (vpdpwssd) and must wrap the same way on the GPU (d = 512: the
+    register-resident kernel, where rows of -128 against centroids clamped
+    to -32768 sum to exactly 2^31; d = 1024: the LDS kernel)."""
     rng = np.random.default_rng(11)
-    n, k, d = 40, 9, 1024
+    n, k = 40, 9
     x = rng.integers(100, 128, size=(n, d)).astype(np.int8)
+    x[:5] = -128
     c = np.full((k, d), 600.0, np.float32)
     c[1::2] *= -1
-    c[:, :5] = rng.normal(size=(k, 5))
+    if d > 512:
+        c[:, :5] = rng.normal(size=(k, 5))
     a = E.HipAssigner(E.I16)
     a.set_centroids(c)
     gl, gd = a.assign(x)
@@ -272,7 +277,7 @@ def test_name_and_state_errors(hip):
         E.HipAssigner(E.I8).set_centroids(np.zeros((1, 40000), np.float32))
 
 
-@pytest.mark.parametrize("prec,rows", [("fp32", "f32"), ("i8", "i8"), ("bf16", "f32"), ("fp8", "f32")])
+@pytest.mark.parametrize("prec,rows", [("fp32", "f32"), ("i8", "i8"), ("i16", "i8"), ("bf16", "f32"), ("fp8", "f32")])
 def test_centroid_split_does_not_change_results(hip, prec, rows):
     """The search split over centroid slices (forced 1..8 and autotuned,
     uneven tile counts) returns the same labels and distances bit for bit."""
@@ -296,6 +301,6 @@ def test_centroid_split_does_not_change_results(hip, prec, rows):
         if ref is None:
             ref = got
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), f"split {ns}"
-    if prec == "i8":
-        cl, cd = O.i8_assign(x, c, bias=bias)
+    if prec in ("i8", "i16"):
+        cl, cd = O.i8_assign(x, c, bits=O.BITS_I16 if prec == "i16" else O.BITS_I8, bias=bias)
         assert np.array_equal(ref[0], cl) and np.array_equal(ref[1], cd)
