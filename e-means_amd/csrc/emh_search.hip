@@ -1039,94 +1039,112 @@ __global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, i
 // -------------------------------------------------------------- rescore ----
 
 // Exact FP32 distances |x - c|^2 + bias to the m candidates of each row;
-// the lowest (score, index) wins and dist = max(score, 0). One wave per row
-// at a time: 16 lanes per (row, candidate) pair stride over d with 16-byte
-// loads, each lane folds its elements in order without FMA, and a fixed
-// butterfly adds the 16 partial sums, so every lane holds identical bits.
-// One row, by one wave.
+// the lowest (score, index) wins and dist = max(score, 0). A group of 16
+// lanes owns one row (four rows per wave): it reads the row's candidate ids
+// once, then streams the row and all M candidate rows together (16-byte
+// loads, every candidate's loads in flight at once); each lane folds its
+// strided elements in order without FMA, per candidate, and a fixed
+// butterfly adds the 16 partial sums, so every lane of the group holds
+// identical bits.
 template <int ROWS, int M>
 __device__ __forceinline__ void rescore_row(const void *x, size_t d, size_t k, const uint32_t *ids, const float *c,
                                             const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
-                                            size_t row, int lane) {
-    constexpr int G = 16, PPW = 64 / G, PASSES = (M + PPW - 1) / PPW;
-    const int grp = lane / G, gl = lane % G;
+                                            size_t row, bool live, int gl) {
+    constexpr int G = 16;
     const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0;
-    const size_t src = ids ? (size_t)ids[row] : row;
+    uint32_t mine = NO_LABEL;
+    if (live && gl < M) {
+        mine = cand[row * M + gl];
+        if (mine >= k) mine = NO_LABEL;  // pad centroids can fill a list when k is small
+    }
+    uint32_t ci[M];
+#pragma unroll
+    for (int t = 0; t < M; t++) ci[t] = (uint32_t)__shfl((int)mine, t, G);
+    const size_t src = live ? (ids ? (size_t)ids[row] : row) : 0;
+    float acc[M];
+#pragma unroll
+    for (int t = 0; t < M; t++) acc[t] = 0.0f;
+    if (live) {
+        if (vec) {
+            const float *xr = (const float *)x + src * d;
+            for (size_t e = 4 * gl; e < d; e += 4 * G) {
+                const float4 xv = *(const float4 *)(xr + e);
+#pragma unroll
+                for (int t = 0; t < M; t++) {
+                    if (ci[t] == NO_LABEL) continue;
+                    const float4 cv = *(const float4 *)(c + (size_t)ci[t] * d + e);
+                    const float t0 = __fsub_rn(xv.x, cv.x), t1 = __fsub_rn(xv.y, cv.y);
+                    const float t2 = __fsub_rn(xv.z, cv.z), t3 = __fsub_rn(xv.w, cv.w);
+                    acc[t] = __fadd_rn(acc[t], __fmul_rn(t0, t0));
+                    acc[t] = __fadd_rn(acc[t], __fmul_rn(t1, t1));
+                    acc[t] = __fadd_rn(acc[t], __fmul_rn(t2, t2));
+                    acc[t] = __fadd_rn(acc[t], __fmul_rn(t3, t3));
+                }
+            }
+        } else {
+            for (size_t e = gl; e < d; e += G) {
+                const float xv = src_elem<ROWS>(x, src * d + e);
+#pragma unroll
+                for (int t = 0; t < M; t++) {
+                    if (ci[t] == NO_LABEL) continue;
+                    const float u = __fsub_rn(xv, c[(size_t)ci[t] * d + e]);
+                    acc[t] = __fadd_rn(acc[t], __fmul_rn(u, u));
+                }
+            }
+        }
+    }
     float best = INFINITY;
     uint32_t bidx = NO_LABEL;
 #pragma unroll
-    for (int p = 0; p < PASSES; p++) {
-        const int slot = p * PPW + grp;
-        uint32_t ci = slot < M ? cand[row * M + slot] : NO_LABEL;
-        if (ci >= k) ci = NO_LABEL;  // pad centroids can fill a list when k is small
-        float acc = 0.0f;
-        if (ci != NO_LABEL) {
-            const float *cp = c + (size_t)ci * d;
-            if (vec) {
-                const float *xr = (const float *)x + src * d;
-                for (size_t e = 4 * gl; e < d; e += 4 * G) {
-                    const float4 xv = *(const float4 *)(xr + e), cv = *(const float4 *)(cp + e);
-                    float t0 = __fsub_rn(xv.x, cv.x), t1 = __fsub_rn(xv.y, cv.y);
-                    float t2 = __fsub_rn(xv.z, cv.z), t3 = __fsub_rn(xv.w, cv.w);
-                    acc = __fadd_rn(acc, __fmul_rn(t0, t0));
-                    acc = __fadd_rn(acc, __fmul_rn(t1, t1));
-                    acc = __fadd_rn(acc, __fmul_rn(t2, t2));
-                    acc = __fadd_rn(acc, __fmul_rn(t3, t3));
-                }
-            } else {
-                for (size_t e = gl; e < d; e += G) {
-                    const float t = __fsub_rn(src_elem<ROWS>(x, src * d + e), cp[e]);
-                    acc = __fadd_rn(acc, __fmul_rn(t, t));
-                }
-            }
-        }
+    for (int t = 0; t < M; t++) {
+        float a = acc[t];
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc = __fadd_rn(acc, __shfl_xor(acc, off, 64));
-        float s = INFINITY;
-        if (ci != NO_LABEL) s = bias ? __fadd_rn(acc, bias[ci]) : acc;
-        uint32_t si = ci;
-#pragma unroll
-        for (int off = G; off < 64; off <<= 1) {
-            const float os = __shfl_xor(s, off, 64);
-            const uint32_t oi = (uint32_t)__shfl_xor((int)si, off, 64);
-            if (os < s || (os == s && oi < si)) {
-                s = os;
-                si = oi;
-            }
-        }
-        if (s < best || (s == best && si < bidx)) {
-            best = s;
-            bidx = si;
+        for (int off = G / 2; off > 0; off >>= 1) a = __fadd_rn(a, __shfl_xor(a, off, 64));
+        if (ci[t] == NO_LABEL) continue;
+        const float sc = bias ? __fadd_rn(a, bias[ci[t]]) : a;
+        if (sc < best || (sc == best && ci[t] < bidx)) {
+            best = sc;
+            bidx = ci[t];
         }
     }
-    if (lane == 0) {
+    if (live && gl == 0) {
         labels[row] = bidx == NO_LABEL ? 0u : bidx;
         dist[row] = (bidx != NO_LABEL && best > 0.0f) ? best : 0.0f;
     }
 }
 
+// Rows per workgroup pass: 4 waves x 4 rows.
+constexpr int RESCORE_ROWS = 16;
+
 template <int ROWS, int M>
 __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, size_t d, size_t k, const uint32_t *ids,
                                                       const float *c, const float *bias, const uint32_t *cand,
                                                       uint32_t *labels, float *dist) {
-    const int lane = threadIdx.x & 63;
-    const size_t nwaves = (size_t)gridDim.x * 4;
-    for (size_t row = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); row < m; row += nwaves)
-        rescore_row<ROWS, M>(x, d, k, ids, c, bias, cand, labels, dist, row, lane);
+    const int gl = threadIdx.x & 15;
+    const size_t stride = (size_t)gridDim.x * RESCORE_ROWS;
+    for (size_t base = (size_t)blockIdx.x * RESCORE_ROWS; base < m; base += stride) {
+        const size_t row = base + (threadIdx.x >> 4);
+        rescore_row<ROWS, M>(x, d, k, ids, c, bias, cand, labels, dist, row, row < m, gl);
+    }
 }
 
 // Every job's rows: slot i of the concatenated batch is row rmap[i].y of job
 // rmap[i].x (live when below that job's row count this step).
 template <int M>
 __global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
-    const int lane = threadIdx.x & 63;
-    const size_t nwaves = (size_t)gridDim.x * 4;
-    for (size_t i = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); i < slots; i += nwaves) {
-        const uint2 r = rmap[i];
-        const JobDev &j = jobs[r.x];
-        if (r.y >= j.m) continue;
-        rescore_row<EMH_ROWS_F32, M>(j.x, j.d, j.k, j.ids, j.cen, j.penalized ? j.bias : nullptr, j.cand, j.labels,
-                                     j.dist, r.y, lane);
+    const int gl = threadIdx.x & 15;
+    const size_t stride = (size_t)gridDim.x * RESCORE_ROWS;
+    for (size_t base = (size_t)blockIdx.x * RESCORE_ROWS; base < slots; base += stride) {
+        const size_t i = base + (threadIdx.x >> 4);
+        uint2 r = make_uint2(0u, 0u);
+        bool live = i < slots;
+        if (live) r = rmap[i];
+        const JobDev *j = live ? &jobs[r.x] : &jobs[0];
+        live = live && r.y < j->m;
+        // a group's d (and so its trip counts) comes from its own job: the
+        // shuffles are within the 16 lanes of the group only
+        rescore_row<EMH_ROWS_F32, M>(j->x, j->d, j->k, j->ids, j->cen, j->penalized ? j->bias : nullptr, j->cand,
+                                     j->labels, j->dist, r.y, live, gl);
     }
 }
 
@@ -1344,41 +1362,73 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
     }
 }
 
-// Per-row e4m3 encoding with a power-of-two scale: one wave per row.
+// Per-row e4m3 encoding with a power-of-two scale: 16 lanes per row, four
+// rows per wave. Pass 1 takes the row's largest finite |v| and |x|^2, pass 2
+// (the row again, now from cache) writes the scaled, packed codes.
 template <int ROWS>
 __global__ __launch_bounds__(256) void rows_fp8_kernel(const void *x, size_t m, size_t d, size_t d_pad,
                                                        const uint32_t *ids, uint8_t *x8, float *row_scale,
                                                        float *row_norm) {
-    const int lane = threadIdx.x & 63;
-    const size_t r = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= m) return;
-    const size_t src = ids ? (size_t)ids[r] : r;
-    float mx = 0.0f, sq = 0.0f;
-    for (size_t e = lane; e < d; e += 64) {
-        const float v = src_elem<ROWS>(x, src * d + e);
-        const float a = fabsf(v);
-        if (a > mx && isfinite(a)) mx = a;
-        sq = fmaf(v, v, sq);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-        sq += __shfl_xor(sq, off, 64);
-    }
-    if (lane == 0) row_norm[r] = sq;
-    const float s = fp8_scale_for(mx);
-    uint32_t *o = (uint32_t *)(x8 + r * d_pad);
-    for (size_t w = lane; w < d_pad / 4; w += 64) {
-        float v[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            size_t e = 4 * w + q;
-            v[q] = e < d ? src_elem<ROWS>(x, src * d + e) * s : 0.0f;
+    constexpr int G = 16;
+    const int gl = threadIdx.x & (G - 1);
+    const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0 && ((uintptr_t)x % 16) == 0;
+    const size_t stride = (size_t)gridDim.x * 16;
+    for (size_t base = (size_t)blockIdx.x * 16; base < m; base += stride) {
+        const size_t r = base + (threadIdx.x >> 4);
+        const bool live = r < m;
+        const size_t src = live ? (ids ? (size_t)ids[r] : r) : 0;
+        float mx = 0.0f, sq = 0.0f;
+        auto take = [&](float v) {
+            const float a = fabsf(v);
+            if (a > mx && isfinite(a)) mx = a;
+            sq = fmaf(v, v, sq);
+        };
+        if (live) {
+            if (vec) {
+                const float *xr = (const float *)x + src * d;
+                for (size_t e = 4 * gl; e < d; e += 4 * G) {
+                    const float4 v = *(const float4 *)(xr + e);
+                    take(v.x);
+                    take(v.y);
+                    take(v.z);
+                    take(v.w);
+                }
+            } else {
+                for (size_t e = gl; e < d; e += G) take(src_elem<ROWS>(x, src * d + e));
+            }
         }
-        int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
-        pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
-        o[w] = (uint32_t)pk;
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+            sq += __shfl_xor(sq, off, 64);
+        }
+        if (!live) continue;
+        const float s = fp8_scale_for(mx);
+        uint32_t *o = (uint32_t *)(x8 + r * d_pad);
+        for (size_t w = gl; w < d_pad / 4; w += G) {
+            float v[4];
+            if (vec && 4 * w + 3 < d) {
+                const float4 q = *(const float4 *)((const float *)x + src * d + 4 * w);
+                v[0] = q.x * s;
+                v[1] = q.y * s;
+                v[2] = q.z * s;
+                v[3] = q.w * s;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const size_t e = 4 * w + q;
+                    v[q] = e < d ? src_elem<ROWS>(x, src * d + e) * s : 0.0f;
+                }
+            }
+            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+            o[w] = (uint32_t)pk;
+        }
+        if (gl == 0) {
+            row_norm[r] = sq;
+            row_scale[r] = s;
+        }
     }
-    if (lane == 0) row_scale[r] = s;
 }
 
 // ------------------------------------------------------------- launchers ---
@@ -1568,7 +1618,7 @@ template <int ROWS>
 static hipError_t rescore_rows(int M, const void *x, size_t m, size_t d, size_t k, const uint32_t *ids, const float *c,
                                const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
                                hipStream_t s) {
-    dim3 grid((unsigned)std::min<size_t>((m + 3) / 4, 16384));
+    dim3 grid((unsigned)std::min<size_t>((m + RESCORE_ROWS - 1) / RESCORE_ROWS, 16384));
     switch (M) {
         case 1: hipLaunchKernelGGL((rescore_kernel<ROWS, 1>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
         case 2: hipLaunchKernelGGL((rescore_kernel<ROWS, 2>), grid, dim3(256), 0, s, x, m, d, k, ids, c, bias, cand, labels, dist); break;
@@ -1685,7 +1735,7 @@ hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_a
 
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
     if (!slots) return hipSuccess;
-    dim3 grid((unsigned)std::min<size_t>((slots + 3) / 4, 16384));
+    dim3 grid((unsigned)std::min<size_t>((slots + RESCORE_ROWS - 1) / RESCORE_ROWS, 16384));
     switch (M) {
         case 1: hipLaunchKernelGGL(rescore_jobs_kernel<1>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
         case 2: hipLaunchKernelGGL(rescore_jobs_kernel<2>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
@@ -1716,7 +1766,7 @@ hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
                            uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s) {
     if (m == 0) return hipSuccess;
-    dim3 grid((unsigned)((m + 3) / 4));
+    dim3 grid((unsigned)std::min<size_t>((m + 15) / 16, 16384));
     if (rows == EMH_ROWS_F32) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
     else if (rows == EMH_ROWS_I8) hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
     else hipLaunchKernelGGL(rows_fp8_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, m, d, d_pad, ids, x8, row_scale, row_norm);
