@@ -429,7 +429,8 @@ int emh_name(const emh_assigner *h, char *buf, size_t cap) {
         n = snprintf(tmp, sizeof tmp, "HIP %s, %s", kind[h->precision], h->device_name.c_str());
     if (h->has_centroids)
         snprintf(tmp + n, sizeof tmp - n, ", k=%zu d=%zu (%s rows%s)", h->g.k, h->g.d,
-                 h->g.resident ? "LDS-resident" : "chunked", h->last_slices > 1 ? ", centroids split" : "");
+                 h->g.reg ? "register-resident" : h->g.resident ? "LDS-resident" : "chunked",
+                 h->last_slices > 1 ? ", centroids split" : "");
     strncpy(buf, tmp, cap - 1);
     buf[cap - 1] = 0;
     return EMH_OK;
