@@ -678,6 +678,89 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
     }
 }
 
+// The candidate searches on two-block tiles (BF16 / FP8, rows of at most
+// 256 bytes): block 0's MFMAs first, then block 1's with block 0's fold
+// between them, so half of the fold runs while the matrix cores work; then
+// block 1's fold and the top-m insertion. Same keys, same minimum, same
+// insertion as reg_mfma + reg_fold.
+template <int OP, int M, int NS, int FR>
+__device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *sH, const uint4 (&xb)[FR][NS],
+                                               const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
+                                               f32x16 (&acc)[FR][2], uint32_t (&bs)[FR][M],
+                                               uint32_t (&bi)[FR][M]) {
+    constexpr int TB = 64, QS = TB * 16;
+    constexpr int STEPS = OP == OP_FP8 ? NS / 2 : NS;  // MFMAs per fragment and block
+    constexpr int PER = 16 / STEPS > 0 ? 16 / STEPS : 1;  // block-0 values folded per step
+    const uint8_t *ab = sT + (h * NS * TB + j) * 16;
+    auto ld = [&](int q, int nb) -> uint4 { return *(const uint4 *)(ab + q * QS + nb * 512); };
+    auto key = [&](float v, uint32_t pos) -> uint32_t { return (__float_as_uint(v) & 0xFFFFFFE0u) | pos; };
+    auto mfma = [&](int st, int fr, const uint4 (&ops)[NS], const f32x16 &c) -> f32x16 {
+        if constexpr (OP == OP_FP8) {
+            const uint4 a0 = ops[2 * st], a1 = ops[2 * st + 1], x0 = xb[fr][2 * st], x1 = xb[fr][2 * st + 1];
+            const i32x8 cv = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+            const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
+            return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, xv, c, 0, 0, 0, sa, 0, sb[fr]);
+        } else {
+            return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ops[st]),
+                                                           __builtin_bit_cast(bf16x8, xb[fr][st]), c, 0, 0, 0);
+        }
+    };
+    uint32_t g[FR];
+    {  // block 0
+        uint4 ops[NS];
+#pragma unroll
+        for (int q = 0; q < NS; q++) ops[q] = ld(q, 0);
+        f32x16 init;
+#pragma unroll
+        for (int r = 0; r < 16; r++) init[r] = sH[8 * (r >> 2) + 4 * h + (r & 3)];
+#pragma unroll
+        for (int st = 0; st < STEPS; st++)
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) acc[fr][0] = mfma(st, fr, ops, st == 0 ? init : acc[fr][0]);
+    }
+    {  // block 1, block 0 folded in between
+        uint4 ops[NS];
+#pragma unroll
+        for (int q = 0; q < NS; q++) ops[q] = ld(q, 1);
+        f32x16 init;
+#pragma unroll
+        for (int r = 0; r < 16; r++) init[r] = sH[32 + 8 * (r >> 2) + 4 * h + (r & 3)];
+#pragma unroll
+        for (int fr = 0; fr < FR; fr++) g[fr] = 0xFFFFFFFFu;
+#pragma unroll
+        for (int st = 0; st < STEPS; st++) {
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) acc[fr][1] = mfma(st, fr, ops, st == 0 ? init : acc[fr][1]);
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+                for (int u = 0; u < PER; u++) {
+                    const int r = st * PER + u;
+                    if (r < 16) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
+                }
+            // keep this step's share of the fold next to its MFMAs: the empty
+            // asm pins the partial minimum here (the optimiser would otherwise
+            // gather every min into one tree after the last MFMA), the
+            // scheduling barrier keeps the MFMAs from running together
+#pragma unroll
+            for (int fr = 0; fr < FR; fr++) asm volatile("" : "+v"(g[fr]));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+            for (int r = STEPS * PER; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
+    }
+#pragma unroll
+    for (int fr = 0; fr < FR; fr++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][1][r], (uint32_t)(16 + r)));
+        const uint32_t li = g[fr] & 31u, nb = li >> 4, r = li & 15u;
+        const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        topm_insert<M, uint32_t>(bs[fr], bi[fr], g[fr], gi);
+    }
+}
+
 // One workgroup's sweep: rows [row0, row0 + 128 FR) of a.x against every
 // centroid tile.
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
@@ -868,8 +951,12 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
         const int tn = (T) + 2 < t1 ? (T) + 2 : t1 - 1;                           \
         dma(BN2, tn);                                                             \
         HV2 = hload(tn);                                                          \
-        reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);                  \
-        reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * CT), a.shift, bs, bi); \
+        if constexpr (CAND && NB == 2) {                                          \
+            reg_tile_cand2<OP, M, NS, FR>(BC, BH, xb, sb, sa, h, j, (uint32_t)((T) * CT), acc, bs, bi); \
+        } else {                                                                  \
+            reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);              \
+            reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * CT), a.shift, bs, bi); \
+        }                                                                         \
         __builtin_amdgcn_sched_barrier(0); /* keep the wait after the compute */  \
         __builtin_amdgcn_s_waitcnt(WAIT_ONE);                                     \
         hstore(BH1, HV1);                                                         \
