@@ -1261,8 +1261,7 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
                                                              int64_t *norm64, const unsigned *cmax_bits,
                                                              unsigned *stats) {
     const int lane = threadIdx.x & 63;
-    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (jc >= k_pad) return;
+    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // < k_pad: the grid is k_pad / 4 blocks
     const bool live = jc < k;
     const float *cp = c + jc * d;
     float cs = 1.0f;
@@ -1305,11 +1304,20 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
         if (lane == 0 && live) norm64[jc] = part;
     }
     if constexpr (OP != OP_I8) {
+        float nv = 0.0f;
         if (live) {
-            const float nv = centroid_norm32(cp, d, lane);
-            if (lane == 0) {
-                norm32[jc] = nv;
-                if (nv >= 0.0f) atomicMax(&stats[0], __float_as_uint(nv));  // NaN norms are skipped
+            nv = centroid_norm32(cp, d, lane);
+            if (lane == 0) norm32[jc] = nv;
+        }
+        // max |c|^2 (the candidate searches' positivity offset): one atomic
+        // per workgroup, not per centroid; NaN norms are skipped
+        if constexpr (OP == OP_BF16 || OP == OP_FP8) {
+            __shared__ unsigned wmax[4];
+            if (lane == 0) wmax[threadIdx.x >> 6] = (live && nv >= 0.0f) ? __float_as_uint(nv) : 0u;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                const unsigned mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+                if (mx) atomicMax(&stats[0], mx);
             }
         }
     }
