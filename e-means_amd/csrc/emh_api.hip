@@ -234,9 +234,15 @@ emh_assigner::~emh_assigner() {
 
 // I8: whether every score fits the packed int32 key of the register-resident
 // kernel (flag written by the bias kernel); otherwise the i64 kernel runs.
-static int refresh_i8_range(emh_assigner *h) {
+static int refresh_i8_range(emh_assigner *h, bool with_bias) {
     h->g.i8_packed = false;
     if (h->op != OP_I8 || !h->g.reg || (int64_t)h->g.d * 32768 >= ((int64_t)1 << 26)) return EMH_OK;
+    // without a bias cn = sum c8^2 <= d 2^14, inside the limit 2^26 - d 2^15
+    // whenever d 3 2^14 < 2^26: no need to read the flag back (no sync)
+    if (!with_bias && (int64_t)h->g.d * 49152 < ((int64_t)1 << 26)) {
+        h->g.i8_packed = true;
+        return EMH_OK;
+    }
     uint32_t flag = 1;
     int rc = check(hipMemcpy(&flag, h->stats.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost), "read range flag");
     if (rc) return rc;
@@ -288,7 +294,7 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
         return rc;
     // The caller may reuse `centroids` on return (pageable source).
     if (!device_src && (rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;
-    if ((rc = refresh_i8_range(h))) return rc;
+    if ((rc = refresh_i8_range(h, false))) return rc;
     h->has_centroids = true;
     return EMH_OK;
 }
@@ -405,7 +411,7 @@ int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
                     "score offsets")))
         return rc;
     if ((rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;
-    if ((rc = refresh_i8_range(h))) return rc;
+    if ((rc = refresh_i8_range(h, true))) return rc;
     h->has_bias = true;
     return EMH_OK;
 }

@@ -253,6 +253,17 @@ struct JobRun {
     size_t nb = 0;
     const uint32_t *ids = nullptr;
     bool penalized = false, is_last = false, last_epoch_start = false;
+    // deferred statistics: a batch's objective and labels are folded in
+    // (finish_stats_deferred, in batch order) after later batches have been
+    // staged; the epoch objective is then recorded by the batch closing it.
+    // Only for runs without the balance penalty, whose next batch does not
+    // depend on the previous one's statistics.
+    bool deferred = false;
+    struct StepMeta {
+        size_t nb;
+        const uint32_t *ids;
+        bool is_last, closes_epoch;
+    };
 
     // input checks (src/lib.rs:349-361), recipe, initial centroids
     // (random data rows, src/lib.rs:374-381)
@@ -302,7 +313,7 @@ struct JobRun {
         if (done) return false;
         if (b0 == 0) {
             Rng(cfg.seed * 1000003ull + (uint64_t)epoch + 1).permutation((uint32_t)n, order);
-            epoch_obj = 0.0;
+            if (!deferred) epoch_obj = 0.0;
         }
         is_last = epoch == R.n_epochs - 1;
         last_epoch_start = is_last && b0 == 0;
@@ -342,6 +353,30 @@ struct JobRun {
         mean_sqdist = batch_obj / (double)nb;
     }
 
+    // The staged batch, for finish_stats_deferred.
+    StepMeta meta() const { return StepMeta{nb, ids, is_last, b0 + nb >= R.epoch_rows[epoch]}; }
+
+    // finish_stats for a batch staged earlier (no balance penalty), then the
+    // epoch objective when the batch closes its epoch. The ids of the last
+    // epoch stay valid: no permutation is drawn after it.
+    void finish_stats_deferred(const StepMeta &m, const uint32_t *lab, const float *dist) {
+        double batch_obj = 0.0;
+        for (size_t p0 = 0; p0 < m.nb; p0 += chunk) {
+            const size_t cm = std::min(chunk, m.nb - p0);
+            double cs = 0.0;
+            for (size_t r = p0; r < p0 + cm; r++) cs += (double)dist[r];
+            batch_obj += cs;
+        }
+        if (m.is_last)
+            for (size_t r = 0; r < m.nb; r++) labels_out[m.ids[r]] = lab[r];
+        epoch_obj += batch_obj;
+        mean_sqdist = batch_obj / (double)m.nb;
+        if (m.closes_epoch) {
+            if (out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+            epoch_obj = 0.0;
+        }
+    }
+
     // This step's EMA rate (src/lib.rs:455) and whether revival follows it
     // (src/lib.rs:463-465).
     float alpha() const { return (float)scheduled_alpha(R.alpha0, step, R.total_steps); }
@@ -352,7 +387,7 @@ struct JobRun {
         step++;
         b0 += nb;
         if (b0 >= R.epoch_rows[epoch]) {
-            if (out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+            if (!deferred && out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
             epoch++;
             b0 = 0;
             if (epoch == R.n_epochs) done = true;

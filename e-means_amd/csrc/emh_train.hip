@@ -10,7 +10,8 @@
 // against device-resident centroids; the last pass's totals accumulate in
 // FP64 on the GPU for the closing mean. Labels and distances come down per
 // batch (objective, last-pass labels), the usage vector only when the bias
-// or a revival needs it.
+// or a revival needs it; without the balance penalty the host folds them in
+// a few batches later, so staging and launching overlap the GPU's work.
 #include <chrono>
 
 #include "emh_recipe.h"
@@ -59,8 +60,52 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         return rc;
     std::vector<std::pair<size_t, size_t>> pairs;
 
+    // Without the balance penalty no batch waits for the previous one's
+    // labels: each batch's labels and distances come down into one of NSLOT
+    // pinned slots behind an event, and the host folds them into the
+    // statistics in batch order when the slot comes round again (or at a
+    // revival, which needs the usage vector, or at the end). The host then
+    // stages batch t + 1 while the GPU still runs batch t.
+    const bool async = cfg.balance <= 0.0f;
+    J.deferred = async;
+    constexpr int NSLOT = 4;
+    HostBuf ids_s[NSLOT], lab_s[NSLOT], dist_s[NSLOT];
+    struct Events {
+        hipEvent_t e[NSLOT] = {};
+        ~Events() {
+            for (hipEvent_t v : e)
+                if (v) (void)hipEventDestroy(v);
+        }
+    } ev;
+    JobRun::StepMeta meta[NSLOT];
+    int pend_head = 0, pend_n = 0;  // pending slots, oldest first
+    if (async)
+        for (int i = 0; i < NSLOT; i++)
+            if ((rc = check(ids_s[i].alloc(batch * 4), "pinned ids")) ||
+                (rc = check(lab_s[i].alloc(batch * 4), "pinned labels")) ||
+                (rc = check(dist_s[i].alloc(batch * 4), "pinned distances")) ||
+                (rc = check(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming), "event")))
+                return rc;
+    auto drain_one = [&]() -> int {
+        const int sl = pend_head;
+        int r;
+        if ((r = check(hipEventSynchronize(ev.e[sl]), "batch"))) return r;
+        J.finish_stats_deferred(meta[sl], lab_s[sl].as<uint32_t>(), dist_s[sl].as<float>());
+        pend_head = (pend_head + 1) % NSLOT;
+        pend_n--;
+        return (int)EMH_OK;
+    };
+    auto drain_all = [&]() -> int {
+        int r;
+        while (pend_n)
+            if ((r = drain_one())) return r;
+        return (int)EMH_OK;
+    };
+
     while (J.next()) {
         const size_t nb = J.nb;
+        const int sl = (int)(J.step % NSLOT);
+        if (async && pend_n == NSLOT && (rc = drain_one())) return rc;  // frees slot sl, the oldest
         if (J.last_epoch_start) {
             if ((rc = check(launch_fill_f64(pass_d.as<double>(), k * d, h->stream), "clear pass sums"))) return rc;
             if ((rc = check(launch_fill_f64((double *)passn_d.p, k, h->stream), "clear pass counts"))) return rc;
@@ -68,8 +113,9 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
         if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true))) return rc;
         if (J.penalized && (rc = emh_set_bias(h, J.bias.data(), k))) return rc;
-        memcpy(ids_h.p, J.ids, nb * 4);
-        if ((rc = check(hipMemcpyAsync(ids_d.p, ids_h.p, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
+        uint32_t *ids_host = async ? ids_s[sl].as<uint32_t>() : ids_h.as<uint32_t>();
+        memcpy(ids_host, J.ids, nb * 4);
+        if ((rc = check(hipMemcpyAsync(ids_d.p, ids_host, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
             return rc;
         // gather + search on the device (src/lib.rs:419-424)
         if ((rc = run_search(h, data.p, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
@@ -91,14 +137,28 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
             return rc;
         const bool revive = J.revive_due();
         const bool need_usage = revive || cfg.balance > 0.0f;
-        if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
-            (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
-            (need_usage &&
-             (rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage"))) ||
-            (rc = check(hipStreamSynchronize(h->stream), "batch")))
-            return rc;
-        if (need_usage) memcpy(J.st.usage.data(), usage_h.p, k * 4);
-        J.finish_stats(lab_h.as<uint32_t>(), dist_h.as<float>());
+        if (async) {
+            if ((rc = check(hipMemcpyAsync(lab_s[sl].p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
+                (rc = check(hipMemcpyAsync(dist_s[sl].p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
+                (rc = check(hipEventRecord(ev.e[sl], h->stream), "event")))
+                return rc;
+            meta[sl] = J.meta();
+            pend_n++;
+            if (revive &&
+                ((rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage")) ||
+                 (rc = check(hipStreamSynchronize(h->stream), "batch")) || (rc = drain_all())))
+                return rc;
+            if (revive) memcpy(J.st.usage.data(), usage_h.p, k * 4);
+        } else {
+            if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
+                (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
+                (need_usage &&
+                 (rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage"))) ||
+                (rc = check(hipStreamSynchronize(h->stream), "batch")))
+                return rc;
+            if (need_usage) memcpy(J.st.usage.data(), usage_h.p, k * 4);
+            J.finish_stats(lab_h.as<uint32_t>(), dist_h.as<float>());
+        }
         if (revive) {
             // src/lib.rs:463-465: planned on the host (its RNG), rows split on the device
             J.st.revive_plan(J.R.unused_threshold, J.split_rng, pairs);
@@ -120,6 +180,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         }
         J.advance();
     }
+    if ((rc = drain_all())) return rc;
 
     // Closing mean on the device from the FP64 pass totals (src/lib.rs:481-489).
     if ((rc = check(launch_closing_mean(pass_d.as<double>(), passn_d.as<uint64_t>(), k, d, cent_d.as<float>(),
