@@ -1050,6 +1050,16 @@ __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR), t0, t1, s);
 }
 
+// The same sweep compiled for two waves per SIMD (a register budget of 256;
+// the compiler keeps what does not fit in scratch).
+template <int OP, int ROWS, int M, int NS, int FR, int NB>
+__global__ __launch_bounds__(256, 2) void search_reg_kernel_2w(SearchArgs a) {
+    const int ntiles = (int)(a.k_pad / reg_centroids<OP, NB>());
+    const int s = blockIdx.y, t0 = s * a.tiles_per_slice;
+    const int t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)blockIdx.x * (128 * FR), t0, t1, s);
+}
+
 // Merges the per-slice results of a split search, one row per thread: the
 // lexicographic (score, index) order of the unsplit sweep, so the merged
 // labels, distances and candidate lists are the unsplit ones bit for bit.
@@ -1570,7 +1580,14 @@ template <int OP, int ROWS, int M, int NS>
 static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
     constexpr int FR = 2, NB = OP == OP_I16 ? 3 : (NS <= 8 ? 2 : 1);
     dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)), (unsigned)a.nslice);
-    hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
+    // I8 on 512-byte rows needs 269 registers at full speed; held to 256 (44
+    // bytes per lane kept in scratch) it runs two waves per SIMD and measures
+    // 16 % faster. The other shapes either fit two waves already or would
+    // spill hundreds of bytes.
+    if constexpr (OP == OP_I8 && NS == 16)
+        hipLaunchKernelGGL((search_reg_kernel_2w<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((search_reg_kernel<OP, ROWS, M, NS, FR, NB>), grid, dim3(256), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.nslice == 1) return e;
     hipLaunchKernelGGL((merge_slices_kernel<OP, M>), dim3((unsigned)((a.m + 255) / 256)), dim3(256), 0, s,
