@@ -16,16 +16,19 @@ def _i8_blobs(n, d, blobs, seed):
     return np.clip(x, -128, 127).astype(np.int8)
 
 
+@pytest.mark.parametrize("bits", ["i8", "i16"])
 @pytest.mark.parametrize("balance", [0.0, 0.1])
-def test_i8_training_is_bit_exact(hip, balance):
-    """int8 rows with the exact integer search: every label, distance and
-    per-cluster sum is an exact integer, so the whole trajectory (centroids,
-    last-pass labels, objective, revivals) must equal the CPU oracle's."""
+def test_i8_training_is_bit_exact(hip, balance, bits):
+    """int8 rows with the exact integer search (CentroidBits::I8 or I16):
+    every label, distance and per-cluster sum is exact, so the whole
+    trajectory (centroids, last-pass labels, objective, revivals) must equal
+    the CPU oracle's."""
+    print(f"{bits} training balance={balance}", flush=True)
     x = _i8_blobs(6000, 48, 60, 1)
     cfg_o = O.Config(nepochs=3, balance=balance, seed=99)
     cfg_e = E.Config(nepochs=3, balance=balance, seed=99)
-    ref = O.train(x, 40, cfg_o, assigner=O.ASSIGNER_I8)
-    got = E.train(x, 40, cfg_e, E.HipAssigner(E.I8))
+    ref = O.train(x, 40, cfg_o, assigner=O.ASSIGNER_I8 if bits == "i8" else O.ASSIGNER_I16)
+    got = E.train(x, 40, cfg_e, E.HipAssigner(E.I8 if bits == "i8" else E.I16))
     assert got.recipe.batch_size == ref.recipe.batch_size
     assert np.array_equal(got.labels, ref.labels)
     assert got.objective == ref.objective
