@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--blobs", type=int, default=16384)
     ap.add_argument("--jobs", type=int, default=64, help="train-c5: number of problems")
-    ap.add_argument("--judge-rows", type=int, default=20000)
+    ap.add_argument("--check-rows", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the comparison passes")
@@ -190,7 +190,7 @@ def run_search(prec, m, x, c_host, steps, warmup):
     }
 
 
-def judge(x, c, res, rows):
+def fp64_check(x, c, res, rows):
     """FP64 brute force (torch, on the GPU) on a fixed subsample of rows."""
     import torch
 
@@ -213,7 +213,7 @@ def judge(x, c, res, rows):
         "label_disagreement": float((lab != best_idx).double().mean()),
         "true_label_errors": float(wrong.double().mean()),
         "dist_err_max_rel_1pd": float(rel.max()),
-        "judged_rows": rows,
+        "checked_rows": rows,
     }
 
 
@@ -274,7 +274,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
     integer = prec in ("i8", "i16")
     if not integer:
         for name, v in variants.items():
-            q = judge(x, c, v, min(args.judge_rows, n))
+            q = fp64_check(x, c, v, min(args.check_rows, n))
             q["mse"] = float(v["dist"].double().mean())
             quality[name] = q
         if "fp32" in quality:

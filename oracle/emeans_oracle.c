@@ -874,7 +874,7 @@ double or_quant_clipped_fraction(const float *x, size_t n, size_t d, const float
 }
 
 /* ========================================================================= */
-/* Brute-force judges                                                        */
+/* Brute-force FP64 / exact references                                       */
 /* ========================================================================= */
 
 static double row_elem(const void *x, int rows, size_t idx) {
@@ -933,7 +933,7 @@ static void brute_block(void *vctx, size_t blk) {
 
 void or_brute_f64(int nthreads, const void *x, int rows, size_t n, size_t d, const float *c, size_t k,
                   const float *bias, uint32_t *labels, double *dist) {
-    /* FP64 exact search with lowest-index ties (the judge of src/cpu.rs:298-306) */
+    /* FP64 exact search with lowest-index ties (the reference of src/cpu.rs:298-306) */
     brute_job J = {x, rows, n, d, k, c, bias, NULL, labels, dist};
     run_pool(nthreads, (n + 63) / 64, &J, brute_block);
 }

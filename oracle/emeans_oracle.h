@@ -107,7 +107,7 @@ float or_report_sqdist(const float *a, const float *b, size_t d);
 void or_report_nearest(int nthreads, const float *c, size_t k, size_t d, const uint32_t *sizes, size_t row0,
                        size_t row1, uint32_t *nearest, float *nearest_dist);
 
-/* ---- Brute-force judges ------------------------------------------------- */
+/* ---- Brute-force FP64 / exact references -------------------------------- */
 void or_brute_f64(int nthreads, const void *x, int rows, size_t n, size_t d, const float *c, size_t k,
                   const float *bias, uint32_t *labels, double *dist);
 void or_dist_f64(int nthreads, const void *x, int rows, size_t n, size_t d, const float *c, const uint32_t *labels,

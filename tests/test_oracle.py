@@ -216,7 +216,7 @@ def test_kernels_match_brute_force(n, k, d):
 
 
 def test_brute_force_is_exact_on_small_case():
-    # the judge itself: a numpy FP64 search on a tiny case, ties to the lowest index
+    # the FP64 reference itself: a numpy FP64 search on a tiny case, ties to the lowest index
     rng = np.random.default_rng(4)
     x = rng.integers(-3, 4, size=(40, 5)).astype(np.float32)
     c = rng.integers(-3, 4, size=(17, 5)).astype(np.float32)
