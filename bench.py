@@ -510,7 +510,7 @@ def c5_problems(njobs, dev, seed=5):
         n = int(round(np.exp(rng.uniform(np.log(5e3), np.log(5e5)))))
         d = int(rng.choice([64, 96, 128, 256]))
         k = int(round(np.exp(rng.uniform(np.log(64), np.log(4096)))))
-        x = mixture(n, d, max(2 * k, 64), dev, seed=1000 + j).cpu().numpy()
+        x = mixture(n, d, max(2 * k, 64), dev, seed=1000 + j)  # rows stay in HBM
         probs.append((x, k, E.Config(nepochs=3.0, balance=0.1 if j % 2 else 0.0, seed=1234 + j)))
     return probs
 
@@ -543,8 +543,9 @@ def train_batch_line(args, njobs, prec, m):
         log("exact re-assignment MSE")
         rel = []
         for (x, k, cfg), g, r in zip(probs, got, ref):
-            a = E.assign(x, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
-            b = E.assign(x, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+            xh = x.cpu().numpy()
+            a = E.assign(xh, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+            b = E.assign(xh, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
             rel.append(a / b - 1.0)
         loop_steps = sum(int(r.recipe.total_steps) for r in ref)
     res = {
@@ -566,6 +567,9 @@ def train_batch_line(args, njobs, prec, m):
         "gpu_seconds": round(bs, 3),
         "graph_replays": int(st.steps),
         "graph_nodes_per_replay": int(st.graph_nodes),
+        "host_seconds": round(st.host_seconds, 3),
+        "replay_wait_seconds": round(st.wait_seconds, 3),
+        "graph_launch_seconds": round(st.launch_seconds, 3),
         "fp32_job_loop": {"rows_per_s": round(rows / ls, 1), "gpu_seconds": round(ls, 3),
                           "train_steps": loop_steps},
         "speedup_vs_fp32_loop": round(ls / bs, 3),
@@ -592,16 +596,14 @@ def train_large_line(args, x, k, nepochs, prec, m):
     import emeans_hip as E
 
     xh = x.cpu().numpy()
-    del x
-    torch.cuda.empty_cache()
     n, d = xh.shape
     log(f"train {prec} warmup")
-    E.train(xh[:65536], 1024, E.Config(nepochs=1), E.HipAssigner(prec, rescore_m=m))
+    E.train(x[:65536], 1024, E.Config(nepochs=1), E.HipAssigner(prec, rescore_m=m))
     runs = {}
     for name, p, mm in ((prec, prec, m), ("fp32", "fp32", 1)):
         log(f"train {name}")
         t = time.perf_counter()
-        out = E.train(xh, k, E.Config(nepochs=nepochs), E.HipAssigner(p, rescore_m=mm))
+        out = E.train(x, k, E.Config(nepochs=nepochs), E.HipAssigner(p, rescore_m=mm))  # rows in HBM
         secs = time.perf_counter() - t
         log(f"exact re-assignment MSE ({name})")
         _, dist = E.assign(xh, out.centroids, E.HipAssigner("fp32"))
@@ -655,10 +657,10 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     n, d = xh.shape
     cfg = E.Config(nepochs=nepochs)
     log(f"train {prec} warmup")
-    E.train(xh[: min(n, 50 * k)], k, E.Config(nepochs=1), E.HipAssigner(prec))
+    E.train(x[: min(n, 50 * k)], k, E.Config(nepochs=1), E.HipAssigner(prec))
     log(f"train {prec}")
     t = time.perf_counter()
-    out = E.train(xh, k, cfg, E.HipAssigner(prec))
+    out = E.train(x, k, cfg, E.HipAssigner(prec))  # rows already in HBM
     gpu_s = time.perf_counter() - t
     rows_per_s = nepochs * n / gpu_s
 

@@ -492,6 +492,29 @@ hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclu
 // c (1 + eps_i), eps_i = +1e-4 on even and -1e-4 on odd dimensions; if the
 // two rows came out equal the copy's first value moves up one ulp. One wave
 // per (donor, unused) pair; pairs never share a row.
+// out[j] = row ids[j] of x widened to f32 (the initial centroids of a run
+// whose rows are already on the device, src/lib.rs:374-381). One wave per row.
+__global__ __launch_bounds__(256) void gather_rows_kernel(const void *x, int rows, size_t d, const uint32_t *ids,
+                                                          size_t m, float *out) {
+    const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= m) return;
+    const size_t src = (size_t)ids[j] * d;
+    for (size_t i = threadIdx.x & 63; i < d; i += 64) {
+        float v;
+        if (rows == EMH_ROWS_I8) v = (float)((const int8_t *)x)[src + i];
+        else if (rows == EMH_ROWS_U8) v = (float)((const uint8_t *)x)[src + i];
+        else v = ((const float *)x)[src + i];
+        out[j * d + i] = v;
+    }
+}
+
+hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t *ids, size_t m, float *out,
+                              hipStream_t s) {
+    if (!m) return hipSuccess;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3((unsigned)((m + 3) / 4)), dim3(256), 0, s, x, rows, d, ids, m, out);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint32_t *pairs, size_t npairs, size_t d) {
     const size_t t = blockIdx.x;
     const int lane = threadIdx.x;

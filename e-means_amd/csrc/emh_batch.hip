@@ -131,7 +131,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         if (jb.rows != EMH_ROWS_F32) return fail(EMH_E_UNSUPPORTED, "train_batch: f32 rows only");
-        if ((rc = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out)))
+        if ((rc = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out,
+                              jb.x_on_device != 0)))
             return rc;
         geo[j] = make_geom(op, jb.k, jb.d);
         if (!geo[j].reg)
@@ -151,13 +152,19 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (slots > 0xFFFFFFFFull || nclusters >= 0xFFFFFFFFull)
         return fail(EMH_E_ARG, "train_batch: more than 2^32 - 1 batch rows or clusters in one call");
 
-    // Resident rows, one table per job.
+    // Resident rows, one table per job (the caller's own when on the device).
     std::vector<DevBuf> rows_d(njobs);
+    std::vector<const float *> rows_ptr(njobs);
     for (size_t j = 0; j < njobs; j++) {
+        if (jobs[j].x_on_device) {
+            rows_ptr[j] = (const float *)jobs[j].x;
+            continue;
+        }
         const size_t bytes = jobs[j].n * jobs[j].d * 4;
         if ((rc = check(rows_d[j].ensure(bytes), "resident rows"))) return rc;
         if ((rc = check(hipMemcpyAsync(rows_d[j].p, jobs[j].x, bytes, hipMemcpyHostToDevice, h->stream), "upload rows")))
             return rc;
+        rows_ptr[j] = rows_d[j].as<float>();
     }
 
     // Staged per-step input (host pinned -> device, one copy) and output
@@ -190,12 +197,38 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     uint8_t *in_hp = (uint8_t *)in_h.p, *out_hp = (uint8_t *)out_h.p;
     float *use_dev = (float *)(out_dp + out_use);
     // initial models: random data rows (JobRun::init), masses 1/65536
+    // (device rows: gathered on the device from the rows JobRun::init picked)
+    size_t ninit = 0;
+    for (size_t j = 0; j < njobs; j++) ninit += run[j].init_rows.size();
+    DevBuf init_d;
+    HostBuf init_h;
+    if (ninit && ((rc = check(init_d.ensure(ninit * 4), "initial rows")) ||
+                  (rc = check(init_h.alloc(ninit * 4), "pinned initial rows"))))
+        return rc;
+    size_t ioff = 0;
     for (size_t j = 0; j < njobs; j++) {
         memcpy(out_hp + out_use + cl_base[j] * 4, run[j].st.usage.data(), jobs[j].k * 4);
-        if ((rc = check(hipMemcpyAsync(cen_model.as<float>() + cen_base[j], run[j].st.centroids.data(),
-                                       jobs[j].k * jobs[j].d * 4, hipMemcpyHostToDevice, h->stream),
-                        "upload centroids")))
+        if (jobs[j].x_on_device) {
+            memcpy(init_h.as<uint32_t>() + ioff, run[j].init_rows.data(), jobs[j].k * 4);
+            ioff += jobs[j].k;
+        } else if ((rc = check(hipMemcpyAsync(cen_model.as<float>() + cen_base[j], run[j].st.centroids.data(),
+                                              jobs[j].k * jobs[j].d * 4, hipMemcpyHostToDevice, h->stream),
+                               "upload centroids"))) {
             return rc;
+        }
+    }
+    if (ninit) {
+        if ((rc = check(hipMemcpyAsync(init_d.p, init_h.p, ninit * 4, hipMemcpyHostToDevice, h->stream), "initial rows")))
+            return rc;
+        ioff = 0;
+        for (size_t j = 0; j < njobs; j++) {
+            if (!jobs[j].x_on_device) continue;
+            if ((rc = check(launch_gather_rows(jobs[j].x, EMH_ROWS_F32, jobs[j].d, init_d.as<uint32_t>() + ioff,
+                                               jobs[j].k, cen_model.as<float>() + cen_base[j], h->stream),
+                            "initial centroids")))
+                return rc;
+            ioff += jobs[j].k;
+        }
     }
     if ((rc = check(hipMemcpyAsync(use_dev, out_hp + out_use, nclusters * 4, hipMemcpyHostToDevice, h->stream),
                     "upload masses")))
@@ -241,7 +274,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (size_t j = 0; j < njobs; j++) {
         JobDev &t = jd[j];
         memset(&t, 0, sizeof t);
-        t.x = rows_d[j].as<float>();
+        t.x = rows_ptr[j];
         t.ids = (const uint32_t *)(in_dp + in_ids) + row_base[j];
         t.cen = cen_model.as<float>() + cen_base[j];
         t.bias = (const float *)(in_dp + in_bias) + cl_base[j];
@@ -329,17 +362,35 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         }
         J.advance();
     };
+    // while a replay runs, the jobs whose batch closes an epoch draw the next
+    // epoch's order (the host's largest per-step cost otherwise)
+    const std::function<void(size_t)> prefetch = [&](size_t j) {
+        if (live[j]) run[j].prefetch_order();
+    };
+    double host_s = 0.0, wait_s = 0.0, launch_s = 0.0;
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     for (;;) {
+        auto th = clk::now();
         pool.run(njobs, stage);
+        host_s += since(th);
         if (std::find(live.begin(), live.end(), (char)1) == live.end()) break;
-        if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch")) ||
-            (rc = check(hipStreamSynchronize(h->stream), "step")))
-            return rc;
+        th = clk::now();
+        if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch"))) return rc;
+        launch_s += since(th);
+        th = clk::now();
+        pool.run(njobs, prefetch);
+        host_s += since(th);
+        th = clk::now();
+        if ((rc = check(hipStreamSynchronize(h->stream), "step"))) return rc;
+        wait_s += since(th);
         steps++;
         // host half per job; revival planned on the downloaded masses, rows
         // split on the device before the next replay (one upload of all
         // triples and masses, one kernel)
+        th = clk::now();
         pool.run(njobs, finish);
+        host_s += since(th);
         uint32_t *tp = pairs_h.as<uint32_t>();
         size_t ntrip = 0;
         for (size_t j = 0; j < njobs; j++)
@@ -382,6 +433,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         stats->steps = steps;
         stats->graph_nodes = nodes;
         stats->seconds = secs;
+        stats->host_seconds = host_s;
+        stats->wait_seconds = wait_s;
+        stats->launch_seconds = launch_s;
     }
     return EMH_OK;
 }

@@ -154,6 +154,9 @@ hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const u
                             float alpha, hipStream_t s);
 // The revival row split of src/lib.rs:330-340 for (donor, unused) pairs.
 hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s);
+// out (m x d f32) = rows ids of x (f32 / i8 / u8), widened
+hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t *ids, size_t m, float *out,
+                              hipStream_t s);
 // Batched: keys = cluster_base + label for live slots (nclusters for spare
 // slots), counts per global cluster, stable sort, then per-job f32 sums and
 // FP64 pass totals. cjob maps a global cluster to its job.

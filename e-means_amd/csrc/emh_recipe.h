@@ -20,7 +20,6 @@ namespace emh {
 static const double ALPHA_MIN_FACTOR = 0.1;
 static const size_t CHECK_UNUSED_EVERY = 5;
 static const float INIT_USAGE = 1.0f / 65536.0f;
-static const float SPLIT_EPSILON = 1e-4f;
 static const float UNUSED_THRESHOLD = 0.1f;
 static const size_t NO_REVIVAL_MAX_D = 2;
 static const size_t AUTO_QUALITY_MAX_D = 16;
@@ -129,37 +128,11 @@ struct State {
     std::vector<float> centroids, usage;
     size_t n_splits = 0;
 
-    // N' = (1-a) N + a n ; c' = c + (a n / N') (s/n - c), all f32 (src/lib.rs:287-309)
-    void apply(const std::vector<uint32_t> &touched, const uint32_t *counts, const float *sums, float alpha) {
-        const bool replace = alpha == 1.0f;
-        if (!replace) {
-            const float decay = 1.0f - alpha;
-            for (float &u : usage) u *= decay;
-        }
-        for (uint32_t jj : touched) {
-            const size_t j = jj;
-            const float n = (float)counts[j];
-            const float retained = replace ? 0.0f : usage[j];
-            const float an = alpha * n;
-            const float mass = retained + an;
-            const float weight = an / mass;
-            float *c = &centroids[j * d];
-            const float *s = sums + j * d;
-            if (weight == 1.0f) {
-                for (size_t i = 0; i < d; i++) c[i] = s[i] / n;
-            } else {
-                for (size_t i = 0; i < d; i++) {
-                    const float mean = s[i] / n;
-                    c[i] = c[i] + weight * (mean - c[i]);
-                }
-            }
-            usage[j] = mass;
-        }
-    }
-
     // Gumbel-top-k donor split of clusters below tau * mean mass
     // (src/lib.rs:314-344), planned on `usage` (updated here): the (donor,
-    // unused) pairs whose centroid rows split_rows then splits.
+    // unused) pairs whose centroid rows the device then splits
+    // (revive_split_kernel). The EMA step itself runs on the device
+    // (ema_apply_kernel); the model's centroids live there.
     void revive_plan(float tau, Rng &rng, std::vector<std::pair<size_t, size_t>> &pairs) {
         pairs.clear();
         float total = 0.0f;
@@ -185,29 +158,6 @@ struct State {
         }
         n_splits += nsplit;
     }
-
-    // donor row c -> c (1 - eps_i) and c (1 + eps_i) with eps_i = +-1e-4 by
-    // parity, nudging the copy up one ulp if both came out equal
-    void split_rows(const std::vector<std::pair<size_t, size_t>> &pairs) {
-        for (const auto &pr : pairs) {
-            const size_t src = pr.first, dst = pr.second;
-            for (size_t i = 0; i < d; i++) {
-                const float offset = (i % 2 == 0) ? SPLIT_EPSILON : -SPLIT_EPSILON;
-                const float c = centroids[src * d + i];
-                centroids[src * d + i] = c * (1.0f - offset);
-                centroids[dst * d + i] = c * (1.0f + offset);
-            }
-            bool same = true;
-            for (size_t i = 0; i < d; i++) same = same && centroids[src * d + i] == centroids[dst * d + i];
-            if (same) centroids[dst * d] = nextafterf(centroids[dst * d], INFINITY);
-        }
-    }
-
-    void revive(float tau, Rng &rng) {
-        std::vector<std::pair<size_t, size_t>> pairs;
-        revive_plan(tau, rng, pairs);
-        split_rows(pairs);
-    }
 };
 
 static float widen(const void *x, int rows, size_t idx) {
@@ -229,8 +179,8 @@ struct HostBuf {
 
 // One training run's host state, advanced one batch at a time: next()
 // stages the batch (src/lib.rs:400-417), the caller runs the device half
-// (gather, search, per-cluster sums), finish_batch() does the rest of
-// src/lib.rs:419-466.
+// (gather, search, per-cluster sums, EMA step, revival split) and folds the
+// batch's labels and distances in with finish_stats (src/lib.rs:419-466).
 struct JobRun {
     const void *x = nullptr;
     int rows = EMH_ROWS_F32;
@@ -242,7 +192,7 @@ struct JobRun {
     emh_recipe R{};
     State st{};
     Rng split_rng{0};
-    std::vector<uint32_t> order, touched;
+    std::vector<uint32_t> order;
     std::vector<float> bias;
     double mean_sqdist = 0.0, epoch_obj = 0.0;
     uint64_t step = 0;
@@ -259,6 +209,10 @@ struct JobRun {
     // Only for runs without the balance penalty, whose next batch does not
     // depend on the previous one's statistics.
     bool deferred = false;
+    // the next epoch's batch order, drawn ahead of time (prefetch_order) while
+    // the GPU works; next() takes it instead of drawing it then
+    std::vector<uint32_t> order_next;
+    uint32_t order_next_epoch = UINT32_MAX;
     struct StepMeta {
         size_t nb;
         const uint32_t *ids;
@@ -267,8 +221,13 @@ struct JobRun {
 
     // input checks (src/lib.rs:349-361), recipe, initial centroids
     // (random data rows, src/lib.rs:374-381)
+    // rows on the device: the caller gathers the initial centroids (rows
+    // init_rows of x, widened to f32) on the GPU; st.centroids stays unset
+    bool rows_on_device = false;
+    std::vector<uint32_t> init_rows;
+
     int init(const void *x_, int rows_, size_t n_, size_t d_, size_t k_, const emh_config &cfg_, float *cout,
-             uint32_t *lout, emh_train_out *out_) {
+             uint32_t *lout, emh_train_out *out_, bool on_device = false) {
         if (d_ == 0 || k_ == 0 || !x_) return fail(EMH_E_ARG, "x must be a non-empty row-major n x d array");
         if (n_ < k_) return fail(EMH_E_ARG, "need at least k = " + std::to_string(k_) + " rows, got " + std::to_string(n_));
         if (n_ > 0xFFFFFFFFull) return fail(EMH_E_ARG, "more than 2^32 - 1 rows: subsample or shard the input");
@@ -286,17 +245,21 @@ struct JobRun {
         out = out_;
         memset(&R, 0, sizeof R);
         if (resolve(n, d, k, cfg, R)) return fail(EMH_E_ARG, "nepochs out of range");
-        st = State{k, d, std::vector<float>(k * d), std::vector<float>(k, INIT_USAGE)};
+        rows_on_device = on_device;
+        st = State{k, d, std::vector<float>(on_device ? 0 : k * d), std::vector<float>(k, INIT_USAGE)};
         {
             std::vector<uint32_t> init;
             Rng(cfg.seed).permutation((uint32_t)n, init);
-            for (size_t j = 0; j < k; j++)
-                for (size_t i = 0; i < d; i++) st.centroids[j * d + i] = widen(x, rows, (size_t)init[j] * d + i);
+            if (on_device) {
+                init_rows.assign(init.begin(), init.begin() + k);
+            } else {
+                for (size_t j = 0; j < k; j++)
+                    for (size_t i = 0; i < d; i++) st.centroids[j * d + i] = widen(x, rows, (size_t)init[j] * d + i);
+            }
         }
         split_rng = Rng(cfg.seed ^ 0x5EED5EED5EED5EEDull);
         chunk = std::min(ASSIGN_CHUNK, (size_t)R.batch_size);
         bias.assign(cfg.balance > 0.0f ? k : 0, 0.0f);
-        touched.reserve(std::min(k, (size_t)R.batch_size));
         for (size_t i = 0; i < n; i++) labels_out[i] = 0xFFFFFFFFu;
         out->n_objective = 0;
         mean_sqdist = 0.0;
@@ -312,7 +275,12 @@ struct JobRun {
     bool next() {
         if (done) return false;
         if (b0 == 0) {
-            Rng(cfg.seed * 1000003ull + (uint64_t)epoch + 1).permutation((uint32_t)n, order);
+            if (order_next_epoch == epoch) {
+                order.swap(order_next);
+                order_next_epoch = UINT32_MAX;
+            } else {
+                Rng(cfg.seed * 1000003ull + (uint64_t)epoch + 1).permutation((uint32_t)n, order);
+            }
             if (!deferred) epoch_obj = 0.0;
         }
         is_last = epoch == R.n_epochs - 1;
@@ -351,6 +319,15 @@ struct JobRun {
             for (size_t r = 0; r < nb; r++) labels_out[ids[r]] = lab[r];
         epoch_obj += batch_obj;
         mean_sqdist = batch_obj / (double)nb;
+    }
+
+    // When the staged batch closes its epoch and another follows, draws that
+    // epoch's order now (the same permutation next() would draw). Reads only
+    // the staged position, so it can run while the batch is on the GPU.
+    void prefetch_order() {
+        if (done || epoch + 1 >= R.n_epochs || b0 + nb < R.epoch_rows[epoch] || order_next_epoch == epoch + 1) return;
+        Rng(cfg.seed * 1000003ull + (uint64_t)(epoch + 1) + 1).permutation((uint32_t)n, order_next);
+        order_next_epoch = epoch + 1;
     }
 
     // The staged batch, for finish_stats_deferred.
@@ -392,19 +369,6 @@ struct JobRun {
             b0 = 0;
             if (epoch == R.n_epochs) done = true;
         }
-    }
-
-    // The host half of the batch with the EMA step on the host copy of the
-    // model: labels, distances (biased when penalized; corrected here),
-    // per-cluster counts and f32 sums.
-    void finish_batch(const uint32_t *lab, float *dist, const uint32_t *counts, const float *sums) {
-        finish_stats(lab, dist);
-        touched.clear();
-        for (size_t j = 0; j < k; j++)
-            if (counts[j]) touched.push_back((uint32_t)j);
-        st.apply(touched, counts, sums, alpha());
-        if (revive_due()) st.revive(R.unused_threshold, split_rng);
-        advance();
     }
 };
 

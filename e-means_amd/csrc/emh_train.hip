@@ -18,11 +18,13 @@
 
 namespace emh {
 
+// x: host rows (uploaded once here) or, with rows_on_device, rows already in
+// HBM (no copy; the initial centroids are gathered on the device).
 static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size_t k, const emh_config &cfg,
-                      float *centroids_out, uint32_t *labels_out, emh_train_out *out) {
+                      float *centroids_out, uint32_t *labels_out, emh_train_out *out, bool rows_on_device) {
     JobRun J;
     int rc;
-    if ((rc = J.init(x, rows, n, d, k, cfg, centroids_out, labels_out, out))) return rc;
+    if ((rc = J.init(x, rows, n, d, k, cfg, centroids_out, labels_out, out, rows_on_device))) return rc;
     if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
     const size_t batch = J.R.batch_size;
@@ -35,8 +37,11 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     // and pulls the usage vector down when the balance bias or a revival
     // needs it.
     DevBuf data, ids_d, lab_d, dist_d, sums_d, counts_d, pass_d, passn_d, cent_d, usage_d, pairs_d;
-    if ((rc = check(data.ensure(n * d * rb), "resident rows"))) return rc;
-    if ((rc = check(hipMemcpyAsync(data.p, x, n * d * rb, hipMemcpyHostToDevice, h->stream), "upload rows"))) return rc;
+    if (!rows_on_device &&
+        ((rc = check(data.ensure(n * d * rb), "resident rows")) ||
+         (rc = check(hipMemcpyAsync(data.p, x, n * d * rb, hipMemcpyHostToDevice, h->stream), "upload rows"))))
+        return rc;
+    const void *rows_dev = rows_on_device ? x : data.p;
     if ((rc = check(ids_d.ensure(batch * 4), "ids"))) return rc;
     if ((rc = check(lab_d.ensure(batch * 4), "labels"))) return rc;
     if ((rc = check(dist_d.ensure(batch * 4), "distances"))) return rc;
@@ -54,9 +59,25 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     if ((rc = check(usage_h.alloc(k * 4), "pinned usage"))) return rc;
     if ((rc = check(pairs_h.alloc(k * 8), "pinned pairs"))) return rc;
     memcpy(usage_h.p, J.st.usage.data(), k * 4);
-    if ((rc = check(hipMemcpyAsync(cent_d.p, J.st.centroids.data(), k * d * 4, hipMemcpyHostToDevice, h->stream),
-                    "upload centroids")) ||
-        (rc = check(hipMemcpyAsync(usage_d.p, usage_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "upload usage")))
+    DevBuf init_d;
+    if (rows_on_device) {
+        // the initial centroids: rows J.init_rows of the device table
+        HostBuf init_h;
+        if ((rc = check(init_h.alloc(k * 4), "pinned initial rows")) ||
+            (rc = check(init_d.ensure(k * 4), "initial rows")))
+            return rc;
+        memcpy(init_h.p, J.init_rows.data(), k * 4);
+        if ((rc = check(hipMemcpyAsync(init_d.p, init_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "initial rows")) ||
+            (rc = check(launch_gather_rows(x, rows, d, init_d.as<uint32_t>(), k, cent_d.as<float>(), h->stream),
+                        "initial centroids")) ||
+            (rc = check(hipStreamSynchronize(h->stream), "initial centroids")))  // before init_h is freed
+            return rc;
+    } else if ((rc = check(hipMemcpyAsync(cent_d.p, J.st.centroids.data(), k * d * 4, hipMemcpyHostToDevice,
+                                          h->stream),
+                           "upload centroids"))) {
+        return rc;
+    }
+    if ((rc = check(hipMemcpyAsync(usage_d.p, usage_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "upload usage")))
         return rc;
     std::vector<std::pair<size_t, size_t>> pairs;
 
@@ -118,10 +139,10 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         if ((rc = check(hipMemcpyAsync(ids_d.p, ids_host, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
             return rc;
         // gather + search on the device (src/lib.rs:419-424)
-        if ((rc = run_search(h, data.p, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
+        if ((rc = run_search(h, rows_dev, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
             return rc;
         // per-cluster counts and sums in batch order (src/lib.rs:431-441)
-        if ((rc = check(launch_accumulate(h->acc, data.p, rows, nb, d, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), k,
+        if ((rc = check(launch_accumulate(h->acc, rows_dev, rows, nb, d, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), k,
                                           sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream),
                         "accumulate")))
             return rc;
@@ -135,6 +156,8 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                                          counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream),
                         "EMA step")))
             return rc;
+        // the next epoch's batch order, drawn while this batch runs
+        J.prefetch_order();
         const bool revive = J.revive_due();
         const bool need_usage = revive || cfg.balance > 0.0f;
         if (async) {
@@ -219,7 +242,13 @@ int emh_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const emh_config *cfg
 int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size_t k, const emh_config *cfg,
               float *centroids_out, uint32_t *labels_out, emh_train_out *out) {
     if (!h || !cfg) return fail(EMH_E_ARG, "train: NULL argument");
-    return train_impl(h, x, rows, n, d, k, *cfg, centroids_out, labels_out, out);
+    return train_impl(h, x, rows, n, d, k, *cfg, centroids_out, labels_out, out, false);
+}
+
+int emh_train_device(emh_assigner *h, const void *x_dev, int rows, size_t n, size_t d, size_t k,
+                     const emh_config *cfg, float *centroids_out, uint32_t *labels_out, emh_train_out *out) {
+    if (!h || !cfg) return fail(EMH_E_ARG, "train_device: NULL argument");
+    return train_impl(h, x_dev, rows, n, d, k, *cfg, centroids_out, labels_out, out, true);
 }
 
 int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,

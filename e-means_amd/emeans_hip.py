@@ -28,7 +28,7 @@ EXPORTS = [
     "emh_create", "emh_destroy", "emh_set_centroids", "emh_set_bias", "emh_name", "emh_assign_f32",
     "emh_assign_i8", "emh_assign_u8", "emh_last_error", "emh_set_stream", "emh_assign_device",
     "emh_synchronize", "emh_accumulate_device", "emh_config_default", "emh_resolve_recipe", "emh_train",
-    "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split",
+    "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split", "emh_train_device",
     "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
     "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device", "emh_report_config_default",
     "emh_report",
@@ -136,6 +136,7 @@ def lib():
     L.emh_config_default.restype = None
     L.emh_resolve_recipe.argtypes = [u64, u64, u64, C.POINTER(Config), C.POINTER(Recipe)]
     L.emh_train.argtypes = [vp, vp, i32, sz, sz, sz, C.POINTER(Config), vp, vp, C.POINTER(TrainOut)]
+    L.emh_train_device.argtypes = [vp, vp, i32, sz, sz, sz, C.POINTER(Config), vp, vp, C.POINTER(TrainOut)]
     L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
     L.emh_train_batch.argtypes = [vp, C.POINTER(Job), sz, C.POINTER(BatchStats)]
     L.emh_set_split.argtypes = [vp, i32]
@@ -284,18 +285,41 @@ class Output:
     seconds: float = 0.0
 
 
+def _device_rows(x):
+    """(pointer, row type) of a contiguous 2-D torch CUDA tensor, else None."""
+    if type(x).__module__.split(".")[0] != "torch" or not getattr(x, "is_cuda", False):
+        return None
+    import torch
+
+    kinds = {torch.float32: ROWS_F32, torch.int8: ROWS_I8, torch.uint8: ROWS_U8}
+    if x.dtype not in kinds or x.dim() != 2 or not x.is_contiguous():
+        raise TypeError("device rows must be a contiguous 2-D f32, i8 or u8 CUDA tensor")
+    return x.data_ptr(), kinds[x.dtype]
+
+
 def train(x, k: int, cfg: Config | None = None, assigner: HipAssigner | None = None) -> Output:
-    """train (src/lib.rs:348-491) with the HIP assigner (FP32 by default)."""
-    x = np.ascontiguousarray(x)
-    if x.dtype not in ROWS or x.ndim != 2:
-        raise TypeError("x must be a 2-D f32, i8 or u8 array")
+    """train (src/lib.rs:348-491) with the HIP assigner (FP32 by default).
+    x: a host array, or a torch CUDA tensor whose rows are used in place
+    (emh_train_device: no copy)."""
     a = assigner or HipAssigner(FP32)
+    dev = _device_rows(x)
+    if dev is None:
+        x = np.ascontiguousarray(x)
+        if x.dtype not in ROWS or x.ndim != 2:
+            raise TypeError("x must be a 2-D f32, i8 or u8 array")
     n, d = x.shape
     cent = np.empty((k, d), np.float32)
     labels = np.empty(n, np.uint32)
     out = TrainOut()
-    _check(lib().emh_train(a.handle, _p(x), ROWS[x.dtype], n, d, k, C.byref(cfg or Config()), _p(cent),
-                           _p(labels), C.byref(out)))
+    if dev is None:
+        _check(lib().emh_train(a.handle, _p(x), ROWS[x.dtype], n, d, k, C.byref(cfg or Config()), _p(cent),
+                               _p(labels), C.byref(out)))
+    else:
+        import torch
+
+        torch.cuda.synchronize()  # the tensor's producers, on torch's streams
+        _check(lib().emh_train_device(a.handle, C.c_void_p(dev[0]), dev[1], n, d, k, C.byref(cfg or Config()),
+                                      _p(cent), _p(labels), C.byref(out)))
     obj = [out.objective_[i] for i in range(out.n_objective)]
     return Output(cent, labels, obj, int(out.n_splits), out.recipe, float(out.seconds))
 
@@ -305,28 +329,42 @@ class Job(C.Structure):
 
     _fields_ = [("x", C.c_void_p), ("rows", C.c_int), ("n", C.c_uint64), ("d", C.c_uint64), ("k", C.c_uint64),
                 ("cfg", Config), ("centroids_out", C.c_void_p), ("labels_out", C.c_void_p),
-                ("out", C.POINTER(TrainOut))]
+                ("out", C.POINTER(TrainOut)), ("x_on_device", C.c_int)]
 
 
 class BatchStats(C.Structure):
-    _fields_ = [("steps", C.c_uint64), ("graph_nodes", C.c_uint64), ("seconds", C.c_double)]
+    _fields_ = [("steps", C.c_uint64), ("graph_nodes", C.c_uint64), ("seconds", C.c_double),
+                ("host_seconds", C.c_double), ("wait_seconds", C.c_double), ("launch_seconds", C.c_double)]
 
 
 def train_batch(problems, assigner: HipAssigner | None = None):
     """Trains independent problems in lockstep, one captured-graph replay per
     training step (emh_train_batch). problems: (x, k, cfg) tuples with f32
-    rows. Each result equals train(x, k, cfg, assigner) on the same assigner
-    precision. Returns (list of Output, BatchStats)."""
+    rows (host arrays, or torch CUDA tensors used in place). Each result
+    equals train(x, k, cfg, assigner) on the same assigner precision.
+    Returns (list of Output, BatchStats)."""
     a = assigner or HipAssigner(FP32)
     keep, jobs = [], (Job * len(problems))()
+    any_dev = False
     for i, (x, k, cfg) in enumerate(problems):
-        x = np.ascontiguousarray(x, np.float32)
+        dev = _device_rows(x)
+        if dev is None:
+            x = np.ascontiguousarray(x, np.float32)
+        elif dev[1] != ROWS_F32:
+            raise TypeError("train_batch takes f32 rows")
+        any_dev = any_dev or dev is not None
         n, d = x.shape
         cent = np.empty((k, d), np.float32)
         labels = np.empty(n, np.uint32)
         out = TrainOut()
         keep.append((x, cent, labels, out))
-        jobs[i] = Job(_p(x), ROWS_F32, n, d, k, cfg or Config(), _p(cent), _p(labels), C.pointer(out))
+        ptr = _p(x) if dev is None else C.c_void_p(dev[0])
+        jobs[i] = Job(ptr, ROWS_F32, n, d, k, cfg or Config(), _p(cent), _p(labels), C.pointer(out),
+                      int(dev is not None))
+    if any_dev:
+        import torch
+
+        torch.cuda.synchronize()
     st = BatchStats()
     _check(lib().emh_train_batch(a.handle, jobs, len(problems), C.byref(st)))
     res = []

@@ -152,6 +152,11 @@ int emh_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const emh_config *cfg
  * the device for the closing mean. centroids_out: k x d; labels_out: n. */
 int emh_train(emh_assigner *h, const void *x, int rows, size_t n, size_t d, size_t k, const emh_config *cfg,
               float *centroids_out, uint32_t *labels_out, emh_train_out *out);
+/* emh_train on rows already in device memory (x_dev, n x d, row-major): no
+ * copy of the rows; the initial centroids are gathered on the device. The
+ * outputs are host arrays as for emh_train, and the results are the same. */
+int emh_train_device(emh_assigner *h, const void *x_dev, int rows, size_t n, size_t d, size_t k,
+                     const emh_config *cfg, float *centroids_out, uint32_t *labels_out, emh_train_out *out);
 
 /* assign (src/lib.rs:512-520): exact labels and distances for all n rows. */
 int emh_assign_all(emh_assigner *h, const void *x, int rows, size_t n, size_t d, const float *centroids, size_t k,
@@ -250,12 +255,16 @@ typedef struct emh_job {
     float *centroids_out; /* k x d */
     uint32_t *labels_out; /* n */
     emh_train_out *out;
+    int x_on_device;      /* nonzero: x points to device memory (rows already in HBM, not copied) */
 } emh_job;
 
 typedef struct emh_batch_stats {
     uint64_t steps;       /* graph replays: one per lockstep training step */
     uint64_t graph_nodes; /* nodes of the per-step graph */
     double seconds;       /* wall time of the call (row uploads included) */
+    double host_seconds;  /* of which the host halves of the steps (staging, statistics, revival plans) */
+    double wait_seconds;  /* and waiting for the replays to finish */
+    double launch_seconds; /* and in the graph launch calls */
 } emh_batch_stats;
 
 int emh_train_batch(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats);

@@ -162,3 +162,44 @@ def test_batched_training_equals_single(hip, prec):
         assert np.array_equal(g.centroids, ref.centroids)
         assert g.objective == ref.objective
         assert g.n_splits == ref.n_splits
+
+
+@pytest.mark.parametrize("prec", ["fp32", "i8"])
+def test_device_rows_training_equals_host_rows(hip, prec):
+    """emh_train_device (rows already in HBM, initial centroids gathered on
+    the device) gives exactly the result of emh_train on the same rows."""
+    import torch
+
+    print(f"device rows {prec}", flush=True)
+    if prec == "i8":
+        x = _i8_blobs(6000, 48, 60, 2)
+        cfg = E.Config(nepochs=2, seed=5)
+        a, b = E.HipAssigner(E.I8), E.HipAssigner(E.I8)
+    else:
+        x = _job_data(21, 9000, 40, 120)
+        cfg = E.Config(nepochs=2, seed=5, balance=0.1)
+        a, b = E.HipAssigner(E.FP32), E.HipAssigner(E.FP32)
+    ref = E.train(x, 60, cfg, a)
+    got = E.train(torch.from_numpy(x).cuda(), 60, cfg, b)
+    assert np.array_equal(got.labels, ref.labels)
+    assert np.array_equal(got.centroids, ref.centroids)
+    assert got.objective == ref.objective and got.n_splits == ref.n_splits
+
+
+def test_batched_device_rows_equal_host_rows(hip):
+    """emh_train_batch with some jobs' rows on the device and some on the host
+    gives every job its host-rows result."""
+    import torch
+
+    specs = [(3000, 24, 40, 0.0), (9000, 64, 200, 0.1), (5000, 33, 17, 0.0)]
+    host, mixed = [], []
+    for i, (n, d, k, bal) in enumerate(specs):
+        x = _job_data(30 + i, n, d, 2 * k)
+        cfg = E.Config(nepochs=2, balance=bal, seed=3 + i)
+        host.append((x, k, cfg))
+        mixed.append((torch.from_numpy(x).cuda() if i != 1 else x, k, cfg))
+    ref, _ = E.train_batch(host, E.HipAssigner("bf16"))
+    got, _ = E.train_batch(mixed, E.HipAssigner("bf16"))
+    for g, r in zip(got, ref):
+        assert np.array_equal(g.labels, r.labels)
+        assert np.array_equal(g.centroids, r.centroids)
