@@ -471,6 +471,18 @@ def report_nearest_line(args, k, d):
                      "note": "3 unfused, unpacked f32 ops per element per unordered pair; that instruction "
                              "mix caps at 1/4 of the packed-FMA peak"},
     }
+    # the whole report::report around this step, on 2M synthetic rows'
+    # labels and squared distances (host statistics + both device steps)
+    n_rows = 2_000_000
+    rng = np.random.default_rng(0)
+    w = rng.pareto(1.5, size=k) + 1e-3
+    lab = rng.choice(k, size=n_rows, p=w / w.sum()).astype(np.uint32)
+    sq = rng.gamma(2.0, 0.05, size=n_rows).astype(np.float32)
+    ch_all = c.cpu().numpy()
+    t = time.perf_counter()
+    rep = E.report(lab, sq, ch_all)
+    res["full_report"] = {"rows": n_rows, "seconds": round(time.perf_counter() - t, 3),
+                          "families": len(rep.families)}
     if not args.no_cpu:
         import oracle as O
 

@@ -23,6 +23,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 #include "emh_assigner.h"
@@ -256,32 +258,52 @@ static int report_impl(int device, const uint32_t *labels, const float *sqdist, 
         std::vector<size_t> cur(start.begin(), start.end() - 1);
         for (size_t r = 0; r < n; r++) grouped[cur[labels[r]]++] = dist[r];
     }
-    float g_med, g_mad;
-    {
+    // corpus median and MAD (src/report.rs:139-142): order statistics, so a
+    // selection gives the elements a full sort would put at those indices
+    float g_med = 0.0f, g_mad = 0.0f;
+    if (n) {
+        auto less = [](float a, float b) { return total_key(a) < total_key(b); };
+        const size_t mi = (size_t)llround((double)(n - 1) * 0.5);
         std::vector<float> all(dist);
-        sort_total(all);
-        median_mad(all, 0, n, g_med, g_mad);
+        std::nth_element(all.begin(), all.begin() + mi, all.end(), less);
+        g_med = all[mi];
+        for (float &v : all) v = fabsf(v - g_med);
+        std::nth_element(all.begin(), all.begin() + mi, all.end(), less);
+        g_mad = all[mi];
     }
     const float g_scale = fmaxf(1.4826f * g_mad, FLT_MIN);
 
-    // per-cluster statistics (src/report.rs:145-167)
+    // per-cluster statistics (src/report.rs:145-167), clusters spread over
+    // host threads (each sorts and folds its own range of `grouped`)
     std::vector<uint32_t> sizes(k);
-    for (size_t j = 0; j < k; j++) {
+    auto cluster = [&](size_t j) {
         emh_cluster_stats &c = o.clusters[j];
         memset(&c, 0, sizeof c);
         const size_t lo = start[j], len = start[j + 1] - lo;
         sizes[j] = (uint32_t)len;
-        if (!len) continue;
+        if (!len) return;
         std::sort(grouped.begin() + lo, grouped.begin() + lo + len,
                   [](float a, float b) { return total_key(a) < total_key(b); });
         c.size = (uint32_t)len;
         median_mad(grouped, lo, len, c.median, c.mad);
-        double sq = 0.0;
+        double sq = 0.0;  // f64, in the sorted order (src/report.rs:157)
         for (size_t t = 0; t < len; t++) sq += (double)grouped[lo + t] * (double)grouped[lo + t];
         c.rms = (float)sqrt(sq / (double)len);
         c.p90 = quantile(grouped, lo, len, 0.9);
         c.p99 = quantile(grouped, lo, len, 0.99);
         c.max = grouped[lo + len - 1];
+    };
+    {
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t j0; (j0 = next.fetch_add(64)) < k;)
+                for (size_t j = j0; j < std::min(k, j0 + 64); j++) cluster(j);
+        };
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < nt && (size_t)t * 64 < k; t++) pool.emplace_back(worker);
+        worker();
+        for (auto &t : pool) t.join();
     }
 
     // nearest live other centroid on the device (src/report.rs:169-183),
