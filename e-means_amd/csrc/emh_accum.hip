@@ -49,6 +49,48 @@ template <int ROWS> __device__ inline float row_elem(const void *x, size_t idx) 
     else return (float)((const uint8_t *)x)[idx];
 }
 
+// The in-order fold of one cluster's n member rows into acc[c] (dimension
+// tg + 64 c + lane, c < CC), by one wave: member row indices come 64 at a
+// time (one per lane, src_of), and U rows' values are loaded ahead of their
+// in-order adds. CC x U = 64 values per lane in flight: the fewer dimensions
+// left, the more rows at once.
+template <int ROWS, int CC, int U, class SrcOf>
+__device__ __forceinline__ void fold_members(const void *x, size_t d, size_t tg, uint32_t n, int lane, SrcOf src_of,
+                                             float (&acc)[4]) {
+    for (uint32_t q0 = 0; q0 < n; q0 += 64) {
+        const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
+        size_t my_src = 0;
+        if ((uint32_t)lane < cnt) my_src = src_of(q0 + lane);
+        for (uint32_t u0 = 0; u0 < cnt; u0 += U) {
+            float v[CC][U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
+#pragma unroll
+                for (int c = 0; c < CC; c++) {
+                    const size_t t = tg + 64 * c + lane;
+                    v[c][u] = (t < d && u0 + u < cnt) ? row_elem<ROWS>(x, src * d + t) : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (u0 + u < cnt)
+#pragma unroll
+                    for (int c = 0; c < CC; c++) acc[c] = __fadd_rn(acc[c], v[c][u]);
+        }
+    }
+}
+
+// fold_members with CC picked from the dimensions left at tg
+template <int ROWS, class SrcOf>
+__device__ __forceinline__ void fold_cluster(const void *x, size_t d, size_t tg, uint32_t n, int lane, SrcOf src_of,
+                                             float (&acc)[4]) {
+    const size_t left = d - tg;
+    if (left > 128) fold_members<ROWS, 4, 16>(x, d, tg, n, lane, src_of, acc);
+    else if (left > 64) fold_members<ROWS, 2, 32>(x, d, tg, n, lane, src_of, acc);
+    else fold_members<ROWS, 1, 64>(x, d, tg, n, lane, src_of, acc);
+}
+
 // One wave per cluster, lanes over dimensions; rows folded in batch order.
 template <int ROWS>
 __global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t d, const uint32_t *ids,
@@ -59,36 +101,14 @@ __global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t 
     if (jc >= k) return;
     const uint32_t n = counts[jc];
     const uint32_t s0 = n ? start[jc] : 0;
-    // one pass over the members per 256 dimensions (4 per lane); member rows
-    // fetched 64 at a time (one per lane), 64 values in flight ahead of the
-    // in-order adds
+    // one pass over the members per 256 dimensions (up to 4 per lane)
+    auto src_of = [&](uint32_t q) -> size_t {
+        const size_t r = sorted_pos[s0 + q];
+        return ids ? (size_t)ids[r] : r;
+    };
     for (size_t tg = 0; tg < d; tg += 256) {
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (uint32_t q0 = 0; q0 < n; q0 += 64) {
-            const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
-            size_t my_src = 0;
-            if ((uint32_t)lane < cnt) {
-                const size_t r = sorted_pos[s0 + q0 + lane];
-                my_src = ids ? (size_t)ids[r] : r;
-            }
-            for (uint32_t u0 = 0; u0 < cnt; u0 += 16) {
-                float v[4][16];
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const size_t t = tg + 64 * c + lane;
-                        v[c][u] = (t < d && u0 + u < cnt) ? row_elem<ROWS>(x, src * d + t) : 0.0f;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 16; u++)
-                    if (u0 + u < cnt)
-#pragma unroll
-                        for (int c = 0; c < 4; c++) acc[c] = __fadd_rn(acc[c], v[c][u]);
-            }
-        }
+        fold_cluster<ROWS>(x, d, tg, n, lane, src_of, acc);
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const size_t t = tg + 64 * c + lane;
@@ -376,30 +396,10 @@ __global__ __launch_bounds__(256) void jobs_segment_sum_kernel(const JobDev *job
     const size_t c = g - j.cluster_base, d = j.d;
     const uint32_t n = counts_all[g];
     const uint32_t s0 = n ? start[g] : 0;
+    auto src_of = [&](uint32_t q) -> size_t { return (size_t)j.ids[sorted_slot[s0 + q] - j.row_base]; };
     for (size_t tg = 0; tg < d; tg += 256) {
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (uint32_t q0 = 0; q0 < n; q0 += 64) {
-            const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
-            size_t my_src = 0;
-            if ((uint32_t)lane < cnt) my_src = (size_t)j.ids[sorted_slot[s0 + q0 + lane] - j.row_base];
-            for (uint32_t u0 = 0; u0 < cnt; u0 += 16) {
-                float v[4][16];
-#pragma unroll
-                for (int u = 0; u < 16; u++) {
-                    const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
-#pragma unroll
-                    for (int cc = 0; cc < 4; cc++) {
-                        const size_t t = tg + 64 * cc + lane;
-                        v[cc][u] = (t < d && u0 + u < cnt) ? j.x[src * d + t] : 0.0f;
-                    }
-                }
-#pragma unroll
-                for (int u = 0; u < 16; u++)
-                    if (u0 + u < cnt)
-#pragma unroll
-                        for (int cc = 0; cc < 4; cc++) acc[cc] = __fadd_rn(acc[cc], v[cc][u]);
-            }
-        }
+        fold_cluster<EMH_ROWS_F32>(j.x, d, tg, n, lane, src_of, acc);
 #pragma unroll
         for (int cc = 0; cc < 4; cc++) {
             const size_t t = tg + 64 * cc + lane;
