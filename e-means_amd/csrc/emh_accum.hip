@@ -59,17 +59,20 @@ __device__ __forceinline__ void fold_members(const void *x, size_t d, size_t tg,
                                              float (&acc)[4]) {
     for (uint32_t q0 = 0; q0 < n; q0 += 64) {
         const uint32_t cnt = n - q0 < 64 ? n - q0 : 64;
-        size_t my_src = 0;
-        if ((uint32_t)lane < cnt) my_src = src_of(q0 + lane);
+        uint32_t my_src = 0;  // row indices are below 2^32
+        if ((uint32_t)lane < cnt) my_src = (uint32_t)src_of(q0 + lane);
         for (uint32_t u0 = 0; u0 < cnt; u0 += U) {
             float v[CC][U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const size_t src = (size_t)__shfl((long long)my_src, (int)(u0 + u) & 63, 64);
+                // the member's row start is uniform (scalar registers); each
+                // lane adds its dimension
+                const uint32_t src = (uint32_t)__builtin_amdgcn_readlane((int)my_src, (int)(u0 + u) & 63);
+                const size_t row0 = (size_t)src * d + tg;
 #pragma unroll
                 for (int c = 0; c < CC; c++) {
-                    const size_t t = tg + 64 * c + lane;
-                    v[c][u] = (t < d && u0 + u < cnt) ? row_elem<ROWS>(x, src * d + t) : 0.0f;
+                    const uint32_t t = 64u * c + (uint32_t)lane;
+                    v[c][u] = (tg + t < d && u0 + u < cnt) ? row_elem<ROWS>(x, row0 + t) : 0.0f;
                 }
             }
 #pragma unroll
@@ -81,40 +84,24 @@ __device__ __forceinline__ void fold_members(const void *x, size_t d, size_t tg,
     }
 }
 
-// fold_members with CC picked from the dimensions left at tg
-template <int ROWS, class SrcOf>
-__device__ __forceinline__ void fold_cluster(const void *x, size_t d, size_t tg, uint32_t n, int lane, SrcOf src_of,
-                                             float (&acc)[4]) {
-    const size_t left = d - tg;
-    if (left > 128) fold_members<ROWS, 4, 16>(x, d, tg, n, lane, src_of, acc);
-    else if (left > 64) fold_members<ROWS, 2, 32>(x, d, tg, n, lane, src_of, acc);
-    else fold_members<ROWS, 1, 64>(x, d, tg, n, lane, src_of, acc);
-}
-
-// One wave per cluster, lanes over dimensions; rows folded in batch order.
+// One wave per (cluster, 64 dimensions), the dimension group major (wave
+// w: group w / k, cluster w % k), lanes over the group's dimensions, 64
+// member rows in flight; rows folded in batch order (sorted_src: the source
+// row of each member, batch order within a cluster).
 template <int ROWS>
-__global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t d, const uint32_t *ids,
-                                                          const uint32_t *sorted_pos, const uint32_t *start,
-                                                          const uint32_t *counts, size_t k, float *sums) {
-    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t d, const uint32_t *sorted_src,
+                                                          const uint32_t *start, const uint32_t *counts, size_t k,
+                                                          float *sums) {
+    const size_t wv = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (jc >= k) return;
+    const size_t grp = wv / k, jc = wv - grp * k, tg = grp * 64;
+    if (tg >= d) return;
     const uint32_t n = counts[jc];
     const uint32_t s0 = n ? start[jc] : 0;
-    // one pass over the members per 256 dimensions (up to 4 per lane)
-    auto src_of = [&](uint32_t q) -> size_t {
-        const size_t r = sorted_pos[s0 + q];
-        return ids ? (size_t)ids[r] : r;
-    };
-    for (size_t tg = 0; tg < d; tg += 256) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        fold_cluster<ROWS>(x, d, tg, n, lane, src_of, acc);
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const size_t t = tg + 64 * c + lane;
-            if (t < d) sums[jc * d + t] = acc[c];
-        }
-    }
+    auto src_of = [&](uint32_t q) -> size_t { return (size_t)sorted_src[s0 + q]; };
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    fold_members<ROWS, 1, 64>(x, d, tg, n, lane, src_of, acc);
+    if (tg + lane < d) sums[jc * d + tg + lane] = acc[0];
 }
 
 __global__ void pass_accumulate_kernel(const float *sums, const uint32_t *counts, size_t k, size_t d,
@@ -149,6 +136,9 @@ static int key_bits(size_t k) {
 
 // ---- stable LSD radix sort of (key, position) pairs, 8-bit digits ----------
 constexpr int SORT_BLOCK = 1024;  // elements per workgroup (256 threads x 4 rounds)
+// Batched sums: clusters with more members than this in one step are folded
+// one wave per 64 dimensions (jobs_big_sum_kernel), the rest as a stream.
+constexpr uint32_t SUM_BIG = 128;
 
 // hist[digit * nblocks + block] = how many of the block's keys have the digit
 __global__ __launch_bounds__(256) void digit_hist_kernel(const uint32_t *keys, size_t m, int shift, uint32_t *hist,
@@ -263,6 +253,9 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         if ((e = hipMalloc(&a.keys_alt, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.vals, m * 4)) != hipSuccess) return e;
         if ((e = hipMalloc(&a.vals_alt, m * 4)) != hipSuccess) return e;
+        hipFree(a.big);
+        a.big = nullptr;
+        if ((e = hipMalloc(&a.big, (2 + 2 * (m / SUM_BIG + 1)) * 4)) != hipSuccess) return e;
         const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
         a.sort_tmp_bytes = 256 * nblocks * 4 + 256 * 4;  // histogram + digit totals
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
@@ -284,15 +277,17 @@ void accum_free(AccumScratch &a) {
     hipFree(a.vals_alt);
     hipFree(a.sort_tmp);
     hipFree(a.offsets);
+    hipFree(a.big);
     a = AccumScratch();
 }
 
-hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, size_t m, int bits, const uint32_t **keys_out,
-                             const uint32_t **vals_out, hipStream_t s) {
+// vals_in: the value carried with each key (nullptr: its position)
+hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uint32_t *vals_in, size_t m, int bits,
+                             const uint32_t **keys_out, const uint32_t **vals_out, hipStream_t s) {
     const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
     uint32_t *hist = (uint32_t *)a.sort_tmp;
-    // ping-pong: pass 0 reads the input keys and writes positions
-    const uint32_t *kin = keys_in, *vin = nullptr;
+    // ping-pong: pass 0 reads the input keys and values
+    const uint32_t *kin = keys_in, *vin = vals_in;
     uint32_t *kb[2] = {a.keys, a.keys_alt}, *vb[2] = {a.vals, a.vals_alt};
     int cur = 0;
     for (int shift = 0; shift < bits; shift += 8) {
@@ -330,23 +325,26 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
     hipError_t e;
     if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
     if ((e = launch_fill_u32(counts, k, 0u, s)) != hipSuccess) return e;
-    const uint32_t *sorted_pos = a.vals;
+    // sorted by label, each carrying its source row (ids[i], or i)
+    const uint32_t *sorted_src = a.vals;
     if (m > 0) {
         unsigned blocks = (unsigned)std::min<size_t>((m + 255) / 256, 4096);
         hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(256), 0, s, labels, m, k, counts);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         const uint32_t *sorted_keys;
-        if ((e = launch_sort_pairs(a, labels, m, key_bits(k), &sorted_keys, &sorted_pos, s)) != hipSuccess) return e;
+        if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess)
+            return e;
         hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, sorted_keys, m, k, a.offsets);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
-    dim3 grid((unsigned)((k + 3) / 4));
+    const size_t waves = k * ((d + 63) / 64);
+    dim3 grid((unsigned)((waves + 3) / 4));
     if (rows == EMH_ROWS_F32)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
     else if (rows == EMH_ROWS_I8)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
     else
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, ids, sorted_pos, a.offsets, counts, k, sums);
+        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
     return hipGetLastError();
 }
 
@@ -355,10 +353,11 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
 // key of batch slot i: its job's first global cluster + label, or nclusters
 // for a slot past the job's rows this step (sorts last, never folded)
 __global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots, uint32_t nclusters,
-                                 uint32_t *keys, uint32_t *counts_all) {
+                                 uint32_t *keys, uint32_t *counts_all, uint32_t *big_count) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *big_count = 0;  // read by the sums after the sort
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (size_t)gridDim.x * blockDim.x) {
         const uint2 r = rmap[i];
-        const JobDev &j = jobs[r.x];
+        const JobDev j = job_at<false>(jobs, r.x);
         uint32_t key = nclusters;
         if (r.y < j.m) {
             const uint32_t l = j.labels[r.y];
@@ -376,66 +375,203 @@ __global__ void jobs_pass_clear_kernel(const JobDev *jobs, const uint32_t *cjob,
     const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (g >= nclusters) return;
-    const JobDev &j = jobs[cjob[g]];
+    const JobDev j = job_at<true>(jobs, cjob[g]);
     if (!j.clear_pass || j.m == 0) return;
     const size_t c = g - j.cluster_base;
     for (size_t t = lane; t < j.d; t += 64) j.pass_sums[c * j.d + t] = 0.0;
     if (lane == 0) j.pass_counts[c] = 0;
 }
 
-// One wave per global cluster: the job's rows folded in batch order (the
-// sorted slots), then the last epoch's FP64 totals.
-__global__ __launch_bounds__(256) void jobs_segment_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
-                                                               size_t nclusters, const uint32_t *sorted_slot,
-                                                               const uint32_t *start, const uint32_t *counts_all) {
-    const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (g >= nclusters) return;
-    const JobDev &j = jobs[cjob[g]];
-    if (j.m == 0) return;
-    const size_t c = g - j.cluster_base, d = j.d;
-    const uint32_t n = counts_all[g];
-    const uint32_t s0 = n ? start[g] : 0;
-    auto src_of = [&](uint32_t q) -> size_t { return (size_t)j.ids[sorted_slot[s0 + q] - j.row_base]; };
-    for (size_t tg = 0; tg < d; tg += 256) {
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        fold_cluster<EMH_ROWS_F32>(j.x, d, tg, n, lane, src_of, acc);
+// ---- batched per-cluster sums: the sorted members as one stream ----------
+//
+// A step's clusters are small (a batch spread over k clusters), so one wave
+// per cluster would spend its time on the prologue. Instead wave w owns the
+// clusters whose first sorted member lies in [w SUM_SPAN, (w + 1) SUM_SPAN)
+// and streams their members in sorted order: a cluster's members are its
+// batch rows in batch order (stable sort), folded one after the other, and
+// its sums written when the next cluster's first member comes up. Rows are
+// loaded U at a time across cluster boundaries, lanes over dimensions
+// (64 c + lane, c < CC).
+constexpr uint32_t SUM_SPAN = 16;
+
+// First position q in [p, lim) with keys[q] != prev (lim if none).
+__device__ inline size_t first_change(const uint32_t *keys, size_t p, size_t lim, uint32_t prev, int lane) {
+    for (size_t q0 = p; q0 < lim; q0 += 64) {
+        const size_t q = q0 + lane;
+        const uint64_t ch = __ballot(q < lim && keys[q] != prev);
+        if (ch) return q0 + (size_t)(__ffsll((long long)ch) - 1);
+    }
+    return lim;
+}
+
+// Writes a finished cluster: its f32 sums, and in the last epoch its FP64
+// pass totals and member count.
+template <int CC>
+__device__ __forceinline__ void flush_cluster(const JobDev &j, uint32_t key, const float (&acc)[CC], uint32_t cnt,
+                                              int lane) {
+    const size_t c = key - j.cluster_base, d = j.d;
 #pragma unroll
-        for (int cc = 0; cc < 4; cc++) {
-            const size_t t = tg + 64 * cc + lane;
-            if (t < d) {
-                j.sums[c * d + t] = acc[cc];
-                if (j.last_epoch && n) j.pass_sums[c * d + t] += (double)acc[cc];
-            }
+    for (int cc = 0; cc < CC; cc++) {
+        const size_t t = 64 * cc + lane;
+        if (t < d) {
+            j.sums[c * d + t] = acc[cc];
+            if (j.last_epoch) j.pass_sums[c * d + t] += (double)acc[cc];
         }
     }
-    if (lane == 0 && j.last_epoch && n) j.pass_counts[c] += n;
+    if (lane == 0 && j.last_epoch) j.pass_counts[c] += cnt;
+}
+
+// Streams job j's clusters from position p (a cluster's first member) while
+// they start before `stop` and belong to j; returns the position after the
+// last member folded.
+template <int CC, int U>
+__device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *keys, const uint32_t *src,
+                                             const uint32_t *counts, size_t slots, size_t p, size_t stop, int lane) {
+    const size_t d = j.d;
+    const float *x = j.x;
+    const uint32_t job_end = j.cluster_base + j.k;
+    float acc[CC];
+#pragma unroll
+    for (int cc = 0; cc < CC; cc++) acc[cc] = 0.0f;
+    uint32_t cur = keys[p], cnt = 0;
+    for (;;) {
+        // the next 64 positions: a member is taken when it continues the
+        // running cluster or starts one of j's clusters before `stop` that
+        // is not a big one
+        const size_t q = p + lane;
+        const uint32_t mk = q < slots ? keys[q] : 0xFFFFFFFFu;
+        uint32_t before = (uint32_t)__shfl_up((int)mk, 1, 64);
+        if (lane == 0) before = cur;
+        bool take = mk == before;
+        if (!take && q < stop && mk < job_end) take = counts[mk] <= SUM_BIG;
+        const uint64_t miss = __ballot(!take);
+        const int ntake = miss ? __ffsll((long long)miss) - 1 : 64;
+        const uint32_t ms = lane < ntake ? src[q] : 0u;
+        for (int u0 = 0; u0 < ntake; u0 += U) {
+            float v[CC][U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                // the member's row address is uniform: scalar registers
+                // hold it, each lane adds its dimension
+                const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)ms, (u0 + u) & 63);
+                const char *xr = (const char *)(x + (size_t)r * d);
+#pragma unroll
+                for (int cc = 0; cc < CC; cc++) {
+                    const uint32_t t = 64u * cc + (uint32_t)lane;
+                    v[cc][u] = (t < d && u0 + u < ntake) ? *(const float *)(xr + 4u * t) : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                if (u0 + u < ntake) {
+                    const uint32_t ku = (uint32_t)__builtin_amdgcn_readlane((int)mk, (u0 + u) & 63);
+                    if (ku != cur) {
+                        flush_cluster<CC>(j, cur, acc, cnt, lane);
+#pragma unroll
+                        for (int cc = 0; cc < CC; cc++) acc[cc] = 0.0f;
+                        cnt = 0;
+                        cur = ku;
+                    }
+#pragma unroll
+                    for (int cc = 0; cc < CC; cc++) acc[cc] = __fadd_rn(acc[cc], v[cc][u]);
+                    cnt++;
+                }
+            }
+        }
+        p += (size_t)ntake;
+        if (ntake < 64) break;
+    }
+    flush_cluster<CC>(j, cur, acc, cnt, lane);
+    return p;
+}
+
+// keys / src: the sorted slots' global clusters (nclusters for spare slots,
+// last) and job-relative rows; counts: members per global cluster. Big
+// clusters are listed in big (after the count word) for jobs_big_sum_kernel.
+__global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
+                                                              uint32_t nclusters, const uint32_t *keys,
+                                                              const uint32_t *src, const uint32_t *counts,
+                                                              size_t slots, uint32_t *big) {
+    const int lane = threadIdx.x & 63;
+    size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * SUM_SPAN;
+    if (p >= slots) return;
+    const size_t stop = p + SUM_SPAN < slots ? p + SUM_SPAN : slots;
+    // the members before the first cluster that starts in this span belong
+    // to an earlier wave
+    if (p > 0) p = first_change(keys, p, stop, keys[p - 1], lane);
+    while (p < stop) {
+        const uint32_t key = keys[p];
+        if (key >= nclusters) break;
+        const uint32_t n = counts[key];
+        if (n > SUM_BIG) {
+            if (lane == 0) {
+                const uint32_t at = atomicAdd(&big[0], 1u);
+                big[2 + 2 * at] = key;
+                big[3 + 2 * at] = (uint32_t)p;
+            }
+            p += n;
+            continue;
+        }
+        const JobDev j = job_at<true>(jobs, cjob[key]);
+        if (j.d > 128) p = stream_job<4, 16>(j, keys, src, counts, slots, p, stop, lane);
+        else if (j.d > 64) p = stream_job<2, 32>(j, keys, src, counts, slots, p, stop, lane);
+        else p = stream_job<1, 64>(j, keys, src, counts, slots, p, stop, lane);
+    }
+}
+
+// The big clusters of this step: wave t of the grid-stride loop folds
+// dimensions [64 (t % 4), + 64) of listed cluster t / 4 (batched rows are at
+// most 256 wide), its members in batch order, 64 rows in flight.
+constexpr unsigned BIG_SUM_BLOCKS = 512;
+
+__global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
+                                                           const uint32_t *src, const uint32_t *counts,
+                                                           const uint32_t *big) {
+    const int lane = threadIdx.x & 63;
+    const size_t total = (size_t)big[0] * 4, waves = (size_t)gridDim.x * 4;
+    for (size_t t = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += waves) {
+        const uint32_t key = big[2 + 2 * (t >> 2)], p0 = big[3 + 2 * (t >> 2)];
+        const size_t tg = (t & 3) * 64;
+        const JobDev j = job_at<true>(jobs, cjob[key]);
+        if (tg >= j.d) continue;
+        const uint32_t n = counts[key];
+        auto src_of = [&](uint32_t q) -> size_t { return (size_t)src[p0 + q]; };
+        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        fold_members<EMH_ROWS_F32, 1, 64>(j.x, j.d, tg, n, lane, src_of, acc);
+        const size_t c = key - j.cluster_base, dd = tg + lane;
+        if (dd < j.d) {
+            j.sums[c * j.d + dd] = acc[0];
+            if (j.last_epoch) j.pass_sums[c * j.d + dd] += (double)acc[0];
+        }
+        if (lane == 0 && tg == 0 && j.last_epoch) j.pass_counts[c] += n;
+    }
 }
 
 hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                                  const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
-                                  hipStream_t s) {
+                                  const uint32_t *ids, const uint32_t *cjob, size_t nclusters, uint32_t *keys,
+                                  uint32_t *counts_all, hipStream_t s) {
     hipError_t e;
     if ((e = accum_reserve(a, slots > 0 ? slots : 1, nclusters)) != hipSuccess) return e;
     dim3 cgrid((unsigned)((nclusters + 3) / 4));
     hipLaunchKernelGGL(jobs_pass_clear_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters);
     if ((e = launch_fill_u32(counts_all, nclusters, 0u, s)) != hipSuccess) return e;
-    const uint32_t *sorted_slot = a.vals;
-    if (slots > 0) {
-        unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
-        hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters,
-                           keys, counts_all);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        const uint32_t *sorted_keys;
-        if ((e = launch_sort_pairs(a, keys, slots, key_bits(nclusters + 1), &sorted_keys, &sorted_slot, s)) !=
-            hipSuccess)
-            return e;
-        hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, sorted_keys, slots, nclusters,
-                           a.offsets);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(jobs_segment_sum_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters, sorted_slot,
-                       (const uint32_t *)a.offsets, (const uint32_t *)counts_all);
+    if (slots == 0) return hipSuccess;
+    unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
+    hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters, keys,
+                       counts_all, a.big);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // slot i's job-relative row is ids[i] (the jobs' id ranges are laid out
+    // in slot order)
+    const uint32_t *sorted_keys, *sorted_src;
+    if ((e = launch_sort_pairs(a, keys, ids, slots, key_bits(nclusters + 1), &sorted_keys, &sorted_src, s)) !=
+        hipSuccess)
+        return e;
+    const size_t waves = (slots + SUM_SPAN - 1) / SUM_SPAN;
+    hipLaunchKernelGGL(jobs_stream_sum_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, jobs, cjob,
+                       (uint32_t)nclusters, sorted_keys, sorted_src, (const uint32_t *)counts_all, slots, a.big);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src,
+                       (const uint32_t *)counts_all, (const uint32_t *)a.big);
     return hipGetLastError();
 }
 
@@ -477,7 +613,7 @@ __global__ __launch_bounds__(256) void ema_apply_kernel(float *cen, float *usage
 __global__ __launch_bounds__(256) void jobs_ema_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters) {
     const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= nclusters) return;
-    const JobDev &j = jobs[cjob[g]];
+    const JobDev j = job_at<true>(jobs, cjob[g]);
     if (j.m == 0) return;
     ema_cluster(j.cen, j.usage, j.sums, j.counts, g - j.cluster_base, j.d, j.alpha, j.decay, (int)j.replace,
                 threadIdx.x & 63);
@@ -548,7 +684,7 @@ __global__ __launch_bounds__(64) void jobs_revive_split_kernel(const JobDev *job
     const size_t t = blockIdx.x;
     const int lane = threadIdx.x;
     if (t >= ntriples) return;
-    const JobDev &j = jobs[triples[3 * t]];
+    const JobDev j = job_at<true>(jobs, triples[3 * t]);
     const size_t d = j.d;
     float *cs = j.cen + (size_t)triples[3 * t + 1] * d, *cd = j.cen + (size_t)triples[3 * t + 2] * d;
     const float lo = 1.0f - 1e-4f, hi = 1.0f + 1e-4f;

@@ -14,9 +14,7 @@
 // keep their slots and idle. Revival splits run between replays.
 #include <atomic>
 #include <chrono>
-#include <condition_variable>
 #include <functional>
-#include <mutex>
 #include <thread>
 
 #include "emh_recipe.h"
@@ -38,68 +36,64 @@ struct Layout {
 };
 
 // Runs fn(0..n-1) on a few persistent host threads plus the caller; each
-// index runs exactly once, and run() returns when all have.
+// index runs exactly once, and run() returns when all have. Between runs the
+// workers poll (yielding the core) instead of sleeping: a step's host halves
+// come tens of microseconds apart, less than a sleeping thread takes to wake.
+// A run is handed to worker w by storing its number in w's `go` (release)
+// after the run's state is written; w reports it with `done` (release), and
+// run() returns only when every worker has, so no worker is still inside a
+// run when the next one rewrites the state.
 class HostPool {
   public:
-    explicit HostPool(unsigned nthreads) {
-        for (unsigned t = 0; t < nthreads; t++) th_.emplace_back([this] { work(); });
+    explicit HostPool(unsigned nthreads) : w_(nthreads) {
+        for (unsigned t = 0; t < nthreads; t++) th_.emplace_back([this, t] { work(w_[t]); });
     }
     ~HostPool() {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-            gen_++;
-        }
-        cv_.notify_all();
+        stop_.store(true, std::memory_order_relaxed);
+        gen_++;
+        for (auto &w : w_) w.go.store(gen_, std::memory_order_release);
         for (auto &t : th_) t.join();
     }
     void run(size_t n, const std::function<void(size_t)> &fn) {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            fn_ = &fn;
-            n_ = n;
-            next_ = 0;
-            done_ = 0;
-            gen_++;
-        }
-        cv_.notify_all();
+        fn_ = &fn;
+        n_ = n;
+        next_.store(0, std::memory_order_relaxed);
+        gen_++;
+        for (auto &w : w_) w.go.store(gen_, std::memory_order_release);
         drain();
-        std::unique_lock<std::mutex> lk(mu_);
-        done_cv_.wait(lk, [&] { return done_ == n_; });
-        fn_ = nullptr;
+        for (auto &w : w_)
+            while (w.done.load(std::memory_order_acquire) != gen_) std::this_thread::yield();
     }
 
   private:
+    struct alignas(64) Slot {
+        std::atomic<uint64_t> go{0}, done{0};
+    };
     void drain() {
         for (;;) {
-            const size_t i = next_.fetch_add(1);
+            const size_t i = next_.fetch_add(1, std::memory_order_relaxed);
             if (i >= n_) return;
             (*fn_)(i);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (++done_ == n_) done_cv_.notify_all();
         }
     }
-    void work() {
+    void work(Slot &w) {
         uint64_t seen = 0;
         for (;;) {
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                cv_.wait(lk, [&] { return gen_ != seen; });
-                seen = gen_;
-                if (stop_) return;
-                if (!fn_) continue;
-            }
+            uint64_t g;
+            while ((g = w.go.load(std::memory_order_acquire)) == seen) std::this_thread::yield();
+            seen = g;
+            if (stop_.load(std::memory_order_relaxed)) return;
             drain();
+            w.done.store(g, std::memory_order_release);
         }
     }
+    std::vector<Slot> w_;
     std::vector<std::thread> th_;
-    std::mutex mu_;
-    std::condition_variable cv_, done_cv_;
     const std::function<void(size_t)> *fn_ = nullptr;
-    size_t n_ = 0, done_ = 0;
+    size_t n_ = 0;
     std::atomic<size_t> next_{0};
     uint64_t gen_ = 0;
-    bool stop_ = false;
+    std::atomic<bool> stop_{false};
 };
 
 struct GraphGuard {
@@ -123,17 +117,37 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
 
-    // Host runners, geometry and slot ranges.
+    // Host threads for the jobs' host halves (each job's recipe state is its
+    // own): the caller plus up to 15 workers.
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    HostPool pool(std::min<unsigned>({hw, 16u, (unsigned)njobs}) - 1);
+
+    // Host runners (set up in parallel: the initial rows come from a
+    // permutation of each job's rows), geometry and slot ranges.
     std::vector<JobRun> run(njobs);
+    std::vector<int> init_rc(njobs, EMH_OK);
+    std::vector<std::string> init_err(njobs);
+    const std::function<void(size_t)> init_job = [&](size_t j) {
+        const emh_job &jb = jobs[j];
+        if (jb.rows != EMH_ROWS_F32) {
+            init_rc[j] = EMH_E_UNSUPPORTED;
+            init_err[j] = "train_batch: f32 rows only";
+            return;
+        }
+        init_rc[j] = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out,
+                                 jb.x_on_device != 0);
+        if (init_rc[j] != EMH_OK) init_err[j] = emh_last_error();  // this thread's message
+    };
+    pool.run(njobs, init_job);
+    for (size_t j = 0; j < njobs; j++)
+        if (init_rc[j] != EMH_OK) return fail(init_rc[j], init_err[j]);
     std::vector<SearchGeom> geo(njobs);
-    std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs);
-    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0;
+    std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs),
+        part_base(njobs);
+    std::vector<JobSlices> slices(njobs);
+    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0;
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
-        if (jb.rows != EMH_ROWS_F32) return fail(EMH_E_UNSUPPORTED, "train_batch: f32 rows only");
-        if ((rc = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out,
-                              jb.x_on_device != 0)))
-            return rc;
         geo[j] = make_geom(op, jb.k, jb.d);
         if (!geo[j].reg)
             return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
@@ -148,6 +162,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         cpad += geo[j].k_pad;
         op_base[j] = op_bytes;
         op_bytes += geo[j].k_pad * (size_t)geo[j].row_bytes;
+        slices[j] = job_slices(geo[j]);
+        part_base[j] = part_words;
+        part_words += run[j].R.batch_size * job_part_words(op, M, slices[j].nslice);
     }
     if (slots > 0xFFFFFFFFull || nclusters >= 0xFFFFFFFFull)
         return fail(EMH_E_ARG, "train_batch: more than 2^32 - 1 batch rows or clusters in one call");
@@ -177,7 +194,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     Layout outl;
     const size_t out_lab = outl.add(slots * 4), out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
     DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts,
-        maps, pairs_d;
+        maps, pairs_d, part_d;
     HostBuf in_h, out_h, pairs_h;
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
@@ -190,7 +207,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(hn.ensure(cpad * 4), "offsets")) || (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
         (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
         (rc = check(keys.ensure(slots * 4), "keys")) || (rc = check(pass_sums.ensure(cen_floats * 8), "pass sums")) ||
-        (rc = check(pass_counts.ensure(nclusters * 8), "pass counts")))
+        (rc = check(pass_counts.ensure(nclusters * 8), "pass counts")) ||
+        (part_words && (rc = check(part_d.ensure(part_words * 4), "slice results"))))
         return rc;
     memset(in_h.p, 0, in.bytes);
     uint8_t *in_dp = (uint8_t *)in_d.p, *out_dp = (uint8_t *)out_d.p;
@@ -243,12 +261,21 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::sort(classes.begin(), classes.end());
     std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
     std::vector<size_t> wg_off, wg_cnt;
+    // One workgroup per (job, 256 batch rows, slice); within a class the
+    // jobs with the most tiles per slice come first, so the longest sweeps
+    // start first.
     for (int rb : classes) {
         wg_off.push_back(wg_all.size());
-        for (size_t j = 0; j < njobs; j++) {
-            if (geo[j].row_bytes != rb) continue;
-            for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += 256) wg_all.push_back(make_uint2((unsigned)j, (unsigned)r0));
-        }
+        std::vector<size_t> order;
+        for (size_t j = 0; j < njobs; j++)
+            if (geo[j].row_bytes == rb) order.push_back(j);
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+            return slices[a].tiles_per_slice > slices[b].tiles_per_slice;
+        });
+        for (size_t j : order)
+            for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += 256)
+                for (uint32_t sl = 0; sl < slices[j].nslice; sl++)
+                    wg_all.push_back(make_uint2((unsigned)j, (unsigned)r0 | sl));
         wg_cnt.push_back(wg_all.size() - wg_off.back());
     }
     std::vector<uint32_t> cjob(nclusters);
@@ -296,6 +323,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.d_pad = (uint32_t)geo[j].d_pad;
         t.row_bytes = (uint32_t)geo[j].row_bytes;
         t.tb = (uint32_t)reg_tile(geo[j]);
+        t.nslice = slices[j].nslice;
+        t.tiles_per_slice = slices[j].tiles_per_slice;
+        t.part = slices[j].nslice > 1 ? part_d.as<uint32_t>() + part_base[j] : nullptr;
         t.row_base = (uint32_t)row_base[j];
         t.cluster_base = (uint32_t)cl_base[j];
     }
@@ -312,9 +342,12 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (size_t c = 0; c < classes.size() && ce == hipSuccess; c++)
         ce = launch_jobs_search(op, M, classes[c], sa_dev, (const uint2 *)(mp + m_wg) + wg_off[c], wg_cnt[c], h->stream);
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    if (ce == hipSuccess && op == OP_F32 && part_words)
+        ce = launch_jobs_merge_f32(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess)
-        ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
-                                    nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+        ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots,
+                                    (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j), nclusters,
+                                    keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
     if (ce == hipSuccess) ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, h->stream);
     if (ce == hipSuccess) ce = hipMemcpyAsync(out_hp, out_dp, outl.bytes, hipMemcpyDeviceToHost, h->stream);
     hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);
@@ -330,8 +363,6 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<char> live(njobs);
     std::vector<std::vector<std::pair<size_t, size_t>>> jpairs(njobs);
     JobDev *jd_h = (JobDev *)(in_hp + in_job);
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    HostPool pool(std::min<unsigned>({hw, 16u, (unsigned)njobs}) - 1);
     const std::function<void(size_t)> stage = [&](size_t j) {
         JobRun &J = run[j];
         JobDev t = jd[j];
@@ -362,34 +393,32 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         }
         J.advance();
     };
-    // while a replay runs, the jobs whose batch closes an epoch draw the next
-    // epoch's order (the host's largest per-step cost otherwise)
-    const std::function<void(size_t)> prefetch = [&](size_t j) {
-        if (live[j]) run[j].prefetch_order();
-    };
     double host_s = 0.0, wait_s = 0.0, launch_s = 0.0;
     using clk = std::chrono::steady_clock;
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
+    // a job's host half of one step and the staging of its next batch run
+    // together, one pool run per step
+    const std::function<void(size_t)> finish_stage = [&](size_t j) {
+        finish(j);
+        stage(j);
+    };
+    auto th = clk::now();
+    pool.run(njobs, stage);
+    host_s += since(th);
     for (;;) {
-        auto th = clk::now();
-        pool.run(njobs, stage);
-        host_s += since(th);
         if (std::find(live.begin(), live.end(), (char)1) == live.end()) break;
         th = clk::now();
         if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch"))) return rc;
         launch_s += since(th);
         th = clk::now();
-        pool.run(njobs, prefetch);
-        host_s += since(th);
-        th = clk::now();
         if ((rc = check(hipStreamSynchronize(h->stream), "step"))) return rc;
         wait_s += since(th);
         steps++;
-        // host half per job; revival planned on the downloaded masses, rows
-        // split on the device before the next replay (one upload of all
-        // triples and masses, one kernel)
+        // host half per job, then its next batch; revival planned on the
+        // downloaded masses, rows split on the device before the next replay
+        // (one upload of all triples and masses, one kernel)
         th = clk::now();
-        pool.run(njobs, finish);
+        pool.run(njobs, finish_stage);
         host_s += since(th);
         uint32_t *tp = pairs_h.as<uint32_t>();
         size_t ntrip = 0;

@@ -108,7 +108,9 @@ struct JobDev {
     uint32_t *counts;        // k
     double *pass_sums;       // k x d
     uint64_t *pass_counts;   // k
+    uint32_t *part;          // centroid-range split: per (row, slice) results, batch slot x nslice
     uint32_t d, k, k_pad, d_pad, row_bytes, tb;
+    uint32_t nslice, tiles_per_slice;  // the search's centroid-range split (1: none)
     uint32_t row_base, cluster_base;  // offsets in the concatenated batch / cluster ranges
     // per step
     uint32_t m;              // rows this step (0: the job is done)
@@ -119,15 +121,78 @@ struct JobDev {
     uint32_t replace;        // alpha == 1
 };
 
-// prep: one wave per (job, centroid < k_pad) of cmap; operand, norms,
-// offsets with this step's bias, stats (zeroed before).
+// A pointer read from a device table addresses global memory, which the
+// compiler cannot see: its loads and stores would take the flat forms, whose
+// completion LDS waits also hold for. Passing it through an empty asm as an
+// address-space-1 pointer says so. global_ptr_s: the value is the same in
+// every lane of the wave (kept in scalar registers); global_ptr_v: any.
+template <class T> __device__ inline T *global_ptr_s(T *p) {
+    const uint64_t v = (uint64_t)p;
+    const uint64_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint64_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    __attribute__((address_space(1))) T *g = (__attribute__((address_space(1))) T *)((hi << 32) | lo);
+    asm("" : "+s"(g));
+    return (T *)g;
+}
+template <class T> __device__ inline T *global_ptr_v(T *p) {
+    __attribute__((address_space(1))) T *g = (__attribute__((address_space(1))) T *)p;
+    asm("" : "+v"(g));
+    return (T *)g;
+}
+
+// jobs[i] with every pointer marked global (UNIFORM: i is the same in every
+// lane of the wave).
+template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size_t i) {
+    JobDev j = jobs[i];
+#define EMH_G(f)                    \
+    if constexpr (UNIFORM)          \
+        j.f = global_ptr_s(j.f);    \
+    else                            \
+        j.f = global_ptr_v(j.f)
+    EMH_G(x);
+    EMH_G(ids);
+    EMH_G(cen);
+    EMH_G(bias);
+    EMH_G(usage);
+    EMH_G(cen_op);
+    EMH_G(norm32);
+    EMH_G(hn);
+    EMH_G(stats);
+    EMH_G(labels);
+    EMH_G(dist);
+    EMH_G(cand);
+    EMH_G(sums);
+    EMH_G(counts);
+    EMH_G(pass_sums);
+    EMH_G(pass_counts);
+    EMH_G(part);
+#undef EMH_G
+    return j;
+}
+
+// prep: every (job, centroid < k_pad) entry of cmap (cmap lists each job's
+// entries together); operand, norms, offsets with this step's bias, stats
+// (zeroed before).
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s);
 // search of every job in one launch per row-width class: wg holds
-// (job, first row) per workgroup; all jobs of the call share row_bytes.
+// (job, first row | slice) per workgroup (first rows are multiples of 256,
+// slices below 256); all jobs of the call share row_bytes.
 hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_args, const uint2 *wg, size_t nwg,
                               hipStream_t s);
-// rescoring of every job's candidates: rmap holds (job, row) per batch slot.
+// rescoring of every job's candidates (merging a split job's slice lists
+// first): rmap holds (job, row) per batch slot.
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
+// FP32 jobs with a split search: each slot's slice results merged into its
+// label and distance.
+hipError_t launch_jobs_merge_f32(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
+// Centroid-range split of one job's search: slices of consecutive tiles
+// sized so that every workgroup sweeps about the same operand volume.
+struct JobSlices {
+    uint32_t nslice, tiles_per_slice;
+};
+JobSlices job_slices(const SearchGeom &g);
+// Words of slice results per batch row of a job (M candidates, or one best).
+size_t job_part_words(int op, int M, uint32_t nslice);
 // Builds a SearchArgs table entry (opaque bytes) for one job's step.
 size_t search_args_bytes();
 void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
@@ -136,6 +201,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
 struct AccumScratch {
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
     uint32_t *offsets = nullptr;
+    uint32_t *big = nullptr;  // batched sums: [0] count, then (cluster, first position) pairs
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     size_t cap_rows = 0, cap_k = 0;
@@ -165,8 +231,8 @@ hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_
 // The EMA step of every job still running, one wave per global cluster.
 hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
 hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                                  const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
-                                  hipStream_t s);
+                                  const uint32_t *ids, const uint32_t *cjob, size_t nclusters, uint32_t *keys,
+                                  uint32_t *counts_all, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
 
 // Error reporting shared by the C ABI.

@@ -8,6 +8,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -209,10 +210,12 @@ struct JobRun {
     // Only for runs without the balance penalty, whose next batch does not
     // depend on the previous one's statistics.
     bool deferred = false;
-    // the next epoch's batch order, drawn ahead of time (prefetch_order) while
-    // the GPU works; next() takes it instead of drawing it then
+    // the next epoch's batch order, drawn on a background thread while the
+    // current epoch runs (started when it starts); next() takes it instead
+    // of drawing it then
     std::vector<uint32_t> order_next;
     uint32_t order_next_epoch = UINT32_MAX;
+    std::future<void> order_job;
     struct StepMeta {
         size_t nb;
         const uint32_t *ids;
@@ -275,11 +278,22 @@ struct JobRun {
     bool next() {
         if (done) return false;
         if (b0 == 0) {
+            if (order_job.valid()) order_job.get();  // this epoch's order, drawn in the background
             if (order_next_epoch == epoch) {
                 order.swap(order_next);
                 order_next_epoch = UINT32_MAX;
             } else {
                 Rng(cfg.seed * 1000003ull + (uint64_t)epoch + 1).permutation((uint32_t)n, order);
+            }
+            // the next epoch's order (the same permutation next() would draw
+            // then) while this epoch runs; order_next no longer holds ids a
+            // pending batch reads: only the last epoch's labels use them
+            if (epoch + 1 < R.n_epochs) {
+                const uint32_t e1 = epoch + 1;
+                order_job = std::async(std::launch::async, [this, e1] {
+                    Rng(cfg.seed * 1000003ull + (uint64_t)e1 + 1).permutation((uint32_t)n, order_next);
+                    order_next_epoch = e1;
+                });
             }
             if (!deferred) epoch_obj = 0.0;
         }
@@ -319,15 +333,6 @@ struct JobRun {
             for (size_t r = 0; r < nb; r++) labels_out[ids[r]] = lab[r];
         epoch_obj += batch_obj;
         mean_sqdist = batch_obj / (double)nb;
-    }
-
-    // When the staged batch closes its epoch and another follows, draws that
-    // epoch's order now (the same permutation next() would draw). Reads only
-    // the staged position, so it can run while the batch is on the GPU.
-    void prefetch_order() {
-        if (done || epoch + 1 >= R.n_epochs || b0 + nb < R.epoch_rows[epoch] || order_next_epoch == epoch + 1) return;
-        Rng(cfg.seed * 1000003ull + (uint64_t)(epoch + 1) + 1).permutation((uint32_t)n, order_next);
-        order_next_epoch = epoch + 1;
     }
 
     // The staged batch, for finish_stats_deferred.

@@ -1112,9 +1112,22 @@ __global__ void merge_slices_kernel(const uint32_t *part, size_t m, int nslice, 
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *jobs, const uint2 *wg) {
     const uint2 w = wg[blockIdx.x];
-    const SearchArgs a = jobs[w.x];
-    if ((size_t)w.y >= a.m) return;  // whole workgroup: before any barrier
-    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, (size_t)w.y, 0, (int)(a.k_pad / (32 * NB)), 0);
+    // one job per workgroup: its table pointers are uniform and global
+    SearchArgs a = jobs[w.x];
+    a.x = global_ptr_s(a.x);
+    a.ids = global_ptr_s(a.ids);
+    a.hn = global_ptr_s(a.hn);
+    a.stats = global_ptr_s(a.stats);
+    a.part = global_ptr_s(a.part);
+    a.cand = global_ptr_s(a.cand);
+    a.labels = global_ptr_s(a.labels);
+    a.dist = global_ptr_s(a.dist);
+    const size_t row0 = w.y & ~255u;
+    if (row0 >= a.m) return;  // whole workgroup: before any barrier
+    // slice s of the job's centroid-range split: tiles [s * per, + per)
+    const int s = (int)(w.y & 255u), ntiles = (int)(a.k_pad / (32 * NB));
+    const int t0 = s * a.tiles_per_slice, t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
+    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, row0, t0, t1, s);
 }
 
 // Copies the row-major operand table into the tile-major order of the
@@ -1143,15 +1156,40 @@ __global__ void tile_layout_kernel(const uint4 *src, uint4 *dst, size_t k_pad, i
 // strided elements in order without FMA, per candidate, and a fixed
 // butterfly adds the 16 partial sums, so every lane of the group holds
 // identical bits.
+// Candidate t of a row whose search ran as nslice centroid ranges: entry t of
+// the lexicographic (key, index) top-m of the slices' lists, which is the
+// unsplit sweep's list (merge_slices_kernel builds the same).
+template <int M>
+__device__ inline uint32_t merged_candidate(const uint32_t *part, int nslice, size_t row, int t) {
+    uint32_t bs[M], bi[M];
+#pragma unroll
+    for (int u = 0; u < M; u++) {
+        bs[u] = 0xFFFFFFFFu;
+        bi[u] = NO_LABEL;
+    }
+    for (int s = 0; s < nslice; s++) {
+        const uint32_t *pp = part + (row * nslice + s) * (2 * M);
+#pragma unroll
+        for (int u = 0; u < M; u++) topm_insert_lex<M, uint32_t>(bs, bi, pp[u], pp[M + u]);
+    }
+    uint32_t r = bi[0];
+#pragma unroll
+    for (int u = 1; u < M; u++) r = u == t ? bi[u] : r;
+    return r;
+}
+
+// part / nslice: the row's candidates are the merge of nslice slice lists
+// (nslice > 1) instead of cand's row.
 template <int ROWS, int M>
 __device__ __forceinline__ void rescore_row(const void *x, size_t d, size_t k, const uint32_t *ids, const float *c,
                                             const float *bias, const uint32_t *cand, uint32_t *labels, float *dist,
-                                            size_t row, bool live, int gl) {
+                                            size_t row, bool live, int gl, const uint32_t *part = nullptr,
+                                            int nslice = 1) {
     constexpr int G = 16;
     const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0;
     uint32_t mine = NO_LABEL;
     if (live && gl < M) {
-        mine = cand[row * M + gl];
+        mine = nslice > 1 ? merged_candidate<M>(part, nslice, row, gl) : cand[row * M + gl];
         if (mine >= k) mine = NO_LABEL;  // pad centroids can fill a list when k is small
     }
     uint32_t ci[M];
@@ -1236,13 +1274,34 @@ __global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, c
         uint2 r = make_uint2(0u, 0u);
         bool live = i < slots;
         if (live) r = rmap[i];
-        const JobDev *j = live ? &jobs[r.x] : &jobs[0];
-        live = live && r.y < j->m;
+        // each group of 16 lanes has its own job
+        const JobDev j = job_at<false>(jobs, live ? r.x : 0u);
+        live = live && r.y < j.m;
         // a group's d (and so its trip counts) comes from its own job: the
         // shuffles are within the 16 lanes of the group only
-        rescore_row<EMH_ROWS_F32, M>(j->x, j->d, j->k, j->ids, j->cen, j->penalized ? j->bias : nullptr, j->cand,
-                                     j->labels, j->dist, r.y, live, gl);
+        rescore_row<EMH_ROWS_F32, M>(j.x, j.d, j.k, j.ids, j.cen, j.penalized ? j.bias : nullptr, j.cand, j.labels,
+                                     j.dist, r.y, live, gl, j.part, (int)j.nslice);
     }
+}
+
+// FP32 jobs whose search ran as several centroid ranges: the lowest
+// (score, index) of the slices' results, as merge_slices_kernel does, one
+// batch slot per thread.
+__global__ void jobs_merge_f32_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= slots) return;
+    const uint2 r = rmap[i];
+    const JobDev j = job_at<false>(jobs, r.x);
+    if (j.nslice < 2 || r.y >= j.m) return;
+    const uint32_t *pp = j.part + (size_t)r.y * j.nslice * 4;
+    uint4 best = *(const uint4 *)pp;
+    for (uint32_t s = 1; s < j.nslice; s++) {
+        const uint4 p = *(const uint4 *)(pp + 4 * s);
+        const float ps = __uint_as_float(p.x), bs = __uint_as_float(best.x);
+        if (ps < bs || (ps == bs && p.y < best.y)) best = p;
+    }
+    j.labels[r.y] = best.y == NO_LABEL ? 0u : best.y;
+    j.dist[r.y] = __uint_as_float(best.z);
 }
 
 // ---------------------------------------------------------- centroid prep --
@@ -1333,50 +1392,87 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     }
 }
 
-// Batched prep: one wave per (job, centroid < k_pad) of cmap. Writes the
-// negated operand straight into the tile-major table, |c|^2, the score
-// offset (|c|^2 + bias) / 2 with this step's bias, and the job's stats, as
-// prep_centroids_kernel + set_bias_kernel do for one job.
+// Batched prep: each wave takes PREP_PER_WAVE consecutive (job, centroid <
+// k_pad) entries of cmap and writes the negated operand straight into the
+// tile-major table, |c|^2, the score offset (|c|^2 + bias) / 2 with this
+// step's bias, as prep_centroids_kernel + set_bias_kernel do for one job.
+// The jobs' stats (max |c|^2, max -bias) take one atomic per job per
+// workgroup: cmap lists each job's entries together.
+constexpr int PREP_PER_WAVE = 4, PREP_ENTRIES = 4 * PREP_PER_WAVE;
+
 template <int OP>
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
-    const int lane = threadIdx.x & 63;
-    const size_t e = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (e >= entries) return;
-    const uint2 cm = cmap[e];
-    const JobDev &jb = jobs[cm.x];
-    if (jb.m == 0) return;  // job done: nothing reads its tables this step
-    const size_t jc = cm.y, d = jb.d;
-    const bool live = jc < jb.k;
-    const float *cp = jb.cen + jc * d;
-    // tile-major: byte b of centroid jc lives in tile jc / tb at quad b / 16
-    const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
-    uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
-    if constexpr (OP == OP_F32) {
-        for (size_t el = lane; el < jb.d_pad; el += 64) {
-            const size_t b = el * 4;
-            *(float *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? -cp[el] : 0.0f;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ uint32_t s_job[PREP_ENTRIES], s_norm[PREP_ENTRIES], s_bias[PREP_ENTRIES];
+    for (int i = 0; i < PREP_PER_WAVE; i++) {
+        const int slot = w * PREP_PER_WAVE + i;
+        const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
+        uint32_t job = NO_LABEL, nbits = 0u, bbits = 0u;
+        if (e < entries) {
+            const uint2 cm = cmap[e];
+            const JobDev jb = job_at<true>(jobs, cm.x);  // one entry per wave
+            if (jb.m != 0) {  // a finished job: nothing reads its tables this step
+                const size_t jc = cm.y, d = jb.d;
+                const bool live = jc < jb.k;
+                const float *cp = jb.cen + jc * d;
+                // tile-major: byte b of centroid jc lives in tile jc / tb at quad b / 16
+                const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
+                uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
+                if constexpr (OP == OP_F32) {
+                    for (size_t el = lane; el < jb.d_pad; el += 64) {
+                        const size_t b = el * 4;
+                        *(float *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? -cp[el] : 0.0f;
+                    }
+                } else {
+                    for (size_t el = lane; el < jb.d_pad; el += 64) {
+                        const size_t b = el * 2;
+                        *(uint16_t *)(tbase + (b >> 4) * tb * 16 + (b & 15)) =
+                            (live && el < d) ? bf16_rne(-cp[el]) : (uint16_t)0;
+                    }
+                }
+                if (!live) {
+                    if (lane == 0) jb.hn[jc] = INFINITY;
+                } else {
+                    const float nv = centroid_norm32(cp, d, lane);
+                    job = cm.x;
+                    nbits = nv >= 0.0f ? __float_as_uint(nv) : 0u;  // NaN norms are skipped
+                    float v = nv;
+                    if (jb.penalized) {
+                        const float bv = jb.bias[jc];
+                        if (bv < 0.0f) bbits = __float_as_uint(-bv);
+                        v = __fadd_rn(nv, bv);
+                    }
+                    if (lane == 0) {
+                        jb.norm32[jc] = nv;
+                        jb.hn[jc] = __fmul_rn(0.5f, v);
+                    }
+                }
+            }
         }
-    } else {
-        for (size_t el = lane; el < jb.d_pad; el += 64) {
-            const size_t b = el * 2;
-            *(uint16_t *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? bf16_rne(-cp[el]) : (uint16_t)0;
+        if (lane == 0) {
+            s_job[slot] = job;
+            s_norm[slot] = nbits;
+            s_bias[slot] = bbits;
         }
     }
-    if (!live) {
-        if (lane == 0) jb.hn[jc] = INFINITY;
-        return;
-    }
-    const float nv = centroid_norm32(cp, d, lane);
-    if (lane == 0) {
-        jb.norm32[jc] = nv;
-        if (nv >= 0.0f) atomicMax(&jb.stats[0], __float_as_uint(nv));
-        float v = nv;
-        if (jb.penalized) {
-            const float b = jb.bias[jc];
-            if (b < 0.0f) atomicMax(&jb.stats[1], __float_as_uint(-b));
-            v = __fadd_rn(nv, b);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t cur = NO_LABEL, mn = 0u, mb = 0u;
+        for (int t = 0; t <= PREP_ENTRIES; t++) {
+            const uint32_t jt = t < PREP_ENTRIES ? s_job[t] : NO_LABEL;
+            if (jt != cur) {
+                if (cur != NO_LABEL) {
+                    if (mn) atomicMax(&jobs[cur].stats[0], mn);
+                    if (mb) atomicMax(&jobs[cur].stats[1], mb);
+                }
+                cur = jt;
+                mn = mb = 0u;
+            }
+            if (t < PREP_ENTRIES) {
+                mn = max(mn, s_norm[t]);
+                mb = max(mb, s_bias[t]);
+            }
         }
-        jb.hn[jc] = __fmul_rn(0.5f, v);
     }
 }
 
@@ -1793,14 +1889,41 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     a.labels = j.labels;
     a.dist = j.dist;
     a.cand = j.cand;
-    a.nslice = 1;
-    a.tiles_per_slice = (int)(g.k_pad / reg_tile(g));
+    a.nslice = (int)j.nslice;
+    a.tiles_per_slice = (int)j.tiles_per_slice;
+    a.part = j.part;
     memcpy(dst, &a, sizeof a);
+}
+
+// A workgroup's sweep costs about (tiles x tile centroids x d_pad) operand
+// elements against its 256 rows, plus loading those rows once. Slices of
+// about 2^16 centroid-dimensions (and at least 256 centroids, so that the row
+// load stays a small share) keep the longest workgroup of a batch short while
+// the merge reads a few words per slice.
+JobSlices job_slices(const SearchGeom &g) {
+    const size_t tc = (size_t)reg_tile(g), ntiles = g.k_pad / tc;
+    size_t per = std::max<size_t>((size_t)1 << 16, 256 * g.d_pad) / (tc * g.d_pad);
+    per = std::max<size_t>(per, 1);
+    size_t ns = std::min<size_t>((ntiles + per - 1) / per, 255);
+    per = (ntiles + ns - 1) / ns;  // balanced, none empty
+    ns = (ntiles + per - 1) / per;
+    return JobSlices{(uint32_t)ns, (uint32_t)per};
+}
+
+size_t job_part_words(int op, int M, uint32_t nslice) {
+    if (nslice < 2) return 0;
+    return (size_t)nslice * (op == OP_BF16 ? 2 * (size_t)M : 4);
+}
+
+hipError_t launch_jobs_merge_f32(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
+    if (!slots) return hipSuccess;
+    hipLaunchKernelGGL(jobs_merge_f32_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, jobs, rmap, slots);
+    return hipGetLastError();
 }
 
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s) {
     if (!entries) return hipSuccess;
-    dim3 grid((unsigned)((entries + 3) / 4));
+    dim3 grid((unsigned)((entries + PREP_ENTRIES - 1) / PREP_ENTRIES));
     if (op == OP_F32) hipLaunchKernelGGL(prep_jobs_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_BF16) hipLaunchKernelGGL(prep_jobs_kernel<OP_BF16>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else return hipErrorInvalidValue;

@@ -156,8 +156,6 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                                          counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream),
                         "EMA step")))
             return rc;
-        // the next epoch's batch order, drawn while this batch runs
-        J.prefetch_order();
         const bool revive = J.revive_due();
         const bool need_usage = revive || cfg.balance > 0.0f;
         if (async) {
