@@ -96,6 +96,22 @@ class HostPool {
     std::atomic<bool> stop_{false};
 };
 
+// A second stream and the events that fork and join it inside a capture.
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t e[4] = {};
+    hipError_t init() {
+        hipError_t r = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+        for (int i = 0; i < 4 && r == hipSuccess; i++) r = hipEventCreateWithFlags(&e[i], hipEventDisableTiming);
+        return r;
+    }
+    ~SideStream() {
+        for (hipEvent_t v : e)
+            if (v) (void)hipEventDestroy(v);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
 struct GraphGuard {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -334,25 +350,58 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     // Capture the step.
     const JobDev *jobs_dev = (const JobDev *)(in_dp + in_job);
     const void *sa_dev = in_dp + in_sa;
+    // The labels' and distances' download is not in the graph: the graph
+    // records an external event once they are final (after the rescore) and
+    // a copy stream downloads them while the per-cluster sums and the EMA
+    // step (which read the labels and write neither) run.
     GraphGuard gg;
+    SideStream side;
+    if ((rc = check(side.init(), "copy stream"))) return rc;
     if ((rc = check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture"))) return rc;
-    hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in.bytes, hipMemcpyHostToDevice, h->stream);
+    // capture info: the nodes the next captured operation would follow
+    auto last_nodes = [&](std::vector<hipGraphNode_t> &out) -> hipError_t {
+        hipStreamCaptureStatus cs;
+        const hipGraphNode_t *deps = nullptr;
+        size_t ndeps = 0;
+        const hipError_t r = hipStreamGetCaptureInfo_v2(h->stream, &cs, nullptr, nullptr, &deps, &ndeps);
+        if (r == hipSuccess) out.assign(deps, deps + ndeps);
+        return r;
+    };
+    // the batch ids are uploaded on the copy stream too, beside the job
+    // tables' upload and the prep kernel; the first search waits for them
+    std::vector<hipGraphNode_t> search_first;
+    hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in_ids, hipMemcpyHostToDevice, h->stream);
     if (ce == hipSuccess) ce = launch_fill_u32(stats_d.as<uint32_t>(), 4 * njobs, 0u, h->stream);
     if (ce == hipSuccess) ce = launch_jobs_prep(op, jobs_dev, (const uint2 *)(mp + m_c), cpad, h->stream);
-    for (size_t c = 0; c < classes.size() && ce == hipSuccess; c++)
+    for (size_t c = 0; c < classes.size() && ce == hipSuccess; c++) {
         ce = launch_jobs_search(op, M, classes[c], sa_dev, (const uint2 *)(mp + m_wg) + wg_off[c], wg_cnt[c], h->stream);
+        if (ce == hipSuccess && search_first.empty()) ce = last_nodes(search_first);
+    }
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess && op == OP_F32 && part_words)
         ce = launch_jobs_merge_f32(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    // the node(s) the labels are final after: an event record node follows
+    // them once the capture ends
+    std::vector<hipGraphNode_t> label_deps;
+    if (ce == hipSuccess) ce = last_nodes(label_deps);
     if (ce == hipSuccess)
         ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots,
                                     (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j), nclusters,
                                     keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
     if (ce == hipSuccess) ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, h->stream);
-    if (ce == hipSuccess) ce = hipMemcpyAsync(out_hp, out_dp, outl.bytes, hipMemcpyDeviceToHost, h->stream);
+    if (ce == hipSuccess)
+        ce = hipMemcpyAsync(out_hp + out_use, out_dp + out_use, outl.bytes - out_use, hipMemcpyDeviceToHost, h->stream);
     hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);
     if ((rc = check(ce, "capture step")) || (rc = check(ee, "end capture"))) return rc;
-    if ((rc = check(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0), "instantiate step"))) return rc;
+    hipGraphNode_t label_event, ids_wait;
+    if (search_first.size() != 1) return fail(EMH_E_DEVICE, "train_batch: unexpected step graph");
+    if ((rc = check(hipGraphAddEventRecordNode(&label_event, gg.graph, label_deps.data(), label_deps.size(),
+                                               side.e[0]),
+                    "labels event")) ||
+        (rc = check(hipGraphAddEventWaitNode(&ids_wait, gg.graph, nullptr, 0, side.e[1]), "ids event")) ||
+        (rc = check(hipGraphAddDependencies(gg.graph, &ids_wait, search_first.data(), 1), "ids event")) ||
+        (rc = check(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0), "instantiate step")))
+        return rc;
     size_t nodes = 0;
     (void)hipGraphGetNodes(gg.graph, nullptr, &nodes);
 
@@ -408,10 +457,19 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (;;) {
         if (std::find(live.begin(), live.end(), (char)1) == live.end()) break;
         th = clk::now();
-        if ((rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch"))) return rc;
+        if ((rc = check(hipMemcpyAsync(in_dp + in_ids, in_hp + in_ids, in.bytes - in_ids, hipMemcpyHostToDevice,
+                                       side.s),
+                        "batch ids")) ||
+            (rc = check(hipEventRecord(side.e[1], side.s), "batch ids")) ||
+            (rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch")) ||
+            (rc = check(hipStreamWaitEvent(side.s, side.e[0], 0), "labels ready")) ||
+            (rc = check(hipMemcpyAsync(out_hp, out_dp, out_use, hipMemcpyDeviceToHost, side.s), "labels")))
+            return rc;
         launch_s += since(th);
         th = clk::now();
-        if ((rc = check(hipStreamSynchronize(h->stream), "step"))) return rc;
+        if ((rc = check(hipStreamSynchronize(h->stream), "step")) ||
+            (rc = check(hipStreamSynchronize(side.s), "labels")))
+            return rc;
         wait_s += since(th);
         steps++;
         // host half per job, then its next batch; revival planned on the
