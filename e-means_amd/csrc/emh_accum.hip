@@ -364,7 +364,13 @@ __global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t s
             if (l < j.k) {
                 key = j.cluster_base + l;
                 atomicAdd(&counts_all[key], 1u);
+                if (j.penalized) {
+                    // the true squared distance: the score less the bias
+                    const float v = __fsub_rn(j.dist[r.y], j.bias[l]);
+                    j.dist[r.y] = v > 0.0f ? v : 0.0f;
+                }
             }
+            if (j.last_epoch) j.labels_all[j.ids[r.y]] = l;
         }
         keys[i] = key;
     }
@@ -547,9 +553,9 @@ __global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, c
     }
 }
 
-hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                                  const uint32_t *ids, const uint32_t *cjob, size_t nclusters, uint32_t *keys,
-                                  uint32_t *counts_all, hipStream_t s) {
+hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
+                            const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                            hipStream_t s) {
     hipError_t e;
     if ((e = accum_reserve(a, slots > 0 ? slots : 1, nclusters)) != hipSuccess) return e;
     dim3 cgrid((unsigned)((nclusters + 3) / 4));
@@ -559,7 +565,14 @@ hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uin
     unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
     hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters, keys,
                        counts_all, a.big);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    return hipGetLastError();
+}
+
+hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
+                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
+                            const uint32_t *counts_all, hipStream_t s) {
+    if (slots == 0) return hipSuccess;
+    hipError_t e;
     // slot i's job-relative row is ids[i] (the jobs' id ranges are laid out
     // in slot order)
     const uint32_t *sorted_keys, *sorted_src;
@@ -568,10 +581,10 @@ hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uin
         return e;
     const size_t waves = (slots + SUM_SPAN - 1) / SUM_SPAN;
     hipLaunchKernelGGL(jobs_stream_sum_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, jobs, cjob,
-                       (uint32_t)nclusters, sorted_keys, sorted_src, (const uint32_t *)counts_all, slots, a.big);
+                       (uint32_t)nclusters, sorted_keys, sorted_src, counts_all, slots, a.big);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src,
-                       (const uint32_t *)counts_all, (const uint32_t *)a.big);
+    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src, counts_all,
+                       (const uint32_t *)a.big);
     return hipGetLastError();
 }
 

@@ -159,9 +159,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         if (init_rc[j] != EMH_OK) return fail(init_rc[j], init_err[j]);
     std::vector<SearchGeom> geo(njobs);
     std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs),
-        part_base(njobs);
+        part_base(njobs), nrow_base(njobs);
     std::vector<JobSlices> slices(njobs);
-    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0;
+    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0, rows_total = 0;
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         geo[j] = make_geom(op, jb.k, jb.d);
@@ -170,6 +170,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
                                                " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256)");
         row_base[j] = slots;
         slots += run[j].R.batch_size;
+        nrow_base[j] = rows_total;
+        rows_total += jb.n;
         cl_base[j] = nclusters;
         nclusters += jb.k;
         cen_base[j] = cen_floats;
@@ -208,9 +210,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     const size_t in_job = in.add(njobs * sizeof(JobDev)), in_sa = in.add(njobs * sa_bytes),
                  in_bias = in.add(nclusters * 4), in_ids = in.add(slots * 4);
     Layout outl;
-    const size_t out_lab = outl.add(slots * 4), out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
+    const size_t out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
     DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts,
-        maps, pairs_d, part_d;
+        maps, pairs_d, part_d, lab_d, labels_all;
     HostBuf in_h, out_h, pairs_h;
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
@@ -224,7 +226,10 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
         (rc = check(keys.ensure(slots * 4), "keys")) || (rc = check(pass_sums.ensure(cen_floats * 8), "pass sums")) ||
         (rc = check(pass_counts.ensure(nclusters * 8), "pass counts")) ||
-        (part_words && (rc = check(part_d.ensure(part_words * 4), "slice results"))))
+        (part_words && (rc = check(part_d.ensure(part_words * 4), "slice results"))) ||
+        (rc = check(lab_d.ensure(slots * 4), "labels")) ||
+        (rc = check(labels_all.ensure(rows_total * 4), "last-pass labels")) ||
+        (rc = check(launch_fill_u32(labels_all.as<uint32_t>(), rows_total, 0xFFFFFFFFu, h->stream), "labels")))
         return rc;
     memset(in_h.p, 0, in.bytes);
     uint8_t *in_dp = (uint8_t *)in_d.p, *out_dp = (uint8_t *)out_d.p;
@@ -326,7 +331,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.norm32 = norm32.as<float>() + cpad_base[j];
         t.hn = hn.as<float>() + cpad_base[j];
         t.stats = stats_d.as<unsigned>() + 4 * j;
-        t.labels = (uint32_t *)(out_dp + out_lab) + row_base[j];
+        t.labels = lab_d.as<uint32_t>() + row_base[j];
+        t.labels_all = labels_all.as<uint32_t>() + nrow_base[j];
         t.dist = (float *)(out_dp + out_dist) + row_base[j];
         t.cand = cand.as<uint32_t>() + row_base[j] * M;
         t.sums = sums_d.as<float>() + cen_base[j];
@@ -380,24 +386,26 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess && op == OP_F32 && part_words)
         ce = launch_jobs_merge_f32(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
-    // the node(s) the labels are final after: an event record node follows
-    // them once the capture ends
-    std::vector<hipGraphNode_t> label_deps;
-    if (ce == hipSuccess) ce = last_nodes(label_deps);
     if (ce == hipSuccess)
-        ce = launch_jobs_accumulate(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots,
-                                    (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j), nclusters,
-                                    keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+        ce = launch_jobs_keys(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
+                              nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+    // the distances are final here (true distances): an event record node
+    // follows these nodes once the capture ends
+    std::vector<hipGraphNode_t> dist_deps;
+    if (ce == hipSuccess) ce = last_nodes(dist_deps);
+    if (ce == hipSuccess)
+        ce = launch_jobs_sums(h->acc, jobs_dev, slots, (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j),
+                              nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
     if (ce == hipSuccess) ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, h->stream);
     if (ce == hipSuccess)
         ce = hipMemcpyAsync(out_hp + out_use, out_dp + out_use, outl.bytes - out_use, hipMemcpyDeviceToHost, h->stream);
     hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);
     if ((rc = check(ce, "capture step")) || (rc = check(ee, "end capture"))) return rc;
-    hipGraphNode_t label_event, ids_wait;
+    hipGraphNode_t dist_event, ids_wait;
     if (search_first.size() != 1) return fail(EMH_E_DEVICE, "train_batch: unexpected step graph");
-    if ((rc = check(hipGraphAddEventRecordNode(&label_event, gg.graph, label_deps.data(), label_deps.size(),
+    if ((rc = check(hipGraphAddEventRecordNode(&dist_event, gg.graph, dist_deps.data(), dist_deps.size(),
                                                side.e[0]),
-                    "labels event")) ||
+                    "distances event")) ||
         (rc = check(hipGraphAddEventWaitNode(&ids_wait, gg.graph, nullptr, 0, side.e[1]), "ids event")) ||
         (rc = check(hipGraphAddDependencies(gg.graph, &ids_wait, search_first.data(), 1), "ids event")) ||
         (rc = check(hipGraphInstantiate(&gg.exec, gg.graph, nullptr, nullptr, 0), "instantiate step")))
@@ -435,7 +443,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         if (!live[j]) return;
         JobRun &J = run[j];
         memcpy(J.st.usage.data(), out_hp + out_use + cl_base[j] * 4, J.k * 4);
-        J.finish_stats((const uint32_t *)(out_hp + out_lab) + row_base[j], (float *)(out_hp + out_dist) + row_base[j]);
+        J.finish_stats_device((const float *)(out_hp + out_dist) + row_base[j]);
         if (J.revive_due()) {
             J.st.revive_plan(J.R.unused_threshold, J.split_rng, jpairs[j]);
             if (!jpairs[j].empty()) memcpy(out_hp + out_use + cl_base[j] * 4, J.st.usage.data(), J.k * 4);
@@ -463,12 +471,12 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
             (rc = check(hipEventRecord(side.e[1], side.s), "batch ids")) ||
             (rc = check(hipGraphLaunch(gg.exec, h->stream), "step launch")) ||
             (rc = check(hipStreamWaitEvent(side.s, side.e[0], 0), "labels ready")) ||
-            (rc = check(hipMemcpyAsync(out_hp, out_dp, out_use, hipMemcpyDeviceToHost, side.s), "labels")))
+            (rc = check(hipMemcpyAsync(out_hp, out_dp, out_use, hipMemcpyDeviceToHost, side.s), "distances")))
             return rc;
         launch_s += since(th);
         th = clk::now();
         if ((rc = check(hipStreamSynchronize(h->stream), "step")) ||
-            (rc = check(hipStreamSynchronize(side.s), "labels")))
+            (rc = check(hipStreamSynchronize(side.s), "distances")))
             return rc;
         wait_s += since(th);
         steps++;
@@ -506,7 +514,10 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
                         "closing mean")) ||
             (rc = check(hipMemcpyAsync(jobs[j].centroids_out, cen_dev + cen_base[j], k * d * 4, hipMemcpyDeviceToHost,
                                        h->stream),
-                        "copy centroids")))
+                        "copy centroids")) ||
+            (rc = check(hipMemcpyAsync(jobs[j].labels_out, labels_all.as<uint32_t>() + nrow_base[j], jobs[j].n * 4,
+                                       hipMemcpyDeviceToHost, h->stream),
+                        "copy labels")))
             return rc;
     }
     if ((rc = check(hipStreamSynchronize(h->stream), "closing mean"))) return rc;

@@ -102,7 +102,8 @@ struct JobDev {
     float *norm32, *hn;      // k_pad each
     unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias (f32 bits)
     uint32_t *labels;        // batch slot
-    float *dist;             // batch slot
+    float *dist;             // batch slot (true distances once the keys kernel has run)
+    uint32_t *labels_all;    // n: the last pass's label of every row (device-resident)
     uint32_t *cand;          // batch slot x M
     float *sums;             // k x d
     uint32_t *counts;        // k
@@ -160,6 +161,7 @@ template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size
     EMH_G(stats);
     EMH_G(labels);
     EMH_G(dist);
+    EMH_G(labels_all);
     EMH_G(cand);
     EMH_G(sums);
     EMH_G(counts);
@@ -223,16 +225,23 @@ hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs,
 // out (m x d f32) = rows ids of x (f32 / i8 / u8), widened
 hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t *ids, size_t m, float *out,
                               hipStream_t s);
-// Batched: keys = cluster_base + label for live slots (nclusters for spare
-// slots), counts per global cluster, stable sort, then per-job f32 sums and
-// FP64 pass totals. cjob maps a global cluster to its job.
+// cjob maps a global cluster to its job.
 // Revival row splits of many jobs: triples (job, donor, unused).
 hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_t ntriples, hipStream_t s);
 // The EMA step of every job still running, one wave per global cluster.
 hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
-hipError_t launch_jobs_accumulate(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                                  const uint32_t *ids, const uint32_t *cjob, size_t nclusters, uint32_t *keys,
-                                  uint32_t *counts_all, hipStream_t s);
+// Batched, first half: the last epoch's first batch zeroes its FP64 totals;
+// keys = cluster_base + label for live slots (nclusters for spare slots),
+// counts per global cluster; penalized jobs' distances lose their bias (the
+// true squared distances, src/lib.rs:425-430); last-epoch labels are stored
+// at their rows in labels_all.
+hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
+                            const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                            hipStream_t s);
+// Second half: one stable sort, then per-job f32 sums and FP64 pass totals.
+hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
+                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
+                            const uint32_t *counts_all, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
 
 // Error reporting shared by the C ABI.

@@ -335,6 +335,20 @@ struct JobRun {
         mean_sqdist = batch_obj / (double)nb;
     }
 
+    // finish_stats for a batch whose true distances (bias taken off) the
+    // device has made and whose last-pass labels it keeps.
+    void finish_stats_device(const float *dist) {
+        double batch_obj = 0.0;
+        for (size_t p0 = 0; p0 < nb; p0 += chunk) {
+            const size_t m = std::min(chunk, nb - p0);
+            double cs = 0.0;
+            for (size_t r = p0; r < p0 + m; r++) cs += (double)dist[r];
+            batch_obj += cs;
+        }
+        epoch_obj += batch_obj;
+        mean_sqdist = batch_obj / (double)nb;
+    }
+
     // The staged batch, for finish_stats_deferred.
     StepMeta meta() const { return StepMeta{nb, ids, is_last, b0 + nb >= R.epoch_rows[epoch]}; }
 
