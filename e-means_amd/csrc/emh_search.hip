@@ -1392,68 +1392,79 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     }
 }
 
-// Batched prep: each wave takes PREP_PER_WAVE consecutive (job, centroid <
-// k_pad) entries of cmap and writes the negated operand straight into the
-// tile-major table, |c|^2, the score offset (|c|^2 + bias) / 2 with this
-// step's bias, as prep_centroids_kernel + set_bias_kernel do for one job.
-// The jobs' stats (max |c|^2, max -bias) take one atomic per job per
-// workgroup: cmap lists each job's entries together.
-constexpr int PREP_PER_WAVE = 4, PREP_ENTRIES = 4 * PREP_PER_WAVE;
+// Batched prep: a group of 16 lanes per (job, centroid < k_pad) entry of
+// cmap, 16 entries per workgroup. The group writes the negated operand
+// straight into the tile-major table a 16-byte quad per lane, folds |c|^2 in
+// the order of centroid_norm32 (lane l: elements l, l + 16, ...; then the 16
+// partial sums in order; then the tail), and sets the score offset
+// (|c|^2 + bias) / 2 with this step's bias, as prep_centroids_kernel +
+// set_bias_kernel do for one job. The jobs' stats (max |c|^2, max -bias)
+// take one atomic per job per workgroup: cmap lists each job's entries
+// together.
+constexpr int PREP_ENTRIES = 16;
 
 template <int OP>
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int EB = OP == OP_F32 ? 4 : 2, EPQ = 16 / EB;  // element bytes, elements per quad
+    const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
     __shared__ uint32_t s_job[PREP_ENTRIES], s_norm[PREP_ENTRIES], s_bias[PREP_ENTRIES];
-    for (int i = 0; i < PREP_PER_WAVE; i++) {
-        const int slot = w * PREP_PER_WAVE + i;
-        const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
-        uint32_t job = NO_LABEL, nbits = 0u, bbits = 0u;
-        if (e < entries) {
-            const uint2 cm = cmap[e];
-            const JobDev jb = job_at<true>(jobs, cm.x);  // one entry per wave
-            if (jb.m != 0) {  // a finished job: nothing reads its tables this step
-                const size_t jc = cm.y, d = jb.d;
-                const bool live = jc < jb.k;
-                const float *cp = jb.cen + jc * d;
-                // tile-major: byte b of centroid jc lives in tile jc / tb at quad b / 16
-                const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
-                uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
-                if constexpr (OP == OP_F32) {
-                    for (size_t el = lane; el < jb.d_pad; el += 64) {
-                        const size_t b = el * 4;
-                        *(float *)(tbase + (b >> 4) * tb * 16 + (b & 15)) = (live && el < d) ? -cp[el] : 0.0f;
-                    }
-                } else {
-                    for (size_t el = lane; el < jb.d_pad; el += 64) {
-                        const size_t b = el * 2;
-                        *(uint16_t *)(tbase + (b >> 4) * tb * 16 + (b & 15)) =
-                            (live && el < d) ? bf16_rne(-cp[el]) : (uint16_t)0;
-                    }
+    const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
+    uint32_t job = NO_LABEL, nbits = 0u, bbits = 0u;
+    if (e < entries) {
+        const uint2 cm = cmap[e];
+        const JobDev jb = job_at<false>(jobs, cm.x);  // groups of a wave may belong to different jobs
+        if (jb.m != 0) {  // a finished job: nothing reads its tables this step
+            const size_t jc = cm.y, d = jb.d;
+            const bool live = jc < jb.k;
+            const float *cp = jb.cen + jc * d;
+            // tile-major: quad q of centroid jc lives in tile jc / tb at quad row q
+            const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
+            uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
+            const size_t nq = jb.row_bytes / 16;
+            for (size_t q = gl; q < nq; q += 16) {
+                uint32_t w[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) w[i] = 0u;
+#pragma unroll
+                for (int i = 0; i < EPQ; i++) {
+                    const size_t el = q * EPQ + i;
+                    const float v = (live && el < d) ? -cp[el] : 0.0f;
+                    if constexpr (OP == OP_F32) w[i] = __float_as_uint(v);
+                    else w[i / 2] |= (uint32_t)bf16_rne(v) << (16 * (i & 1));
                 }
-                if (!live) {
-                    if (lane == 0) jb.hn[jc] = INFINITY;
-                } else {
-                    const float nv = centroid_norm32(cp, d, lane);
-                    job = cm.x;
-                    nbits = nv >= 0.0f ? __float_as_uint(nv) : 0u;  // NaN norms are skipped
-                    float v = nv;
-                    if (jb.penalized) {
-                        const float bv = jb.bias[jc];
-                        if (bv < 0.0f) bbits = __float_as_uint(-bv);
-                        v = __fadd_rn(nv, bv);
-                    }
-                    if (lane == 0) {
-                        jb.norm32[jc] = nv;
-                        jb.hn[jc] = __fmul_rn(0.5f, v);
-                    }
+                *(uint4 *)(tbase + q * tb * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+            }
+            if (!live) {
+                if (gl == 0) jb.hn[jc] = INFINITY;
+            } else {
+                // |c|^2 in f32 as src/cpu.rs:170-180 folds it
+                const size_t body = d & ~(size_t)15;
+                float part = 0.0f;
+                for (size_t i = gl; i < body; i += 16) part = __fadd_rn(part, __fmul_rn(cp[i], cp[i]));
+                float nv = 0.0f;
+                for (int t = 0; t < 16; t++) nv = __fadd_rn(nv, __shfl(part, t, 16));
+                float tail = 0.0f;
+                for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
+                nv = __fadd_rn(nv, tail);
+                job = cm.x;
+                nbits = nv >= 0.0f ? __float_as_uint(nv) : 0u;  // NaN norms are skipped
+                float v = nv;
+                if (jb.penalized) {
+                    const float bv = jb.bias[jc];
+                    if (bv < 0.0f) bbits = __float_as_uint(-bv);
+                    v = __fadd_rn(nv, bv);
+                }
+                if (gl == 0) {
+                    jb.norm32[jc] = nv;
+                    jb.hn[jc] = __fmul_rn(0.5f, v);
                 }
             }
         }
-        if (lane == 0) {
-            s_job[slot] = job;
-            s_norm[slot] = nbits;
-            s_bias[slot] = bbits;
-        }
+    }
+    if (gl == 0) {
+        s_job[slot] = job;
+        s_norm[slot] = nbits;
+        s_bias[slot] = bbits;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
