@@ -255,7 +255,7 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         if ((e = hipMalloc(&a.vals_alt, m * 4)) != hipSuccess) return e;
         hipFree(a.big);
         a.big = nullptr;
-        if ((e = hipMalloc(&a.big, (2 + 2 * (m / SUM_BIG + 1)) * 4)) != hipSuccess) return e;
+        if ((e = hipMalloc(&a.big, (2 + 3 * (m / SUM_BIG + 1)) * 4)) != hipSuccess) return e;
         const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
         a.sort_tmp_bytes = 256 * nblocks * 4 + 256 * 4;  // histogram + digit totals
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
@@ -351,9 +351,11 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
 // ---- many jobs per launch ----
 
 // key of batch slot i: its job's first global cluster + label, or nclusters
-// for a slot past the job's rows this step (sorts last, never folded)
+// for a slot past the job's rows this step (sorts last, never folded); the
+// slot's true distance (penalized jobs) and, in the last epoch, its row's
+// label. The first thread clears the big-cluster count for the sums.
 __global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots, uint32_t nclusters,
-                                 uint32_t *keys, uint32_t *counts_all, uint32_t *big_count) {
+                                 uint32_t *keys, uint32_t *big_count) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *big_count = 0;  // read by the sums after the sort
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (size_t)gridDim.x * blockDim.x) {
         const uint2 r = rmap[i];
@@ -363,7 +365,6 @@ __global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t s
             const uint32_t l = j.labels[r.y];
             if (l < j.k) {
                 key = j.cluster_base + l;
-                atomicAdd(&counts_all[key], 1u);
                 if (j.penalized) {
                     // the true squared distance: the score less the bias
                     const float v = __fsub_rn(j.dist[r.y], j.bias[l]);
@@ -410,8 +411,8 @@ __device__ inline size_t first_change(const uint32_t *keys, size_t p, size_t lim
     return lim;
 }
 
-// Writes a finished cluster: its f32 sums, and in the last epoch its FP64
-// pass totals and member count.
+// Writes a finished cluster: its member count, its f32 sums, and in the
+// last epoch its FP64 pass totals and member count.
 template <int CC>
 __device__ __forceinline__ void flush_cluster(const JobDev &j, uint32_t key, const float (&acc)[CC], uint32_t cnt,
                                               int lane) {
@@ -424,15 +425,18 @@ __device__ __forceinline__ void flush_cluster(const JobDev &j, uint32_t key, con
             if (j.last_epoch) j.pass_sums[c * d + t] += (double)acc[cc];
         }
     }
-    if (lane == 0 && j.last_epoch) j.pass_counts[c] += cnt;
+    if (lane == 0) {
+        j.counts[c] = cnt;
+        if (j.last_epoch) j.pass_counts[c] += cnt;
+    }
 }
 
 // Streams job j's clusters from position p (a cluster's first member) while
 // they start before `stop` and belong to j; returns the position after the
 // last member folded.
 template <int CC, int U>
-__device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *keys, const uint32_t *src,
-                                             const uint32_t *counts, size_t slots, size_t p, size_t stop, int lane) {
+__device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *keys, const uint32_t *src, size_t slots,
+                                             size_t p, size_t stop, int lane) {
     const size_t d = j.d;
     const float *x = j.x;
     const uint32_t job_end = j.cluster_base + j.k;
@@ -442,14 +446,17 @@ __device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *ke
     uint32_t cur = keys[p], cnt = 0;
     for (;;) {
         // the next 64 positions: a member is taken when it continues the
-        // running cluster or starts one of j's clusters before `stop` that
-        // is not a big one
+        // running cluster, or starts one of j's clusters before `stop` that
+        // also ends in this window (so it is not a big one; the caller
+        // measures a cluster that runs past the window)
         const size_t q = p + lane;
         const uint32_t mk = q < slots ? keys[q] : 0xFFFFFFFFu;
         uint32_t before = (uint32_t)__shfl_up((int)mk, 1, 64);
         if (lane == 0) before = cur;
-        bool take = mk == before;
-        if (!take && q < stop && mk < job_end) take = counts[mk] <= SUM_BIG;
+        const bool start = mk != before;
+        const uint64_t starts = __ballot(start);
+        const uint64_t later = lane < 63 ? starts >> (lane + 1) : 0ull;
+        const bool take = !start || (q < stop && mk < job_end && later != 0ull);
         const uint64_t miss = __ballot(!take);
         const int ntake = miss ? __ffsll((long long)miss) - 1 : 64;
         const uint32_t ms = lane < ntake ? src[q] : 0u;
@@ -492,12 +499,13 @@ __device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *ke
 }
 
 // keys / src: the sorted slots' global clusters (nclusters for spare slots,
-// last) and job-relative rows; counts: members per global cluster. Big
-// clusters are listed in big (after the count word) for jobs_big_sum_kernel.
+// last) and job-relative rows. Each cluster's member count goes to counts
+// (a global cluster's entry); big clusters are listed in big (after the
+// count word) for jobs_big_sum_kernel.
 __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
                                                               uint32_t nclusters, const uint32_t *keys,
-                                                              const uint32_t *src, const uint32_t *counts,
-                                                              size_t slots, uint32_t *big) {
+                                                              const uint32_t *src, uint32_t *counts, size_t slots,
+                                                              uint32_t *big) {
     const int lane = threadIdx.x & 63;
     size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * SUM_SPAN;
     if (p >= slots) return;
@@ -508,20 +516,22 @@ __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs
     while (p < stop) {
         const uint32_t key = keys[p];
         if (key >= nclusters) break;
-        const uint32_t n = counts[key];
+        const size_t n = first_change(keys, p, slots, key, lane) - p;  // the cluster's members
         if (n > SUM_BIG) {
             if (lane == 0) {
                 const uint32_t at = atomicAdd(&big[0], 1u);
-                big[2 + 2 * at] = key;
-                big[3 + 2 * at] = (uint32_t)p;
+                big[2 + 3 * at] = key;
+                big[3 + 3 * at] = (uint32_t)p;
+                big[4 + 3 * at] = (uint32_t)n;
+                counts[key] = (uint32_t)n;
             }
             p += n;
             continue;
         }
         const JobDev j = job_at<true>(jobs, cjob[key]);
-        if (j.d > 128) p = stream_job<4, 16>(j, keys, src, counts, slots, p, stop, lane);
-        else if (j.d > 64) p = stream_job<2, 32>(j, keys, src, counts, slots, p, stop, lane);
-        else p = stream_job<1, 64>(j, keys, src, counts, slots, p, stop, lane);
+        if (j.d > 128) p = stream_job<4, 16>(j, keys, src, slots, p, stop, lane);
+        else if (j.d > 64) p = stream_job<2, 32>(j, keys, src, slots, p, stop, lane);
+        else p = stream_job<1, 64>(j, keys, src, slots, p, stop, lane);
     }
 }
 
@@ -531,16 +541,14 @@ __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs
 constexpr unsigned BIG_SUM_BLOCKS = 512;
 
 __global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
-                                                           const uint32_t *src, const uint32_t *counts,
-                                                           const uint32_t *big) {
+                                                           const uint32_t *src, const uint32_t *big) {
     const int lane = threadIdx.x & 63;
     const size_t total = (size_t)big[0] * 4, waves = (size_t)gridDim.x * 4;
     for (size_t t = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += waves) {
-        const uint32_t key = big[2 + 2 * (t >> 2)], p0 = big[3 + 2 * (t >> 2)];
+        const uint32_t key = big[2 + 3 * (t >> 2)], p0 = big[3 + 3 * (t >> 2)], n = big[4 + 3 * (t >> 2)];
         const size_t tg = (t & 3) * 64;
         const JobDev j = job_at<true>(jobs, cjob[key]);
         if (tg >= j.d) continue;
-        const uint32_t n = counts[key];
         auto src_of = [&](uint32_t q) -> size_t { return (size_t)src[p0 + q]; };
         float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
         fold_members<EMH_ROWS_F32, 1, 64>(j.x, j.d, tg, n, lane, src_of, acc);
@@ -564,13 +572,13 @@ hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rm
     if (slots == 0) return hipSuccess;
     unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
     hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters, keys,
-                       counts_all, a.big);
+                       a.big);
     return hipGetLastError();
 }
 
 hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
-                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
-                            const uint32_t *counts_all, hipStream_t s) {
+                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys, uint32_t *counts_all,
+                            hipStream_t s) {
     if (slots == 0) return hipSuccess;
     hipError_t e;
     // slot i's job-relative row is ids[i] (the jobs' id ranges are laid out
@@ -583,7 +591,7 @@ hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, c
     hipLaunchKernelGGL(jobs_stream_sum_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, jobs, cjob,
                        (uint32_t)nclusters, sorted_keys, sorted_src, counts_all, slots, a.big);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src, counts_all,
+    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src,
                        (const uint32_t *)a.big);
     return hipGetLastError();
 }

@@ -203,7 +203,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
 struct AccumScratch {
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
     uint32_t *offsets = nullptr;
-    uint32_t *big = nullptr;  // batched sums: [0] count, then (cluster, first position) pairs
+    uint32_t *big = nullptr;  // batched sums: [0] count, then (cluster, first position, members) triples
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     size_t cap_rows = 0, cap_k = 0;
@@ -232,16 +232,17 @@ hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_
 hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
 // Batched, first half: the last epoch's first batch zeroes its FP64 totals;
 // keys = cluster_base + label for live slots (nclusters for spare slots),
-// counts per global cluster; penalized jobs' distances lose their bias (the
+// counts_all zeroed; penalized jobs' distances lose their bias (the
 // true squared distances, src/lib.rs:425-430); last-epoch labels are stored
 // at their rows in labels_all.
 hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
                             const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
                             hipStream_t s);
-// Second half: one stable sort, then per-job f32 sums and FP64 pass totals.
+// Second half: one stable sort, then per-job member counts (counts_all,
+// zeroed by the first half), f32 sums and FP64 pass totals.
 hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
-                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
-                            const uint32_t *counts_all, hipStream_t s);
+                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys, uint32_t *counts_all,
+                            hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
 
 // Error reporting shared by the C ABI.
