@@ -282,9 +282,14 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::sort(classes.begin(), classes.end());
     std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
     std::vector<size_t> wg_off, wg_cnt;
-    // One workgroup per (job, 256 batch rows, slice); within a class the
+    // One workgroup per (job, slice, 256 batch rows); within a class the
     // jobs with the most tiles per slice come first, so the longest sweeps
-    // start first.
+    // start first. Workgroups go to the 8 XCDs in turn (dispatch i runs on
+    // XCD i % 8), so the list is dealt out in runs of XCD_RUN consecutive
+    // entries, one XCD per run: the row blocks of one slice then share that
+    // XCD's L2 for the slice's centroid tiles. Padding entries (first row
+    // past every batch) exit at once.
+    constexpr size_t XCDS = 8, XCD_RUN = 8;
     for (int rb : classes) {
         wg_off.push_back(wg_all.size());
         std::vector<size_t> order;
@@ -293,10 +298,18 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
             return slices[a].tiles_per_slice > slices[b].tiles_per_slice;
         });
+        std::vector<uint2> list;
         for (size_t j : order)
-            for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += 256)
-                for (uint32_t sl = 0; sl < slices[j].nslice; sl++)
-                    wg_all.push_back(make_uint2((unsigned)j, (unsigned)r0 | sl));
+            for (uint32_t sl = 0; sl < slices[j].nslice; sl++)
+                for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += 256)
+                    list.push_back(make_uint2((unsigned)j, (unsigned)r0 | sl));
+        std::vector<std::vector<uint2>> per(XCDS);
+        for (size_t i = 0; i < list.size(); i++) per[(i / XCD_RUN) % XCDS].push_back(list[i]);
+        size_t rounds = 0;
+        for (const auto &v : per) rounds = std::max(rounds, v.size());
+        const uint2 pad = make_uint2(0u, 0xFFFFFF00u);
+        for (size_t t = 0; t < rounds; t++)
+            for (size_t x = 0; x < XCDS; x++) wg_all.push_back(t < per[x].size() ? per[x][t] : pad);
         wg_cnt.push_back(wg_all.size() - wg_off.back());
     }
     std::vector<uint32_t> cjob(nclusters);

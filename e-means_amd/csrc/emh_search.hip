@@ -1264,7 +1264,9 @@ __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, s
 }
 
 // Every job's rows: slot i of the concatenated batch is row rmap[i].y of job
-// rmap[i].x (live when below that job's row count this step).
+// rmap[i].x (live when below that job's row count this step). (Giving each
+// XCD a contiguous run of slots, for L2 reuse of the candidates, measured
+// slower than this grid-stride order.)
 template <int M>
 __global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
     const int gl = threadIdx.x & 15;
@@ -1908,12 +1910,13 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
 
 // A workgroup's sweep costs about (tiles x tile centroids x d_pad) operand
 // elements against its 256 rows, plus loading those rows once. Slices of
-// about 2^16 centroid-dimensions (and at least 256 centroids, so that the row
+// about 2^17 centroid-dimensions (and at least 256 centroids, so that the row
 // load stays a small share) keep the longest workgroup of a batch short while
-// the merge reads a few words per slice.
+// every slice re-reads its rows and the merge reads a few words per slice
+// (2^16 and 2^18 measured slower on uneven batches).
 JobSlices job_slices(const SearchGeom &g) {
     const size_t tc = (size_t)reg_tile(g), ntiles = g.k_pad / tc;
-    size_t per = std::max<size_t>((size_t)1 << 16, 256 * g.d_pad) / (tc * g.d_pad);
+    size_t per = std::max<size_t>((size_t)1 << 17, 256 * g.d_pad) / (tc * g.d_pad);
     per = std::max<size_t>(per, 1);
     size_t ns = std::min<size_t>((ntiles + per - 1) / per, 255);
     per = (ntiles + ns - 1) / ns;  // balanced, none empty
