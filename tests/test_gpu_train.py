@@ -164,6 +164,29 @@ def test_batched_training_equals_single(hip, prec):
         assert g.n_splits == ref.n_splits
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batched_slices_and_big_clusters_equal_single(hip, prec):
+    """The batched step's own paths: a job whose centroid tiles split into
+    two slices (k = 2100, d = 64), jobs whose clusters get hundreds of
+    members per step (k = 3 and 5 with batches of 2048: the per-64-dimension
+    folds of big clusters, with balance on one), beside an ordinary job; each
+    ends with exactly its single-problem result."""
+    specs = [(8000, 64, 2100, 0.0, None), (20000, 24, 3, 0.1, 2048), (12000, 200 if prec == "bf16" else 100, 5, 0.0, 2048),
+             (3000, 40, 50, 0.0, None)]
+    problems = []
+    for i, (n, d, k, bal, bs) in enumerate(specs):
+        cfg = E.Config(nepochs=2, balance=bal, seed=40 + i, batch_size=bs)
+        problems.append((_job_data(50 + i, n, d, 2 * k), k, cfg))
+    print(f"batch slices/big {prec}", flush=True)
+    got, _ = E.train_batch(problems, E.HipAssigner(prec))
+    for (x, k, cfg), g in zip(problems, got):
+        ref = E.train(x, k, cfg, E.HipAssigner(prec))
+        assert np.array_equal(g.labels, ref.labels)
+        assert np.array_equal(g.centroids, ref.centroids)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+
+
 @pytest.mark.parametrize("prec", ["fp32", "i8"])
 def test_device_rows_training_equals_host_rows(hip, prec):
     """emh_train_device (rows already in HBM, initial centroids gathered on
