@@ -4,10 +4,10 @@
 // counts[l] += 1; sums[l] += row (f32), src/lib.rs:431-441, and keeps the
 // last pass's totals for the closing mean, src/lib.rs:446-454, 481-489.
 // Here the batch is grouped by label with a stable LSD radix sort (row
-// positions keep their batch order inside each label), and one lane per
-// (cluster, dimension) folds that cluster's rows in that order, so every f32
-// sum is bit-identical to the host loop. The last pass's totals accumulate in
-// FP64.
+// positions keep their batch order inside each label), and the sorted rows
+// are folded as one stream, one lane per dimension, each cluster's rows in
+// that order, so every f32 sum is bit-identical to the host loop. The last
+// pass's totals accumulate in FP64.
 //
 // The sort is this file's own (8-bit digits: block histograms, one-workgroup
 // scan, stable scatter with wave ballots). It issues kernels only, no memset
@@ -26,21 +26,6 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t count, uint32_t v) {
 __global__ void fill_f64_kernel(double *p, size_t count) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
-}
-
-__global__ void count_kernel(const uint32_t *labels, size_t m, size_t k, uint32_t *counts) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
-        uint32_t l = labels[i];
-        if (l < k) atomicAdd(&counts[l], 1u);
-    }
-}
-
-// start[label] = first sorted position holding that label.
-__global__ void segment_start_kernel(const uint32_t *sorted_labels, size_t m, size_t k, uint32_t *start) {
-    for (size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x; p < m; p += (size_t)gridDim.x * blockDim.x) {
-        uint32_t l = sorted_labels[p];
-        if (l < k && (p == 0 || sorted_labels[p - 1] != l)) start[l] = (uint32_t)p;
-    }
 }
 
 template <int ROWS> __device__ inline float row_elem(const void *x, size_t idx) {
@@ -82,26 +67,6 @@ __device__ __forceinline__ void fold_members(const void *x, size_t d, size_t tg,
                     for (int c = 0; c < CC; c++) acc[c] = __fadd_rn(acc[c], v[c][u]);
         }
     }
-}
-
-// One wave per (cluster, 64 dimensions), the dimension group major (wave
-// w: group w / k, cluster w % k), lanes over the group's dimensions, 64
-// member rows in flight; rows folded in batch order (sorted_src: the source
-// row of each member, batch order within a cluster).
-template <int ROWS>
-__global__ __launch_bounds__(256) void segment_sum_kernel(const void *x, size_t d, const uint32_t *sorted_src,
-                                                          const uint32_t *start, const uint32_t *counts, size_t k,
-                                                          float *sums) {
-    const size_t wv = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    const size_t grp = wv / k, jc = wv - grp * k, tg = grp * 64;
-    if (tg >= d) return;
-    const uint32_t n = counts[jc];
-    const uint32_t s0 = n ? start[jc] : 0;
-    auto src_of = [&](uint32_t q) -> size_t { return (size_t)sorted_src[s0 + q]; };
-    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    fold_members<ROWS, 1, 64>(x, d, tg, n, lane, src_of, acc);
-    if (tg + lane < d) sums[jc * d + tg + lane] = acc[0];
 }
 
 __global__ void pass_accumulate_kernel(const float *sums, const uint32_t *counts, size_t k, size_t d,
@@ -240,6 +205,7 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
 }
 
 hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
+    (void)k;  // the scratch depends on the batch only
     hipError_t e = hipSuccess;
     if (m > a.cap_rows) {
         hipFree(a.keys);
@@ -261,12 +227,6 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
         a.cap_rows = m;
     }
-    if (k > a.cap_k) {
-        hipFree(a.offsets);
-        a.offsets = nullptr;
-        if ((e = hipMalloc(&a.offsets, k * 4)) != hipSuccess) return e;
-        a.cap_k = k;
-    }
     return e;
 }
 
@@ -276,7 +236,6 @@ void accum_free(AccumScratch &a) {
     hipFree(a.vals);
     hipFree(a.vals_alt);
     hipFree(a.sort_tmp);
-    hipFree(a.offsets);
     hipFree(a.big);
     a = AccumScratch();
 }
@@ -317,34 +276,6 @@ hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s) {
     if (!count) return hipSuccess;
     unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 4096);
     hipLaunchKernelGGL(fill_f64_kernel, dim3(blocks), dim3(256), 0, s, p, count);
-    return hipGetLastError();
-}
-
-hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
-                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s) {
-    hipError_t e;
-    if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
-    if ((e = launch_fill_u32(counts, k, 0u, s)) != hipSuccess) return e;
-    // sorted by label, each carrying its source row (ids[i], or i)
-    const uint32_t *sorted_src = a.vals;
-    if (m > 0) {
-        unsigned blocks = (unsigned)std::min<size_t>((m + 255) / 256, 4096);
-        hipLaunchKernelGGL(count_kernel, dim3(blocks), dim3(256), 0, s, labels, m, k, counts);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        const uint32_t *sorted_keys;
-        if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess)
-            return e;
-        hipLaunchKernelGGL(segment_start_kernel, dim3(blocks), dim3(256), 0, s, sorted_keys, m, k, a.offsets);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
-    const size_t waves = k * ((d + 63) / 64);
-    dim3 grid((unsigned)((waves + 3) / 4));
-    if (rows == EMH_ROWS_F32)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
-    else if (rows == EMH_ROWS_I8)
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
-    else
-        hipLaunchKernelGGL(segment_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, sorted_src, a.offsets, counts, k, sums);
     return hipGetLastError();
 }
 
@@ -411,44 +342,67 @@ __device__ inline size_t first_change(const uint32_t *keys, size_t p, size_t lim
     return lim;
 }
 
-// Writes a finished cluster: its member count, its f32 sums, and in the
-// last epoch its FP64 pass totals and member count.
+// One problem's view for the streamed fold: its rows, where its clusters'
+// sums and counts go, and its global cluster range [base, end).
+struct SumView {
+    const void *x;             // n x d rows (f32, i8 or u8)
+    size_t d;
+    float *sums;               // (cluster - base) x d
+    uint32_t *counts;          // (cluster - base)
+    double *pass_sums;         // nullptr: no FP64 pass totals this step
+    uint64_t *pass_counts;
+    uint32_t base, end;
+};
+
+__device__ inline SumView job_view(const JobDev &j) {
+    return SumView{j.x, j.d, j.sums, j.counts, j.last_epoch ? j.pass_sums : nullptr, j.pass_counts, j.cluster_base,
+                   j.cluster_base + j.k};
+}
+
+// Element t of a row that starts at byte xr (uniform, scalar registers).
+template <int ROWS> __device__ inline float row_at(const char *xr, uint32_t t) {
+    if constexpr (ROWS == EMH_ROWS_F32) return *(const float *)(xr + 4u * t);
+    else if constexpr (ROWS == EMH_ROWS_I8) return (float)*(const int8_t *)(xr + t);
+    else return (float)*(const uint8_t *)(xr + t);
+}
+
+// Writes a finished cluster's dimensions [tg, tg + 64 CC): its f32 sums, and
+// (the first group) its member count; with pass totals, those too.
 template <int CC>
-__device__ __forceinline__ void flush_cluster(const JobDev &j, uint32_t key, const float (&acc)[CC], uint32_t cnt,
-                                              int lane) {
-    const size_t c = key - j.cluster_base, d = j.d;
+__device__ __forceinline__ void flush_cluster(const SumView &v, uint32_t key, size_t tg, const float (&acc)[CC],
+                                              uint32_t cnt, int lane) {
+    const size_t c = key - v.base, d = v.d;
 #pragma unroll
     for (int cc = 0; cc < CC; cc++) {
-        const size_t t = 64 * cc + lane;
+        const size_t t = tg + 64 * cc + lane;
         if (t < d) {
-            j.sums[c * d + t] = acc[cc];
-            if (j.last_epoch) j.pass_sums[c * d + t] += (double)acc[cc];
+            v.sums[c * d + t] = acc[cc];
+            if (v.pass_sums) v.pass_sums[c * d + t] += (double)acc[cc];
         }
     }
-    if (lane == 0) {
-        j.counts[c] = cnt;
-        if (j.last_epoch) j.pass_counts[c] += cnt;
+    if (lane == 0 && tg == 0) {
+        v.counts[c] = cnt;
+        if (v.pass_sums) v.pass_counts[c] += cnt;
     }
 }
 
-// Streams job j's clusters from position p (a cluster's first member) while
-// they start before `stop` and belong to j; returns the position after the
-// last member folded.
-template <int CC, int U>
-__device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *keys, const uint32_t *src, size_t slots,
-                                             size_t p, size_t stop, int lane) {
-    const size_t d = j.d;
-    const float *x = j.x;
-    const uint32_t job_end = j.cluster_base + j.k;
+// Streams the problem's clusters from position p (a cluster's first member)
+// while they start before `stop` and belong to it, over dimensions
+// [tg, tg + 64 CC); returns the position after the last member folded.
+template <int ROWS, int CC, int U>
+__device__ __forceinline__ size_t stream_clusters(const SumView &v, const uint32_t *keys, const uint32_t *src,
+                                                  size_t slots, size_t p, size_t stop, size_t tg, int lane) {
+    constexpr size_t ES = ROWS == EMH_ROWS_F32 ? 4 : 1;
+    const size_t d = v.d;
     float acc[CC];
 #pragma unroll
     for (int cc = 0; cc < CC; cc++) acc[cc] = 0.0f;
     uint32_t cur = keys[p], cnt = 0;
     for (;;) {
         // the next 64 positions: a member is taken when it continues the
-        // running cluster, or starts one of j's clusters before `stop` that
-        // also ends in this window (so it is not a big one; the caller
-        // measures a cluster that runs past the window)
+        // running cluster, or starts one of the problem's clusters before
+        // `stop` that also ends in this window (so it is not a big one; the
+        // caller measures a cluster that runs past the window)
         const size_t q = p + lane;
         const uint32_t mk = q < slots ? keys[q] : 0xFFFFFFFFu;
         uint32_t before = (uint32_t)__shfl_up((int)mk, 1, 64);
@@ -456,22 +410,22 @@ __device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *ke
         const bool start = mk != before;
         const uint64_t starts = __ballot(start);
         const uint64_t later = lane < 63 ? starts >> (lane + 1) : 0ull;
-        const bool take = !start || (q < stop && mk < job_end && later != 0ull);
+        const bool take = !start || (q < stop && mk < v.end && later != 0ull);
         const uint64_t miss = __ballot(!take);
         const int ntake = miss ? __ffsll((long long)miss) - 1 : 64;
         const uint32_t ms = lane < ntake ? src[q] : 0u;
         for (int u0 = 0; u0 < ntake; u0 += U) {
-            float v[CC][U];
+            float val[CC][U];
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 // the member's row address is uniform: scalar registers
                 // hold it, each lane adds its dimension
                 const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)ms, (u0 + u) & 63);
-                const char *xr = (const char *)(x + (size_t)r * d);
+                const char *xr = (const char *)v.x + ((size_t)r * d + tg) * ES;
 #pragma unroll
                 for (int cc = 0; cc < CC; cc++) {
                     const uint32_t t = 64u * cc + (uint32_t)lane;
-                    v[cc][u] = (t < d && u0 + u < ntake) ? *(const float *)(xr + 4u * t) : 0.0f;
+                    val[cc][u] = (tg + t < d && u0 + u < ntake) ? row_at<ROWS>(xr, t) : 0.0f;
                 }
             }
 #pragma unroll
@@ -479,14 +433,14 @@ __device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *ke
                 if (u0 + u < ntake) {
                     const uint32_t ku = (uint32_t)__builtin_amdgcn_readlane((int)mk, (u0 + u) & 63);
                     if (ku != cur) {
-                        flush_cluster<CC>(j, cur, acc, cnt, lane);
+                        flush_cluster<CC>(v, cur, tg, acc, cnt, lane);
 #pragma unroll
                         for (int cc = 0; cc < CC; cc++) acc[cc] = 0.0f;
                         cnt = 0;
                         cur = ku;
                     }
 #pragma unroll
-                    for (int cc = 0; cc < CC; cc++) acc[cc] = __fadd_rn(acc[cc], v[cc][u]);
+                    for (int cc = 0; cc < CC; cc++) acc[cc] = __fadd_rn(acc[cc], val[cc][u]);
                     cnt++;
                 }
             }
@@ -494,8 +448,28 @@ __device__ __forceinline__ size_t stream_job(const JobDev &j, const uint32_t *ke
         p += (size_t)ntake;
         if (ntake < 64) break;
     }
-    flush_cluster<CC>(j, cur, acc, cnt, lane);
+    flush_cluster<CC>(v, cur, tg, acc, cnt, lane);
     return p;
+}
+
+// stream_clusters with CC picked from the dimensions left at tg
+template <int ROWS>
+__device__ __forceinline__ size_t stream_dims(const SumView &v, const uint32_t *keys, const uint32_t *src,
+                                              size_t slots, size_t p, size_t stop, size_t tg, int lane) {
+    const size_t left = v.d - tg;
+    if (left > 128) return stream_clusters<ROWS, 4, 16>(v, keys, src, slots, p, stop, tg, lane);
+    if (left > 64) return stream_clusters<ROWS, 2, 32>(v, keys, src, slots, p, stop, tg, lane);
+    return stream_clusters<ROWS, 1, 64>(v, keys, src, slots, p, stop, tg, lane);
+}
+
+// A big cluster's entry in the list (after the count word): its key, first
+// sorted position and members; its count is written here.
+__device__ inline void list_big(uint32_t *big, uint32_t *count_slot, uint32_t key, size_t p, size_t n) {
+    const uint32_t at = atomicAdd(&big[0], 1u);
+    big[2 + 3 * at] = key;
+    big[3 + 3 * at] = (uint32_t)p;
+    big[4 + 3 * at] = (uint32_t)n;
+    *count_slot = (uint32_t)n;
 }
 
 // keys / src: the sorted slots' global clusters (nclusters for spare slots,
@@ -518,27 +492,34 @@ __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs
         if (key >= nclusters) break;
         const size_t n = first_change(keys, p, slots, key, lane) - p;  // the cluster's members
         if (n > SUM_BIG) {
-            if (lane == 0) {
-                const uint32_t at = atomicAdd(&big[0], 1u);
-                big[2 + 3 * at] = key;
-                big[3 + 3 * at] = (uint32_t)p;
-                big[4 + 3 * at] = (uint32_t)n;
-                counts[key] = (uint32_t)n;
-            }
+            if (lane == 0) list_big(big, &counts[key], key, p, n);
             p += n;
             continue;
         }
         const JobDev j = job_at<true>(jobs, cjob[key]);
-        if (j.d > 128) p = stream_job<4, 16>(j, keys, src, slots, p, stop, lane);
-        else if (j.d > 64) p = stream_job<2, 32>(j, keys, src, slots, p, stop, lane);
-        else p = stream_job<1, 64>(j, keys, src, slots, p, stop, lane);
+        p = stream_dims<EMH_ROWS_F32>(job_view(j), keys, src, slots, p, stop, 0, lane);  // batched rows: d <= 256
     }
 }
 
 // The big clusters of this step: wave t of the grid-stride loop folds
-// dimensions [64 (t % 4), + 64) of listed cluster t / 4 (batched rows are at
-// most 256 wide), its members in batch order, 64 rows in flight.
+// dimensions [64 (t % G), + 64) of listed cluster t / G (G = 4 for batched
+// rows, at most 256 wide; G = ceil(d / 64) for one problem), its members in
+// batch order, 64 rows in flight.
 constexpr unsigned BIG_SUM_BLOCKS = 512;
+
+template <int ROWS>
+__device__ __forceinline__ void fold_big(const SumView &v, const uint32_t *src, uint32_t key, uint32_t p0, uint32_t n,
+                                         size_t tg, int lane) {
+    auto src_of = [&](uint32_t q) -> size_t { return (size_t)src[p0 + q]; };
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    fold_members<ROWS, 1, 64>(v.x, v.d, tg, n, lane, src_of, acc);
+    const size_t c = key - v.base, dd = tg + lane;
+    if (dd < v.d) {
+        v.sums[c * v.d + dd] = acc[0];
+        if (v.pass_sums) v.pass_sums[c * v.d + dd] += (double)acc[0];
+    }
+    if (lane == 0 && tg == 0 && v.pass_sums) v.pass_counts[c] += n;
+}
 
 __global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
                                                            const uint32_t *src, const uint32_t *big) {
@@ -549,16 +530,87 @@ __global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, c
         const size_t tg = (t & 3) * 64;
         const JobDev j = job_at<true>(jobs, cjob[key]);
         if (tg >= j.d) continue;
-        auto src_of = [&](uint32_t q) -> size_t { return (size_t)src[p0 + q]; };
-        float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        fold_members<EMH_ROWS_F32, 1, 64>(j.x, j.d, tg, n, lane, src_of, acc);
-        const size_t c = key - j.cluster_base, dd = tg + lane;
-        if (dd < j.d) {
-            j.sums[c * j.d + dd] = acc[0];
-            if (j.last_epoch) j.pass_sums[c * j.d + dd] += (double)acc[0];
-        }
-        if (lane == 0 && tg == 0 && j.last_epoch) j.pass_counts[c] += n;
+        fold_big<EMH_ROWS_F32>(job_view(j), src, key, p0, n, tg, lane);
     }
+}
+
+// One problem (any row width): keys are the sorted labels (< k), src the
+// rows; wave w streams the clusters starting in its span, 256 dimensions at
+// a time; big clusters go to the list for big_sum_kernel.
+template <int ROWS>
+__global__ __launch_bounds__(256) void stream_sum_kernel(const void *x, size_t d, const uint32_t *keys,
+                                                         const uint32_t *src, size_t m, size_t k, float *sums,
+                                                         uint32_t *counts, uint32_t *big) {
+    const int lane = threadIdx.x & 63;
+    size_t p0 = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * SUM_SPAN;
+    if (p0 >= m) return;
+    const size_t stop = p0 + SUM_SPAN < m ? p0 + SUM_SPAN : m;
+    if (p0 > 0) p0 = first_change(keys, p0, stop, keys[p0 - 1], lane);
+    const SumView v{x, d, sums, counts, nullptr, nullptr, 0u, (uint32_t)k};
+    for (size_t tg = 0; tg < d; tg += 256) {
+        size_t p = p0;
+        while (p < stop) {
+            const uint32_t key = keys[p];
+            if (key >= k) break;
+            const size_t n = first_change(keys, p, m, key, lane) - p;
+            if (n > SUM_BIG) {
+                if (lane == 0 && tg == 0) list_big(big, &counts[key], key, p, n);
+                p += n;
+                continue;
+            }
+            p = stream_dims<ROWS>(v, keys, src, m, p, stop, tg, lane);
+        }
+    }
+}
+
+template <int ROWS>
+__global__ __launch_bounds__(256) void big_sum_kernel(const void *x, size_t d, const uint32_t *src, size_t k,
+                                                      float *sums, uint32_t *counts, const uint32_t *big) {
+    const int lane = threadIdx.x & 63;
+    const size_t groups = (d + 63) / 64;
+    const size_t total = (size_t)big[0] * groups, waves = (size_t)gridDim.x * 4;
+    const SumView v{x, d, sums, counts, nullptr, nullptr, 0u, (uint32_t)k};
+    for (size_t t = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += waves) {
+        const size_t e = t / groups;
+        fold_big<ROWS>(v, src, big[2 + 3 * e], big[3 + 3 * e], big[4 + 3 * e], (t - e * groups) * 64, lane);
+    }
+}
+
+// Per-cluster counts and f32 sums of one batch, each cluster's rows folded
+// in batch order (src/lib.rs:431-441): labels sorted stably, carrying their
+// rows (ids[i], or i), then the sorted stream; empty clusters keep zeros.
+hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
+                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s) {
+    hipError_t e;
+    if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
+    if ((e = launch_fill_u32(counts, k, 0u, s)) != hipSuccess) return e;
+    if ((e = launch_fill_u32((uint32_t *)sums, k * d, 0u, s)) != hipSuccess) return e;  // +0.0f
+    if (m == 0) return hipSuccess;
+    if ((e = launch_fill_u32(a.big, 1, 0u, s)) != hipSuccess) return e;
+    const uint32_t *sorted_keys, *sorted_src;
+    if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess) return e;
+    const dim3 grid((unsigned)(((m + SUM_SPAN - 1) / SUM_SPAN + 3) / 4)), bgrid(BIG_SUM_BLOCKS);
+    switch (rows) {
+        case EMH_ROWS_F32:
+            hipLaunchKernelGGL(stream_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, sorted_keys, sorted_src, m,
+                               k, sums, counts, a.big);
+            hipLaunchKernelGGL(big_sum_kernel<EMH_ROWS_F32>, bgrid, dim3(256), 0, s, x, d, sorted_src, k, sums, counts,
+                               (const uint32_t *)a.big);
+            break;
+        case EMH_ROWS_I8:
+            hipLaunchKernelGGL(stream_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, sorted_keys, sorted_src, m, k,
+                               sums, counts, a.big);
+            hipLaunchKernelGGL(big_sum_kernel<EMH_ROWS_I8>, bgrid, dim3(256), 0, s, x, d, sorted_src, k, sums, counts,
+                               (const uint32_t *)a.big);
+            break;
+        default:
+            hipLaunchKernelGGL(stream_sum_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, sorted_keys, sorted_src, m, k,
+                               sums, counts, a.big);
+            hipLaunchKernelGGL(big_sum_kernel<EMH_ROWS_U8>, bgrid, dim3(256), 0, s, x, d, sorted_src, k, sums, counts,
+                               (const uint32_t *)a.big);
+            break;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,

@@ -202,11 +202,10 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
 // Device-side per-cluster accumulation (emh_accum.hip).
 struct AccumScratch {
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
-    uint32_t *offsets = nullptr;
     uint32_t *big = nullptr;  // batched sums: [0] count, then (cluster, first position, members) triples
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
-    size_t cap_rows = 0, cap_k = 0;
+    size_t cap_rows = 0;
 };
 hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k);
 void accum_free(AccumScratch &a);

@@ -101,16 +101,18 @@ def test_f32_training_tracks_oracle(hip):
     assert abs(got.objective[-1] / ref.objective[-1] - 1) < 0.02
 
 
-@pytest.mark.parametrize("n,m,k", [(5000, 3000, 50), (20000, 7001, 300), (70000, 66000, 70000), (9000, 1, 1)])
-def test_device_accumulate_is_batch_order_f32(hip, n, m, k):
-    """The stable sort + segmented fold equals the sequential host loop bit
+@pytest.mark.parametrize("n,m,k,d", [(5000, 3000, 50, 37), (20000, 7001, 300, 37), (70000, 66000, 70000, 37),
+                                     (9000, 1, 1, 37), (20000, 15000, 7, 37), (6000, 5000, 40, 300)])
+def test_device_accumulate_is_batch_order_f32(hip, n, m, k, d):
+    """The stable sort + streamed fold equals the sequential host loop bit
     for bit: one, two and three 8-bit sort passes, blocks with a ragged end,
-    a single row, empty clusters."""
+    a single row, empty clusters, clusters of thousands of rows (folded one
+    wave per 64 dimensions) and rows wider than 256 (streamed 256 dimensions
+    at a time)."""
     import torch
 
-    print(f"accumulate n={n} m={m} k={k}", flush=True)
+    print(f"accumulate n={n} m={m} k={k} d={d}", flush=True)
     rng = np.random.default_rng(3)
-    d = 37
     x = rng.normal(scale=100, size=(n, d)).astype(np.float32)
     ids = rng.permutation(n)[:m].astype(np.uint32)
     labels = rng.integers(0, k, m).astype(np.uint32)
