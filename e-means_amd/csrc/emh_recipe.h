@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <future>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -182,6 +183,20 @@ struct HostBuf {
 // stages the batch (src/lib.rs:400-417), the caller runs the device half
 // (gather, search, per-cluster sums, EMA step, revival split) and folds the
 // batch's labels and distances in with finish_stats (src/lib.rs:419-466).
+// p[0..n) = v, on a few threads for large n (the first touch of a fresh
+// output array is page faults, which spread across cores).
+inline void fill_u32_host(uint32_t *p, size_t n, uint32_t v) {
+    const size_t nt = n < (size_t)1 << 20 ? 1 : std::min<size_t>(8, std::max(1u, std::thread::hardware_concurrency()));
+    if (nt == 1) {
+        std::fill(p, p + n, v);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (size_t t = 0; t < nt; t++)
+        th.emplace_back([=] { std::fill(p + n * t / nt, p + n * (t + 1) / nt, v); });
+    for (auto &x : th) x.join();
+}
+
 struct JobRun {
     const void *x = nullptr;
     int rows = EMH_ROWS_F32;
@@ -250,6 +265,13 @@ struct JobRun {
         if (resolve(n, d, k, cfg, R)) return fail(EMH_E_ARG, "nepochs out of range");
         rows_on_device = on_device;
         st = State{k, d, std::vector<float>(on_device ? 0 : k * d), std::vector<float>(k, INIT_USAGE)};
+        // the first epoch's order (next() takes it) drawn beside the initial
+        // rows' permutation: two independent streams of the same length
+        if (R.n_epochs > 0)
+            order_job = std::async(std::launch::async, [this] {
+                Rng(cfg.seed * 1000003ull + 1).permutation((uint32_t)n, order_next);
+                order_next_epoch = 0;
+            });
         {
             std::vector<uint32_t> init;
             Rng(cfg.seed).permutation((uint32_t)n, init);
@@ -263,7 +285,7 @@ struct JobRun {
         split_rng = Rng(cfg.seed ^ 0x5EED5EED5EED5EEDull);
         chunk = std::min(ASSIGN_CHUNK, (size_t)R.batch_size);
         bias.assign(cfg.balance > 0.0f ? k : 0, 0.0f);
-        for (size_t i = 0; i < n; i++) labels_out[i] = 0xFFFFFFFFu;
+        fill_u32_host(labels_out, n, 0xFFFFFFFFu);
         out->n_objective = 0;
         mean_sqdist = 0.0;
         step = 0;

@@ -1323,71 +1323,92 @@ __device__ inline float centroid_norm32(const float *cp, size_t d, int lane) {
     return __fadd_rn(s, tail);
 }
 
-// One wave per centroid: operand row (zero padded), and the f32 squared norm
-// folded as src/cpu.rs:170-180 does (16 strided partial sums, then the tail),
-// or the exact i64 norm of the rounded i8 centroid (src/cpu_i8.rs:83-93).
+// A group of 16 lanes per centroid, 16 centroids per workgroup: the
+// operand row (zero padded) a 16-byte quad per lane, and the f32 squared
+// norm folded as src/cpu.rs:170-180 does (lane l: elements l, l + 16, ...;
+// then the 16 partial sums in order; then the tail), or the exact i64 norm
+// of the rounded i8 centroid (src/cpu_i8.rs:83-93).
+constexpr int PREP_CENTROIDS = 16;
+
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
                                                              int64_t *norm64, const unsigned *cmax_bits,
                                                              unsigned *stats) {
-    const int lane = threadIdx.x & 63;
-    const size_t jc = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);  // < k_pad: the grid is k_pad / 4 blocks
+    constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
+    const int gl = threadIdx.x & 15;
+    const size_t jc = (size_t)blockIdx.x * PREP_CENTROIDS + (threadIdx.x >> 4);  // < k_pad (a multiple of 64)
     const bool live = jc < k;
     const float *cp = c + jc * d;
     float cs = 1.0f;
     if constexpr (OP == OP_FP8) cs = fp8_scale_for(__uint_as_float(*cmax_bits));
     // operand row: the float searches store -c, so that an MFMA accumulator
     // started at |c|^2/2 ends at |c|^2/2 - x.c
-    if constexpr (OP == OP_F32) {
-        float *o = (float *)cen_op + jc * d_pad;
-        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? -cp[e] : 0.0f;
-    } else if constexpr (OP == OP_BF16) {
-        uint16_t *o = (uint16_t *)cen_op + jc * d_pad;
-        for (size_t e = lane; e < d_pad; e += 64) o[e] = (live && e < d) ? bf16_rne(-cp[e]) : (uint16_t)0;
-    } else if constexpr (OP == OP_FP8) {
-        uint32_t *o = (uint32_t *)(cen_op + jc * d_pad);
-        for (size_t w = lane; w < d_pad / 4; w += 64) {
-            float v[4];
+    uint8_t *orow = cen_op + jc * d_pad * EB;
+    int64_t part64 = 0;
+    for (size_t q = gl; q < d_pad / EPQ; q += 16) {
+        uint32_t w[4] = {0u, 0u, 0u, 0u};
+        if constexpr (OP == OP_FP8) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                size_t e = 4 * w + q;
-                v[q] = (live && e < d) ? -cp[e] * cs : 0.0f;
+            for (int wi = 0; wi < 4; wi++) {
+                float v[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    const size_t e = q * EPQ + 4 * wi + t;
+                    v[t] = (live && e < d) ? -cp[e] * cs : 0.0f;
+                }
+                int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+                pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+                w[wi] = (uint32_t)pk;
             }
-            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
-            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
-            o[w] = (uint32_t)pk;
         }
-    } else {  // OP_I8: round half away from zero, clamp (src/cpu_i8.rs:83)
-        int8_t *o = (int8_t *)cen_op + jc * d_pad;
-        int64_t part = 0;
-        for (size_t e = lane; e < d_pad; e += 64) {
-            int8_t q = 0;
-            if (live && e < d) {
-                float r = roundf(cp[e]);
-                r = r != r ? 0.0f : fminf(fmaxf(r, -128.0f), 127.0f);
-                q = (int8_t)r;
+#pragma unroll
+        for (int i = 0; i < EPQ; i++) {
+            const size_t e = q * EPQ + i;
+            const bool in = live && e < d;
+            if constexpr (OP == OP_F32) {
+                w[i] = __float_as_uint(in ? -cp[e] : 0.0f);
+            } else if constexpr (OP == OP_BF16) {
+                w[i / 2] |= (uint32_t)bf16_rne(in ? -cp[e] : 0.0f) << (16 * (i & 1));
+            } else if constexpr (OP == OP_FP8) {
+                (void)in;  // packed above
+            } else {  // OP_I8: round half away from zero, clamp (src/cpu_i8.rs:83)
+                int32_t qv = 0;
+                if (in) {
+                    float r = roundf(cp[e]);
+                    r = r != r ? 0.0f : fminf(fmaxf(r, -128.0f), 127.0f);
+                    qv = (int32_t)r;
+                }
+                w[i / 4] |= ((uint32_t)qv & 0xFFu) << (8 * (i & 3));
+                part64 += (int64_t)qv * qv;
             }
-            o[e] = q;
-            part += (int64_t)q * q;
         }
-        for (int off = 32; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
-        if (lane == 0 && live) norm64[jc] = part;
+        *(uint4 *)(orow + q * 16) = make_uint4(w[0], w[1], w[2], w[3]);
     }
-    if constexpr (OP != OP_I8) {
+    if constexpr (OP == OP_I8) {
+        for (int off = 8; off > 0; off >>= 1) part64 += shfl_xor_i64(part64, off);
+        if (gl == 0 && live) norm64[jc] = part64;
+    } else {
         float nv = 0.0f;
         if (live) {
-            nv = centroid_norm32(cp, d, lane);
-            if (lane == 0) norm32[jc] = nv;
+            const size_t body = d & ~(size_t)15;
+            float part = 0.0f;
+            for (size_t i = gl; i < body; i += 16) part = __fadd_rn(part, __fmul_rn(cp[i], cp[i]));
+            for (int t = 0; t < 16; t++) nv = __fadd_rn(nv, __shfl(part, t, 16));
+            float tail = 0.0f;
+            for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
+            nv = __fadd_rn(nv, tail);
+            if (gl == 0) norm32[jc] = nv;
         }
         // max |c|^2 (the candidate searches' positivity offset): one atomic
         // per workgroup, not per centroid; NaN norms are skipped
         if constexpr (OP == OP_BF16 || OP == OP_FP8) {
-            __shared__ unsigned wmax[4];
-            if (lane == 0) wmax[threadIdx.x >> 6] = (live && nv >= 0.0f) ? __float_as_uint(nv) : 0u;
+            __shared__ unsigned gmax[PREP_CENTROIDS];
+            if (gl == 0) gmax[threadIdx.x >> 4] = (live && nv >= 0.0f) ? __float_as_uint(nv) : 0u;
             __syncthreads();
             if (threadIdx.x == 0) {
-                const unsigned mx = max(max(wmax[0], wmax[1]), max(wmax[2], wmax[3]));
+                unsigned mx = 0u;
+                for (int t = 0; t < PREP_CENTROIDS; t++) mx = max(mx, gmax[t]);
                 if (mx) atomicMax(&stats[0], mx);
             }
         }
@@ -1519,14 +1540,21 @@ __global__ __launch_bounds__(256) void prep_i16_kernel(const float *c, size_t k,
     if (lane == 0 && live) norm64[jc] = part;
 }
 
-__global__ void absmax_kernel(const float *v, size_t count, unsigned *out_bits) {
+// max |v| over the finite values, one atomic per workgroup (256 threads)
+__global__ __launch_bounds__(256) void absmax_kernel(const float *v, size_t count, unsigned *out_bits) {
+    __shared__ float wmax[4];
     float mx = 0.0f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
         float a = fabsf(v[i]);
         if (a > mx && isfinite(a)) mx = a;
     }
     for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out_bits, __float_as_uint(mx));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        if (mx > 0.0f) atomicMax(out_bits, __float_as_uint(mx));
+    }
 }
 
 // hn_j = (|c_j|^2 + bias_j) / 2 for the float searches; for I8
@@ -1861,14 +1889,14 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
                                  const unsigned *cmax, unsigned *stats, hipStream_t s) {
-    dim3 grid((unsigned)((g.k_pad + 3) / 4));
+    const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;
     switch (g.op) {
         case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
         case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
-        case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
+        case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid_wave, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
@@ -1996,7 +2024,7 @@ hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, siz
 }
 
 hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s) {
-    unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 1024);
+    unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 512);
     if (blocks == 0) blocks = 1;
     hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, v, count, out_bits);
     return hipGetLastError();
