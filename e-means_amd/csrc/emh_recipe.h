@@ -55,16 +55,40 @@ class Rng {
         return r;
     }
     uint64_t below(uint64_t bound) {  // Lemire
-        for (;;) {
-            unsigned __int128 m = (unsigned __int128)next() * bound;
-            if ((uint64_t)m >= (0 - bound) % bound) return (uint64_t)(m >> 64);
+        // accept x when the low word of x * bound is >= (2^64 - bound) % bound;
+        // a low word >= bound always is, so the division waits for the rare
+        // low word below bound (the same draws, the same results)
+        unsigned __int128 m = (unsigned __int128)next() * bound;
+        if ((uint64_t)m < bound) {
+            const uint64_t t = (0 - bound) % bound;
+            while ((uint64_t)m < t) m = (unsigned __int128)next() * bound;
         }
+        return (uint64_t)(m >> 64);
     }
     double open01() { return ((double)(next() >> 11) + 0.5) / 9007199254740992.0; }
+    // Fisher-Yates from the top: for i = n - 1 down to 1, swap p[i] with
+    // p[below(i + 1)]. The draws do not depend on the array, so they are
+    // taken LA steps ahead (in the same order) and the random slot is
+    // prefetched.
     void permutation(uint32_t n, std::vector<uint32_t> &p) {
         p.resize(n);
         for (uint32_t i = 0; i < n; i++) p[i] = i;
-        for (uint32_t i = n; i-- > 1;) std::swap(p[i], p[below((uint64_t)i + 1)]);
+        if (n < 2) return;
+        constexpr uint32_t LA = 16;
+        uint32_t ahead[LA];
+        uint32_t next_i = n - 1;  // the next step whose draw is taken
+        auto draw = [&]() {
+            const uint32_t j = (uint32_t)below((uint64_t)next_i + 1);
+            ahead[next_i % LA] = j;
+            __builtin_prefetch(&p[j], 1);
+            next_i--;
+        };
+        for (uint32_t t = 0; t < LA && next_i >= 1; t++) draw();
+        for (uint32_t i = n - 1; i >= 1; i--) {
+            const uint32_t j = ahead[i % LA];
+            if (next_i >= 1) draw();  // step i - LA, into the slot just read
+            std::swap(p[i], p[j]);
+        }
     }
 
   private:
