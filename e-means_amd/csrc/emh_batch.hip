@@ -273,31 +273,22 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
                     "upload masses")))
         return rc;
 
-    // Static maps: search workgroups per row-width class, prep entries, batch
-    // slots, clusters.
-    std::vector<int> classes;
-    for (size_t j = 0; j < njobs; j++)
-        if (std::find(classes.begin(), classes.end(), geo[j].row_bytes) == classes.end())
-            classes.push_back(geo[j].row_bytes);
-    std::sort(classes.begin(), classes.end());
-    std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
-    std::vector<size_t> wg_off, wg_cnt;
-    // One workgroup per (job, slice, 256 batch rows); within a class the
-    // jobs with the most tiles per slice come first, so the longest sweeps
-    // start first. Workgroups go to the 8 XCDs in turn (dispatch i runs on
-    // XCD i % 8), so the list is dealt out in runs of XCD_RUN consecutive
-    // entries, one XCD per run: the row blocks of one slice then share that
-    // XCD's L2 for the slice's centroid tiles. Padding entries (first row
-    // past every batch) exit at once.
+    // Static maps: search workgroups, prep entries, batch slots, clusters.
+    // One workgroup per (job, slice, 256 batch rows), every row width in the
+    // same launch; the jobs whose slices are the most work (tiles x tile
+    // centroids x row length) come first, so the longest sweeps start first.
+    // Workgroups go to the 8 XCDs in turn (dispatch i runs on XCD i % 8), so
+    // the list is dealt out in runs of XCD_RUN consecutive entries, one XCD
+    // per run: the row blocks of one slice then share that XCD's L2 for the
+    // slice's centroid tiles. Padding entries (first row past every batch)
+    // exit at once.
     constexpr size_t XCDS = 8, XCD_RUN = 8;
-    for (int rb : classes) {
-        wg_off.push_back(wg_all.size());
-        std::vector<size_t> order;
-        for (size_t j = 0; j < njobs; j++)
-            if (geo[j].row_bytes == rb) order.push_back(j);
-        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-            return slices[a].tiles_per_slice > slices[b].tiles_per_slice;
-        });
+    std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
+    {
+        std::vector<size_t> order(njobs);
+        for (size_t j = 0; j < njobs; j++) order[j] = j;
+        auto work = [&](size_t j) { return (size_t)slices[j].tiles_per_slice * reg_tile(geo[j]) * geo[j].d_pad; };
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
         std::vector<uint2> list;
         for (size_t j : order)
             for (uint32_t sl = 0; sl < slices[j].nslice; sl++)
@@ -310,7 +301,6 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         const uint2 pad = make_uint2(0u, 0xFFFFFF00u);
         for (size_t t = 0; t < rounds; t++)
             for (size_t x = 0; x < XCDS; x++) wg_all.push_back(t < per[x].size() ? per[x][t] : pad);
-        wg_cnt.push_back(wg_all.size() - wg_off.back());
     }
     std::vector<uint32_t> cjob(nclusters);
     for (size_t j = 0; j < njobs; j++) {
@@ -392,10 +382,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in_ids, hipMemcpyHostToDevice, h->stream);
     if (ce == hipSuccess) ce = launch_fill_u32(stats_d.as<uint32_t>(), 4 * njobs, 0u, h->stream);
     if (ce == hipSuccess) ce = launch_jobs_prep(op, jobs_dev, (const uint2 *)(mp + m_c), cpad, h->stream);
-    for (size_t c = 0; c < classes.size() && ce == hipSuccess; c++) {
-        ce = launch_jobs_search(op, M, classes[c], sa_dev, (const uint2 *)(mp + m_wg) + wg_off[c], wg_cnt[c], h->stream);
-        if (ce == hipSuccess && search_first.empty()) ce = last_nodes(search_first);
-    }
+    if (ce == hipSuccess) ce = launch_jobs_search(op, M, sa_dev, (const uint2 *)(mp + m_wg), wg_all.size(), h->stream);
+    if (ce == hipSuccess) ce = last_nodes(search_first);
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess && op == OP_F32 && part_words)
         ce = launch_jobs_merge_f32(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);

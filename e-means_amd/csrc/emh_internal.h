@@ -176,11 +176,9 @@ template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size
 // entries together); operand, norms, offsets with this step's bias, stats
 // (zeroed before).
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s);
-// search of every job in one launch per row-width class: wg holds
-// (job, first row | slice) per workgroup (first rows are multiples of 256,
-// slices below 256); all jobs of the call share row_bytes.
-hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_args, const uint2 *wg, size_t nwg,
-                              hipStream_t s);
+// search of every job in one launch: wg holds (job, first row | slice) per
+// workgroup (first rows are multiples of 256, slices below 256).
+hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint2 *wg, size_t nwg, hipStream_t s);
 // rescoring of every job's candidates (merging a split job's slice lists
 // first): rmap holds (job, row) per batch slot.
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);

@@ -762,10 +762,12 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
 }
 
 // One workgroup's sweep: rows [row0, row0 + 128 FR) of a.x against every
-// centroid tile.
+// centroid tile. LDS: three tile slots sC0..2 (TILE bytes each), three
+// offset slots sH0..2 (256 floats) and sRed (4 floats).
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
-__device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_t row0, const int t0, const int t1,
-                                                const int slice) {
+__device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_t row0, const int t0, const int t1,
+                                                const int slice, uint8_t *sC0, uint8_t *sC1, uint8_t *sC2,
+                                                float *sH0, float *sH1, float *sH2, float *sRed) {
     constexpr int ES = OpT<OP>::ES;
     constexpr int RB = 32 * NS;           // operand bytes per row
     constexpr int HB = RB / 2;            // bytes per lane half
@@ -777,11 +779,6 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     using Score = typename RegScore<OP>::T;
     static_assert(NS % 2 == 0 && PER_T >= 1 && TILE % 4096 == 0, "shape");
     static_assert(OP != OP_I16 || NB == 3, "an I16 tile is three planes");
-    __shared__ __align__(16) uint8_t sC0[TILE];
-    __shared__ __align__(16) uint8_t sC1[TILE];
-    __shared__ __align__(16) uint8_t sC2[TILE];
-    __shared__ __align__(16) float sH0[256], sH1[256], sH2[256];
-    __shared__ float sRed[4];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -1042,6 +1039,19 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     }
 }
 
+// search_reg_core on this instantiation's own LDS
+template <int OP, int ROWS, int M, int NS, int FR, int NB>
+__device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_t row0, const int t0, const int t1,
+                                                const int slice) {
+    constexpr int TILE = 32 * NB * 32 * NS;
+    __shared__ __align__(16) uint8_t sC0[TILE];
+    __shared__ __align__(16) uint8_t sC1[TILE];
+    __shared__ __align__(16) uint8_t sC2[TILE];
+    __shared__ __align__(16) float sH0[256], sH1[256], sH2[256];
+    __shared__ float sRed[4];
+    search_reg_core<OP, ROWS, M, NS, FR, NB>(a, row0, t0, t1, slice, sC0, sC1, sC2, sH0, sH1, sH2, sRed);
+}
+
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __global__ __launch_bounds__(256) void search_reg_kernel(SearchArgs a) {
     const int ntiles = (int)(a.k_pad / reg_centroids<OP, NB>());
@@ -1106,11 +1116,31 @@ __global__ void merge_slices_kernel(const uint32_t *part, size_t m, int nslice, 
 }
 
 // Many independent problems in one launch: workgroup b sweeps rows
-// [wg[b].y, + 128 FR) of problem wg[b].x, whose arguments (row count of this
-// step included) sit in a device table. Problems with fewer rows this step
-// leave their spare workgroups idle.
-template <int OP, int ROWS, int M, int NS, int FR, int NB>
+// [wg[b].y & ~255, + 256) of problem wg[b].x against slice wg[b].y & 255 of
+// its centroid tiles; the problem's arguments (row count of this step
+// included) sit in a device table. Every row width runs in the same launch
+// (the workgroup picks its instantiation from the problem's row bytes), so
+// the jobs of all widths fill the chip together; the LDS is one block sized
+// for the largest tile. Problems with fewer rows this step leave their spare
+// workgroups idle.
+constexpr int JOBS_TILE_MAX = 16384;  // 64 centroids x 256 B, or 32 x 512 B
+
+template <int OP, int M, int NS>
+__device__ __forceinline__ void jobs_sweep(const SearchArgs &a, size_t row0, int s, uint8_t (*sC)[JOBS_TILE_MAX],
+                                           float (*sH)[256], float *sRed) {
+    constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
+    static_assert(32 * NB * 32 * NS <= JOBS_TILE_MAX, "tile");
+    const int ntiles = (int)(a.k_pad / (32 * NB));
+    const int t0 = s * a.tiles_per_slice, t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
+    search_reg_core<OP, EMH_ROWS_F32, M, NS, FR, NB>(a, row0, t0, t1, s, sC[0], sC[1], sC[2], sH[0], sH[1], sH[2],
+                                                      sRed);
+}
+
+template <int OP, int M>
 __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *jobs, const uint2 *wg) {
+    __shared__ __align__(16) uint8_t sC[3][JOBS_TILE_MAX];
+    __shared__ __align__(16) float sH[3][256];
+    __shared__ float sRed[4];
     const uint2 w = wg[blockIdx.x];
     // one job per workgroup: its table pointers are uniform and global
     SearchArgs a = jobs[w.x];
@@ -1124,10 +1154,16 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
     a.dist = global_ptr_s(a.dist);
     const size_t row0 = w.y & ~255u;
     if (row0 >= a.m) return;  // whole workgroup: before any barrier
-    // slice s of the job's centroid-range split: tiles [s * per, + per)
-    const int s = (int)(w.y & 255u), ntiles = (int)(a.k_pad / (32 * NB));
-    const int t0 = s * a.tiles_per_slice, t1 = t0 + a.tiles_per_slice < ntiles ? t0 + a.tiles_per_slice : ntiles;
-    search_reg_body<OP, ROWS, M, NS, FR, NB>(a, row0, t0, t1, s);
+    const int s = (int)(w.y & 255u);  // the slice
+    switch (__builtin_amdgcn_readfirstlane(a.row_bytes) / 32) {
+        case 2: jobs_sweep<OP, M, 2>(a, row0, s, sC, sH, sRed); break;
+        case 4: jobs_sweep<OP, M, 4>(a, row0, s, sC, sH, sRed); break;
+        case 6: jobs_sweep<OP, M, 6>(a, row0, s, sC, sH, sRed); break;
+        case 8: jobs_sweep<OP, M, 8>(a, row0, s, sC, sH, sRed); break;
+        case 12: jobs_sweep<OP, M, 12>(a, row0, s, sC, sH, sRed); break;
+        case 16: jobs_sweep<OP, M, 16>(a, row0, s, sC, sH, sRed); break;
+        default: break;
+    }
 }
 
 // Copies the row-major operand table into the tile-major order of the
@@ -1972,40 +2008,25 @@ hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_
     return hipGetLastError();
 }
 
-template <int OP, int M, int NS>
-static hipError_t launch_jobs_search_ns(const void *args, const uint2 *wg, size_t nwg, hipStream_t s) {
-    constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
-    hipLaunchKernelGGL((search_reg_jobs_kernel<OP, EMH_ROWS_F32, M, NS, FR, NB>), dim3((unsigned)nwg), dim3(256), 0, s,
-                       (const SearchArgs *)args, wg);
+template <int OP, int M>
+static hipError_t launch_jobs_search_m(const void *args, const uint2 *wg, size_t nwg, hipStream_t s) {
+    hipLaunchKernelGGL((search_reg_jobs_kernel<OP, M>), dim3((unsigned)nwg), dim3(256), 0, s, (const SearchArgs *)args,
+                       wg);
     return hipGetLastError();
 }
 
-template <int OP, int M>
-static hipError_t launch_jobs_search_m(int row_bytes, const void *args, const uint2 *wg, size_t nwg, hipStream_t s) {
-    switch (row_bytes / 32) {
-        case 2: return launch_jobs_search_ns<OP, M, 2>(args, wg, nwg, s);
-        case 4: return launch_jobs_search_ns<OP, M, 4>(args, wg, nwg, s);
-        case 6: return launch_jobs_search_ns<OP, M, 6>(args, wg, nwg, s);
-        case 8: return launch_jobs_search_ns<OP, M, 8>(args, wg, nwg, s);
-        case 12: return launch_jobs_search_ns<OP, M, 12>(args, wg, nwg, s);
-        case 16: return launch_jobs_search_ns<OP, M, 16>(args, wg, nwg, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-hipError_t launch_jobs_search(int op, int M, int row_bytes, const void *search_args, const uint2 *wg, size_t nwg,
-                              hipStream_t s) {
+hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint2 *wg, size_t nwg, hipStream_t s) {
     if (!nwg) return hipSuccess;
     if (op == OP_F32) {
         if (M != 1) return hipErrorInvalidValue;
-        return launch_jobs_search_m<OP_F32, 1>(row_bytes, search_args, wg, nwg, s);
+        return launch_jobs_search_m<OP_F32, 1>(search_args, wg, nwg, s);
     }
     if (op != OP_BF16) return hipErrorInvalidValue;
     switch (M) {
-        case 1: return launch_jobs_search_m<OP_BF16, 1>(row_bytes, search_args, wg, nwg, s);
-        case 2: return launch_jobs_search_m<OP_BF16, 2>(row_bytes, search_args, wg, nwg, s);
-        case 4: return launch_jobs_search_m<OP_BF16, 4>(row_bytes, search_args, wg, nwg, s);
-        case 8: return launch_jobs_search_m<OP_BF16, 8>(row_bytes, search_args, wg, nwg, s);
+        case 1: return launch_jobs_search_m<OP_BF16, 1>(search_args, wg, nwg, s);
+        case 2: return launch_jobs_search_m<OP_BF16, 2>(search_args, wg, nwg, s);
+        case 4: return launch_jobs_search_m<OP_BF16, 4>(search_args, wg, nwg, s);
+        case 8: return launch_jobs_search_m<OP_BF16, 8>(search_args, wg, nwg, s);
         default: return hipErrorInvalidValue;
     }
 }
