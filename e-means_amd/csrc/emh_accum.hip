@@ -23,6 +23,16 @@ __global__ void fill_u32_kernel(uint32_t *p, size_t count, uint32_t v) {
         p[i] = v;
 }
 
+// Zeroes three word ranges in one launch.
+__global__ void zero3_kernel(uint32_t *a, size_t na, uint32_t *b, size_t nb, uint32_t *c, size_t nc) {
+    const size_t total = na + nb + nc;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        if (i < na) a[i] = 0u;
+        else if (i < na + nb) b[i - na] = 0u;
+        else c[i - na - nb] = 0u;
+    }
+}
+
 __global__ void fill_f64_kernel(double *p, size_t count) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
         p[i] = 0.0;
@@ -583,10 +593,12 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
                              const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s) {
     hipError_t e;
     if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
-    if ((e = launch_fill_u32(counts, k, 0u, s)) != hipSuccess) return e;
-    if ((e = launch_fill_u32((uint32_t *)sums, k * d, 0u, s)) != hipSuccess) return e;  // +0.0f
+    // counts, sums (+0.0f) and the big-cluster count start at zero
+    const size_t zn = k + k * d + 1;
+    hipLaunchKernelGGL(zero3_kernel, dim3((unsigned)std::min<size_t>((zn + 255) / 256, 4096)), dim3(256), 0, s, counts,
+                       k, (uint32_t *)sums, k * d, a.big, (size_t)1);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (m == 0) return hipSuccess;
-    if ((e = launch_fill_u32(a.big, 1, 0u, s)) != hipSuccess) return e;
     const uint32_t *sorted_keys, *sorted_src;
     if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess) return e;
     const dim3 grid((unsigned)(((m + SUM_SPAN - 1) / SUM_SPAN + 3) / 4)), bgrid(BIG_SUM_BLOCKS);

@@ -1366,11 +1366,13 @@ __device__ inline float centroid_norm32(const float *cp, size_t d, int lane) {
 // of the rounded i8 centroid (src/cpu_i8.rs:83-93).
 constexpr int PREP_CENTROIDS = 16;
 
+// tile_op (register-resident path, else nullptr): the same quads in the
+// tile-major order, quad q of centroid c of tile t at quad (t nq + q) tb + c.
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
                                                              int64_t *norm64, const unsigned *cmax_bits,
-                                                             unsigned *stats) {
+                                                             unsigned *stats, uint8_t *tile_op, int tb) {
     constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
     const int gl = threadIdx.x & 15;
     const size_t jc = (size_t)blockIdx.x * PREP_CENTROIDS + (threadIdx.x >> 4);  // < k_pad (a multiple of 64)
@@ -1419,7 +1421,12 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
                 part64 += (int64_t)qv * qv;
             }
         }
-        *(uint4 *)(orow + q * 16) = make_uint4(w[0], w[1], w[2], w[3]);
+        const uint4 quad = make_uint4(w[0], w[1], w[2], w[3]);
+        *(uint4 *)(orow + q * 16) = quad;
+        if (tile_op) {
+            const size_t nq = d_pad / EPQ, t = jc / (size_t)tb, cl = jc - t * (size_t)tb;
+            *(uint4 *)(tile_op + ((t * nq + q) * (size_t)tb + cl) * 16) = quad;
+        }
     }
     if constexpr (OP == OP_I8) {
         for (int off = 8; off > 0; off >>= 1) part64 += shfl_xor_i64(part64, off);
@@ -1927,16 +1934,20 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
                                  const unsigned *cmax, unsigned *stats, hipStream_t s) {
     const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;
+    // the register-resident path's tile-major copy, written by the same
+    // kernel (I16's three planes go through tile_layout_kernel)
+    uint8_t *tile = g.reg && g.op != OP_I16 ? (uint8_t *)reg_operand(g, cen_op) : nullptr;
+    const int tbc = g.reg ? reg_tile(g) : 1;
     switch (g.op) {
-        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
-        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
-        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
-        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats); break;
+        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
+        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
+        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
+        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
         case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid_wave, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
         default: return hipErrorInvalidValue;
     }
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess || !g.reg) return e;
+    if (e != hipSuccess || !g.reg || g.op != OP_I16) return e;
     const int nq = g.row_bytes / 16, tb = reg_tile(g), planes = g.op == OP_I16 ? 3 : 1;
     const size_t quads = (size_t)planes * g.k_pad * (size_t)nq;
     hipLaunchKernelGGL(tile_layout_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s,
