@@ -189,6 +189,26 @@ def test_batched_slices_and_big_clusters_equal_single(hip, prec):
         assert g.n_splits == ref.n_splits
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batched_edge_jobs_equal_single(hip, prec):
+    """Edge shapes in one batch: a single cluster, as many rows as clusters,
+    three- and five-wide rows (row lengths not a multiple of four), a row
+    width whose operand rows round up to 384 bytes (BF16; FP32 stays at
+    d <= 128), tiny batches beside a large one."""
+    wide = 130 if prec == "bf16" else 120
+    specs = [(50, 3, 1, 0.0), (64, 7, 64, 0.0), (1000, wide, 700, 0.1), (300, 5, 2, 0.1), (30000, 32, 40, 0.0)]
+    problems = [(_job_data(70 + i, n, d, max(2, 2 * k)), k, E.Config(nepochs=2, balance=bal, seed=60 + i))
+                for i, (n, d, k, bal) in enumerate(specs)]
+    print(f"batch edge jobs {prec}", flush=True)
+    got, _ = E.train_batch(problems, E.HipAssigner(prec))
+    for (x, k, cfg), g in zip(problems, got):
+        ref = E.train(x, k, cfg, E.HipAssigner(prec))
+        assert np.array_equal(g.labels, ref.labels)
+        assert np.array_equal(g.centroids, ref.centroids)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+
+
 @pytest.mark.parametrize("prec", ["fp32", "i8"])
 def test_device_rows_training_equals_host_rows(hip, prec):
     """emh_train_device (rows already in HBM, initial centroids gathered on
