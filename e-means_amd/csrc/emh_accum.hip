@@ -632,7 +632,7 @@ hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rm
     if ((e = accum_reserve(a, slots > 0 ? slots : 1, nclusters)) != hipSuccess) return e;
     dim3 cgrid((unsigned)((nclusters + 3) / 4));
     hipLaunchKernelGGL(jobs_pass_clear_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters);
-    if ((e = launch_fill_u32(counts_all, nclusters, 0u, s)) != hipSuccess) return e;
+    (void)counts_all;  // zero here: set up so, and left so by jobs_ema_kernel
     if (slots == 0) return hipSuccess;
     unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
     hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters, keys,
@@ -695,17 +695,27 @@ __global__ __launch_bounds__(256) void ema_apply_kernel(float *cen, float *usage
     ema_cluster(cen, usage, sums, counts, j, d, alpha, decay, replace, threadIdx.x & 63);
 }
 
-__global__ __launch_bounds__(256) void jobs_ema_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters) {
+// The step's last kernel also leaves the next step's accumulators at zero:
+// each cluster's count once the EMA has read it, and (workgroup 0) the
+// jobs' stats words, which the search has read.
+__global__ __launch_bounds__(256) void jobs_ema_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters,
+                                                       unsigned *stats, size_t nstats) {
+    if (blockIdx.x == 0)
+        for (size_t i = threadIdx.x; i < nstats; i += 256) stats[i] = 0u;
     const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= nclusters) return;
     const JobDev j = job_at<true>(jobs, cjob[g]);
     if (j.m == 0) return;
-    ema_cluster(j.cen, j.usage, j.sums, j.counts, g - j.cluster_base, j.d, j.alpha, j.decay, (int)j.replace,
-                threadIdx.x & 63);
+    const int lane = threadIdx.x & 63;
+    const size_t c = g - j.cluster_base;
+    ema_cluster(j.cen, j.usage, j.sums, j.counts, c, j.d, j.alpha, j.decay, (int)j.replace, lane);
+    if (lane == 0) j.counts[c] = 0u;
 }
 
-hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s) {
-    hipLaunchKernelGGL(jobs_ema_kernel, dim3((unsigned)((nclusters + 3) / 4)), dim3(256), 0, s, jobs, cjob, nclusters);
+hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, unsigned *stats, size_t nstats,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(jobs_ema_kernel, dim3((unsigned)((nclusters + 3) / 4)), dim3(256), 0, s, jobs, cjob, nclusters,
+                       stats, nstats);
     return hipGetLastError();
 }
 

@@ -229,7 +229,10 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (part_words && (rc = check(part_d.ensure(part_words * 4), "slice results"))) ||
         (rc = check(lab_d.ensure(slots * 4), "labels")) ||
         (rc = check(labels_all.ensure(rows_total * 4), "last-pass labels")) ||
-        (rc = check(launch_fill_u32(labels_all.as<uint32_t>(), rows_total, 0xFFFFFFFFu, h->stream), "labels")))
+        (rc = check(launch_fill_u32(labels_all.as<uint32_t>(), rows_total, 0xFFFFFFFFu, h->stream), "labels")) ||
+        // the step graph expects these at zero; its last kernel leaves them so
+        (rc = check(launch_fill_u32(stats_d.as<uint32_t>(), 4 * njobs, 0u, h->stream), "stats")) ||
+        (rc = check(launch_fill_u32(counts_d.as<uint32_t>(), nclusters, 0u, h->stream), "counts")))
         return rc;
     memset(in_h.p, 0, in.bytes);
     uint8_t *in_dp = (uint8_t *)in_d.p, *out_dp = (uint8_t *)out_d.p;
@@ -380,7 +383,6 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     // tables' upload and the prep kernel; the first search waits for them
     std::vector<hipGraphNode_t> search_first;
     hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in_ids, hipMemcpyHostToDevice, h->stream);
-    if (ce == hipSuccess) ce = launch_fill_u32(stats_d.as<uint32_t>(), 4 * njobs, 0u, h->stream);
     if (ce == hipSuccess) ce = launch_jobs_prep(op, jobs_dev, (const uint2 *)(mp + m_c), cpad, h->stream);
     if (ce == hipSuccess) ce = launch_jobs_search(op, M, sa_dev, (const uint2 *)(mp + m_wg), wg_all.size(), h->stream);
     if (ce == hipSuccess) ce = last_nodes(search_first);
@@ -397,7 +399,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess)
         ce = launch_jobs_sums(h->acc, jobs_dev, slots, (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j),
                               nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
-    if (ce == hipSuccess) ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, h->stream);
+    if (ce == hipSuccess)
+        ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, stats_d.as<unsigned>(), 4 * njobs,
+                             h->stream);
     if (ce == hipSuccess)
         ce = hipMemcpyAsync(out_hp + out_use, out_dp + out_use, outl.bytes - out_use, hipMemcpyDeviceToHost, h->stream);
     hipError_t ee = hipStreamEndCapture(h->stream, &gg.graph);

@@ -226,10 +226,13 @@ hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t 
 // Revival row splits of many jobs: triples (job, donor, unused).
 hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_t ntriples, hipStream_t s);
 // The EMA step of every job still running, one wave per global cluster.
-hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, hipStream_t s);
+// It also zeroes each cluster's count after reading it and (nstats words)
+// the jobs' stats, for the next step.
+hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclusters, unsigned *stats, size_t nstats,
+                           hipStream_t s);
 // Batched, first half: the last epoch's first batch zeroes its FP64 totals;
 // keys = cluster_base + label for live slots (nclusters for spare slots),
-// counts_all zeroed; penalized jobs' distances lose their bias (the
+// counts_all (all zero on entry); penalized jobs' distances lose their bias (the
 // true squared distances, src/lib.rs:425-430); last-epoch labels are stored
 // at their rows in labels_all.
 hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
