@@ -15,6 +15,7 @@
 #include <atomic>
 #include <chrono>
 #include <functional>
+#include <future>
 #include <thread>
 
 #include "emh_recipe.h"
@@ -455,6 +456,40 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         }
         J.advance();
     };
+    // Each job's closing mean (src/lib.rs:481-489, from its FP64 pass totals
+    // over its final EMA centroids) and the download of its centroids and
+    // last-pass labels run on a closing stream, from a background thread, as
+    // soon as its last step (and that step's revivals) is done: later steps
+    // never touch a finished job's data, and the copies into the caller's
+    // arrays block only that thread.
+    float *cen_dev = cen_model.as<float>();
+    SideStream closer;
+    if ((rc = check(closer.init(), "closing stream"))) return rc;
+    const auto close_job = [&](size_t j) -> std::pair<int, std::string> {
+        int r;
+        const size_t k = jobs[j].k, d = jobs[j].d;
+        if ((r = check(hipSetDevice(h->device), "hipSetDevice")) ||
+            (r = check(launch_closing_mean(jd[j].pass_sums, jd[j].pass_counts, k, d, cen_dev + cen_base[j], closer.s),
+                       "closing mean")) ||
+            (r = check(hipMemcpyAsync(jobs[j].centroids_out, cen_dev + cen_base[j], k * d * 4, hipMemcpyDeviceToHost,
+                                      closer.s),
+                       "copy centroids")) ||
+            (r = check(hipMemcpyAsync(jobs[j].labels_out, labels_all.as<uint32_t>() + nrow_base[j], jobs[j].n * 4,
+                                      hipMemcpyDeviceToHost, closer.s),
+                       "copy labels")) ||
+            (r = check(hipStreamSynchronize(closer.s), "closing mean")))
+            return {r, emh_last_error()};
+        return {(int)EMH_OK, std::string()};
+    };
+    std::vector<char> closed(njobs, 0);
+    std::vector<std::future<std::pair<int, std::string>>> closing;
+    const auto close_finished = [&]() {
+        for (size_t j = 0; j < njobs; j++)
+            if (!live[j] && !closed[j]) {
+                closed[j] = 1;
+                closing.push_back(std::async(std::launch::async, close_job, j));
+            }
+    };
     double host_s = 0.0, wait_s = 0.0, launch_s = 0.0;
     using clk = std::chrono::steady_clock;
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
@@ -508,24 +543,15 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
              (rc = check(launch_jobs_revive(jobs_dev, pairs_d.as<uint32_t>(), ntrip, h->stream), "revival"))))
             return rc;
         if (revived && (rc = check(hipStreamSynchronize(h->stream), "revival"))) return rc;
+        close_finished();
     }
 
-    // Closing means (src/lib.rs:481-489) from the FP64 pass totals, per job,
-    // over the final EMA centroids.
-    float *cen_dev = cen_model.as<float>();
-    for (size_t j = 0; j < njobs; j++) {
-        const size_t k = jobs[j].k, d = jobs[j].d;
-        if ((rc = check(launch_closing_mean(jd[j].pass_sums, jd[j].pass_counts, k, d, cen_dev + cen_base[j], h->stream),
-                        "closing mean")) ||
-            (rc = check(hipMemcpyAsync(jobs[j].centroids_out, cen_dev + cen_base[j], k * d * 4, hipMemcpyDeviceToHost,
-                                       h->stream),
-                        "copy centroids")) ||
-            (rc = check(hipMemcpyAsync(jobs[j].labels_out, labels_all.as<uint32_t>() + nrow_base[j], jobs[j].n * 4,
-                                       hipMemcpyDeviceToHost, h->stream),
-                        "copy labels")))
-            return rc;
+    close_finished();  // jobs that never stepped
+    for (auto &f : closing) {
+        const auto r = f.get();
+        if (r.first != EMH_OK && rc == EMH_OK) rc = fail(r.first, r.second);
     }
-    if ((rc = check(hipStreamSynchronize(h->stream), "closing mean"))) return rc;
+    if (rc) return rc;
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (size_t j = 0; j < njobs; j++) {
         jobs[j].out->n_splits = run[j].st.n_splits;
