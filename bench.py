@@ -541,7 +541,10 @@ def train_batch_line(args, njobs, prec, m):
     rows = sum(cfg.nepochs * x.shape[0] for x, _, cfg in probs)
     work = sum(2.0 * cfg.nepochs * x.shape[0] * k * x.shape[1] for x, k, cfg in probs)
     log("batched warmup")
-    E.train_batch([(x[:2 * k], k, E.Config(nepochs=1.0)) for x, k, _ in probs[:4]], E.HipAssigner(prec, rescore_m=m))
+    # the same jobs for one epoch: first-call costs (allocations, first
+    # launches of every shape) stay out of the timed call
+    E.train_batch([(x, k, E.Config(nepochs=1.0, balance=cfg.balance, seed=cfg.seed)) for x, k, cfg in probs],
+                  E.HipAssigner(prec, rescore_m=m))
     log(f"batched {prec} ({njobs} jobs)")
     t = time.perf_counter()
     got, st = E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
@@ -549,6 +552,8 @@ def train_batch_line(args, njobs, prec, m):
     rel, ls, loop_steps = [0.0], float("nan"), 0
     if not args.no_compare:
         log("per-job fp32 loop")
+        for x, k, cfg in probs:  # the same warm-up: each job's shapes once, one epoch
+            E.train(x, k, E.Config(nepochs=1.0, balance=cfg.balance, seed=cfg.seed), E.HipAssigner("fp32"))
         t = time.perf_counter()
         ref = [E.train(x, k, cfg, E.HipAssigner("fp32")) for x, k, cfg in probs]
         ls = time.perf_counter() - t
@@ -610,7 +615,11 @@ def train_large_line(args, x, k, nepochs, prec, m):
     xh = x.cpu().numpy()
     n, d = xh.shape
     log(f"train {prec} warmup")
-    E.train(x[:65536], 1024, E.Config(nepochs=1), E.HipAssigner(prec, rescore_m=m))
+    # the same k and batch size for two batches: the search's per-shape
+    # split choice and first-call costs stay out of the timed call
+    bs = E.resolve_recipe(n, d, k, E.Config(nepochs=nepochs)).batch_size
+    for p, mm in ((prec, m), ("fp32", 1)):
+        E.train(x[: min(n, max(2 * bs, k))], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(p, rescore_m=mm))
     runs = {}
     for name, p, mm in ((prec, prec, m), ("fp32", "fp32", 1)):
         log(f"train {name}")
@@ -669,7 +678,9 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     n, d = xh.shape
     cfg = E.Config(nepochs=nepochs)
     log(f"train {prec} warmup")
-    E.train(x[: min(n, 50 * k)], k, E.Config(nepochs=1), E.HipAssigner(prec))
+    # the same k and batch size: first-call costs stay out of the timed call
+    bs = E.resolve_recipe(n, d, k, cfg).batch_size
+    E.train(x[: min(n, max(2 * bs, k))], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(prec))
     log(f"train {prec}")
     t = time.perf_counter()
     out = E.train(x, k, cfg, E.HipAssigner(prec))  # rows already in HBM
