@@ -130,9 +130,10 @@ __global__ __launch_bounds__(256) void digit_hist_kernel(const uint32_t *keys, s
     hist[(size_t)threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of the digit-major histogram (hist[digit * nblocks + block])
-// in two launches of 256 workgroups: each digit's row over the blocks, then
-// each row plus the total of the smaller digits.
+// Exclusive scan of the digit-major histogram (hist[digit * nblocks + block]):
+// one launch of 256 workgroups scans each digit's row over the blocks and
+// records the digit's total; the scatter adds the total of the smaller
+// digits.
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t *sm, uint32_t *total) {
     // 256 threads: wave prefix by shuffles, then the 4 wave totals
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -165,26 +166,28 @@ __global__ __launch_bounds__(256) void digit_row_scan_kernel(uint32_t *hist, siz
     if (threadIdx.x == 0) totals[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(256) void digit_base_kernel(uint32_t *hist, size_t nblocks, const uint32_t *totals) {
-    __shared__ uint32_t sm[4];
-    const uint32_t v = threadIdx.x < blockIdx.x ? totals[threadIdx.x] : 0u;
-    uint32_t base;
-    (void)block_exclusive_scan(v, sm, &base);  // base = sum of the smaller digits' totals
-    uint32_t *row = hist + (size_t)blockIdx.x * nblocks;
-    for (size_t b = threadIdx.x; b < nblocks; b += 256) row[b] += base;
-}
 
 // Stable scatter: an element's destination is the start of its digit in its
 // block (scanned histogram) plus the number of equal digits before it in the
 // block. Inside a wave the equal digits are found with one ballot per digit
 // bit; waves and rounds are ordered through LDS counters, so the block keeps
 // its input order.
+// offs holds each digit's row scanned over the blocks (digit_row_scan_kernel);
+// the digit's base (the total of the smaller digits) is added here from the
+// digit totals.
 __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
                                                             int shift, const uint32_t *offs, size_t nblocks,
-                                                            uint32_t *keys_out, uint32_t *vals_out) {
+                                                            const uint32_t *totals, uint32_t *keys_out,
+                                                            uint32_t *vals_out) {
     __shared__ uint32_t run[256];      // equal digits in the rounds before this one
     __shared__ uint32_t wcnt[4][256];  // per wave, this round
+    __shared__ uint32_t sbase[256];    // total of the smaller digits
+    __shared__ uint32_t ssm[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    {
+        uint32_t all;
+        sbase[tid] = block_exclusive_scan(totals[tid], ssm, &all);
+    }
     run[tid] = 0;
     const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -204,7 +207,7 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
         if (live && before == 0) wcnt[w][dg] = (uint32_t)__popcll(same);
         __syncthreads();
         if (live) {
-            uint32_t pos = offs[(size_t)dg * nblocks + blockIdx.x] + run[dg] + before;
+            uint32_t pos = sbase[dg] + offs[(size_t)dg * nblocks + blockIdx.x] + run[dg] + before;
             for (int q = 0; q < w; q++) pos += wcnt[q][dg];
             keys_out[pos] = key;
             vals_out[pos] = vals ? vals[i] : (uint32_t)i;
@@ -262,10 +265,9 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, shift, hist, nblocks);
         hipLaunchKernelGGL(digit_row_scan_kernel, dim3(256), dim3(256), 0, s, hist, nblocks, hist + 256 * nblocks);
-        hipLaunchKernelGGL(digit_base_kernel, dim3(256), dim3(256), 0, s, hist, nblocks,
-                           (const uint32_t *)(hist + 256 * nblocks));
         hipLaunchKernelGGL(digit_scatter_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m, shift,
-                           (const uint32_t *)hist, nblocks, kb[cur], vb[cur]);
+                           (const uint32_t *)hist, nblocks, (const uint32_t *)(hist + 256 * nblocks), kb[cur],
+                           vb[cur]);
         kin = kb[cur];
         vin = vb[cur];
         cur ^= 1;
