@@ -4,14 +4,15 @@
 // (src/lib.rs:165-491): recipe laws, xoshiro256++ batch order, step
 // schedule, balance bias and the revival plan. What moves to the device is
 // the data path of each batch and the model it updates: the rows are
-// uploaded once and per batch only the ids (4 bytes per row) go up; the
+// uploaded once and per epoch only its batch order (4 bytes per row); the
 // gather, the search, the per-cluster sums/counts and the EMA step (f32
 // operations in the host loop's order, so bit-identical) run on the GPU
 // against device-resident centroids; the last pass's totals accumulate in
-// FP64 on the GPU for the closing mean. Labels and distances come down per
-// batch (objective, last-pass labels), the usage vector only when the bias
-// or a revival needs it; without the balance penalty the host folds them in
-// a few batches later, so staging and launching overlap the GPU's work.
+// FP64 on the GPU for the closing mean. Each batch's ids are a slice of
+// the uploaded order; labels and distances come down per batch in one copy
+// (objective, last-pass labels), the usage vector only when the bias or a
+// revival needs it; without the balance penalty the host folds them in a
+// few batches later, so staging and launching overlap the GPU's work.
 #include <chrono>
 
 #include "emh_recipe.h"
@@ -36,15 +37,19 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     // buffers. The EMA step runs on the device; the host keeps the recipe
     // and pulls the usage vector down when the balance bias or a revival
     // needs it.
-    DevBuf data, ids_d, lab_d, dist_d, sums_d, counts_d, pass_d, passn_d, cent_d, usage_d, pairs_d;
+    DevBuf data, ids_d, out_d, sums_d, counts_d, pass_d, passn_d, cent_d, usage_d, pairs_d;
     if (!rows_on_device &&
         ((rc = check(data.ensure(n * d * rb), "resident rows")) ||
          (rc = check(hipMemcpyAsync(data.p, x, n * d * rb, hipMemcpyHostToDevice, h->stream), "upload rows"))))
         return rc;
     const void *rows_dev = rows_on_device ? x : data.p;
-    if ((rc = check(ids_d.ensure(batch * 4), "ids"))) return rc;
-    if ((rc = check(lab_d.ensure(batch * 4), "labels"))) return rc;
-    if ((rc = check(dist_d.ensure(batch * 4), "distances"))) return rc;
+    size_t epoch_max = batch;
+    for (size_t e : J.R.epoch_rows) epoch_max = std::max(epoch_max, e);
+    if ((rc = check(ids_d.ensure(epoch_max * 4), "epoch order"))) return rc;
+    // labels [0, batch) then distances [batch, 2 batch): one download a batch
+    if ((rc = check(out_d.ensure(2 * batch * 4), "labels and distances"))) return rc;
+    uint32_t *lab_dev = out_d.as<uint32_t>();
+    float *dist_dev = out_d.as<float>() + batch;
     if ((rc = check(sums_d.ensure(k * d * 4), "sums"))) return rc;
     if ((rc = check(counts_d.ensure(k * 4), "counts"))) return rc;
     if ((rc = check(pass_d.ensure(k * d * 8), "pass sums"))) return rc;
@@ -52,10 +57,9 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     if ((rc = check(cent_d.ensure(k * d * 4), "centroids"))) return rc;
     if ((rc = check(usage_d.ensure(k * 4), "usage"))) return rc;
     if ((rc = check(pairs_d.ensure(k * 8), "revival pairs"))) return rc;
-    HostBuf ids_h, lab_h, dist_h, usage_h, pairs_h;
-    if ((rc = check(ids_h.alloc(batch * 4), "pinned ids"))) return rc;
-    if ((rc = check(lab_h.alloc(batch * 4), "pinned labels"))) return rc;
-    if ((rc = check(dist_h.alloc(batch * 4), "pinned distances"))) return rc;
+    HostBuf ids_h, out_h, usage_h, pairs_h;
+    if ((rc = check(ids_h.alloc(epoch_max * 4), "pinned epoch order"))) return rc;
+    if ((rc = check(out_h.alloc(2 * batch * 4), "pinned labels and distances"))) return rc;
     if ((rc = check(usage_h.alloc(k * 4), "pinned usage"))) return rc;
     if ((rc = check(pairs_h.alloc(k * 8), "pinned pairs"))) return rc;
     memcpy(usage_h.p, J.st.usage.data(), k * 4);
@@ -90,28 +94,29 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     const bool async = cfg.balance <= 0.0f;
     J.deferred = async;
     constexpr int NSLOT = 4;
-    HostBuf ids_s[NSLOT], lab_s[NSLOT], dist_s[NSLOT];
+    HostBuf out_s[NSLOT];
     struct Events {
-        hipEvent_t e[NSLOT] = {};
+        hipEvent_t e[NSLOT] = {}, order = nullptr;  // order: the epoch order's upload
         ~Events() {
             for (hipEvent_t v : e)
                 if (v) (void)hipEventDestroy(v);
+            if (order) (void)hipEventDestroy(order);
         }
     } ev;
     JobRun::StepMeta meta[NSLOT];
     int pend_head = 0, pend_n = 0;  // pending slots, oldest first
     if (async)
         for (int i = 0; i < NSLOT; i++)
-            if ((rc = check(ids_s[i].alloc(batch * 4), "pinned ids")) ||
-                (rc = check(lab_s[i].alloc(batch * 4), "pinned labels")) ||
-                (rc = check(dist_s[i].alloc(batch * 4), "pinned distances")) ||
+            if ((rc = check(out_s[i].alloc(2 * batch * 4), "pinned labels and distances")) ||
                 (rc = check(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming), "event")))
                 return rc;
+    if ((rc = check(hipEventCreateWithFlags(&ev.order, hipEventDisableTiming), "event"))) return rc;
+    bool order_pending = false;
     auto drain_one = [&]() -> int {
         const int sl = pend_head;
         int r;
         if ((r = check(hipEventSynchronize(ev.e[sl]), "batch"))) return r;
-        J.finish_stats_deferred(meta[sl], lab_s[sl].as<uint32_t>(), dist_s[sl].as<float>());
+        J.finish_stats_deferred(meta[sl], out_s[sl].as<uint32_t>(), out_s[sl].as<float>() + batch);
         pend_head = (pend_head + 1) % NSLOT;
         pend_n--;
         return (int)EMH_OK;
@@ -134,15 +139,25 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
         if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true))) return rc;
         if (J.penalized && (rc = emh_set_bias(h, J.bias.data(), k))) return rc;
-        uint32_t *ids_host = async ? ids_s[sl].as<uint32_t>() : ids_h.as<uint32_t>();
-        memcpy(ids_host, J.ids, nb * 4);
-        if ((rc = check(hipMemcpyAsync(ids_d.p, ids_host, nb * 4, hipMemcpyHostToDevice, h->stream), "copy ids")))
-            return rc;
+        if (J.b0 == 0) {
+            // The epoch's batch order goes up once; batch ids are slices of
+            // it. The device copy is stream-ordered behind the last epoch's
+            // readers; the pinned staging waits for its previous upload.
+            const size_t epoch_n = J.R.epoch_rows[J.epoch];
+            if (order_pending && (rc = check(hipEventSynchronize(ev.order), "epoch order"))) return rc;
+            memcpy(ids_h.p, J.order.data(), epoch_n * 4);
+            if ((rc = check(hipMemcpyAsync(ids_d.p, ids_h.p, epoch_n * 4, hipMemcpyHostToDevice, h->stream),
+                            "copy epoch order")) ||
+                (rc = check(hipEventRecord(ev.order, h->stream), "event")))
+                return rc;
+            order_pending = true;
+        }
+        const uint32_t *ids_dev = ids_d.as<uint32_t>() + J.b0;
         // gather + search on the device (src/lib.rs:419-424)
-        if ((rc = run_search(h, rows_dev, rows, nb, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), dist_d.as<float>())))
+        if ((rc = run_search(h, rows_dev, rows, nb, ids_dev, lab_dev, dist_dev)))
             return rc;
         // per-cluster counts and sums in batch order (src/lib.rs:431-441)
-        if ((rc = check(launch_accumulate(h->acc, rows_dev, rows, nb, d, ids_d.as<uint32_t>(), lab_d.as<uint32_t>(), k,
+        if ((rc = check(launch_accumulate(h->acc, rows_dev, rows, nb, d, ids_dev, lab_dev, k,
                                           sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream),
                         "accumulate")))
             return rc;
@@ -159,8 +174,8 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         const bool revive = J.revive_due();
         const bool need_usage = revive || cfg.balance > 0.0f;
         if (async) {
-            if ((rc = check(hipMemcpyAsync(lab_s[sl].p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
-                (rc = check(hipMemcpyAsync(dist_s[sl].p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
+            if ((rc = check(hipMemcpyAsync(out_s[sl].p, out_d.p, (batch + nb) * 4, hipMemcpyDeviceToHost, h->stream),
+                            "labels and distances")) ||
                 (rc = check(hipEventRecord(ev.e[sl], h->stream), "event")))
                 return rc;
             meta[sl] = J.meta();
@@ -171,14 +186,14 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                 return rc;
             if (revive) memcpy(J.st.usage.data(), usage_h.p, k * 4);
         } else {
-            if ((rc = check(hipMemcpyAsync(lab_h.p, lab_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "labels")) ||
-                (rc = check(hipMemcpyAsync(dist_h.p, dist_d.p, nb * 4, hipMemcpyDeviceToHost, h->stream), "dist")) ||
+            if ((rc = check(hipMemcpyAsync(out_h.p, out_d.p, (batch + nb) * 4, hipMemcpyDeviceToHost, h->stream),
+                            "labels and distances")) ||
                 (need_usage &&
                  (rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage"))) ||
                 (rc = check(hipStreamSynchronize(h->stream), "batch")))
                 return rc;
             if (need_usage) memcpy(J.st.usage.data(), usage_h.p, k * 4);
-            J.finish_stats(lab_h.as<uint32_t>(), dist_h.as<float>());
+            J.finish_stats(out_h.as<uint32_t>(), out_h.as<float>() + batch);
         }
         if (revive) {
             // src/lib.rs:463-465: planned on the host (its RNG), rows split on the device
