@@ -298,6 +298,25 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
     h->has_centroids = true;
     return EMH_OK;
 }
+
+int set_bias_impl(emh_assigner *h, const float *bias, size_t k, bool sync) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_bias: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (!bias || k != h->g.k) return fail(EMH_E_ARG, "set_bias: bias must have k entries");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    if ((rc = check(h->bias.ensure(k * 4), "bias buffer"))) return rc;
+    if ((rc = check(hipMemcpyAsync(h->bias.p, bias, k * 4, hipMemcpyHostToDevice, h->stream), "copy bias"))) return rc;
+    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>() + 1, 2, 0u, h->stream), "fill"))) return rc;
+    if ((rc = check(launch_set_bias(h->g, h->norm32.as<float>(), h->norm64.as<int64_t>(), h->bias.as<float>(),
+                                    h->hn.as<float>(), h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
+                    "score offsets")))
+        return rc;
+    if ((sync || h->op == OP_I8) && (rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;
+    if ((rc = refresh_i8_range(h, true))) return rc;
+    h->has_bias = true;
+    return EMH_OK;
+}
 }  // namespace emh
 
 extern "C" {
@@ -397,24 +416,7 @@ int emh_set_centroids(emh_assigner *h, const float *centroids, size_t k, size_t 
     return set_centroids_impl(h, centroids, k, d, false);
 }
 
-int emh_set_bias(emh_assigner *h, const float *bias, size_t k) {
-    if (!h) return fail(EMH_E_ARG, "emh_set_bias: NULL handle");
-    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
-    if (!bias || k != h->g.k) return fail(EMH_E_ARG, "set_bias: bias must have k entries");
-    int rc;
-    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
-    if ((rc = check(h->bias.ensure(k * 4), "bias buffer"))) return rc;
-    if ((rc = check(hipMemcpyAsync(h->bias.p, bias, k * 4, hipMemcpyHostToDevice, h->stream), "copy bias"))) return rc;
-    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>() + 1, 2, 0u, h->stream), "fill"))) return rc;
-    if ((rc = check(launch_set_bias(h->g, h->norm32.as<float>(), h->norm64.as<int64_t>(), h->bias.as<float>(),
-                                    h->hn.as<float>(), h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
-                    "score offsets")))
-        return rc;
-    if ((rc = check(hipStreamSynchronize(h->stream), "set_bias"))) return rc;
-    if ((rc = refresh_i8_range(h, true))) return rc;
-    h->has_bias = true;
-    return EMH_OK;
-}
+int emh_set_bias(emh_assigner *h, const float *bias, size_t k) { return emh::set_bias_impl(h, bias, k, true); }
 
 int emh_set_split(emh_assigner *h, int nslice) {
     if (!h) return fail(EMH_E_ARG, "emh_set_split: NULL handle");

@@ -81,6 +81,10 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
 int fail(int code, const std::string &msg);
 // set_centroids from host (synchronous) or device (stream-ordered) memory.
 int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src);
+// set_bias from host memory; `sync` false leaves it stream-ordered (the
+// caller keeps `bias`, pinned, unchanged until the stream has passed it),
+// except for the I8 search, which reads its range flag back.
+int set_bias_impl(emh_assigner *h, const float *bias, size_t k, bool sync);
 int check(hipError_t e, const char *what);
 // Run body(stream) on `device` with a stream of its own, destroyed afterwards
 // (the host-pointer entry points that take a device ordinal).

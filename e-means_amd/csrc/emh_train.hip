@@ -57,7 +57,8 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
     if ((rc = check(cent_d.ensure(k * d * 4), "centroids"))) return rc;
     if ((rc = check(usage_d.ensure(k * 4), "usage"))) return rc;
     if ((rc = check(pairs_d.ensure(k * 8), "revival pairs"))) return rc;
-    HostBuf ids_h, out_h, usage_h, pairs_h;
+    HostBuf ids_h, out_h, usage_h, pairs_h, bias_h;
+    if ((rc = check(bias_h.alloc(k * 4), "pinned bias"))) return rc;
     if ((rc = check(ids_h.alloc(epoch_max * 4), "pinned epoch order"))) return rc;
     if ((rc = check(out_h.alloc(2 * batch * 4), "pinned labels and distances"))) return rc;
     if ((rc = check(usage_h.alloc(k * 4), "pinned usage"))) return rc;
@@ -138,7 +139,12 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         }
         // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
         if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true))) return rc;
-        if (J.penalized && (rc = emh_set_bias(h, J.bias.data(), k))) return rc;
+        if (J.penalized) {
+            // pinned and stream-ordered: the step ends with a stream sync
+            // before the next bias is staged (penalized runs are not async)
+            memcpy(bias_h.p, J.bias.data(), k * 4);
+            if ((rc = set_bias_impl(h, bias_h.as<float>(), k, false))) return rc;
+        }
         if (J.b0 == 0) {
             // The epoch's batch order goes up once; batch ids are slices of
             // it. The device copy is stream-ordered behind the last epoch's
