@@ -592,14 +592,18 @@ __global__ __launch_bounds__(256) void big_sum_kernel(const void *x, size_t d, c
 // in batch order (src/lib.rs:431-441): labels sorted stably, carrying their
 // rows (ids[i], or i), then the sorted stream; empty clusters keep zeros.
 hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
-                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s) {
+                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s,
+                             bool cleared) {
     hipError_t e;
+    uint32_t *big_before = a.big;
     if ((e = accum_reserve(a, m > 0 ? m : 1, k)) != hipSuccess) return e;
-    // counts, sums (+0.0f) and the big-cluster count start at zero
-    const size_t zn = k + k * d + 1;
-    hipLaunchKernelGGL(zero3_kernel, dim3((unsigned)std::min<size_t>((zn + 255) / 256, 4096)), dim3(256), 0, s, counts,
-                       k, (uint32_t *)sums, k * d, a.big, (size_t)1);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!cleared || a.big != big_before) {
+        // counts, sums (+0.0f) and the big-cluster count start at zero
+        const size_t zn = k + k * d + 1;
+        hipLaunchKernelGGL(zero3_kernel, dim3((unsigned)std::min<size_t>((zn + 255) / 256, 4096)), dim3(256), 0, s,
+                           counts, k, (uint32_t *)sums, k * d, a.big, (size_t)1);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     if (m == 0) return hipSuccess;
     const uint32_t *sorted_keys, *sorted_src;
     if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess) return e;
@@ -689,12 +693,18 @@ __device__ __forceinline__ void ema_cluster(float *cen, float *usage, const floa
     if (lane == 0) usage[j] = u;
 }
 
+// clear_counts (the counts array itself, or null): each count is zeroed once
+// its wave has read it, and workgroup 0 zeroes *clear_word, so the next
+// step's accumulate needs no clearing launch.
 __global__ __launch_bounds__(256) void ema_apply_kernel(float *cen, float *usage, const float *sums,
                                                         const uint32_t *counts, size_t k, size_t d, float alpha,
-                                                        float decay, int replace) {
+                                                        float decay, int replace, uint32_t *clear_counts,
+                                                        uint32_t *clear_word) {
+    if (clear_word && blockIdx.x == 0 && threadIdx.x == 0) *clear_word = 0u;
     const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= k) return;
     ema_cluster(cen, usage, sums, counts, j, d, alpha, decay, replace, threadIdx.x & 63);
+    if (clear_counts && (threadIdx.x & 63) == 0) clear_counts[j] = 0u;
 }
 
 // The step's last kernel also leaves the next step's accumulators at zero:
@@ -767,11 +777,11 @@ __global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint
 }
 
 hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
-                            float alpha, hipStream_t s) {
+                            float alpha, hipStream_t s, uint32_t *clear_counts, uint32_t *clear_word) {
     const int replace = alpha == 1.0f;
     const float decay = 1.0f - alpha;
     hipLaunchKernelGGL(ema_apply_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
-                       alpha, decay, replace);
+                       alpha, decay, replace, clear_counts, clear_word);
     return hipGetLastError();
 }
 

@@ -207,16 +207,22 @@ struct AccumScratch {
 };
 hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k);
 void accum_free(AccumScratch &a);
+// `cleared`: counts and the big-cluster count are already zero (an EMA step
+// with clear_counts left them so) and the sums of clusters without members
+// are not read; the clearing launch is skipped.
 hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
-                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s);
+                             const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s,
+                             bool cleared = false);
 hipError_t launch_pass_accumulate(const float *sums, const uint32_t *counts, size_t k, size_t d, double *pass_sums,
                                   uint64_t *pass_counts, hipStream_t s);
 hipError_t launch_closing_mean(const double *pass_sums, const uint64_t *pass_counts, size_t k, size_t d,
                                float *centroids, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s);
 // The EMA step of src/lib.rs:287-309 on device centroids/usage (k x d, k).
+// With clear_counts it also zeroes each count once read, and *clear_word.
 hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
-                            float alpha, hipStream_t s);
+                            float alpha, hipStream_t s, uint32_t *clear_counts = nullptr,
+                            uint32_t *clear_word = nullptr);
 // The revival row split of src/lib.rs:330-340 for (donor, unused) pairs.
 hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s);
 // out (m x d f32) = rows ids of x (f32 / i8 / u8), widened

@@ -112,7 +112,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                 (rc = check(hipEventCreateWithFlags(&ev.e[i], hipEventDisableTiming), "event")))
                 return rc;
     if ((rc = check(hipEventCreateWithFlags(&ev.order, hipEventDisableTiming), "event"))) return rc;
-    bool order_pending = false;
+    bool order_pending = false, counts_cleared = false;
     auto drain_one = [&]() -> int {
         const int sl = pend_head;
         int r;
@@ -163,8 +163,10 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         if ((rc = run_search(h, rows_dev, rows, nb, ids_dev, lab_dev, dist_dev)))
             return rc;
         // per-cluster counts and sums in batch order (src/lib.rs:431-441)
+        // after the first step the EMA step leaves the counts and the
+        // big-cluster count at zero (no clearing launch)
         if ((rc = check(launch_accumulate(h->acc, rows_dev, rows, nb, d, ids_dev, lab_dev, k,
-                                          sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream),
+                                          sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream, counts_cleared),
                         "accumulate")))
             return rc;
         if (J.is_last &&
@@ -174,9 +176,11 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
             return rc;
         // the EMA step (src/lib.rs:455-456)
         if ((rc = check(launch_ema_apply(cent_d.as<float>(), usage_d.as<float>(), sums_d.as<float>(),
-                                         counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream),
+                                         counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream,
+                                         counts_d.as<uint32_t>(), h->acc.big),
                         "EMA step")))
             return rc;
+        counts_cleared = true;
         const bool revive = J.revive_due();
         const bool need_usage = revive || cfg.balance > 0.0f;
         if (async) {
