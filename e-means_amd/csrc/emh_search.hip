@@ -230,6 +230,117 @@ __device__ inline void topm_insert_lex(S (&bs)[M], uint32_t (&bi)[M], S v, uint3
     }
 }
 
+// ---- candidate lists of the BF16 / FP8 searches ----
+//
+// A candidate key is a score's f32 bits (the scores are kept positive, so
+// their bits order like the scores) with the low 5 bits replaced by the
+// value's position in the lane's tile: the keys of one tile are distinct and
+// a key names its centroid. Each lane keeps the M smallest keys it has seen,
+// exactly: per tile it takes the two smallest keys of its values with a
+// min3 / med3 tree, inserts both, and only when the second one enters the
+// list (several of the lane's best in one tile, which grows rare as the
+// list fills) extracts the tile's next keys one at a time.
+
+__device__ __forceinline__ uint32_t cand_key(float v, uint32_t pos) {
+    return (__float_as_uint(v) & 0xFFFFFFE0u) | pos;
+}
+__device__ __forceinline__ uint32_t umin3(uint32_t a, uint32_t b, uint32_t c) { return min(min(a, b), c); }
+// the median of three (one v_med3_u32)
+__device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
+    return max(min(a, b), min(max(a, b), c));
+}
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// The two smallest keys of one 32x32 block's 16 values of a lane, built in
+// 8 phases so a caller can spread them between MFMAs: phases 0-4 take
+// triples (3p, 3p+1, 3p+2) to (min, median); phase 5 merges triples 3 and 4
+// with value 15; phase 6 merges triples 0-2; phase 7 merges the two halves.
+// (Of pairs lo_i <= hi_i, the second smallest is min(second smallest lo,
+// hi of the smallest lo), and every other hi is at least the second
+// smallest lo: hence min(med3(lo), min3(hi)).)
+struct Top2 {
+    uint32_t pl[5], ph[5], ql[2], qh[2], lo, hi;
+};
+template <bool PIN>
+__device__ __forceinline__ void top2_phase(Top2 &s, const f32x16 &a, uint32_t pos0, int p) {
+    if (p < 5) {
+        const uint32_t k0 = cand_key(a[3 * p], pos0 + 3 * p), k1 = cand_key(a[3 * p + 1], pos0 + 3 * p + 1);
+        const uint32_t k2 = cand_key(a[3 * p + 2], pos0 + 3 * p + 2);
+        s.pl[p] = umin3(k0, k1, k2);
+        s.ph[p] = umed3(k0, k1, k2);
+        if constexpr (PIN) asm volatile("" : "+v"(s.pl[p]), "+v"(s.ph[p]));
+    } else if (p == 5) {
+        const uint32_t k = cand_key(a[15], pos0 + 15);
+        s.ql[1] = umin3(s.pl[3], s.pl[4], k);
+        s.qh[1] = umin3(umed3(s.pl[3], s.pl[4], k), s.ph[3], s.ph[4]);
+        if constexpr (PIN) asm volatile("" : "+v"(s.ql[1]), "+v"(s.qh[1]));
+    } else if (p == 6) {
+        s.ql[0] = umin3(s.pl[0], s.pl[1], s.pl[2]);
+        s.qh[0] = min(umed3(s.pl[0], s.pl[1], s.pl[2]), umin3(s.ph[0], s.ph[1], s.ph[2]));
+        if constexpr (PIN) asm volatile("" : "+v"(s.ql[0]), "+v"(s.qh[0]));
+    } else {
+        s.lo = min(s.ql[0], s.ql[1]);
+        s.hi = umin3(max(s.ql[0], s.ql[1]), s.qh[0], s.qh[1]);
+        if constexpr (PIN) asm volatile("" : "+v"(s.lo), "+v"(s.hi));
+    }
+}
+
+// Sorted insertion of key v (centroid vi); strict '<', so an equal key
+// already in the list (an earlier tile: a lower index) stays first. A key
+// not below the last entry changes nothing.
+template <int M>
+__device__ __forceinline__ void cand_insert(uint32_t (&bs)[M], uint32_t (&bi)[M], uint32_t v, uint32_t vi) {
+    bool c[M];
+#pragma unroll
+    for (int t = 0; t < M; t++) c[t] = v < bs[t];
+#pragma unroll
+    for (int t = M - 1; t > 0; t--) {
+        bi[t] = c[t - 1] ? bi[t - 1] : (c[t] ? vi : bi[t]);
+        bs[t] = umed3(bs[t - 1], bs[t], v);
+    }
+    bi[0] = c[0] ? vi : bi[0];
+    bs[0] = min(bs[0], v);
+}
+
+// Adds one tile of a lane (NB blocks of 16 values in a, the tile's first
+// centroid c0) to its list, given the tile's two smallest keys m1 < m2.
+// Exact: when m2 enters the list the tile's next keys are extracted in
+// order (the smallest key above the last one: one subtraction and a min per
+// value) until one stays out; at most M keys of a tile can enter.
+template <int M, int NB>
+__device__ __forceinline__ void cand_settle(const f32x16 (&a)[NB], uint32_t m1, uint32_t m2, int h, uint32_t c0,
+                                            uint32_t (&bs)[M], uint32_t (&bi)[M]) {
+    auto gidx = [&](uint32_t key) -> uint32_t {
+        const uint32_t li = key & 31u, nb = li >> 4, r = li & 15u;
+        return c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+    };
+    if (!wave_any(m1 < bs[M - 1])) return;
+    cand_insert<M>(bs, bi, m1, gidx(m1));
+    if constexpr (M >= 2) {
+        bool more = m2 < bs[M - 1];
+        if (!wave_any(more)) return;
+        cand_insert<M>(bs, bi, m2, gidx(m2));
+        uint32_t last = m2;
+#pragma unroll 1
+        for (int it = 2; it < M && it < 16 * NB; it++) {
+            uint32_t u = 0xFFFFFFFFu;
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int r = 0; r < 16; r++) {
+                    float v = a[nb][r];
+                    asm volatile("" : "+v"(v));  // recomputed here, not kept alive from the tree
+                    u = min(u, cand_key(v, (uint32_t)(nb * 16 + r)) - last - 1u);
+                }
+            const uint32_t nx = u + last + 1u;  // the smallest key above `last`
+            more = more && nx > last && nx < bs[M - 1];
+            if (!wave_any(more)) return;
+            cand_insert<M>(bs, bi, more ? nx : 0xFFFFFFFFu, gidx(nx));
+            last = nx;
+        }
+    }
+}
+
 __device__ inline int64_t shfl_xor_i64(int64_t v, int mask) {
     int lo = __shfl_xor((int)(uint32_t)v, mask, 64);
     int hi = __shfl_xor((int)(uint32_t)((uint64_t)v >> 32), mask, 64);
@@ -623,16 +734,28 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
                                          int shift, typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
-        if constexpr (OP == OP_BF16 || OP == OP_FP8) {
+        if constexpr ((OP == OP_BF16 || OP == OP_FP8) && M == 1) {
             uint32_t g = 0xFFFFFFFFu;
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-                for (int r = 0; r < 16; r++)
-                    g = min(g, (__float_as_uint(acc[fr][nb][r]) & 0xFFFFFFE0u) | (uint32_t)(nb * 16 + r));
+                for (int r = 0; r < 16; r++) g = min(g, cand_key(acc[fr][nb][r], (uint32_t)(nb * 16 + r)));
             const uint32_t li = g & 31u, nb = li >> 4, r = li & 15u;
             const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
             topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
+        } else if constexpr (OP == OP_BF16 || OP == OP_FP8) {
+            // the tile's two smallest keys, then the exact list update
+            Top2 t[NB];
+#pragma unroll
+            for (int nb = 0; nb < NB; nb++)
+#pragma unroll
+                for (int p = 0; p < 8; p++) top2_phase<false>(t[nb], acc[fr][nb], (uint32_t)(nb * 16), p);
+            uint32_t m1 = t[0].lo, m2 = t[0].hi;
+            if constexpr (NB == 2) {
+                m1 = min(t[0].lo, t[1].lo);
+                m2 = umin3(max(t[0].lo, t[1].lo), t[0].hi, t[1].hi);
+            }
+            cand_settle<M, NB>(acc[fr], m1, m2, h, c0, bs[fr], bi[fr]);
         } else if constexpr (OP == OP_F32) {
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
@@ -681,8 +804,8 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
 // The candidate searches on two-block tiles (BF16 / FP8, rows of at most
 // 256 bytes): block 0's MFMAs first, then block 1's with block 0's fold
 // between them, so half of the fold runs while the matrix cores work; then
-// block 1's fold and the top-m insertion. Same keys, same minimum, same
-// insertion as reg_mfma + reg_fold.
+// block 1's fold and the list update. Same keys and the same list as
+// reg_mfma + reg_fold.
 template <int OP, int M, int NS, int FR>
 __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *sH, const uint4 (&xb)[FR][NS],
                                                const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
@@ -706,6 +829,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
         }
     };
     uint32_t g[FR];
+    Top2 t0[FR];  // M > 1: block 0's two smallest keys
     {  // block 0
         uint4 ops[NS];
 #pragma unroll
@@ -731,33 +855,55 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
         for (int st = 0; st < STEPS; st++) {
 #pragma unroll
             for (int fr = 0; fr < FR; fr++) acc[fr][1] = mfma(st, fr, ops, st == 0 ? init : acc[fr][1]);
+            if constexpr (M == 1) {
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+                    for (int u = 0; u < PER; u++) {
+                        const int r = st * PER + u;
+                        if (r < 16) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
+                    }
+                // keep this step's share of the fold next to its MFMAs: the
+                // empty asm pins the partial minimum here (the optimiser
+                // would otherwise gather every min into one tree after the
+                // last MFMA), the scheduling barrier keeps the MFMAs from
+                // running together
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++) asm volatile("" : "+v"(g[fr]));
+            } else {
+                // this step's phases of block 0's two-smallest tree, pinned
+                // here the same way
+#pragma unroll
+                for (int fr = 0; fr < FR; fr++)
+#pragma unroll
+                    for (int p = 0; p < 8; p++)
+                        if ((p * STEPS) / 8 == st) top2_phase<true>(t0[fr], acc[fr][0], 0u, p);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if constexpr (M == 1) {
 #pragma unroll
             for (int fr = 0; fr < FR; fr++)
 #pragma unroll
-                for (int u = 0; u < PER; u++) {
-                    const int r = st * PER + u;
-                    if (r < 16) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
-                }
-            // keep this step's share of the fold next to its MFMAs: the empty
-            // asm pins the partial minimum here (the optimiser would otherwise
-            // gather every min into one tree after the last MFMA), the
-            // scheduling barrier keeps the MFMAs from running together
-#pragma unroll
-            for (int fr = 0; fr < FR; fr++) asm volatile("" : "+v"(g[fr]));
-            __builtin_amdgcn_sched_barrier(0);
+                for (int r = STEPS * PER; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
         }
-#pragma unroll
-        for (int fr = 0; fr < FR; fr++)
-#pragma unroll
-            for (int r = STEPS * PER; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][0][r], (uint32_t)r));
     }
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
+        if constexpr (M == 1) {
 #pragma unroll
-        for (int r = 0; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][1][r], (uint32_t)(16 + r)));
-        const uint32_t li = g[fr] & 31u, nb = li >> 4, r = li & 15u;
-        const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
-        topm_insert<M, uint32_t>(bs[fr], bi[fr], g[fr], gi);
+            for (int r = 0; r < 16; r++) g[fr] = min(g[fr], key(acc[fr][1][r], (uint32_t)(16 + r)));
+            const uint32_t li = g[fr] & 31u, nb = li >> 4, r = li & 15u;
+            const uint32_t gi = c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+            topm_insert<M, uint32_t>(bs[fr], bi[fr], g[fr], gi);
+        } else {
+            Top2 t1;
+#pragma unroll
+            for (int p = 0; p < 8; p++) top2_phase<false>(t1, acc[fr][1], 16u, p);
+            const uint32_t m1 = min(t0[fr].lo, t1.lo);
+            const uint32_t m2 = umin3(max(t0[fr].lo, t1.lo), t0[fr].hi, t1.hi);
+            cand_settle<M, 2>(acc[fr], m1, m2, h, c0, bs[fr], bi[fr]);
+        }
     }
 }
 

@@ -6,6 +6,8 @@ vs the CpuI8Assigner restatement (src/cpu_i8.rs:294-324). BF16/FP8 + FP32
 rescore: label disagreement and distance error vs FP64 brute force, with the
 tolerances written in each test.
 """
+import functools
+
 import numpy as np
 import pytest
 
@@ -218,23 +220,89 @@ def _mixture(n, d, k_blobs, seed, scale=4.0):
     return x.astype(np.float32)
 
 
-@pytest.mark.parametrize("prec,m,max_dis,d", [("bf16", 4, 1e-3, 128), ("bf16", 1, 2e-2, 128), ("fp8", 8, 5e-3, 128),
-                                              ("bf16", 4, 1e-3, 256), ("fp8", 4, 5e-3, 512), ("fp8", 8, 5e-3, 256)])
-def test_rescored_search_vs_fp64(hip, prec, m, max_dis, d):
-    """Label disagreement vs FP64 at most max_dis (true errors only: a label
-    whose FP64 distance exceeds the best by more than 1e-6(1+best)), and the
-    returned distance within 1e-4(1+best) of FP64 at that label."""
-    print(f"{prec} m={m} d={d}", flush=True)
-    x = _mixture(20000, d, 2048, 3)
-    c = x[O.permutation(1234, 20000)[:1024]]
+CAND_N = 100_000
+CAND_CASES = [(k, d, prec, m) for k in (64, 128, 256, 1024, 16384) for d in (64, 128, 256)
+              for prec, m in (("bf16", 4), ("fp8", 8))]
+
+
+@functools.lru_cache(maxsize=1)
+def _judged_mixture(k, d):
+    """100,000 mixture rows (2048 blobs, or k blobs at k = 16384 as in the
+    2M x 128 workload), centroids = the rows at permutation positions 0..k
+    (src/lib.rs:375-379), and the FP64 brute-force best distances of the
+    judged rows: all of them, or a fixed 20,000-row sample at k = 16384."""
+    x = _mixture(CAND_N, d, max(2048, k), 100 + d)
+    c = x[O.permutation(1234, CAND_N)[:k]].copy()
+    rows = np.arange(CAND_N) if k <= 1024 else np.sort(O.permutation(77, CAND_N)[:20000])
+    _, bd = O.brute_f64(x[rows], c)
+    return x, c, rows, bd
+
+
+@pytest.mark.parametrize("k,d,prec,m", CAND_CASES)
+def test_rescored_search_vs_fp64(hip, k, d, prec, m):
+    """BF16 top-4 and FP8 top-8 + FP32 rescore against FP64 brute force:
+    true label errors (a label whose FP64 distance exceeds the best by more
+    than 1e-6(1+best)) at most 1e-4 of the judged rows, and the returned
+    distance within 1e-4(1+d) of FP64 at the returned label (the FP32
+    rescore). The candidate list is the exact m smallest keys, so a row is
+    lost only when the true nearest is not among the m best low-precision
+    scores."""
+    print(f"{prec} top-{m} k={k} d={d}", flush=True)
+    x, c, rows, bd = _judged_mixture(k, d)
     a = E.HipAssigner(prec, rescore_m=m)
     a.set_centroids(c)
     labels, dist = a.assign(x)
-    bl, bd = O.brute_f64(x, c)
+    got = O.dist_f64(x[rows], c, labels[rows])
+    wrong = int((got > bd + 1e-6 * (1 + bd)).sum())
+    assert wrong <= 1e-4 * len(rows), f"{prec} top-{m} k={k} d={d}: {wrong} of {len(rows)} true label errors"
+    assert np.all(np.abs(dist[rows] - got) <= 1e-4 * (1 + got))
+
+
+def _near_duplicates(k, d, m, where, seed):
+    """k centroids far from `base`, except m of them equal to base up to
+    offsets of at most 1.4e-3 in dimension 0, which BF16 (spacing 2^-7 at
+    1.0) and e4m3 (2^-3) round away: their low-precision scores tie exactly.
+    The truly nearest (offset 0) is the one with the highest index, so only a
+    list holding all m of them lets the FP32 rescore find it."""
+    rng = np.random.default_rng(seed)
+    base = rng.normal(size=d).astype(np.float32)
+    base[0] = 1.0
+    c = (base + rng.normal(scale=3.0, size=(k, d))).astype(np.float32)
+    lane_half0 = [8 * (r >> 2) + (r & 3) for r in range(16)]  # positions one lane sees in a 32-block
+    if where == "block":  # one lane, one 32-centroid block of the first tile
+        idx = lane_half0[:m]
+    elif where == "spread":  # one lane, both blocks of the first 64-centroid tile
+        idx = sorted([p for r in range(8) for p in (lane_half0[r], 32 + lane_half0[r])][:m])
+    else:  # "late": one lane and block of tile 37
+        idx = [64 * 37 + p for p in lane_half0[:m]]
+    for j, i in enumerate(idx):
+        c[i] = base
+        c[i, 0] = np.float32(1.0) + np.float32((m - 1 - j) * 2e-4)
+    x = np.concatenate([np.repeat(base[None], 8, 0), rng.normal(size=(56, d)).astype(np.float32)])
+    return x, c, idx
+
+
+@pytest.mark.parametrize("prec,m", [("bf16", 4), ("bf16", 8), ("fp8", 8), ("fp8", 4)])
+@pytest.mark.parametrize("k,d,where,split", [(64, 64, "block", 0), (64, 128, "spread", 0), (300, 256, "spread", 0),
+                                             (4096, 128, "late", 0), (4096, 128, "late", 3), (200, 512, "block", 0),
+                                             (100, 300, "block", 0)])
+def test_candidates_keep_tied_centroids(hip, prec, m, k, d, where, split):
+    """m centroids whose BF16 / FP8 scores tie exactly, all inside one lane's
+    share of one tile: the list must hold every one of them (the m smallest
+    keys), and the FP32 rescore then returns the truly nearest, the last.
+    (BF16 d = 300 and d = 512 rows are wider than 512 bytes and take the LDS
+    kernel; FP8 d = 300 and 512 the register kernel's 32-centroid tiles.)"""
+    print(f"{prec} top-{m} k={k} d={d} {where} split={split}", flush=True)
+    x, c, idx = _near_duplicates(k, d, m, where, seed=k + d + m)
+    a = E.HipAssigner(prec, rescore_m=m)
+    a.set_split(split)
+    a.set_centroids(c)
+    labels, dist = a.assign(x)
+    assert np.all(labels[:8] == idx[-1]), f"labels {labels[:8]}, expected {idx[-1]} (tied set {idx})"
+    assert np.all(dist[:8] == 0.0)
+    _, bd = O.brute_f64(x, c)
     got = O.dist_f64(x, c, labels)
-    wrong = got > bd + 1e-6 * (1 + bd)
-    assert wrong.mean() <= max_dis, f"{prec} top-{m}: {wrong.mean():.2e} true label errors"
-    assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+    assert np.all(got <= bd + 1e-6 * (1 + bd))
 
 
 def test_bf16_odd_shapes(hip):
