@@ -522,3 +522,78 @@ def test_report_quantiles_and_spread_closed_forms():
     assert c["nearest_centroid"][0] == 0 and np.isinf(c["nearest_dist"][0]) and np.isnan(c["isolation"][0])
     # the only cluster is its own family holding every row
     assert len(r["families"]) == 1 and r["families"][0]["share"] == 1.0
+
+
+def _balance_ladder(lam):
+    """1-D, k = 3, one batch per epoch, 2 epochs, alpha0 = 1, no revival.
+    Initial centroids (permutation positions 0..2, src/lib.rs:375-379) are
+    0, 10 and 100; the rows are a core {0, 0, 0, 0, 1, -1} around 0, two
+    rungs 4.0 and 4.9 between 0 and 10, one more 10 and one more 100."""
+    n = 11
+    perm = O.permutation(1234, n)
+    vals = [0.0, 10.0, 100.0, 0.0, 0.0, 0.0, 1.0, -1.0, 4.0, 4.9, 10.0]
+    x = np.empty((n, 1), np.float32)
+    for pos, v in zip(perm, vals):
+        x[pos, 0] = v
+    cfg = O.Config(nepochs=2.0, batch_size=n, alpha0=1.0, balance=lam, seed=1234)
+    return x, perm, O.train(x, 3, cfg)
+
+
+def test_balance_penalty_hand_derived():
+    """src/lib.rs:411-430 on a case worked by hand. Step 0 has no penalty
+    (s = 0): every row below 5 joins centroid 0, so after the replace step
+    (alpha = 1) c = (8.9/8, 10, 100) = (1.1125, 10, 100), N = (8, 2, 1) and
+    s = (4 * 0 + 1 + 1 + 16 + 24.01) / 11 = 3.81909.
+    Step 1 biases b_j = lambda s / mean(N) * N_j with mean(N) = 11/3: the
+    rung 4.9 leaves cluster 0 once 6 * scale > 5.1^2 - (4.9 - 1.1125)^2
+    = 11.66 (scale > 1.944); the rung 4.0 once 6 * scale > 6^2 - 2.8875^2
+    = 27.66 (scale > 4.610). lambda = 2.88 gives scale = 3.000: exactly the
+    4.9 rung moves. The epoch-1 objective is the unpenalized distance sum
+    (the bias is taken back off, src/lib.rs:425-430) and the closing mean
+    (src/lib.rs:481-489) averages the last pass's members."""
+    x, perm, r = _balance_ladder(2.88)
+    lab = {float(x[i, 0]): int(r.labels[i]) for i in range(11)}
+    assert lab[4.0] == 0 and abs(float(x[perm[8], 0]) - 4.0) < 1e-6
+    assert int(r.labels[perm[9]]) == 1, "the 4.9 rung must move to centroid 10 under the penalty"
+    assert int(r.labels[perm[8]]) == 0, "the 4.0 rung must stay (scale below 4.61)"
+    c0 = np.float32(4.0) * (np.float32(1) / np.float32(7))
+    c1 = np.float32(24.9) * (np.float32(1) / np.float32(3))
+    assert abs(r.centroids[0, 0] - c0) <= 1e-6 and abs(r.centroids[1, 0] - c1) <= 2e-6
+    assert r.centroids[2, 0] == np.float32(100.0)
+    # epoch 0: distances to the initial centroids; epoch 1: to (1.1125, 10, 100)
+    assert abs(r.objective[0] - 42.01) <= 1e-4
+    c = 1.1125
+    e1 = 4 * c * c + (1 - c) ** 2 + (1 + c) ** 2 + (4 - c) ** 2 + 5.1 ** 2
+    assert abs(r.objective[1] - e1) <= 1e-4 * e1
+    # without the penalty nothing moves: centroid 0 keeps both rungs
+    _, _, r0 = _balance_ladder(0.0)
+    assert int(r0.labels[perm[9]]) == 0 and int(r0.labels[perm[8]]) == 0
+    assert abs(r0.centroids[0, 0] - np.float32(8.9) * (np.float32(1) / np.float32(8))) <= 1e-6
+    # a scale past 4.61 moves both rungs (lambda = 5: scale = 5.21)
+    _, _, r5 = _balance_ladder(5.0)
+    assert int(r5.labels[perm[9]]) == 1 and int(r5.labels[perm[8]]) == 1
+
+
+def _smooth(n, d, seed):
+    rng = np.random.default_rng(seed)
+    scales = np.linspace(1.0, 0.2, d).astype(np.float32)
+    return (rng.normal(size=(n, d)) * scales).astype(np.float32)
+
+
+def test_balance_closing_mean_invariant_and_imbalance():
+    """lambda = 0.1 keeps the closing-mean invariant (src/lib.rs:575-590:
+    each centroid is the mean of its last-pass members); on smooth data the
+    imbalance factor k sum n_j^2 / n^2 of the penalized labels falls as
+    lambda rises (README.md:111-122)."""
+    x = _smooth(20000, 24, 3)
+    k = 64
+    imb = []
+    for lam in (0.0, 0.1, 1.0):
+        r = O.train(x, k, O.Config(nepochs=3.0, balance=lam))
+        if lam == 0.1:
+            for j in range(k):
+                members = x[r.labels == j]
+                if len(members):
+                    assert np.allclose(r.centroids[j], members.astype(np.float64).mean(0), atol=1e-3)
+        imb.append(O.balance_stats(r.labels, k)[0])
+    assert imb[0] > imb[1] > imb[2], f"imbalance {imb}"
