@@ -302,11 +302,22 @@ __device__ __forceinline__ void cand_insert(uint32_t (&bs)[M], uint32_t (&bi)[M]
     bs[0] = min(bs[0], v);
 }
 
+// The same register of the other lane half (lane l ^ 32), by one
+// permlane32 swap: after it r[0] holds lanes 0-31 twice and r[1] lanes
+// 32-63 twice.
+__device__ __forceinline__ uint32_t other_half(uint32_t v, int h) {
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return h ? r[0] : r[1];
+}
+
 // Adds one tile of a lane (NB blocks of 16 values in a, the tile's first
 // centroid c0) to its list, given the tile's two smallest keys m1 < m2.
-// Exact: when m2 enters the list the tile's next keys are extracted in
-// order (the smallest key above the last one: one subtraction and a min per
-// value) until one stays out; at most M keys of a tile can enter.
+// A key enters only below the lane's own last entry and at most equal to
+// the other lane half's last entry (the row's list is the merge of the two
+// halves' lists, so a key above M keys of either half never reaches it;
+// equal keys are kept, the merge orders them by index). Exact: when m2
+// enters, the tile's next keys are extracted in order (the smallest key
+// above the last one) until one stays out; at most M keys of a tile enter.
 template <int M, int NB>
 __device__ __forceinline__ void cand_settle(const f32x16 (&a)[NB], uint32_t m1, uint32_t m2, int h, uint32_t c0,
                                             uint32_t (&bs)[M], uint32_t (&bi)[M]) {
@@ -314,26 +325,25 @@ __device__ __forceinline__ void cand_settle(const f32x16 (&a)[NB], uint32_t m1, 
         const uint32_t li = key & 31u, nb = li >> 4, r = li & 15u;
         return c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
     };
-    if (!wave_any(m1 < bs[M - 1])) return;
-    cand_insert<M>(bs, bi, m1, gidx(m1));
+    const uint32_t po = other_half(bs[M - 1], h);
+    const uint32_t pl = po == 0xFFFFFFFFu ? po : po + 1u;  // keys below pl may still reach the row's list
+    if (!wave_any(m1 < min(bs[M - 1], pl))) return;
+    cand_insert<M>(bs, bi, m1 < pl ? m1 : 0xFFFFFFFFu, gidx(m1));
     if constexpr (M >= 2) {
-        bool more = m2 < bs[M - 1];
+        bool more = m2 < min(bs[M - 1], pl);
         if (!wave_any(more)) return;
-        cand_insert<M>(bs, bi, m2, gidx(m2));
+        cand_insert<M>(bs, bi, more ? m2 : 0xFFFFFFFFu, gidx(m2));
         uint32_t last = m2;
 #pragma unroll 1
         for (int it = 2; it < M && it < 16 * NB; it++) {
+            const uint32_t above = last + 1u;
             uint32_t u = 0xFFFFFFFFu;
 #pragma unroll
             for (int nb = 0; nb < NB; nb++)
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    float v = a[nb][r];
-                    asm volatile("" : "+v"(v));  // recomputed here, not kept alive from the tree
-                    u = min(u, cand_key(v, (uint32_t)(nb * 16 + r)) - last - 1u);
-                }
-            const uint32_t nx = u + last + 1u;  // the smallest key above `last`
-            more = more && nx > last && nx < bs[M - 1];
+                for (int r = 0; r < 16; r++) u = min(u, cand_key(a[nb][r], (uint32_t)(nb * 16 + r)) - above);
+            const uint32_t nx = u + above;  // the smallest key above `last`
+            more = more && nx > last && nx < min(bs[M - 1], pl);
             if (!wave_any(more)) return;
             cand_insert<M>(bs, bi, more ? nx : 0xFFFFFFFFu, gidx(nx));
             last = nx;
