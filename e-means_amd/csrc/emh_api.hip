@@ -83,27 +83,38 @@ bool autotune_enabled() {
 }
 }  // namespace
 
-// Search launch with `nslice` centroid slices (1: unsplit).
+// Search launch with `nslice` centroid slices (1: unsplit); `part` holds
+// slice_part_bytes of per-slice results when nslice > 1.
 static int search_launch(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
                          const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
-                         int nslice) {
+                         uint32_t *cand, int nslice, uint32_t *part) {
     const SearchGeom &g = h->g;
-    uint32_t *part = nullptr;
-    if (nslice > 1) {
-        int rc = check(h->part.ensure(slice_part_bytes(M, m, nslice, h->op)), "split buffer");
-        if (rc) return rc;
-        part = h->part.as<uint32_t>();
-    }
     h->last_slices = nslice;
     return check(launch_search(g, M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(), h->cn.as<int64_t>(), x8,
-                               rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(), labels_dev, dist_dev,
-                               h->cand.as<uint32_t>(), h->stream, nslice, part),
+                               rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(), labels_dev, dist_dev, cand,
+                               h->stream, nslice, nslice > 1 ? part : nullptr),
                  "search kernel");
+}
+
+// The split rule without timing: the smallest power-of-two split that puts
+// two workgroups on every CU (1 when the launch fills the chip already).
+static int split_rule(const emh_assigner *h, size_t m) {
+    const SearchGeom &g = h->g;
+    const size_t rowblocks = (m + 255) / 256;
+    const int nt = reg_tiles(g);
+    const size_t target = 2 * (size_t)h->cu_count;
+    if (!g.reg || rowblocks >= target || nt < 4) return 1;
+    int best = 1;
+    for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) {
+        best = s;
+        if (rowblocks * (size_t)s >= target) break;
+    }
+    return best;
 }
 
 static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
                          const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
-                         int *out) {
+                         uint32_t *cand, int *out) {
     *out = 1;
     const SearchGeom &g = h->g;
     if (h->forced_slices > 0) {
@@ -115,9 +126,9 @@ static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, si
     const int nt = reg_tiles(g);
     const size_t target = 2 * (size_t)h->cu_count;  // the kernel runs two workgroups per CU
     if (rowblocks >= target || nt < 4) return EMH_OK;
-    std::vector<int> cand;
-    for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) cand.push_back(s);
-    if (cand.size() < 2) return EMH_OK;
+    std::vector<int> cand_s;
+    for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) cand_s.push_back(s);
+    if (cand_s.size() < 2) return EMH_OK;
     const TuneKey key{h->device, h->op, rows, M, g.row_bytes, nt, rowblocks};
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -127,30 +138,28 @@ static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, si
             return EMH_OK;
         }
     }
-    int best = cand.back();
+    int best = cand_s.back();
     if (!autotune_enabled()) {
-        for (int s : cand)
-            if (rowblocks * (size_t)s >= target) {
-                best = s;
-                break;
-            }
+        best = split_rule(h, m);
     } else {
-        hipEvent_t e0, e1;
         int rc;
+        if ((rc = check(h->part.ensure(slice_part_bytes(M, m, cand_s.back(), h->op)), "split buffer"))) return rc;
+        hipEvent_t e0, e1;
         if ((rc = check(hipEventCreate(&e0), "tune event")) || (rc = check(hipEventCreate(&e1), "tune event")))
             return rc;
         float best_ms = 0.0f;
-        for (int s : cand) {
+        uint32_t *part = h->part.as<uint32_t>();
+        for (int s : cand_s) {
             // one warm-up launch, then the timed one
-            if ((rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, s)) ||
+            if ((rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, s, part)) ||
                 (rc = check(hipEventRecord(e0, h->stream), "tune event")) ||
-                (rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, s)) ||
+                (rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, s, part)) ||
                 (rc = check(hipEventRecord(e1, h->stream), "tune event")) ||
                 (rc = check(hipEventSynchronize(e1), "tune sync")))
                 break;
             float ms = 0.0f;
             (void)hipEventElapsedTime(&ms, e0, e1);
-            if (s == cand.front() || ms < best_ms) {
+            if (s == cand_s.front() || ms < best_ms) {
                 best_ms = ms;
                 best = s;
             }
@@ -165,6 +174,50 @@ static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, si
     return EMH_OK;
 }
 
+// The row-side buffers of one search call.
+struct RowScratch {
+    uint32_t *cand = nullptr;  // m x M candidate ids (BF16 / FP8)
+    uint8_t *x8 = nullptr;     // FP8: m x d_pad encoded rows
+    float *rs = nullptr;       // FP8: m row scales
+    float *rn = nullptr;       // FP8: m squared row norms
+    uint32_t *part = nullptr;  // nslice > 1: per-slice results
+};
+
+// Encoding (FP8), search and rescore (BF16 / FP8) of one call on the
+// buffers of `rs_`, with `ns` centroid slices; allocates nothing.
+static int search_stages(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                         uint32_t *labels_dev, float *dist_dev, const RowScratch &b, int ns, hipEvent_t *ev,
+                         bool timed) {
+    auto mk = [&](int i) { return timed ? mark(h, &ev[i]) : EMH_OK; };
+    const SearchGeom &g = h->g;
+    int rc;
+    if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
+        if ((rc = mk(1))) return rc;
+        if ((rc = search_launch(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev,
+                                nullptr, ns, b.part)))
+            return rc;
+        if ((rc = mk(2))) return rc;
+        return mk(3);
+    }
+    if ((rc = mk(1))) return rc;
+    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, ns, b.part)))
+        return rc;
+    if ((rc = mk(2))) return rc;
+    if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen_f32.as<float>(),
+                                   h->has_bias ? h->bias.as<float>() : nullptr, g.k, b.cand, labels_dev, dist_dev,
+                                   h->stream),
+                    "rescore kernel")))
+        return rc;
+    return mk(3);
+}
+
+static int encode_rows(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                       const RowScratch &b) {
+    const SearchGeom &g = h->g;
+    return check(launch_rows_fp8(x_dev, rows, m, g.d, g.d_pad, ids_dev, b.x8, b.rs, b.rn, h->stream),
+                 "fp8 row encoding");
+}
+
 int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev, uint32_t *labels_dev,
                float *dist_dev) {
     if (m == 0) return EMH_OK;
@@ -172,42 +225,105 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     int rc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if ((rc = mark(h, &ev[0]))) return rc;
-    if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
-        int ns = 1;
-        if ((rc = choose_slices(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev, &ns)))
-            return rc;
-        if ((rc = mark(h, &ev[1]))) return rc;
-        if ((rc = search_launch(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev, ns)))
-            return rc;
-        if ((rc = mark(h, &ev[2]))) return rc;
-        return mark(h, &ev[3]);
+    RowScratch b;
+    const bool cand_search = h->op == OP_BF16 || h->op == OP_FP8;
+    if (cand_search) {
+        if ((rc = check(h->cand.ensure(m * h->M * 4), "candidate buffer"))) return rc;
+        b.cand = h->cand.as<uint32_t>();
     }
-    if ((rc = check(h->cand.ensure(m * h->M * 4), "candidate buffer"))) return rc;
-    const uint8_t *x8 = nullptr;
-    const float *rs = nullptr, *rn = nullptr;
     if (h->op == OP_FP8) {
         if ((rc = check(h->x8.ensure(m * g.d_pad), "fp8 row buffer"))) return rc;
         if ((rc = check(h->row_scale.ensure(m * 4), "row scale buffer"))) return rc;
         if ((rc = check(h->row_norm.ensure(m * 4), "row norm buffer"))) return rc;
-        if ((rc = check(launch_rows_fp8(x_dev, rows, m, g.d, g.d_pad, ids_dev, h->x8.as<uint8_t>(),
-                                        h->row_scale.as<float>(), h->row_norm.as<float>(), h->stream),
-                        "fp8 row encoding")))
-            return rc;
-        x8 = h->x8.as<uint8_t>();
-        rs = h->row_scale.as<float>();
-        rn = h->row_norm.as<float>();
+        b.x8 = h->x8.as<uint8_t>();
+        b.rs = h->row_scale.as<float>();
+        b.rn = h->row_norm.as<float>();
+        if ((rc = encode_rows(h, x_dev, rows, m, ids_dev, b))) return rc;
     }
+    const int M = cand_search ? h->M : 1;
     int ns = 1;
-    if ((rc = choose_slices(h, h->M, x_dev, rows, m, ids_dev, x8, rs, rn, nullptr, nullptr, &ns))) return rc;
-    if ((rc = mark(h, &ev[1]))) return rc;
-    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, x8, rs, rn, nullptr, nullptr, ns))) return rc;
-    if ((rc = mark(h, &ev[2]))) return rc;
-    if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen_f32.as<float>(),
-                                   h->has_bias ? h->bias.as<float>() : nullptr, g.k, h->cand.as<uint32_t>(),
-                                   labels_dev, dist_dev, h->stream),
-                    "rescore kernel")))
+    if ((rc = choose_slices(h, M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, cand_search ? nullptr : labels_dev,
+                            cand_search ? nullptr : dist_dev, b.cand, &ns)))
         return rc;
-    return mark(h, &ev[3]);
+    if (ns > 1) {
+        if ((rc = check(h->part.ensure(slice_part_bytes(M, m, ns, h->op)), "split buffer"))) return rc;
+        b.part = h->part.as<uint32_t>();
+    }
+    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, ns, ev, true);
+}
+
+// ---- the search without allocations ----
+// The row-side buffers of emh_assign_device_noalloc live in this array (one
+// copy per GPU, held until the process ends). Its device address on each
+// GPU is taken once, by emh_create on that GPU, and kept in g_scratch.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[MAX_DEVICES];
+
+// Called with `device` current.
+static int scratch_setup(int device) {
+    if (device < 0 || device >= MAX_DEVICES) return fail(EMH_E_UNSUPPORTED, "device ordinal above 63");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return EMH_OK;
+    void *ptr = nullptr;
+    int rc = check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (rc) return rc;
+    g_scratch[device] = ptr;
+    return EMH_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// Scratch layout for m rows and ns slices: cand | x8 | row scales | row
+// norms | slice results; returns the bytes used.
+static size_t noalloc_layout(const emh_assigner *h, size_t m, int ns, uint8_t *base, RowScratch *b) {
+    const SearchGeom &g = h->g;
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> uint8_t * {
+        uint8_t *p = base ? base + off : nullptr;
+        off += align256(bytes);
+        return p;
+    };
+    if (h->op == OP_BF16 || h->op == OP_FP8) b->cand = (uint32_t *)take(m * h->M * 4);
+    if (h->op == OP_FP8) {
+        b->x8 = take(m * g.d_pad);
+        b->rs = (float *)take(m * 4);
+        b->rn = (float *)take(m * 4);
+    }
+    if (ns > 1) b->part = (uint32_t *)take(slice_part_bytes(h->op == OP_BF16 || h->op == OP_FP8 ? h->M : 1, m, ns, h->op));
+    return off;
+}
+
+int run_search_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                       uint32_t *labels_dev, float *dist_dev) {
+    if (m == 0) return EMH_OK;
+    uint8_t *base = h->device >= 0 && h->device < MAX_DEVICES ? (uint8_t *)g_scratch[h->device] : nullptr;
+    if (!base) return fail(EMH_E_STATE, "no-allocation search: the scratch array of this device is not set up");
+    RowScratch probe;
+    const size_t need1 = noalloc_layout(h, m, 1, nullptr, &probe);
+    if (need1 > NOALLOC_SCRATCH_BYTES) {
+        const size_t per_row = need1 / m + 1;
+        return fail(EMH_E_ARG, "no-allocation search: " + std::to_string(m) + " rows need " + std::to_string(need1) +
+                                   " bytes of scratch, above the 67108864-byte maximum (at most about " +
+                                   std::to_string(NOALLOC_SCRATCH_BYTES / per_row) + " rows per call at this d and m)");
+    }
+    // the split the rule picks, when its slice results fit as well
+    int ns = h->forced_slices > 0 ? h->forced_slices : split_rule(h, m);
+    if (ns > 1) {
+        const int nt = reg_tiles(h->g);
+        const int per = (nt + ns - 1) / ns;
+        ns = per > 0 ? (nt + per - 1) / per : 1;  // the slices launch_search forms
+        RowScratch p2;
+        if (!h->g.reg || noalloc_layout(h, m, ns, nullptr, &p2) > NOALLOC_SCRATCH_BYTES) ns = 1;
+    }
+    RowScratch b;
+    noalloc_layout(h, m, ns, base, &b);
+    int rc;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (h->op == OP_FP8 && (rc = encode_rows(h, x_dev, rows, m, ids_dev, b))) return rc;
+    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, ns, ev, false);
 }
 
 }  // namespace emh
@@ -350,6 +466,11 @@ int emh_create(int device, int precision, int rescore_m, emh_assigner **out) {
     }
     h->device_name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    // the device address of this GPU's copy of the no-allocation scratch
+    if ((rc = scratch_setup(device))) {
+        delete h;
+        return rc;
+    }
     if ((rc = check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate"))) {
         delete h;
         return rc;
@@ -501,6 +622,20 @@ int emh_assign_device(emh_assigner *h, const void *x_dev, int rows, size_t m, co
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
     return run_search(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev);
+}
+
+int emh_assign_device_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                              uint32_t *labels_dev, float *dist_dev) {
+    if (!h) return fail(EMH_E_ARG, "assign: NULL handle");
+    if (!h->has_centroids) return fail(EMH_E_STATE, "set_centroids must be called first");
+    if (rows < EMH_ROWS_F32 || rows > EMH_ROWS_U8) return fail(EMH_E_ARG, "assign: unknown row type");
+    if (m == 0) return EMH_OK;
+    if (!x_dev || !labels_dev || !dist_dev) return fail(EMH_E_ARG, "assign: NULL buffer");
+    if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
+        return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
+    int rc;
+    if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
+    return run_search_noalloc(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev);
 }
 
 int emh_accumulate_device(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,

@@ -31,7 +31,7 @@ EXPORTS = [
     "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split", "emh_train_device",
     "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
     "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device", "emh_report_config_default",
-    "emh_report",
+    "emh_report", "emh_assign_device_noalloc",
 ]
 
 
@@ -128,6 +128,7 @@ def lib():
     L.emh_last_error.restype = C.c_char_p
     L.emh_set_stream.argtypes = [vp, vp]
     L.emh_assign_device.argtypes = [vp, vp, i32, sz, vp, vp, vp]
+    L.emh_assign_device_noalloc.argtypes = [vp, vp, i32, sz, vp, vp, vp]
     L.emh_synchronize.argtypes = [vp]
     L.emh_set_timing.argtypes = [vp, i32]
     L.emh_get_timing.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]
@@ -241,6 +242,15 @@ class HipAssigner:
         _check(lib().emh_assign_device(self._h, C.c_void_p(x.data_ptr()), rows, m,
                                        None if ids is None else C.c_void_p(ids.data_ptr()),
                                        C.c_void_p(labels.data_ptr()), C.c_void_p(dist.data_ptr())))
+
+    def assign_device_noalloc(self, x, labels, dist, ids=None):
+        """assign_device without allocating: the row-side buffers live in the
+        library's static per-GPU scratch (EMH_NOALLOC_SCRATCH_BYTES)."""
+        rows = {"torch.float32": ROWS_F32, "torch.int8": ROWS_I8, "torch.uint8": ROWS_U8}[str(x.dtype)]
+        m = labels.shape[0]
+        _check(lib().emh_assign_device_noalloc(self._h, C.c_void_p(x.data_ptr()), rows, m,
+                                               None if ids is None else C.c_void_p(ids.data_ptr()),
+                                               C.c_void_p(labels.data_ptr()), C.c_void_p(dist.data_ptr())))
 
     def accumulate_device(self, x, labels, sums, counts, ids=None):
         rows = {"torch.float32": ROWS_F32, "torch.int8": ROWS_I8, "torch.uint8": ROWS_U8}[str(x.dtype)]

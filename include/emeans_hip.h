@@ -96,6 +96,22 @@ int emh_assign_device(emh_assigner *h, const void *x_dev, int rows, size_t m, co
                       uint32_t *labels_dev, float *dist_dev);
 int emh_synchronize(emh_assigner *h);
 
+/* emh_assign_device without any allocation (no counterpart in the
+ * reference: for callers that cannot allocate device memory, such as code
+ * under HIP graph capture or tight serving loops). Same arguments, results
+ * and stream as emh_assign_device; the call itself allocates nothing: the
+ * row-side buffers (candidate lists, FP8 row encodings, centroid-split
+ * results) live in one static device array of EMH_NOALLOC_SCRATCH_BYTES per
+ * GPU, held until the process ends. A call whose rows need more than that
+ * fails with EMH_E_ARG (BF16 top-4: 16 bytes per row; FP8 top-8 at d = 256:
+ * 296 bytes per row; FP32 / I8 / I16 need none unless split). Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads, other handles) race, so order them. No timing
+ * events are recorded. */
+#define EMH_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int emh_assign_device_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                              uint32_t *labels_dev, float *dist_dev);
+
 /* Per-kernel timing with HIP events on the assigner's stream. While on, each
  * search call records events around its three stages; emh_get_timing waits,
  * returns the summed milliseconds of {row encoding, search, rescore} in
