@@ -38,6 +38,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
     if _p not in sys.path:
@@ -66,7 +68,8 @@ def parse():
     ap.add_argument("--k", type=int, default=None)
     ap.add_argument("--blobs", type=int, default=16384)
     ap.add_argument("--jobs", type=int, default=64, help="train-c5: number of problems")
-    ap.add_argument("--check-rows", type=int, default=20000)
+    ap.add_argument("--check-rows", type=int, default=0,
+                    help="rows judged by FP64 brute force (0: every row)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip the comparison passes")
@@ -190,31 +193,69 @@ def run_search(prec, m, x, c_host, steps, warmup):
     }
 
 
-def fp64_check(x, c, res, rows):
-    """FP64 brute force (torch, on the GPU) on a fixed subsample of rows."""
+def fp64_check(x, c, labels, dist, rows=None):
+    """FP64 brute force (torch, on the GPU) over every row (rows=None) or a
+    fixed sample: label disagreements (label != FP64 argmin, lowest index on
+    ties), true label errors (the label's FP64 distance above the best by
+    more than 1e-6(1+best)), as counts, and the largest distance error
+    relative to 1 + d (the form of src/cpu.rs:303)."""
     import torch
 
-    g = torch.Generator().manual_seed(7)
-    ids = torch.randperm(x.shape[0], generator=g)[:rows].to(x.device)
-    xs = x[ids].double()
+    n = x.shape[0]
+    if rows is None or rows >= n:
+        ids = None
+        rows = n
+    else:
+        g = torch.Generator().manual_seed(7)
+        ids = torch.randperm(n, generator=g)[:rows].to(x.device)
     cd = c.double()
-    best_idx = torch.empty(rows, dtype=torch.long, device=x.device)
-    for r0 in range(0, rows, 4096):
-        xb = xs[r0:r0 + 4096]
-        d2 = (xb * xb).sum(1, keepdim=True) - 2.0 * (xb @ cd.T) + (cd * cd).sum(1)[None]
-        best_idx[r0:r0 + 4096] = d2.argmin(1)
-    lab = res["labels"][ids].long()
-    d64 = ((xs - cd[lab]) ** 2).sum(1)
-    best_exact = ((xs - cd[best_idx]) ** 2).sum(1)
-    wrong = d64 > best_exact + 1e-6 * (1 + best_exact)
-    # error relative to 1 + d, the form of the reference's tolerance (src/cpu.rs:303)
-    rel = (res["dist"][ids].double() - d64).abs() / (1.0 + d64)
+    cn = (cd * cd).sum(1)
+    lab_all = labels.long()
+    dis = wrong = 0
+    rel_max = 0.0
+    step = 8192
+    for r0 in range(0, rows, step):
+        sel = slice(r0, min(rows, r0 + step))
+        idx = ids[sel] if ids is not None else torch.arange(sel.start, sel.stop, device=x.device)
+        xb = x[idx].double()
+        d2 = (xb * xb).sum(1, keepdim=True) - 2.0 * (xb @ cd.T) + cn[None]
+        best_idx = d2.argmin(1)
+        lab = lab_all[idx]
+        d64 = ((xb - cd[lab]) ** 2).sum(1)
+        best = ((xb - cd[best_idx]) ** 2).sum(1)
+        dis += int((lab != best_idx).sum())
+        wrong += int((d64 > best + 1e-6 * (1 + best)).sum())
+        rel_max = max(rel_max, float(((dist[idx].double() - d64).abs() / (1.0 + d64)).max()))
     return {
-        "label_disagreement": float((lab != best_idx).double().mean()),
-        "true_label_errors": float(wrong.double().mean()),
-        "dist_err_max_rel_1pd": float(rel.max()),
         "checked_rows": rows,
+        "label_disagreements": dis,
+        "true_label_errors": wrong,
+        "label_disagreement": dis / rows,
+        "true_label_error_rate": wrong / rows,
+        "dist_err_max_rel_1pd": rel_max,
     }
+
+
+def exact_reassign(x, cent, sample_rows=20000):
+    """Exact (FP32 search) re-assignment of every row of the CUDA tensor x to
+    the centroids `cent`: the final MSE (f64 mean), and the FP64 brute-force
+    check of those labels and distances on a fixed row sample."""
+    import torch
+
+    import emeans_hip as E
+
+    a = E.HipAssigner("fp32")
+    a.set_centroids(cent)
+    n = x.shape[0]
+    labels = torch.empty(n, dtype=torch.int32, device=x.device)
+    dist = torch.empty(n, dtype=torch.float32, device=x.device)
+    a.assign_device(x, labels, dist)
+    a.synchronize()
+    q = None
+    if sample_rows:
+        c = torch.from_numpy(np.ascontiguousarray(cent)).to(x.device)
+        q = fp64_check(x if x.dtype == torch.float32 else x.float(), c, labels, dist, sample_rows)
+    return float(dist.double().mean()), q
 
 
 def cpu_search_baseline(x, c, seconds, i8=False, i16=False):
@@ -224,7 +265,8 @@ def cpu_search_baseline(x, c, seconds, i8=False, i16=False):
     import oracle as O
 
     cores = O.default_threads()
-    xh = x.cpu().numpy() if hasattr(x, "cpu") else x
+    # at most 1M rows come to the host: a sample of 10-30 s of CPU work is far smaller
+    xh = x[:1_000_000].cpu().numpy() if hasattr(x, "cpu") else x
     ch = c.cpu().numpy() if hasattr(c, "cpu") else c
 
     def run(rows):
@@ -244,13 +286,16 @@ def cpu_search_baseline(x, c, seconds, i8=False, i16=False):
         sample = int(min(xh.shape[0], max(2 * sample, sample * 0.9 * seconds / max(dt, 1e-3))))
     kind = ("AVX-512 VNNI" if O.has_vnni() else "portable") if i8 else \
         ("AVX-512" if O.lib().or_simd_best() == O.SIMD_AVX512 else "portable")
+    pinned = O.pins_threads()
     return {
         "value": sample / dt,
         "unit": "rows/s",
         "cores": cores,
         "kind": "port",
+        "isa": kind,
+        "threads_pinned": pinned,
         "sample": f"first {sample} rows of the same workload against all {ch.shape[0]} centroids "
-                  f"({kind} kernel, {cores} threads, {dt:.1f} s)",
+                  f"({kind} kernel, {cores} threads{' pinned one per core' if pinned else ''}, {dt:.1f} s)",
         "gops": 2.0 * sample * ch.shape[0] * ch.shape[1] / dt / 1e9,
     }
 
@@ -268,20 +313,20 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
     variants[prec] = head
     for cprec, cm in compare:
         log(f"timing {cprec} (top-{cm})")
-        steps = max(2, args.steps // 4) if cprec == "fp32" else args.steps
-        variants[cprec] = run_search(cprec, cm, x, c_host, steps, 1)
+        variants[cprec] = run_search(cprec, cm, x, c_host, args.steps, args.warmup)
     quality = {}
     integer = prec in ("i8", "i16")
     if not integer:
         for name, v in variants.items():
-            q = fp64_check(x, c, v, min(args.check_rows, n))
+            log(f"fp64 check ({name}, {'all' if args.check_rows <= 0 else args.check_rows} rows)")
+            q = fp64_check(x, c, v["labels"], v["dist"], None if args.check_rows <= 0 else args.check_rows)
             q["mse"] = float(v["dist"].double().mean())
             quality[name] = q
         if "fp32" in quality:
             for name, q in quality.items():
                 q["mse_rel_vs_fp32"] = q["mse"] / quality["fp32"]["mse"] - 1.0
     flops = 2.0 * n * k * d
-    achieved = OPS_PER_PRODUCT.get(prec, 1) * flops / (head["search_ms"] * 1e-3) / 1e12
+    achieved = flops / (head["search_ms"] * 1e-3) / 1e12
     roofline = {
         "bound": "mfma",
         "achieved": round(achieved, 2),
@@ -292,6 +337,12 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         "kernel": f"search_reg_kernel ({prec})",
         "kernel_ms": round(head["search_ms"], 4),
     }
+    if prec in OPS_PER_PRODUCT:
+        # I16 runs three i8 MFMA planes per product: the executed rate beside
+        # the algorithmic one
+        ex = OPS_PER_PRODUCT[prec] * achieved
+        roofline["executed"] = {"achieved": round(ex, 2), "frac": round(ex / PEAK[prec], 4),
+                                "planes": OPS_PER_PRODUCT[prec]}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline")
@@ -408,13 +459,16 @@ def quantize_line(args, n, d):
 
         log("cpu baseline (oracle quantizer on a sample)")
         xs = x[: min(n, 200000)].cpu().numpy()
+        th = O.default_threads()
         t = time.perf_counter()
-        q = O.Quantizer.fit(xs)
-        q.encode(xs)
+        q = O.Quantizer.fit(xs, nthreads=th)
+        q.encode(xs, nthreads=th)
         dt = time.perf_counter() - t
-        same = O.Quantizer(d, shift.cpu().numpy(), scale).encode(xs)
-        res["cpu_baseline"] = {"value": round(xs.shape[0] / dt, 1), "unit": "rows/s", "cores": 1, "kind": "port",
-                               "sample": f"fit + encode of the first {xs.shape[0]} rows (scalar C, {dt:.2f} s)"}
+        same = O.Quantizer(d, shift.cpu().numpy(), scale).encode(xs, nthreads=th)
+        res["cpu_baseline"] = {"value": round(xs.shape[0] / dt, 1), "unit": "rows/s", "cores": th, "kind": "port",
+                               "threads_pinned": O.pins_threads(),
+                               "sample": f"fit + encode of the first {xs.shape[0]} rows (C, the reference's parallel "
+                                         f"loops on {th} pinned threads, sequential f64 mean; {dt:.2f} s)"}
         res["sample_codes_equal_cpu"] = bool(np.array_equal(same, codes[: xs.shape[0]].cpu().numpy()))
     return res
 
@@ -549,7 +603,7 @@ def train_batch_line(args, njobs, prec, m):
     t = time.perf_counter()
     got, st = E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
     bs = time.perf_counter() - t
-    rel, ls, loop_steps = [0.0], float("nan"), 0
+    rel, seed_rel, ls, loop_steps = [0.0], [0.0], float("nan"), 0
     if not args.no_compare:
         log("per-job fp32 loop")
         for x, k, cfg in probs:  # the same warm-up: each job's shapes once, one epoch
@@ -558,12 +612,16 @@ def train_batch_line(args, njobs, prec, m):
         ref = [E.train(x, k, cfg, E.HipAssigner("fp32")) for x, k, cfg in probs]
         ls = time.perf_counter() - t
         log("exact re-assignment MSE")
-        rel = []
+        rel, seed_rel = [], []
         for (x, k, cfg), g, r in zip(probs, got, ref):
-            xh = x.cpu().numpy()
-            a = E.assign(xh, g.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
-            b = E.assign(xh, r.centroids, E.HipAssigner("fp32"))[1].astype(np.float64).mean()
+            a = exact_reassign(x, g.centroids, 0)[0]
+            b = exact_reassign(x, r.centroids, 0)[0]
             rel.append(a / b - 1.0)
+            # the same FP32 loop with the next seed: how far two k-means
+            # trajectories of one job land apart without any search change
+            r2 = E.train(x, k, E.Config(nepochs=cfg.nepochs, balance=cfg.balance, seed=cfg.seed + 1),
+                         E.HipAssigner("fp32"))
+            seed_rel.append(exact_reassign(x, r2.centroids, 0)[0] / b - 1.0)
         loop_steps = sum(int(r.recipe.total_steps) for r in ref)
     res = {
         "metric": "train rows/s (sum over jobs of epochs x n / wall time)",
@@ -590,7 +648,11 @@ def train_batch_line(args, njobs, prec, m):
         "fp32_job_loop": {"rows_per_s": round(rows / ls, 1), "gpu_seconds": round(ls, 3),
                           "train_steps": loop_steps},
         "speedup_vs_fp32_loop": round(ls / bs, 3),
-        "mse_rel_vs_fp32_loop": {"mean": float(np.mean(rel)), "max": float(np.max(rel)), "min": float(np.min(rel))},
+        "mse_rel_vs_fp32_loop": {"mean": float(np.mean(rel)), "max": float(np.max(rel)), "min": float(np.min(rel)),
+                                 "abs_mean": float(np.mean(np.abs(rel)))},
+        "mse_rel_fp32_next_seed": ({"mean": float(np.mean(seed_rel)), "max": float(np.max(seed_rel)),
+                                    "min": float(np.min(seed_rel)), "abs_mean": float(np.mean(np.abs(seed_rel)))}
+                                   if not args.no_compare else None),
     }
     if not args.no_cpu:
         log("cpu baseline (search on a sample of the largest job)")
@@ -612,8 +674,7 @@ def train_large_line(args, x, k, nepochs, prec, m):
 
     import emeans_hip as E
 
-    xh = x.cpu().numpy()
-    n, d = xh.shape
+    n, d = x.shape
     log(f"train {prec} warmup")
     # the same k and batch size for two batches: the search's per-shape
     # split choice and first-call costs stay out of the timed call
@@ -626,10 +687,9 @@ def train_large_line(args, x, k, nepochs, prec, m):
         t = time.perf_counter()
         out = E.train(x, k, E.Config(nepochs=nepochs), E.HipAssigner(p, rescore_m=mm))  # rows in HBM
         secs = time.perf_counter() - t
-        log(f"exact re-assignment MSE ({name})")
-        _, dist = E.assign(xh, out.centroids, E.HipAssigner("fp32"))
-        runs[name] = {"seconds": secs, "loop_seconds": out.seconds, "out": out,
-                      "mse": float(dist.astype(np.float64).mean())}
+        log(f"exact re-assignment MSE + fp64 check ({name})")
+        mse_v, q = exact_reassign(x, out.centroids)
+        runs[name] = {"seconds": secs, "loop_seconds": out.seconds, "out": out, "mse": mse_v, "fp64": q}
     head = runs[prec]
     out = head["out"]
     res = {
@@ -650,6 +710,7 @@ def train_large_line(args, x, k, nepochs, prec, m):
         "gpu_seconds": round(head["seconds"], 3),
         "gpu_loop_seconds": round(head["loop_seconds"], 3),
         "final_mse_exact_reassign": head["mse"],
+        "final_assign_vs_fp64": head["fp64"],
         "fp32_search_run": {"rows_per_s": round(nepochs * n / runs["fp32"]["seconds"], 1),
                             "gpu_seconds": round(runs["fp32"]["seconds"], 3),
                             "final_mse_exact_reassign": runs["fp32"]["mse"]},
@@ -660,7 +721,7 @@ def train_large_line(args, x, k, nepochs, prec, m):
     }
     if not args.no_cpu:
         log("cpu baseline (search on a sample)")
-        cpu = cpu_search_baseline(xh, out.centroids, args.cpu_seconds)
+        cpu = cpu_search_baseline(x, out.centroids, args.cpu_seconds)
         cpu["sample"] = "search only: " + cpu["sample"]
         res["cpu_baseline"] = cpu
     return res
@@ -688,8 +749,7 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     rows_per_s = nepochs * n / gpu_s
 
     def mse(cent):
-        lab, dist = E.assign(mse_x if mse_x is not None else xh, cent, E.HipAssigner("fp32"))
-        return float(dist.astype(np.float64).mean())
+        return exact_reassign(x, cent)
 
     res = {
         "metric": "train rows/s (epochs x n / wall time)",
@@ -708,10 +768,11 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
                    "batch": int(out.recipe.batch_size), "precision": prec},
         "gpu_seconds": round(gpu_s, 3),
         "gpu_loop_seconds": round(out.seconds, 3),
-        "final_mse_exact_reassign": mse(out.centroids),
         "n_splits": out.n_splits,
         "objective": out.objective,
     }
+    log("exact re-assignment + fp64 check")
+    res["final_mse_exact_reassign"], res["final_assign_vs_fp64"] = mse(out.centroids)
     if not args.no_cpu:
         import oracle as O
 
@@ -727,7 +788,7 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
             "sample": f"the full training run ({cpu_s:.1f} s) with the oracle's train and "
                       f"{'int8 VNNI' if oracle_assigner else 'AVX-512 f32'} search",
         }
-        res["cpu_final_mse_exact_reassign"] = mse(ref.centroids)
+        res["cpu_final_mse_exact_reassign"] = mse(ref.centroids)[0]
         res["identical_to_cpu"] = bool(np.array_equal(ref.centroids, out.centroids)
                                        and np.array_equal(ref.labels, out.labels))
     return res

@@ -124,6 +124,9 @@ def lib():
         L.or_i8_search.argtypes = [i32, i32, i32, vp, sz, sz, vp, sz, vp, vp, vp]
         L.or_quant_fit.argtypes = [vp, sz, sz, C.c_float, C.c_double, i32, vp, vp]
         L.or_quant_encode.argtypes = [vp, sz, sz, vp, C.c_float, vp]
+        L.or_quant_fit_mt.argtypes = [i32, vp, sz, sz, C.c_float, C.c_double, i32, vp, vp]
+        L.or_quant_encode_mt.argtypes = [i32, vp, sz, sz, vp, C.c_float, vp]
+        L.or_pool_pins_threads.restype = i32
         L.or_quant_decode.argtypes = [vp, sz, sz, vp, C.c_float, vp]
         L.or_quant_clipped_fraction.argtypes = [vp, sz, sz, vp, C.c_float]
         L.or_quant_clipped_fraction.restype = C.c_double
@@ -142,6 +145,11 @@ def lib():
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def pins_threads() -> bool:
+    """The oracle's thread pool pins worker t to the t-th allowed CPU."""
+    return bool(lib().or_pool_pins_threads())
 
 
 def default_threads() -> int:
@@ -435,19 +443,20 @@ class Quantizer:
     scale: float
 
     @staticmethod
-    def fit(sample, target_peak=112.0, quantile=1.0, power_of_two=True) -> "Quantizer":
+    def fit(sample, target_peak=112.0, quantile=1.0, power_of_two=True, nthreads=1) -> "Quantizer":
         sample = np.ascontiguousarray(sample, np.float32)
         n, d = sample.shape
         shift = np.empty(d, np.float32)
         scale = C.c_float()
-        lib().or_quant_fit(_p(sample), n, d, C.c_float(target_peak), C.c_double(quantile), int(power_of_two),
-                           _p(shift), C.byref(scale))
+        lib().or_quant_fit_mt(int(nthreads), _p(sample), n, d, C.c_float(target_peak), C.c_double(quantile),
+                              int(power_of_two), _p(shift), C.byref(scale))
         return Quantizer(d, shift, float(scale.value))
 
-    def encode(self, x):
+    def encode(self, x, nthreads=1):
         x = np.ascontiguousarray(x, np.float32).reshape(-1, self.d)
         out = np.empty(x.shape, np.int8)
-        lib().or_quant_encode(_p(x), x.shape[0], self.d, _p(self.shift), C.c_float(self.scale), _p(out))
+        lib().or_quant_encode_mt(int(nthreads), _p(x), x.shape[0], self.d, _p(self.shift), C.c_float(self.scale),
+                                 _p(out))
         return out
 
     def decode(self, codes):

@@ -413,6 +413,19 @@ static void *pool_worker(void *arg) {
     return NULL;
 }
 
+/* Worker t runs on the t-th CPU of the caller's affinity mask (the
+ * caller is worker 0 and gets its own mask back afterwards), so a timed run
+ * uses each allowed core once. */
+static int allowed_cpus(int *cpus, int cap) {
+    cpu_set_t all;
+    CPU_ZERO(&all);
+    if (sched_getaffinity(0, sizeof all, &all) != 0) return 0;
+    int n = 0;
+    for (int c = 0; c < CPU_SETSIZE && n < cap; c++)
+        if (CPU_ISSET(c, &all)) cpus[n++] = c;
+    return n;
+}
+
 static void run_pool(int nthreads, size_t nblocks, void *ctx, void (*fn)(void *, size_t)) {
     or_pool p;
     atomic_init(&p.next, 0);
@@ -426,9 +439,36 @@ static void run_pool(int nthreads, size_t nblocks, void *ctx, void (*fn)(void *,
     if ((size_t)nthreads > nblocks) nthreads = (int)nblocks;
     pthread_t th[256];
     if (nthreads > 256) nthreads = 256;
-    for (int t = 1; t < nthreads; t++) pthread_create(&th[t], NULL, pool_worker, &p);
+    int cpus[256];
+    const int ncpu = allowed_cpus(cpus, 256);
+    for (int t = 1; t < nthreads; t++) {
+        pthread_attr_t at;
+        pthread_attr_init(&at);
+        if (ncpu > 0) {
+            cpu_set_t one;
+            CPU_ZERO(&one);
+            CPU_SET(cpus[t % ncpu], &one);
+            pthread_attr_setaffinity_np(&at, sizeof one, &one);
+        }
+        pthread_create(&th[t], &at, pool_worker, &p);
+        pthread_attr_destroy(&at);
+    }
+    cpu_set_t mine;
+    const int saved = ncpu > 0 && pthread_getaffinity_np(pthread_self(), sizeof mine, &mine) == 0;
+    if (saved) {
+        cpu_set_t one;
+        CPU_ZERO(&one);
+        CPU_SET(cpus[0], &one);
+        pthread_setaffinity_np(pthread_self(), sizeof one, &one);
+    }
     pool_worker(&p);
     for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+    if (saved) pthread_setaffinity_np(pthread_self(), sizeof mine, &mine);
+}
+
+int or_pool_pins_threads(void) {
+    int cpus[1];
+    return allowed_cpus(cpus, 1) > 0;
 }
 
 typedef struct {
@@ -827,9 +867,37 @@ static float select_nth_total(float *a, size_t len, size_t idx) {
     return a[idx];
 }
 
-void or_quant_fit(const float *sample, size_t n, size_t d, float target_peak, double quantile, int power_of_two,
-                  float *shift, float *scale) {
-    /* Int8Quantizer::fit, src/quantize.rs:45-62 */
+typedef struct {
+    const float *x;
+    size_t n, d;
+    const float *shift;
+    float scale;
+    int8_t *out;   /* encode */
+    float *mags;   /* fit: |x - shift| */
+} quant_job;
+
+#define QUANT_BLOCK 1024
+
+static void mags_block(void *ctx, size_t b) {
+    const quant_job *J = ctx;
+    const size_t r1 = (b + 1) * QUANT_BLOCK < J->n ? (b + 1) * QUANT_BLOCK : J->n;
+    for (size_t r = b * QUANT_BLOCK; r < r1; r++)
+        for (size_t t = 0; t < J->d; t++) J->mags[r * J->d + t] = fabsf(J->x[r * J->d + t] - J->shift[t]);
+}
+
+static void encode_block(void *ctx, size_t b) {
+    const quant_job *J = ctx;
+    const size_t r1 = (b + 1) * QUANT_BLOCK < J->n ? (b + 1) * QUANT_BLOCK : J->n;
+    for (size_t r = b * QUANT_BLOCK; r < r1; r++)
+        for (size_t t = 0; t < J->d; t++)
+            J->out[r * J->d + t] = (int8_t)clampf(roundf((J->x[r * J->d + t] - J->shift[t]) * J->scale), -128.0f, 127.0f);
+}
+
+void or_quant_fit_mt(int nthreads, const float *sample, size_t n, size_t d, float target_peak, double quantile,
+                     int power_of_two, float *shift, float *scale) {
+    /* Int8Quantizer::fit, src/quantize.rs:45-62: the per-dimension means in
+     * row order (a sequential f64 fold, as the reference), the magnitudes
+     * in parallel row blocks (par_chunks_exact), one selection */
     double *mean = calloc(d, sizeof(double));
     for (size_t r = 0; r < n; r++)
         for (size_t t = 0; t < d; t++) mean[t] += (double)sample[r * d + t];
@@ -837,8 +905,8 @@ void or_quant_fit(const float *sample, size_t n, size_t d, float target_peak, do
     free(mean);
     size_t len = n * d;
     float *mags = malloc(len * sizeof(float));
-    for (size_t r = 0; r < n; r++)
-        for (size_t t = 0; t < d; t++) mags[r * d + t] = fabsf(sample[r * d + t] - shift[t]);
+    quant_job J = {sample, n, d, shift, 0.0f, NULL, mags};
+    run_pool(nthreads, (n + QUANT_BLOCK - 1) / QUANT_BLOCK, &J, mags_block);
     double q = quantile < 0.0 ? 0.0 : (quantile > 1.0 ? 1.0 : quantile);
     size_t idx = (size_t)round((double)(len - 1) * q);
     float peak = select_nth_total(mags, len, idx);
@@ -848,11 +916,20 @@ void or_quant_fit(const float *sample, size_t n, size_t d, float target_peak, do
     *scale = s;
 }
 
+void or_quant_fit(const float *sample, size_t n, size_t d, float target_peak, double quantile, int power_of_two,
+                  float *shift, float *scale) {
+    or_quant_fit_mt(1, sample, n, d, target_peak, quantile, power_of_two, shift, scale);
+}
+
+void or_quant_encode_mt(int nthreads, const float *x, size_t n, size_t d, const float *shift, float scale,
+                        int8_t *out) {
+    /* src/quantize.rs:65-73 (par_chunks_mut over rows) */
+    quant_job J = {x, n, d, shift, scale, out, NULL};
+    run_pool(nthreads, (n + QUANT_BLOCK - 1) / QUANT_BLOCK, &J, encode_block);
+}
+
 void or_quant_encode(const float *x, size_t n, size_t d, const float *shift, float scale, int8_t *out) {
-    /* src/quantize.rs:65-73 */
-    for (size_t r = 0; r < n; r++)
-        for (size_t t = 0; t < d; t++)
-            out[r * d + t] = (int8_t)clampf(roundf((x[r * d + t] - shift[t]) * scale), -128.0f, 127.0f);
+    or_quant_encode_mt(1, x, n, d, shift, scale, out);
 }
 
 void or_quant_decode(const float *codes, size_t n, size_t d, const float *shift, float scale, float *out) {

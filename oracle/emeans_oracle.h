@@ -97,6 +97,13 @@ int or_i8_search(int bits, int vnni, int nthreads, const int8_t *x, size_t n, si
 void or_quant_fit(const float *sample, size_t n, size_t d, float target_peak, double quantile, int power_of_two,
                   float *shift, float *scale);
 void or_quant_encode(const float *x, size_t n, size_t d, const float *shift, float scale, int8_t *out);
+/* the same with the reference's parallel loops on nthreads pinned threads */
+void or_quant_fit_mt(int nthreads, const float *sample, size_t n, size_t d, float target_peak, double quantile,
+                     int power_of_two, float *shift, float *scale);
+void or_quant_encode_mt(int nthreads, const float *x, size_t n, size_t d, const float *shift, float scale,
+                        int8_t *out);
+/* nonzero: pool threads are pinned one per CPU of the affinity mask */
+int or_pool_pins_threads(void);
 void or_quant_decode(const float *codes, size_t n, size_t d, const float *shift, float scale, float *out);
 double or_quant_clipped_fraction(const float *x, size_t n, size_t d, const float *shift, float scale);
 
