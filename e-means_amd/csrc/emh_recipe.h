@@ -105,7 +105,7 @@ static uint64_t round_pow2(double v) {
 }
 
 static int resolve(uint64_t n, uint64_t d, uint64_t k, const emh_config &cfg, emh_recipe &r) {
-    if (!(cfg.nepochs > 0.0) || cfg.nepochs >= 63.0) return -1;
+    if (!(cfg.nepochs > 0.0) || cfg.nepochs >= (double)(EMH_MAX_EPOCHS - 1)) return -1;
     int law = cfg.batch_law;
     if (law == 0) law = d <= AUTO_QUALITY_MAX_D ? 2 : 1;
     uint64_t batch;
@@ -414,7 +414,7 @@ struct JobRun {
         epoch_obj += batch_obj;
         mean_sqdist = batch_obj / (double)m.nb;
         if (m.closes_epoch) {
-            if (out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+            if (out->n_objective < EMH_MAX_EPOCHS) out->objective[out->n_objective++] = epoch_obj;
             epoch_obj = 0.0;
         }
     }
@@ -429,7 +429,7 @@ struct JobRun {
         step++;
         b0 += nb;
         if (b0 >= R.epoch_rows[epoch]) {
-            if (!deferred && out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+            if (!deferred && out->n_objective < EMH_MAX_EPOCHS) out->objective[out->n_objective++] = epoch_obj;
             epoch++;
             b0 = 0;
             if (epoch == R.n_epochs) done = true;

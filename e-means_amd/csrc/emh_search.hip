@@ -108,10 +108,17 @@ template <int ROWS> __device__ inline float src_elem(const void *x, size_t idx) 
     else return (float)((const uint8_t *)x)[idx];
 }
 
-// OCP e4m3 power-of-two scale mapping |v| <= amax into [0, 448].
+// OCP e4m3 power-of-two scale 2^p mapping |v| <= amax into [0, 448]: the
+// largest p with amax 2^p <= 448 = 0.875 2^9, from amax's exact exponent,
+// clamped so that 2^p is a normal float and 2^-p an E8M0 value (tiny
+// inputs then keep a finite scale).
 __device__ inline float fp8_scale_for(float amax) {
     if (!(amax > 0.0f) || !isfinite(amax)) return 1.0f;
-    return exp2f(floorf(log2f(448.0f / amax)));
+    int e;
+    const float m = frexpf(amax, &e);  // amax = m 2^e, m in [0.5, 1)
+    int p = (m <= 0.875f ? 9 : 8) - e;
+    p = p < -126 ? -126 : (p > 127 ? 127 : p);
+    return ldexpf(1.0f, p);
 }
 
 struct SearchArgs {

@@ -79,7 +79,7 @@ class Recipe(C.Structure):
         ("total_steps", C.c_uint64),
         ("unused_threshold", C.c_float),
         ("n_epochs", C.c_uint32),
-        ("epoch_rows_", C.c_uint64 * 64),
+        ("epoch_rows_", C.c_uint64 * 1024),
     ]
 
     @property
@@ -91,7 +91,7 @@ class TrainOut(C.Structure):
     _fields_ = [
         ("n_splits", C.c_uint64),
         ("n_objective", C.c_uint32),
-        ("objective_", C.c_double * 64),
+        ("objective_", C.c_double * 1024),
         ("recipe", Recipe),
         ("seconds", C.c_double),
     ]

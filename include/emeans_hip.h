@@ -139,6 +139,11 @@ typedef struct {
     uint64_t seed;
 } emh_config;
 
+/* Entries of the per-epoch tables below: nepochs must be below
+ * EMH_MAX_EPOCHS - 1 (1023 epochs; resolve/train return EMH_E_ARG beyond).
+ * The reference's Vec has no bound (src/lib.rs:159, 180-192). */
+#define EMH_MAX_EPOCHS 1024
+
 /* Recipe (src/lib.rs:142-149). */
 typedef struct {
     uint64_t batch_size;
@@ -146,13 +151,13 @@ typedef struct {
     uint64_t total_steps;
     float unused_threshold;
     uint32_t n_epochs;
-    uint64_t epoch_rows[64];
+    uint64_t epoch_rows[EMH_MAX_EPOCHS];
 } emh_recipe;
 
 typedef struct {
     uint64_t n_splits;
     uint32_t n_objective;
-    double objective[64];
+    double objective[EMH_MAX_EPOCHS];
     emh_recipe recipe;
     double seconds;               /* wall time of the training loop */
 } emh_train_out;

@@ -160,7 +160,7 @@ static uint32_t epoch_rows(uint64_t n, double nepochs, uint64_t *out) {
         out[m++] = rem;
         return m;
     }
-    for (uint64_t i = 0; i + 1 < whole && m < 62; i++) out[m++] = n;
+    for (uint64_t i = 0; i + 1 < whole && m < 1021; i++) out[m++] = n;
     if (rem > 0) out[m++] = rem;
     out[m++] = n;
     return m;
@@ -175,7 +175,7 @@ double or_scheduled_alpha(double alpha0, uint64_t step, uint64_t total) {
 
 int or_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const or_config *cfg, or_recipe *out) {
     /* src/lib.rs:201-225 */
-    if (cfg->nepochs >= 63.0) return -1; /* epoch_rows table bound of this oracle */
+    if (cfg->nepochs >= 1023.0) return -1; /* epoch_rows table bound of this oracle */
     int law = cfg->batch_law;
     if (law == OR_LAW_AUTO) law = d <= AUTO_QUALITY_MAX_D ? OR_LAW_QUALITY : OR_LAW_THROUGHPUT;
     uint64_t batch;
@@ -1206,7 +1206,7 @@ int or_train(const void *x, int rows, size_t n, size_t d, size_t k, const or_con
                 n_splits += or_state_revive(k, d, C, usage, R.unused_threshold, &split_rng);
             step++;
         }
-        if (out->n_objective < 64) out->objective[out->n_objective++] = epoch_obj;
+        if (out->n_objective < 1024) out->objective[out->n_objective++] = epoch_obj;
     }
     if (rc == 0) {
         /* closing mean, src/lib.rs:481-489 */

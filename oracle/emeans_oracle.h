@@ -48,7 +48,7 @@ typedef struct {
     uint64_t total_steps;
     float unused_threshold;
     uint32_t n_epochs;          /* entries in epoch_rows */
-    uint64_t epoch_rows[64];
+    uint64_t epoch_rows[1024];
 } or_recipe;
 void or_config_default(or_config *cfg);
 int or_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const or_config *cfg, or_recipe *out);
@@ -126,7 +126,7 @@ enum { OR_ASSIGNER_F32 = 0, OR_ASSIGNER_I8 = 1, OR_ASSIGNER_I16 = 2 };
 typedef struct {
     uint64_t n_splits;
     uint32_t n_objective;
-    double objective[64];
+    double objective[1024];
     or_recipe recipe;
 } or_train_out;
 int or_train(const void *x, int rows, size_t n, size_t d, size_t k, const or_config *cfg, int assigner, int simd,

@@ -305,6 +305,27 @@ def test_candidates_keep_tied_centroids(hip, prec, m, k, d, where, split):
     assert np.all(got <= bd + 1e-6 * (1 + bd))
 
 
+@pytest.mark.parametrize("mag", [1e-39, 1e-30, 1.0, 1e15])
+def test_fp8_rows_of_extreme_magnitude(hip, mag):
+    """FP8 rows scaled to the f32 denormal range (largest |value| near
+    1e-39, where 448 / amax overflows), to 1e-30 and to 1e15 each get a
+    finite power-of-two scale: labels match FP64 brute force and distances
+    are finite and exact to the FP32 rescore. (One magnitude per call: the
+    positivity offset of the keys is taken per block of 256 rows.)"""
+    rng = np.random.default_rng(21)
+    d, k = 64, 200
+    c = rng.normal(size=(k, d)).astype(np.float32)
+    x = (rng.normal(size=(300, d)) * mag).astype(np.float32)
+    a = E.HipAssigner("fp8", rescore_m=8)
+    a.set_centroids(c)
+    labels, dist = a.assign(x)
+    assert np.all(np.isfinite(dist))
+    bl, bd = O.brute_f64(x, c)
+    got = O.dist_f64(x, c, labels)
+    assert np.all(got <= bd + 1e-6 * (1 + bd)), np.nonzero(got > bd + 1e-6 * (1 + bd))[0][:10]
+    assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+
+
 def test_bf16_odd_shapes(hip):
     for n, k, d in SHAPES:
         x = O.lcg_f32(n * d, 1).reshape(n, d)

@@ -36,6 +36,16 @@ def test_i8_training_is_bit_exact(hip, balance, bits):
     assert np.array_equal(got.centroids, ref.centroids)
 
 
+def test_i8_training_past_63_epochs(hip):
+    """100 epochs (past the old 64-entry tables) of int8 training equal the
+    oracle's bit for bit, with one objective entry per epoch."""
+    x = _i8_blobs(1500, 16, 12, 5)
+    ref = O.train(x, 10, O.Config(nepochs=100.0, seed=3), assigner=O.ASSIGNER_I8)
+    got = E.train(x, 10, E.Config(nepochs=100.0, seed=3), E.HipAssigner(E.I8))
+    assert len(got.objective) == 100 and got.objective == ref.objective
+    assert np.array_equal(got.centroids, ref.centroids) and np.array_equal(got.labels, ref.labels)
+
+
 def _blobs_6d():
     n, d = 20_000, 6
     r = O.OracleRng(7)

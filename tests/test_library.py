@@ -41,6 +41,20 @@ def test_host_recipe_matches_oracle(n, d, k, nep):
     assert a.unused_threshold == b.unused_threshold
 
 
+def test_many_epochs_up_to_the_documented_limit():
+    """EMH_MAX_EPOCHS = 1024 table entries: nepochs up to 1022.x resolves as
+    the oracle does; 1023 and above are refused (the reference's Vec has no
+    bound)."""
+    for nep in (63.0, 100.5, 1022.25):
+        a = E.resolve_recipe(5000, 16, 10, E.Config(nepochs=nep))
+        b = O.resolve_recipe(5000, 16, 10, O.Config(nepochs=nep))
+        assert a.epoch_rows == b.epoch_rows and a.total_steps == b.total_steps and a.batch_size == b.batch_size
+        assert len(a.epoch_rows) == int(np.ceil(nep))
+    with pytest.raises(E.EmeansError) as ei:
+        E.resolve_recipe(5000, 16, 10, E.Config(nepochs=1023.0))
+    assert ei.value.code == E.E_ARG
+
+
 def test_host_recipe_golden():
     # src/lib.rs:526-540 (faiss 1.16.0)
     r = E.resolve_recipe(200_000, 96, 1024)
