@@ -217,6 +217,87 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
     }
 }
 
+// The same sort in one pass over 11-bit digits (keys below 2048), for the
+// cluster labels of problems with k <= 2048: three launches instead of six.
+constexpr int WIDE_BINS = 2048;
+
+__global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *keys, size_t m, uint32_t *hist,
+                                                              size_t nblocks) {
+    __shared__ uint32_t h[WIDE_BINS];
+    for (int i = threadIdx.x; i < WIDE_BINS; i += 256) h[i] = 0;
+    __syncthreads();
+    const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
+    for (int r = 0; r < SORT_BLOCK / 256; r++) {
+        const size_t i = b0 + (size_t)r * 256 + threadIdx.x;
+        if (i < m) atomicAdd(&h[keys[i] & (WIDE_BINS - 1)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < WIDE_BINS; i += 256) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
+}
+
+// As digit_scatter_kernel; with 2048 digits the per-wave counts are not
+// cleared wholesale each round: the leader of a digit group adds its group
+// to the digit's running count and clears its own entry after the round.
+__global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
+                                                                 const uint32_t *offs, size_t nblocks,
+                                                                 const uint32_t *totals, uint32_t *keys_out,
+                                                                 uint32_t *vals_out) {
+    constexpr int PER = WIDE_BINS / 256;
+    __shared__ uint32_t run[WIDE_BINS];
+    __shared__ uint32_t wcnt[4][WIDE_BINS];
+    __shared__ uint32_t sbase[WIDE_BINS];
+    __shared__ uint32_t ssm[4];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    {
+        // exclusive scan of the digit totals, PER consecutive digits a thread
+        uint32_t loc[PER], tsum = 0;
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            loc[i] = tsum;
+            tsum += totals[tid * PER + i];
+        }
+        uint32_t all;
+        const uint32_t base = block_exclusive_scan(tsum, ssm, &all);
+#pragma unroll
+        for (int i = 0; i < PER; i++) {
+            sbase[tid * PER + i] = base + loc[i];
+            run[tid * PER + i] = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) wcnt[q][tid * PER + i] = 0;
+        }
+    }
+    __syncthreads();
+    const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+    for (int r = 0; r < SORT_BLOCK / 256; r++) {
+        const size_t i = b0 + (size_t)r * 256 + tid;
+        const bool live = i < m;
+        const uint32_t key = live ? keys[i] : 0u;
+        const uint32_t dg = key & (WIDE_BINS - 1);
+        uint64_t same = __ballot(live);
+        for (int bit = 0; bit < 11; bit++) {
+            const uint64_t bl = __ballot(live && ((dg >> bit) & 1u));
+            same &= ((dg >> bit) & 1u) ? bl : ~bl;
+        }
+        const uint32_t before = (uint32_t)__popcll(same & lt), cnt = (uint32_t)__popcll(same);
+        const bool leader = live && before == 0;
+        if (leader) wcnt[w][dg] = cnt;
+        __syncthreads();
+        if (live) {
+            uint32_t pos = sbase[dg] + offs[(size_t)dg * nblocks + blockIdx.x] + run[dg] + before;
+            for (int q = 0; q < w; q++) pos += wcnt[q][dg];
+            keys_out[pos] = key;
+            vals_out[pos] = vals ? vals[i] : (uint32_t)i;
+        }
+        __syncthreads();
+        if (leader) {
+            atomicAdd(&run[dg], cnt);
+            wcnt[w][dg] = 0;
+        }
+        __syncthreads();
+    }
+}
+
 hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
     (void)k;  // the scratch depends on the batch only
     hipError_t e = hipSuccess;
@@ -236,7 +317,7 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         a.big = nullptr;
         if ((e = hipMalloc(&a.big, (2 + 3 * (m / SUM_BIG + 1)) * 4)) != hipSuccess) return e;
         const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
-        a.sort_tmp_bytes = 256 * nblocks * 4 + 256 * 4;  // histogram + digit totals
+        a.sort_tmp_bytes = WIDE_BINS * nblocks * 4 + WIDE_BINS * 4;  // histogram + digit totals
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
         a.cap_rows = m;
     }
@@ -262,6 +343,17 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
     const uint32_t *kin = keys_in, *vin = vals_in;
     uint32_t *kb[2] = {a.keys, a.keys_alt}, *vb[2] = {a.vals, a.vals_alt};
     int cur = 0;
+    if (bits > 8 && bits <= 11) {
+        hipLaunchKernelGGL(digit_hist_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist, nblocks);
+        hipLaunchKernelGGL(digit_row_scan_kernel, dim3(WIDE_BINS), dim3(256), 0, s, hist, nblocks,
+                           hist + (size_t)WIDE_BINS * nblocks);
+        hipLaunchKernelGGL(digit_scatter_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m,
+                           (const uint32_t *)hist, nblocks, (const uint32_t *)(hist + (size_t)WIDE_BINS * nblocks),
+                           kb[0], vb[0]);
+        *keys_out = kb[0];
+        *vals_out = vb[0];
+        return hipGetLastError();
+    }
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, shift, hist, nblocks);
         hipLaunchKernelGGL(digit_row_scan_kernel, dim3(256), dim3(256), 0, s, hist, nblocks, hist + 256 * nblocks);
@@ -696,14 +788,26 @@ __device__ __forceinline__ void ema_cluster(float *cen, float *usage, const floa
 // clear_counts (the counts array itself, or null): each count is zeroed once
 // its wave has read it, and workgroup 0 zeroes *clear_word, so the next
 // step's accumulate needs no clearing launch.
+// pass_sums / pass_counts (the last epoch, else null): each cluster's f32
+// sums and count are first added to the FP64 pass totals (what
+// pass_accumulate_kernel does, before the EMA consumes them).
 __global__ __launch_bounds__(256) void ema_apply_kernel(float *cen, float *usage, const float *sums,
                                                         const uint32_t *counts, size_t k, size_t d, float alpha,
                                                         float decay, int replace, uint32_t *clear_counts,
-                                                        uint32_t *clear_word) {
+                                                        uint32_t *clear_word, double *pass_sums,
+                                                        uint64_t *pass_counts) {
     if (clear_word && blockIdx.x == 0 && threadIdx.x == 0) *clear_word = 0u;
     const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= k) return;
-    ema_cluster(cen, usage, sums, counts, j, d, alpha, decay, replace, threadIdx.x & 63);
+    const int lane = threadIdx.x & 63;
+    if (pass_sums) {
+        const uint32_t n = counts[j];
+        if (n) {
+            for (size_t t = lane; t < d; t += 64) pass_sums[j * d + t] += (double)sums[j * d + t];
+            if (lane == 0) pass_counts[j] += n;
+        }
+    }
+    ema_cluster(cen, usage, sums, counts, j, d, alpha, decay, replace, lane);
     if (clear_counts && (threadIdx.x & 63) == 0) clear_counts[j] = 0u;
 }
 
@@ -777,11 +881,12 @@ __global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint
 }
 
 hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
-                            float alpha, hipStream_t s, uint32_t *clear_counts, uint32_t *clear_word) {
+                            float alpha, hipStream_t s, uint32_t *clear_counts, uint32_t *clear_word,
+                            double *pass_sums, uint64_t *pass_counts) {
     const int replace = alpha == 1.0f;
     const float decay = 1.0f - alpha;
     hipLaunchKernelGGL(ema_apply_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
-                       alpha, decay, replace, clear_counts, clear_word);
+                       alpha, decay, replace, clear_counts, clear_word, pass_sums, pass_counts);
     return hipGetLastError();
 }
 

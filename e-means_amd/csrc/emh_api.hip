@@ -203,7 +203,7 @@ static int search_stages(emh_assigner *h, const void *x_dev, int rows, size_t m,
     if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, ns, b.part)))
         return rc;
     if ((rc = mk(2))) return rc;
-    if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen_f32.as<float>(),
+    if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen(),
                                    h->has_bias ? h->bias.as<float>() : nullptr, g.k, b.cand, labels_dev, dist_dev,
                                    h->stream),
                     "rescore kernel")))
@@ -369,7 +369,7 @@ static int refresh_i8_range(emh_assigner *h, bool with_bias) {
 namespace emh {
 // set_centroids from host memory (synchronous: the caller may reuse the
 // table) or from device memory (stream-ordered).
-int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src) {
+int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src, bool borrow) {
     if (!h) return fail(EMH_E_ARG, "emh_set_centroids: NULL handle");
     if (k == 0 || d == 0 || !centroids) return fail(EMH_E_ARG, "set_centroids: need k > 0, d > 0 and a k x d table");
     if ((h->op == OP_I8 || h->op == OP_I16) && d > 32768)
@@ -378,7 +378,10 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
     SearchGeom g = make_geom(h->op, k, d);
-    if ((rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
+    // the rescoring searches read the f32 table after later model updates
+    // may have changed the caller's: they always keep a copy
+    borrow = borrow && device_src && h->op != OP_BF16 && h->op != OP_FP8;
+    if (!borrow && (rc = check(h->cen_f32.ensure(k * d * 4), "centroid buffer"))) return rc;
     if ((rc = check(h->cen_op.ensure(operand_bytes(g)), "operand buffer"))) return rc;
     if ((rc = check(h->norm32.ensure(g.k_pad * 4), "norm buffer"))) return rc;
     if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
@@ -389,24 +392,30 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
     h->g = g;
     h->has_centroids = false;
     h->has_bias = false;
-    if ((rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4,
-                                   device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
-                    "copy centroids")))
+    h->cen_ref = borrow ? centroids : nullptr;
+    if (!borrow && (rc = check(hipMemcpyAsync(h->cen_f32.p, centroids, k * d * 4,
+                                              device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, h->stream),
+                               "copy centroids")))
         return rc;
-    if ((rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
+    // FP32, and I8 where its range flag is not read (no bias, d below 1366),
+    // take their unbiased score offsets from the prep kernel itself: no
+    // stats words to clear, no separate offsets pass
+    const bool fused = h->op == OP_F32 || (h->op == OP_I8 && (int64_t)d * 49152 < ((int64_t)1 << 26));
+    if (!fused && (rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
     if (h->op == OP_FP8) {
         if ((rc = check(launch_fill_u32(h->cmax.as<uint32_t>(), 1, 0u, h->stream), "fill"))) return rc;
-        if ((rc = check(launch_absmax(h->cen_f32.as<float>(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax")))
-            return rc;
+        if ((rc = check(launch_absmax(h->cen(), k * d, h->cmax.as<unsigned>(), h->stream), "absmax"))) return rc;
     }
-    if ((rc = check(launch_prep_centroids(g, h->cen_f32.as<float>(), h->cen_op.p, h->norm32.as<float>(),
-                                          h->norm64.as<int64_t>(), h->cmax.as<unsigned>(), h->stats.as<unsigned>(),
-                                          h->stream),
+    if ((rc = check(launch_prep_centroids(g, h->cen(), h->cen_op.p, h->norm32.as<float>(), h->norm64.as<int64_t>(),
+                                          h->cmax.as<unsigned>(), h->stats.as<unsigned>(), h->stream,
+                                          fused && h->op == OP_F32 ? h->hn.as<float>() : nullptr,
+                                          fused && h->op == OP_I8 ? h->cn.as<int64_t>() : nullptr),
                     "centroid prep")))
         return rc;
-    if ((rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr, h->hn.as<float>(),
-                                    h->cn.as<int64_t>(), h->stats.as<unsigned>(), h->stream),
-                    "score offsets")))
+    if (!fused && (rc = check(launch_set_bias(g, h->norm32.as<float>(), h->norm64.as<int64_t>(), nullptr,
+                                              h->hn.as<float>(), h->cn.as<int64_t>(), h->stats.as<unsigned>(),
+                                              h->stream),
+                              "score offsets")))
         return rc;
     // The caller may reuse `centroids` on return (pageable source).
     if (!device_src && (rc = check(hipStreamSynchronize(h->stream), "set_centroids"))) return rc;

@@ -55,6 +55,11 @@ struct emh_assigner {
 
     // centroid side (set_centroids / set_bias)
     emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax, stats;
+    // the f32 centroid table the search reads: cen_f32, or (training's
+    // per-step snapshot of a search without a rescore) the caller's device
+    // table itself, valid until the next set_centroids
+    const float *cen_ref = nullptr;
+    const float *cen() const { return cen_ref ? cen_ref : cen_f32.as<float>(); }
     // row side (per assign call)
     emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale, row_norm;
     // centroid-range split of the register-resident search (autotuned)
@@ -80,7 +85,11 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
                float *dist_dev);
 int fail(int code, const std::string &msg);
 // set_centroids from host (synchronous) or device (stream-ordered) memory.
-int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src);
+// borrow (device source, FP32 / I8 / I16 only): read the caller's table in
+// place instead of copying it; it must stay unchanged until the searches
+// queued before the next set_centroids have run.
+int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t d, bool device_src,
+                       bool borrow = false);
 // set_bias from host memory; `sync` false leaves it stream-ordered (the
 // caller keeps `bias`, pinned, unchanged until the stream has passed it),
 // except for the I8 search, which reads its range flag back.

@@ -65,8 +65,11 @@ inline const void *reg_operand(const SearchGeom &g, const void *cen_op) {
 // stats (2 x u32, f32 bits): [0] max_j |c_j|^2 (set by the prep kernel),
 // [1] max(0, -min_j bias_j) (set by the bias kernel); both start at 0.
 // cen_op holds operand_bytes(g): the prep launcher fills both layouts.
+// hn_out / cn_out: also write the unbiased score offsets (what
+// launch_set_bias without a bias writes) in the same pass.
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax_bits, unsigned *stats, hipStream_t s);
+                                 const unsigned *cmax_bits, unsigned *stats, hipStream_t s, float *hn_out = nullptr,
+                                 int64_t *cn_out = nullptr);
 hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s);
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
                            float *hn, int64_t *cn, unsigned *stats, hipStream_t s);
@@ -220,9 +223,12 @@ hipError_t launch_closing_mean(const double *pass_sums, const uint64_t *pass_cou
 hipError_t launch_fill_u32(uint32_t *p, size_t count, uint32_t v, hipStream_t s);
 // The EMA step of src/lib.rs:287-309 on device centroids/usage (k x d, k).
 // With clear_counts it also zeroes each count once read, and *clear_word.
+// With pass_sums / pass_counts (the last epoch) it first adds each
+// cluster's sums and count to the FP64 pass totals (launch_pass_accumulate).
 hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const uint32_t *counts, size_t k, size_t d,
                             float alpha, hipStream_t s, uint32_t *clear_counts = nullptr,
-                            uint32_t *clear_word = nullptr);
+                            uint32_t *clear_word = nullptr, double *pass_sums = nullptr,
+                            uint64_t *pass_counts = nullptr);
 // The revival row split of src/lib.rs:330-340 for (donor, unused) pairs.
 hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s);
 // out (m x d f32) = rows ids of x (f32 / i8 / u8), widened

@@ -1531,11 +1531,14 @@ constexpr int PREP_CENTROIDS = 16;
 
 // tile_op (register-resident path, else nullptr): the same quads in the
 // tile-major order, quad q of centroid c of tile t at quad (t nq + q) tb + c.
+// hn_out / cn_out (else nullptr): the unbiased score offsets set_bias_kernel
+// would write, (|c|^2)/2 (+inf for pad centroids) or sum c8^2 (2^26).
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
                                                              int64_t *norm64, const unsigned *cmax_bits,
-                                                             unsigned *stats, uint8_t *tile_op, int tb) {
+                                                             unsigned *stats, uint8_t *tile_op, int tb,
+                                                             float *hn_out, int64_t *cn_out) {
     constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
     const int gl = threadIdx.x & 15;
     const size_t jc = (size_t)blockIdx.x * PREP_CENTROIDS + (threadIdx.x >> 4);  // < k_pad (a multiple of 64)
@@ -1594,6 +1597,7 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     if constexpr (OP == OP_I8) {
         for (int off = 8; off > 0; off >>= 1) part64 += shfl_xor_i64(part64, off);
         if (gl == 0 && live) norm64[jc] = part64;
+        if (gl == 0 && cn_out) cn_out[jc] = live ? part64 : ((int64_t)1 << 26);
     } else {
         float nv = 0.0f;
         if (live) {
@@ -1606,6 +1610,7 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
             nv = __fadd_rn(nv, tail);
             if (gl == 0) norm32[jc] = nv;
         }
+        if (gl == 0 && hn_out) hn_out[jc] = live ? __fmul_rn(0.5f, nv) : INFINITY;
         // max |c|^2 (the candidate searches' positivity offset): one atomic
         // per workgroup, not per centroid; NaN norms are skipped
         if constexpr (OP == OP_BF16 || OP == OP_FP8) {
@@ -2094,7 +2099,8 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 }
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax, unsigned *stats, hipStream_t s) {
+                                 const unsigned *cmax, unsigned *stats, hipStream_t s, float *hn_out,
+                                 int64_t *cn_out) {
     const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;
     // the register-resident path's tile-major copy, written by the same
@@ -2102,10 +2108,10 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     uint8_t *tile = g.reg && g.op != OP_I16 ? (uint8_t *)reg_operand(g, cen_op) : nullptr;
     const int tbc = g.reg ? reg_tile(g) : 1;
     switch (g.op) {
-        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
-        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
-        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
-        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc); break;
+        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
         case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid_wave, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
         default: return hipErrorInvalidValue;
     }

@@ -25,6 +25,12 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                       float *centroids_out, uint32_t *labels_out, emh_train_out *out, bool rows_on_device) {
     JobRun J;
     int rc;
+    // the steps' searches read the device model in place (FP32 / I8 / I16):
+    // the assigner must not keep that pointer once the model is freed
+    struct Unborrow {
+        emh_assigner *h;
+        ~Unborrow() { h->cen_ref = nullptr; }
+    } unborrow{h};
     if ((rc = J.init(x, rows, n, d, k, cfg, centroids_out, labels_out, out, rows_on_device))) return rc;
     if ((h->op == OP_I8 || h->op == OP_I16) && rows != EMH_ROWS_I8)
         return fail(EMH_E_UNSUPPORTED, "the integer search takes i8 rows only (AssignRows<i8>)");
@@ -137,8 +143,10 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
             if ((rc = check(launch_fill_f64(pass_d.as<double>(), k * d, h->stream), "clear pass sums"))) return rc;
             if ((rc = check(launch_fill_f64((double *)passn_d.p, k, h->stream), "clear pass counts"))) return rc;
         }
-        // One batch against a fixed centroid snapshot (src/lib.rs:408-417).
-        if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true))) return rc;
+        // One batch against a fixed centroid snapshot (src/lib.rs:408-417):
+        // the searches without a rescore read the model in place (the EMA
+        // changes it only after they have run, in stream order)
+        if ((rc = set_centroids_impl(h, cent_d.as<float>(), k, d, true, true))) return rc;
         if (J.penalized) {
             // pinned and stream-ordered: the step ends with a stream sync
             // before the next bias is staged (penalized runs are not async)
@@ -169,15 +177,14 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                                           sums_d.as<float>(), counts_d.as<uint32_t>(), h->stream, counts_cleared),
                         "accumulate")))
             return rc;
-        if (J.is_last &&
-            (rc = check(launch_pass_accumulate(sums_d.as<float>(), counts_d.as<uint32_t>(), k, d, pass_d.as<double>(),
-                                               passn_d.as<uint64_t>(), h->stream),
-                        "pass accumulate")))
-            return rc;
-        // the EMA step (src/lib.rs:455-456)
+        // the EMA step (src/lib.rs:455-456); in the last epoch the same
+        // launch first adds the batch's sums and counts to the FP64 pass
+        // totals (src/lib.rs:446-454)
         if ((rc = check(launch_ema_apply(cent_d.as<float>(), usage_d.as<float>(), sums_d.as<float>(),
                                          counts_d.as<uint32_t>(), k, d, J.alpha(), h->stream,
-                                         counts_d.as<uint32_t>(), h->acc.big),
+                                         counts_d.as<uint32_t>(), h->acc.big,
+                                         J.is_last ? pass_d.as<double>() : nullptr,
+                                         J.is_last ? passn_d.as<uint64_t>() : nullptr),
                         "EMA step")))
             return rc;
         counts_cleared = true;
@@ -190,9 +197,11 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                 return rc;
             meta[sl] = J.meta();
             pend_n++;
+            // the revival plan needs this step's masses, nothing of the
+            // batches still pending: those fold in later as usual
             if (revive &&
                 ((rc = check(hipMemcpyAsync(usage_h.p, usage_d.p, k * 4, hipMemcpyDeviceToHost, h->stream), "usage")) ||
-                 (rc = check(hipStreamSynchronize(h->stream), "batch")) || (rc = drain_all())))
+                 (rc = check(hipStreamSynchronize(h->stream), "batch"))))
                 return rc;
             if (revive) memcpy(J.st.usage.data(), usage_h.p, k * 4);
         } else {
@@ -215,12 +224,14 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                     pp[2 * t + 1] = (uint32_t)pairs[t].second;
                 }
                 memcpy(usage_h.p, J.st.usage.data(), k * 4);
+                // stream-ordered: the pinned staging is next written after
+                // the next revival's synchronisation, when these copies are
+                // long done
                 if ((rc = check(hipMemcpyAsync(pairs_d.p, pairs_h.p, pairs.size() * 8, hipMemcpyHostToDevice, h->stream),
                                 "revival pairs")) ||
                     (rc = check(hipMemcpyAsync(usage_d.p, usage_h.p, k * 4, hipMemcpyHostToDevice, h->stream), "usage")) ||
                     (rc = check(launch_revive_split(cent_d.as<float>(), pairs_d.as<uint32_t>(), pairs.size(), d, h->stream),
-                                "revival")) ||
-                    (rc = check(hipStreamSynchronize(h->stream), "revival")))
+                                "revival")))
                     return rc;
             }
         }
@@ -258,7 +269,7 @@ void emh_config_default(emh_config *cfg) {
 int emh_resolve_recipe(uint64_t n, uint64_t d, uint64_t k, const emh_config *cfg, emh_recipe *out) {
     if (!cfg || !out) return fail(EMH_E_ARG, "resolve_recipe: NULL argument");
     memset(out, 0, sizeof *out);
-    if (resolve(n, d, k, *cfg, *out)) return fail(EMH_E_ARG, "nepochs must be in (0, 63)");
+    if (resolve(n, d, k, *cfg, *out)) return fail(EMH_E_ARG, "nepochs must be in (0, 1023)");
     return EMH_OK;
 }
 

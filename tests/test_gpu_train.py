@@ -112,7 +112,8 @@ def test_f32_training_tracks_oracle(hip):
 
 
 @pytest.mark.parametrize("n,m,k,d", [(5000, 3000, 50, 37), (20000, 7001, 300, 37), (70000, 66000, 70000, 37),
-                                     (9000, 1, 1, 37), (20000, 15000, 7, 37), (6000, 5000, 40, 300)])
+                                     (9000, 1, 1, 37), (20000, 15000, 7, 37), (6000, 5000, 40, 300),
+                                     (20000, 17001, 2048, 37), (9000, 4097, 257, 96)])
 def test_device_accumulate_is_batch_order_f32(hip, n, m, k, d):
     """The stable sort + streamed fold equals the sequential host loop bit
     for bit: one, two and three 8-bit sort passes, blocks with a ragged end,
