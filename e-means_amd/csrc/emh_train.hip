@@ -13,7 +13,9 @@
 // (objective, last-pass labels), the usage vector only when the bias or a
 // revival needs it; without the balance penalty the host folds them in a
 // few batches later, so staging and launching overlap the GPU's work.
+#include <atomic>
 #include <chrono>
+#include <thread>
 
 #include "emh_recipe.h"
 
@@ -94,10 +96,10 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
 
     // Without the balance penalty no batch waits for the previous one's
     // labels: each batch's labels and distances come down into one of NSLOT
-    // pinned slots behind an event, and the host folds them into the
-    // statistics in batch order when the slot comes round again (or at a
-    // revival, which needs the usage vector, or at the end). The host then
-    // stages batch t + 1 while the GPU still runs batch t.
+    // pinned slots behind an event, and a helper thread folds them into the
+    // statistics in batch order (the objective's f64 chain and the last
+    // pass's labels) while this thread stages and launches the next
+    // batches; a slot is reused once its batch is folded.
     const bool async = cfg.balance <= 0.0f;
     J.deferred = async;
     constexpr int NSLOT = 4;
@@ -111,7 +113,6 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         }
     } ev;
     JobRun::StepMeta meta[NSLOT];
-    int pend_head = 0, pend_n = 0;  // pending slots, oldest first
     if (async)
         for (int i = 0; i < NSLOT; i++)
             if ((rc = check(out_s[i].alloc(2 * batch * 4), "pinned labels and distances")) ||
@@ -119,26 +120,49 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                 return rc;
     if ((rc = check(hipEventCreateWithFlags(&ev.order, hipEventDisableTiming), "event"))) return rc;
     bool order_pending = false, counts_cleared = false;
-    auto drain_one = [&]() -> int {
-        const int sl = pend_head;
-        int r;
-        if ((r = check(hipEventSynchronize(ev.e[sl]), "batch"))) return r;
-        J.finish_stats_deferred(meta[sl], out_s[sl].as<uint32_t>(), out_s[sl].as<float>() + batch);
-        pend_head = (pend_head + 1) % NSLOT;
-        pend_n--;
-        return (int)EMH_OK;
-    };
-    auto drain_all = [&]() -> int {
-        int r;
-        while (pend_n)
-            if ((r = drain_one())) return r;
-        return (int)EMH_OK;
+    // posted: batches whose slot is queued; folded: batches the helper has
+    // taken into the statistics (both count from the first batch)
+    struct Folder {
+        std::atomic<uint64_t> posted{0}, folded{0};
+        std::atomic<bool> stop{false};
+        std::atomic<int> err{0};
+        std::thread th;
+        ~Folder() {
+            stop.store(true, std::memory_order_release);
+            if (th.joinable()) th.join();
+        }
+    } fold;
+    if (async) {
+        const int dev = h->device;
+        fold.th = std::thread([&, dev] {
+            (void)hipSetDevice(dev);
+            uint64_t next = 0;
+            for (;;) {
+                while (fold.posted.load(std::memory_order_acquire) == next) {
+                    if (fold.stop.load(std::memory_order_acquire) &&
+                        fold.posted.load(std::memory_order_acquire) == next)
+                        return;
+                    std::this_thread::yield();
+                }
+                const int sl = (int)(next % NSLOT);
+                if (hipEventSynchronize(ev.e[sl]) != hipSuccess) fold.err.store(1, std::memory_order_release);
+                J.finish_stats_deferred(meta[sl], out_s[sl].as<uint32_t>(), out_s[sl].as<float>() + batch);
+                fold.folded.store(++next, std::memory_order_release);
+            }
+        });
+    }
+    // waits until the helper has folded the first `count` batches
+    auto folded_up_to = [&](uint64_t count) -> int {
+        while (fold.folded.load(std::memory_order_acquire) < count) std::this_thread::yield();
+        return fold.err.load(std::memory_order_acquire) ? fail(EMH_E_DEVICE, "batch statistics: a step failed")
+                                                        : (int)EMH_OK;
     };
 
     while (J.next()) {
         const size_t nb = J.nb;
         const int sl = (int)(J.step % NSLOT);
-        if (async && pend_n == NSLOT && (rc = drain_one())) return rc;  // frees slot sl, the oldest
+        // slot sl held batch step - NSLOT: reused once that batch is folded
+        if (async && J.step >= (uint64_t)NSLOT && (rc = folded_up_to(J.step - NSLOT + 1))) return rc;
         if (J.last_epoch_start) {
             if ((rc = check(launch_fill_f64(pass_d.as<double>(), k * d, h->stream), "clear pass sums"))) return rc;
             if ((rc = check(launch_fill_f64((double *)passn_d.p, k, h->stream), "clear pass counts"))) return rc;
@@ -196,7 +220,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                 (rc = check(hipEventRecord(ev.e[sl], h->stream), "event")))
                 return rc;
             meta[sl] = J.meta();
-            pend_n++;
+            fold.posted.store(J.step + 1, std::memory_order_release);
             // the revival plan needs this step's masses, nothing of the
             // batches still pending: those fold in later as usual
             if (revive &&
@@ -237,7 +261,7 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
         }
         J.advance();
     }
-    if ((rc = drain_all())) return rc;
+    if (async && (rc = folded_up_to(fold.posted.load(std::memory_order_acquire)))) return rc;
 
     // Closing mean on the device from the FP64 pass totals (src/lib.rs:481-489).
     if ((rc = check(launch_closing_mean(pass_d.as<double>(), passn_d.as<uint64_t>(), k, d, cent_d.as<float>(),
