@@ -181,7 +181,9 @@ def run_search(prec, m, x, c_host, steps, warmup):
     ms = t0.elapsed_time(t1) / steps
     enc, search, rescore, calls = a.get_timing()
     a.set_timing(False)
+    fr, ns = a.last_tile()
     return {
+        "tile": {"rows_per_workgroup": 128 * fr, "centroid_slices": ns},
         "ms_per_step": ms,
         "rows_per_s": n / (ms * 1e-3),
         "search_ms": search / max(calls, 1),
@@ -336,6 +338,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         "traffic": traffic_from_profile(args.workload, prec),
         "kernel": f"search_reg_kernel ({prec})",
         "kernel_ms": round(head["search_ms"], 4),
+        "tile": head["tile"],
     }
     if prec in OPS_PER_PRODUCT:
         # I16 runs three i8 MFMA planes per product: the executed rate beside
@@ -373,6 +376,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
                 "rescore_ms": round(v["rescore_ms"], 4),
                 "encode_ms": round(v["encode_ms"], 4),
                 "search_tflops": round(flops / (v["search_ms"] * 1e-3) / 1e12, 2),
+                "tile": v["tile"],
                 "name": v["name"],
                 **quality.get(k_, {}),
             }

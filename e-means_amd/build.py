@@ -17,8 +17,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libemeans_hip.so")
-SOURCES = ["emh_search.hip", "emh_accum.hip", "emh_api.hip", "emh_train.hip", "emh_batch.hip", "emh_quant.hip",
-           "emh_report.hip"]
+# the heaviest translation units (most template instantiations) first
+SOURCES = ["emh_search_reg_bf16.hip", "emh_search_reg_fp8.hip", "emh_search_reg_int.hip", "emh_search_reg_f32.hip",
+           "emh_search_lds.hip", "emh_search_jobs.hip", "emh_search.hip", "emh_accum.hip", "emh_api.hip",
+           "emh_train.hip", "emh_batch.hip", "emh_quant.hip", "emh_report.hip"]
 ARCH = os.environ.get("EMH_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -50,7 +52,7 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 4) -> str:
+def build(jobs: int = min(8, os.cpu_count() or 4)) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, SOURCES))
@@ -66,5 +68,5 @@ def build(jobs: int = 4) -> str:
 
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
-    ap.add_argument("--jobs", type=int, default=4)
+    ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 4))
     print(build(ap.parse_args().jobs))

@@ -8,6 +8,8 @@
 // the reference's panics.
 #include <stdio.h>
 #include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <new>
@@ -57,119 +59,186 @@ static int mark(emh_assigner *h, hipEvent_t *slot) {
     return check(hipEventRecord(*slot, h->stream), "hipEventRecord");
 }
 
-// ---- split autotuner ----
-// A launch with few row blocks leaves most of the chip idle; splitting the
-// centroid tiles of each row block over several workgroups fills it. The
-// split changes nothing in the results (merge_slices_kernel), so the tuner
-// times the candidate splits on the call itself, the first time a shape is
-// seen, and keeps the fastest for the process. EMH_AUTOTUNE=0 in the
-// environment replaces the timing by a rule: the smallest split that puts
-// two workgroups on every CU.
+// ---- tile autotuner ----
+// A register-resident search launch sweeps 128 fr rows per workgroup (fr
+// 32-row fragments per wave, 1 or 2) and may split every row block's
+// centroid tiles over nslice workgroups (merge_slices_kernel merges them).
+// Neither changes the results, so for launches that leave the chip short of
+// two workgroups per CU the tuner times the candidate (fr, nslice) pairs on
+// the call itself, the first time a shape (op, row type, m, row bytes,
+// tiles, row blocks) is seen, keeps the fastest, and records it in a JSON
+// file that later processes read: EMH_TUNE_CACHE names the file ("" keeps
+// the choices in memory only), by default
+// $XDG_CACHE_HOME/emeans_hip/tiles.json or ~/.cache/emeans_hip/tiles.json.
+// EMH_AUTOTUNE=0 replaces the timing by a rule: fr = 2 and the smallest
+// split that puts two workgroups on every CU.
 namespace {
-struct TuneKey {
-    int device, op, rows, M, row_bytes, ntiles;
-    size_t rowblocks;
-    bool operator<(const TuneKey &o) const {
-        return std::tie(device, op, rows, M, row_bytes, ntiles, rowblocks) <
-               std::tie(o.device, o.op, o.rows, o.M, o.row_bytes, o.ntiles, o.rowblocks);
-    }
-};
 std::mutex g_tune_mu;
-std::map<TuneKey, int> g_tune;
+std::map<std::string, Tile> g_tiles;
+bool g_tiles_loaded = false;
 
 bool autotune_enabled() {
     const char *e = getenv("EMH_AUTOTUNE");
     return !(e && e[0] == '0');
 }
+
+std::string tune_cache_path() {
+    const char *e = getenv("EMH_TUNE_CACHE");
+    if (e) return std::string(e);
+    const char *x = getenv("XDG_CACHE_HOME");
+    if (x && x[0]) return std::string(x) + "/emeans_hip/tiles.json";
+    const char *home = getenv("HOME");
+    if (home && home[0]) return std::string(home) + "/.cache/emeans_hip/tiles.json";
+    return std::string();
+}
+
+// The file is one JSON object, one entry a line:
+//   "<arch>|<op>|<rows>|<M>|<row bytes>|<tiles>|<row blocks>": {"fr": 2, "nslice": 4, "ms": 0.1234},
+// (called with g_tune_mu held)
+void load_tiles() {
+    if (g_tiles_loaded) return;
+    g_tiles_loaded = true;
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    FILE *f = fopen(path.c_str(), "r");
+    if (!f) return;
+    char line[512], key[400];
+    while (fgets(line, sizeof line, f)) {
+        int fr = 0, ns = 0;
+        if (sscanf(line, " \"%399[^\"]\": {\"fr\": %d, \"nslice\": %d", key, &fr, &ns) == 3 && (fr == 1 || fr == 2) &&
+            ns >= 1 && ns <= 4096)
+            g_tiles[key] = Tile{fr, ns, 0.0f};
+    }
+    fclose(f);
+}
+
+// (called with g_tune_mu held); best effort: a cache that cannot be
+// written leaves the choices in memory
+void save_tiles() {
+    const std::string path = tune_cache_path();
+    if (path.empty()) return;
+    for (size_t p = path.find('/', 1); p != std::string::npos; p = path.find('/', p + 1))
+        (void)mkdir(path.substr(0, p).c_str(), 0755);
+    const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return;
+    fprintf(f, "{\n");
+    size_t i = 0;
+    for (const auto &kv : g_tiles)
+        fprintf(f, " \"%s\": {\"fr\": %d, \"nslice\": %d, \"ms\": %.4f}%s\n", kv.first.c_str(), kv.second.fr,
+                kv.second.ns, kv.second.ms, ++i < g_tiles.size() ? "," : "");
+    fprintf(f, "}\n");
+    const bool ok = fclose(f) == 0;
+    if (!ok || rename(tmp.c_str(), path.c_str()) != 0) (void)remove(tmp.c_str());
+}
 }  // namespace
 
-// Search launch with `nslice` centroid slices (1: unsplit); `part` holds
-// slice_part_bytes of per-slice results when nslice > 1.
+// Search launch with tile t; `part` holds slice_part_bytes of per-slice
+// results when t.ns > 1.
 static int search_launch(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
                          const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
-                         uint32_t *cand, int nslice, uint32_t *part) {
+                         uint32_t *cand, Tile t, uint32_t *part) {
     const SearchGeom &g = h->g;
-    h->last_slices = nslice;
+    h->last_tile = t;
     return check(launch_search(g, M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(), h->cn.as<int64_t>(), x8,
                                rs, rn, h->cmax.as<unsigned>(), h->stats.as<unsigned>(), labels_dev, dist_dev, cand,
-                               h->stream, nslice, nslice > 1 ? part : nullptr),
+                               h->stream, t.ns, t.ns > 1 ? part : nullptr, t.fr),
                  "search kernel");
 }
 
-// The split rule without timing: the smallest power-of-two split that puts
-// two workgroups on every CU (1 when the launch fills the chip already).
-static int split_rule(const emh_assigner *h, size_t m) {
+// fr = 1 is instantiated for f32 rows of the FP32 and BF16 searches, and
+// for the FP8 and I8 searches
+static bool fr1_available(const emh_assigner *h, int rows) {
+    if (!h->g.reg) return false;
+    if (h->op == OP_F32 || h->op == OP_BF16) return rows == EMH_ROWS_F32;
+    return h->op == OP_FP8 || (h->op == OP_I8 && h->g.i8_packed);
+}
+
+// The rule without timing: fr = 2 and the smallest power-of-two split that
+// puts two workgroups on every CU (1 when the launch fills the chip already).
+static Tile split_rule(const emh_assigner *h, size_t m) {
     const SearchGeom &g = h->g;
     const size_t rowblocks = (m + 255) / 256;
     const int nt = reg_tiles(g);
     const size_t target = 2 * (size_t)h->cu_count;
-    if (!g.reg || rowblocks >= target || nt < 4) return 1;
-    int best = 1;
+    Tile t;
+    if (!g.reg || rowblocks >= target || nt < 4) return t;
     for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) {
-        best = s;
+        t.ns = s;
         if (rowblocks * (size_t)s >= target) break;
     }
-    return best;
+    return t;
 }
 
-static int choose_slices(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
-                         const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
-                         uint32_t *cand, int *out) {
-    *out = 1;
+// a forced tile (emh_set_tile / emh_set_split), if any
+static bool forced_tile(const emh_assigner *h, int rows, Tile *t) {
+    if (h->forced_fr <= 0 && h->forced_slices <= 0) return false;
+    t->fr = h->forced_fr == 1 && fr1_available(h, rows) ? 1 : 2;
+    t->ns = h->forced_slices > 0 ? h->forced_slices : 1;
+    return true;
+}
+
+static int choose_tile(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
+                       const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev, float *dist_dev,
+                       uint32_t *cand, Tile *out) {
+    *out = Tile{};
     const SearchGeom &g = h->g;
-    if (h->forced_slices > 0) {
-        *out = h->forced_slices;
-        return EMH_OK;
-    }
+    if (forced_tile(h, rows, out)) return EMH_OK;
     if (!g.reg || h->timing) return EMH_OK;
     const size_t rowblocks = (m + 255) / 256;
     const int nt = reg_tiles(g);
     const size_t target = 2 * (size_t)h->cu_count;  // the kernel runs two workgroups per CU
     if (rowblocks >= target || nt < 4) return EMH_OK;
-    std::vector<int> cand_s;
-    for (int s = 1; s <= nt / 2 && rowblocks * (size_t)s <= 4 * target; s *= 2) cand_s.push_back(s);
-    if (cand_s.size() < 2) return EMH_OK;
-    const TuneKey key{h->device, h->op, rows, M, g.row_bytes, nt, rowblocks};
+    std::vector<Tile> cands;
+    for (int fr = 2; fr >= 1; fr--) {
+        if (fr == 1 && !fr1_available(h, rows)) break;
+        const size_t rb = (m + 128 * fr - 1) / (128 * fr);
+        for (int s = 1; s <= nt / 2 && rb * (size_t)s <= 4 * target; s *= 2) cands.push_back(Tile{fr, s, 0.0f});
+    }
+    if (cands.size() < 2) return EMH_OK;
+    const std::string key = h->arch + "|" + std::to_string(h->op) + "|" + std::to_string(rows) + "|" +
+                            std::to_string(M) + "|" + std::to_string(g.row_bytes) + "|" + std::to_string(nt) + "|" +
+                            std::to_string(rowblocks);
+    if (!autotune_enabled()) {
+        *out = split_rule(h, m);
+        return EMH_OK;
+    }
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
+        load_tiles();
+        auto it = g_tiles.find(key);
+        if (it != g_tiles.end()) {
             *out = it->second;
+            if (out->fr == 1 && !fr1_available(h, rows)) out->fr = 2;
             return EMH_OK;
         }
     }
-    int best = cand_s.back();
-    if (!autotune_enabled()) {
-        best = split_rule(h, m);
-    } else {
-        int rc;
-        if ((rc = check(h->part.ensure(slice_part_bytes(M, m, cand_s.back(), h->op)), "split buffer"))) return rc;
-        hipEvent_t e0, e1;
-        if ((rc = check(hipEventCreate(&e0), "tune event")) || (rc = check(hipEventCreate(&e1), "tune event")))
-            return rc;
-        float best_ms = 0.0f;
-        uint32_t *part = h->part.as<uint32_t>();
-        for (int s : cand_s) {
-            // one warm-up launch, then the timed one
-            if ((rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, s, part)) ||
-                (rc = check(hipEventRecord(e0, h->stream), "tune event")) ||
-                (rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, s, part)) ||
-                (rc = check(hipEventRecord(e1, h->stream), "tune event")) ||
-                (rc = check(hipEventSynchronize(e1), "tune sync")))
-                break;
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, e0, e1);
-            if (s == cand_s.front() || ms < best_ms) {
-                best_ms = ms;
-                best = s;
-            }
-        }
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (rc) return rc;
+    int rc;
+    size_t part_bytes = 0;
+    for (const Tile &t : cands) part_bytes = std::max(part_bytes, slice_part_bytes(M, m, t.ns, h->op));
+    if ((rc = check(h->part.ensure(part_bytes), "split buffer"))) return rc;
+    hipEvent_t e0, e1;
+    if ((rc = check(hipEventCreate(&e0), "tune event")) || (rc = check(hipEventCreate(&e1), "tune event"))) return rc;
+    Tile best = cands.front();
+    uint32_t *part = h->part.as<uint32_t>();
+    for (size_t i = 0; i < cands.size(); i++) {
+        Tile t = cands[i];
+        // one warm-up launch, then the timed one
+        if ((rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, t, part)) ||
+            (rc = check(hipEventRecord(e0, h->stream), "tune event")) ||
+            (rc = search_launch(h, M, x_dev, rows, m, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, t, part)) ||
+            (rc = check(hipEventRecord(e1, h->stream), "tune event")) ||
+            (rc = check(hipEventSynchronize(e1), "tune sync")))
+            break;
+        (void)hipEventElapsedTime(&t.ms, e0, e1);
+        if (i == 0 || t.ms < best.ms) best = t;
     }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune[key] = best;
+    g_tiles[key] = best;
+    save_tiles();
     *out = best;
     return EMH_OK;
 }
@@ -186,7 +255,7 @@ struct RowScratch {
 // Encoding (FP8), search and rescore (BF16 / FP8) of one call on the
 // buffers of `rs_`, with `ns` centroid slices; allocates nothing.
 static int search_stages(emh_assigner *h, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
-                         uint32_t *labels_dev, float *dist_dev, const RowScratch &b, int ns, hipEvent_t *ev,
+                         uint32_t *labels_dev, float *dist_dev, const RowScratch &b, Tile t, hipEvent_t *ev,
                          bool timed) {
     auto mk = [&](int i) { return timed ? mark(h, &ev[i]) : EMH_OK; };
     const SearchGeom &g = h->g;
@@ -194,13 +263,13 @@ static int search_stages(emh_assigner *h, const void *x_dev, int rows, size_t m,
     if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
         if ((rc = mk(1))) return rc;
         if ((rc = search_launch(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev,
-                                nullptr, ns, b.part)))
+                                nullptr, t, b.part)))
             return rc;
         if ((rc = mk(2))) return rc;
         return mk(3);
     }
     if ((rc = mk(1))) return rc;
-    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, ns, b.part)))
+    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, t, b.part)))
         return rc;
     if ((rc = mk(2))) return rc;
     if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen(),
@@ -241,15 +310,15 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
         if ((rc = encode_rows(h, x_dev, rows, m, ids_dev, b))) return rc;
     }
     const int M = cand_search ? h->M : 1;
-    int ns = 1;
-    if ((rc = choose_slices(h, M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, cand_search ? nullptr : labels_dev,
-                            cand_search ? nullptr : dist_dev, b.cand, &ns)))
+    Tile t;
+    if ((rc = choose_tile(h, M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, cand_search ? nullptr : labels_dev,
+                          cand_search ? nullptr : dist_dev, b.cand, &t)))
         return rc;
-    if (ns > 1) {
-        if ((rc = check(h->part.ensure(slice_part_bytes(M, m, ns, h->op)), "split buffer"))) return rc;
+    if (t.ns > 1) {
+        if ((rc = check(h->part.ensure(slice_part_bytes(M, m, t.ns, h->op)), "split buffer"))) return rc;
         b.part = h->part.as<uint32_t>();
     }
-    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, ns, ev, true);
+    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, t, ev, true);
 }
 
 // ---- the search without allocations ----
@@ -309,21 +378,22 @@ int run_search_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, c
                                    " bytes of scratch, above the 67108864-byte maximum (at most about " +
                                    std::to_string(NOALLOC_SCRATCH_BYTES / per_row) + " rows per call at this d and m)");
     }
-    // the split the rule picks, when its slice results fit as well
-    int ns = h->forced_slices > 0 ? h->forced_slices : split_rule(h, m);
-    if (ns > 1) {
+    // a forced tile, else the rule's; its split when the slice results fit
+    Tile t;
+    if (!forced_tile(h, rows, &t)) t = split_rule(h, m);
+    if (t.ns > 1) {
         const int nt = reg_tiles(h->g);
-        const int per = (nt + ns - 1) / ns;
-        ns = per > 0 ? (nt + per - 1) / per : 1;  // the slices launch_search forms
+        const int per = (nt + t.ns - 1) / t.ns;
+        t.ns = per > 0 ? (nt + per - 1) / per : 1;  // the slices launch_search forms
         RowScratch p2;
-        if (!h->g.reg || noalloc_layout(h, m, ns, nullptr, &p2) > NOALLOC_SCRATCH_BYTES) ns = 1;
+        if (!h->g.reg || noalloc_layout(h, m, t.ns, nullptr, &p2) > NOALLOC_SCRATCH_BYTES) t.ns = 1;
     }
     RowScratch b;
-    noalloc_layout(h, m, ns, base, &b);
+    noalloc_layout(h, m, t.ns, base, &b);
     int rc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (h->op == OP_FP8 && (rc = encode_rows(h, x_dev, rows, m, ids_dev, b))) return rc;
-    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, ns, ev, false);
+    return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, t, ev, false);
 }
 
 }  // namespace emh
@@ -475,6 +545,7 @@ int emh_create(int device, int precision, int rescore_m, emh_assigner **out) {
     }
     h->device_name = std::string(prop.name) + " (" + prop.gcnArchName + ")";
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+    h->arch = std::string(prop.gcnArchName) + "/" + std::to_string(h->cu_count);
     // the device address of this GPU's copy of the no-allocation scratch
     if ((rc = scratch_setup(device))) {
         delete h;
@@ -555,6 +626,20 @@ int emh_set_split(emh_assigner *h, int nslice) {
     return EMH_OK;
 }
 
+int emh_set_tile(emh_assigner *h, int fragments) {
+    if (!h) return fail(EMH_E_ARG, "emh_set_tile: NULL handle");
+    if (fragments < 0 || fragments > 2) return fail(EMH_E_ARG, "emh_set_tile: fragments must be 0 (tuned), 1 or 2");
+    h->forced_fr = fragments;
+    return EMH_OK;
+}
+
+int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice) {
+    if (!h || !fragments || !nslice) return fail(EMH_E_ARG, "emh_last_tile: NULL argument");
+    *fragments = h->last_tile.fr;
+    *nslice = h->last_tile.ns;
+    return EMH_OK;
+}
+
 int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     if (!h || !buf || cap == 0) return fail(EMH_E_ARG, "emh_name: bad arguments");
     static const char *kind[] = {"FP32 fused MFMA argmin", "BF16 MFMA + FP32 rescore", "FP8-e4m3 MFMA + FP32 rescore",
@@ -568,7 +653,7 @@ int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     if (h->has_centroids)
         snprintf(tmp + n, sizeof tmp - n, ", k=%zu d=%zu (%s rows%s)", h->g.k, h->g.d,
                  h->g.reg ? "register-resident" : h->g.resident ? "LDS-resident" : "chunked",
-                 h->last_slices > 1 ? ", centroids split" : "");
+                 h->last_tile.ns > 1 ? ", centroids split" : "");
     strncpy(buf, tmp, cap - 1);
     buf[cap - 1] = 0;
     return EMH_OK;

@@ -38,6 +38,14 @@ struct DevBuf {
     template <class T> T *as() const { return (T *)p; }
 };
 
+// Tile of a register-resident search launch: fr 32-row fragments per wave
+// (128 fr rows per workgroup), ns centroid slices per row block; ms: the
+// tuner's timing of it.
+struct Tile {
+    int fr = 2, ns = 1;
+    float ms = 0.0f;
+};
+
 }  // namespace emh
 
 struct emh_assigner {
@@ -48,6 +56,7 @@ struct emh_assigner {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string device_name;
+    std::string arch;  // gcnArchName / CU count: the tile cache's device key
 
     emh::SearchGeom g{};
     bool has_centroids = false;
@@ -62,11 +71,13 @@ struct emh_assigner {
     const float *cen() const { return cen_ref ? cen_ref : cen_f32.as<float>(); }
     // row side (per assign call)
     emh::DevBuf x_stage, labels_stage, dist_stage, cand, x8, row_scale, row_norm;
-    // centroid-range split of the register-resident search (autotuned)
+    // tile of the register-resident search (autotuned): centroid-range
+    // split results, the last launch's tile, forced values (0: tuned)
     emh::DevBuf part;
     int cu_count = 256;
-    int last_slices = 1;
-    int forced_slices = 0;  // 0: the autotuner decides
+    emh::Tile last_tile;
+    int forced_slices = 0;
+    int forced_fr = 0;
     // per-cluster accumulation
     emh::AccumScratch acc;
 

@@ -76,6 +76,8 @@ hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
                            uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s);
 bool reg_path(const SearchGeom &g);
+// fr (register-resident path): 32-row fragments per wave, 1 or 2 (128 or 256
+// rows per workgroup); the results do not depend on it.
 // nslice > 1 (register-resident path only): the centroid tiles are split
 // over nslice workgroups per row block (fewer when tiles run out), each
 // writing partial results to `part` (slice_part_bytes), then merged into
@@ -84,7 +86,7 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
                          const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
                          const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
-                         int nslice = 1, uint32_t *part = nullptr);
+                         int nslice = 1, uint32_t *part = nullptr, int fr = 2);
 int reg_tiles(const SearchGeom &g);
 size_t slice_part_bytes(int M, size_t m, int nslice, int op);
 hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, const uint32_t *ids, const float *c,

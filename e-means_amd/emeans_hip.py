@@ -31,7 +31,7 @@ EXPORTS = [
     "emh_assign_all", "emh_set_timing", "emh_get_timing", "emh_train_batch", "emh_set_split", "emh_train_device",
     "emh_quant_fit", "emh_quant_encode", "emh_quant_decode", "emh_quant_clipped_fraction", "emh_quant_fit_device",
     "emh_quant_encode_device", "emh_report_nearest", "emh_report_nearest_device", "emh_report_config_default",
-    "emh_report", "emh_assign_device_noalloc",
+    "emh_report", "emh_assign_device_noalloc", "emh_set_tile", "emh_last_tile",
 ]
 
 
@@ -141,6 +141,8 @@ def lib():
     L.emh_assign_all.argtypes = [vp, vp, i32, sz, sz, vp, sz, vp, vp]
     L.emh_train_batch.argtypes = [vp, C.POINTER(Job), sz, C.POINTER(BatchStats)]
     L.emh_set_split.argtypes = [vp, i32]
+    L.emh_set_tile.argtypes = [vp, i32]
+    L.emh_last_tile.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     f32 = C.c_float
     L.emh_quant_fit.argtypes = [i32, vp, u64, u64, f32, C.c_double, i32, vp, C.POINTER(f32)]
     L.emh_quant_encode.argtypes = [i32, vp, u64, u64, vp, f32, vp]
@@ -265,6 +267,16 @@ class HipAssigner:
     def set_split(self, nslice: int):
         """Centroid-range split of the search: 0 = autotuned, n >= 1 forced."""
         _check(lib().emh_set_split(self._h, int(nslice)))
+
+    def set_tile(self, fragments: int):
+        """Rows per workgroup of the search, 128 x fragments: 0 = autotuned, 1 or 2 forced."""
+        _check(lib().emh_set_tile(self._h, int(fragments)))
+
+    def last_tile(self):
+        """(fragments per wave, centroid slices) of the last search launch."""
+        fr, ns = C.c_int(), C.c_int()
+        _check(lib().emh_last_tile(self._h, C.byref(fr), C.byref(ns)))
+        return fr.value, ns.value
 
     def set_timing(self, on: bool):
         _check(lib().emh_set_timing(self._h, int(on)))

@@ -259,6 +259,18 @@ int emh_report(int device, const uint32_t *labels, const float *sqdist, uint64_t
  * nslice >= 1 forces that many slices (fewer when the tiles run out). */
 int emh_set_split(emh_assigner *h, int nslice);
 
+/* Tile of the register-resident search (no counterpart in the reference):
+ * a workgroup sweeps 128 x fragments rows (fragments = 1 or 2 per wave)
+ * against its centroid slice. 0 (the default) lets the autotuner time the
+ * candidate (fragments, split) pairs the first time a launch shape that
+ * leaves the chip short of work is seen; its choices persist in a JSON file
+ * (EMH_TUNE_CACHE, default ~/.cache/emeans_hip/tiles.json; "" keeps them in
+ * memory) that later processes read. The results are the same for every
+ * tile. emh_last_tile reports the tile of the assigner's last search launch
+ * (fragments, centroid slices). */
+int emh_set_tile(emh_assigner *h, int fragments);
+int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice);
+
 /* Many independent problems, trained in lockstep (no counterpart in the
  * reference, which trains one problem per train() call, src/lib.rs:348).
  * Each job is one train() call's arguments; every job ends with the results

@@ -366,14 +366,17 @@ def test_name_and_state_errors(hip):
         E.HipAssigner(E.I8).set_centroids(np.zeros((1, 40000), np.float32))
 
 
-@pytest.mark.parametrize("prec,rows", [("fp32", "f32"), ("i8", "i8"), ("i16", "i8"), ("bf16", "f32"), ("fp8", "f32")])
-def test_centroid_split_does_not_change_results(hip, prec, rows):
-    """The search split over centroid slices (forced 1..8 and autotuned,
-    uneven tile counts) returns the same labels and distances bit for bit."""
+@pytest.mark.parametrize("prec,rows", [("fp32", "f32"), ("i8", "i8"), ("i16", "i8"), ("bf16", "f32"), ("fp8", "f32"),
+                                       ("fp32", "u8"), ("bf16", "i8")])
+def test_tiles_do_not_change_results(hip, prec, rows):
+    """Every tile of the search (128 or 256 rows per workgroup, the centroid
+    tiles split 1..8 ways, or autotuned; uneven tile counts) returns the same
+    labels and distances bit for bit."""
     rng = np.random.default_rng(13)
     n, k, d = 3000, 1300, 64
-    if rows == "i8":
-        x = rng.integers(-128, 128, size=(n, d)).astype(np.int8)
+    if rows in ("i8", "u8"):
+        lo, hi = (-128, 128) if rows == "i8" else (0, 256)
+        x = rng.integers(lo, hi, size=(n, d)).astype(np.int8 if rows == "i8" else np.uint8)
         c = rng.normal(scale=40, size=(k, d)).astype(np.float32)
     else:
         x = _mixture(n, d, 600, 14)
@@ -381,15 +384,53 @@ def test_centroid_split_does_not_change_results(hip, prec, rows):
     bias = rng.uniform(0, 2, size=k).astype(np.float32)
     a = E.HipAssigner(prec, rescore_m=4)
     ref = None
-    for ns in (1, 2, 3, 5, 8, 0):
-        print(f"{prec} split {ns}", flush=True)
-        a.set_split(ns)
-        a.set_centroids(c)
-        a.set_bias(bias)
-        got = a.assign(x)
-        if ref is None:
-            ref = got
-        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), f"split {ns}"
+    for fr in (2, 1, 0):
+        for ns in (1, 2, 3, 5, 8, 0):
+            if fr == 0 and ns != 0:
+                continue
+            print(f"{prec} rows {rows} tile fr={fr} split={ns}", flush=True)
+            a.set_tile(fr)
+            a.set_split(ns)
+            a.set_centroids(c)
+            a.set_bias(bias)
+            got = a.assign(x)
+            if ref is None:
+                ref = got
+            assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), f"tile {fr}/{ns}"
     if prec in ("i8", "i16"):
         cl, cd = O.i8_assign(x, c, bits=O.BITS_I16 if prec == "i16" else O.BITS_I8, bias=bias)
         assert np.array_equal(ref[0], cl) and np.array_equal(ref[1], cd)
+
+
+def test_tile_choices_persist(hip, tmp_path):
+    """The tuner's choice for a launch shape is written to EMH_TUNE_CACHE by
+    one process and used by the next without timing again."""
+    import json
+    import subprocess
+    import sys
+
+    cache = tmp_path / "tiles.json"
+    prog = (
+        "import sys, numpy as np\n"
+        "sys.path[:0] = [%r, %r]\n"
+        "import torch; torch.cuda.init()\n"
+        "import emeans_hip as E\n"
+        "rng = np.random.default_rng(1)\n"
+        "x = rng.normal(size=(5000, 96)).astype(np.float32)\n"
+        "a = E.HipAssigner('bf16', rescore_m=4)\n"
+        "a.set_centroids(x[:2000])\n"
+        "lab, dist = a.assign(x)\n"
+        "print(*a.last_tile(), int(lab.sum()))\n"
+    ) % (E.__file__.rsplit("/", 1)[0], E.__file__.rsplit("/", 2)[0])
+    env = dict(__import__("os").environ, EMH_TUNE_CACHE=str(cache))
+    first = subprocess.run([sys.executable, "-c", prog], env=env, capture_output=True, text=True, timeout=300)
+    assert first.returncode == 0, first.stderr[-2000:]
+    entries = json.loads(cache.read_text())
+    assert len(entries) == 1
+    (key, tile), = entries.items()
+    assert tile["fr"] in (1, 2) and tile["nslice"] >= 1 and "|" in key
+    second = subprocess.run([sys.executable, "-c", prog], env=env, capture_output=True, text=True, timeout=300)
+    assert second.returncode == 0, second.stderr[-2000:]
+    fr1, ns1, s1 = map(int, first.stdout.split())
+    fr2, ns2, s2 = map(int, second.stdout.split())
+    assert (fr2, ns2) == (tile["fr"], tile["nslice"]) == (fr1, ns1) and s1 == s2
