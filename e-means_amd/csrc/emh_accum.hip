@@ -579,7 +579,10 @@ __device__ inline void list_big(uint32_t *big, uint32_t *count_slot, uint32_t ke
 // keys / src: the sorted slots' global clusters (nclusters for spare slots,
 // last) and job-relative rows. Each cluster's member count goes to counts
 // (a global cluster's entry); big clusters are listed in big (after the
-// count word) for jobs_big_sum_kernel.
+// count word) for jobs_big_sum_kernel. A run of clusters streamed together
+// belongs to one job (stream_clusters stops at the job's last cluster), so
+// its dimensions are taken 256 at a time over the same run.
+template <int ROWS>
 __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
                                                               uint32_t nclusters, const uint32_t *keys,
                                                               const uint32_t *src, uint32_t *counts, size_t slots,
@@ -601,14 +604,17 @@ __global__ __launch_bounds__(256) void jobs_stream_sum_kernel(const JobDev *jobs
             continue;
         }
         const JobDev j = job_at<true>(jobs, cjob[key]);
-        p = stream_dims<EMH_ROWS_F32>(job_view(j), keys, src, slots, p, stop, 0, lane);  // batched rows: d <= 256
+        const SumView v = job_view(j);
+        size_t next = p;
+        for (size_t tg = 0; tg < v.d; tg += 256) next = stream_dims<ROWS>(v, keys, src, slots, p, stop, tg, lane);
+        p = next;
     }
 }
 
 // The big clusters of this step: wave t of the grid-stride loop folds
-// dimensions [64 (t % G), + 64) of listed cluster t / G (G = 4 for batched
-// rows, at most 256 wide; G = ceil(d / 64) for one problem), its members in
-// batch order, 64 rows in flight.
+// dimensions [64 (t % G), + 64) of listed cluster t / G (G = ceil(d / 64)
+// of the widest job in the batched case, of the problem otherwise), its
+// members in batch order, 64 rows in flight.
 constexpr unsigned BIG_SUM_BLOCKS = 512;
 
 template <int ROWS>
@@ -625,16 +631,18 @@ __device__ __forceinline__ void fold_big(const SumView &v, const uint32_t *src, 
     if (lane == 0 && tg == 0 && v.pass_sums) v.pass_counts[c] += n;
 }
 
+template <int ROWS>
 __global__ __launch_bounds__(256) void jobs_big_sum_kernel(const JobDev *jobs, const uint32_t *cjob,
-                                                           const uint32_t *src, const uint32_t *big) {
+                                                           const uint32_t *src, const uint32_t *big, uint32_t groups) {
     const int lane = threadIdx.x & 63;
-    const size_t total = (size_t)big[0] * 4, waves = (size_t)gridDim.x * 4;
+    const size_t total = (size_t)big[0] * groups, waves = (size_t)gridDim.x * 4;
     for (size_t t = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6); t < total; t += waves) {
-        const uint32_t key = big[2 + 3 * (t >> 2)], p0 = big[3 + 3 * (t >> 2)], n = big[4 + 3 * (t >> 2)];
-        const size_t tg = (t & 3) * 64;
+        const size_t e = t / groups;
+        const uint32_t key = big[2 + 3 * e], p0 = big[3 + 3 * e], n = big[4 + 3 * e];
+        const size_t tg = (t - e * groups) * 64;
         const JobDev j = job_at<true>(jobs, cjob[key]);
         if (tg >= j.d) continue;
-        fold_big<EMH_ROWS_F32>(job_view(j), src, key, p0, n, tg, lane);
+        fold_big<ROWS>(job_view(j), src, key, p0, n, tg, lane);
     }
 }
 
@@ -738,10 +746,11 @@ hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rm
     return hipGetLastError();
 }
 
-hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
-                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys, uint32_t *counts_all,
-                            hipStream_t s) {
+hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, int rows, size_t d_max, size_t slots,
+                            const uint32_t *ids, const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
+                            uint32_t *counts_all, hipStream_t s) {
     if (slots == 0) return hipSuccess;
+    if (rows != EMH_ROWS_F32 && rows != EMH_ROWS_I8) return hipErrorInvalidValue;
     hipError_t e;
     // slot i's job-relative row is ids[i] (the jobs' id ranges are laid out
     // in slot order)
@@ -750,11 +759,21 @@ hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, c
         hipSuccess)
         return e;
     const size_t waves = (slots + SUM_SPAN - 1) / SUM_SPAN;
-    hipLaunchKernelGGL(jobs_stream_sum_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, jobs, cjob,
-                       (uint32_t)nclusters, sorted_keys, sorted_src, counts_all, slots, a.big);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    hipLaunchKernelGGL(jobs_big_sum_kernel, dim3(BIG_SUM_BLOCKS), dim3(256), 0, s, jobs, cjob, sorted_src,
-                       (const uint32_t *)a.big);
+    const dim3 grid((unsigned)((waves + 3) / 4)), bgrid(BIG_SUM_BLOCKS);
+    const uint32_t groups = (uint32_t)((d_max + 63) / 64);
+    if (rows == EMH_ROWS_F32) {
+        hipLaunchKernelGGL(jobs_stream_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, jobs, cjob,
+                           (uint32_t)nclusters, sorted_keys, sorted_src, counts_all, slots, a.big);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(jobs_big_sum_kernel<EMH_ROWS_F32>, bgrid, dim3(256), 0, s, jobs, cjob, sorted_src,
+                           (const uint32_t *)a.big, groups);
+    } else {
+        hipLaunchKernelGGL(jobs_stream_sum_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, jobs, cjob,
+                           (uint32_t)nclusters, sorted_keys, sorted_src, counts_all, slots, a.big);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        hipLaunchKernelGGL(jobs_big_sum_kernel<EMH_ROWS_I8>, bgrid, dim3(256), 0, s, jobs, cjob, sorted_src,
+                           (const uint32_t *)a.big, groups);
+    }
     return hipGetLastError();
 }
 

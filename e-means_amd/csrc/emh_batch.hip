@@ -6,7 +6,8 @@
 // slot range of one concatenated batch and its clusters a range of one
 // concatenated cluster list, and the step (upload of the job table, batch
 // ids and biases; prep of every job's centroid operand; the search per
-// row-width class; the FP32 rescore; the per-cluster sums with one stable
+// row-width class; the FP32 rescore (BF16) or the merge of a split search
+// (FP32, I8); the per-cluster sums with one stable
 // sort; the EMA step; download of labels, distances and masses) is captured
 // once as a hipGraph and replayed once per step. The kernels read the step's
 // row counts, rates and flags from the device job table that the replay's
@@ -127,9 +128,12 @@ struct GraphGuard {
 static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats) {
     if (!jobs || njobs == 0) return fail(EMH_E_ARG, "train_batch: need at least one job");
     if (njobs > 0xFFFFu) return fail(EMH_E_ARG, "train_batch: at most 65535 jobs per call");
-    if (h->op != OP_F32 && h->op != OP_BF16)
-        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32 and BF16 searches");
+    if (h->op != OP_F32 && h->op != OP_BF16 && h->op != OP_I8)
+        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32, BF16 and I8 searches");
     const int op = h->op, M = op == OP_BF16 ? h->M : 1;
+    // the I8 search takes i8 rows (AssignRows<i8>), the float searches f32
+    const int rows = op == OP_I8 ? EMH_ROWS_I8 : EMH_ROWS_F32;
+    const size_t rb = op == OP_I8 ? 1 : 4;
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
@@ -146,9 +150,10 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<std::string> init_err(njobs);
     const std::function<void(size_t)> init_job = [&](size_t j) {
         const emh_job &jb = jobs[j];
-        if (jb.rows != EMH_ROWS_F32) {
+        if (jb.rows != rows) {
             init_rc[j] = EMH_E_UNSUPPORTED;
-            init_err[j] = "train_batch: f32 rows only";
+            init_err[j] = op == OP_I8 ? "train_batch: the I8 search takes i8 rows only"
+                                      : "train_batch: the float searches take f32 rows only";
             return;
         }
         init_rc[j] = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out,
@@ -162,13 +167,15 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs),
         part_base(njobs), nrow_base(njobs);
     std::vector<JobSlices> slices(njobs);
-    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0, rows_total = 0;
+    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0, rows_total = 0, d_max = 0;
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         geo[j] = make_geom(op, jb.k, jb.d);
         if (!geo[j].reg)
             return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
-                                               " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256)");
+                                               " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256, "
+                                               "I8 d <= 512)");
+        d_max = std::max(d_max, (size_t)jb.d);
         row_base[j] = slots;
         slots += run[j].R.batch_size;
         nrow_base[j] = rows_total;
@@ -190,17 +197,17 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
 
     // Resident rows, one table per job (the caller's own when on the device).
     std::vector<DevBuf> rows_d(njobs);
-    std::vector<const float *> rows_ptr(njobs);
+    std::vector<const void *> rows_ptr(njobs);
     for (size_t j = 0; j < njobs; j++) {
         if (jobs[j].x_on_device) {
-            rows_ptr[j] = (const float *)jobs[j].x;
+            rows_ptr[j] = jobs[j].x;
             continue;
         }
-        const size_t bytes = jobs[j].n * jobs[j].d * 4;
+        const size_t bytes = jobs[j].n * jobs[j].d * rb;
         if ((rc = check(rows_d[j].ensure(bytes), "resident rows"))) return rc;
         if ((rc = check(hipMemcpyAsync(rows_d[j].p, jobs[j].x, bytes, hipMemcpyHostToDevice, h->stream), "upload rows")))
             return rc;
-        rows_ptr[j] = rows_d[j].as<float>();
+        rows_ptr[j] = rows_d[j].p;
     }
 
     // Staged per-step input (host pinned -> device, one copy) and output
@@ -212,8 +219,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
                  in_bias = in.add(nclusters * 4), in_ids = in.add(slots * 4);
     Layout outl;
     const size_t out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
-    DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, stats_d, cand, keys, pass_sums, pass_counts,
-        maps, pairs_d, part_d, lab_d, labels_all;
+    DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, cn, stats_d, cand, keys, pass_sums,
+        pass_counts, maps, pairs_d, part_d, lab_d, labels_all;
     HostBuf in_h, out_h, pairs_h;
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
@@ -223,7 +230,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(cen_model.ensure(cen_floats * 4), "centroids")) ||
         (rc = check(sums_d.ensure(cen_floats * 4), "sums")) || (rc = check(counts_d.ensure(nclusters * 4), "counts")) ||
         (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
-        (rc = check(hn.ensure(cpad * 4), "offsets")) || (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
+        (rc = check(hn.ensure(cpad * 4), "offsets")) ||
+        (op == OP_I8 && (rc = check(cn.ensure(cpad * 8), "integer offsets"))) ||
+        (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
         (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
         (rc = check(keys.ensure(slots * 4), "keys")) || (rc = check(pass_sums.ensure(cen_floats * 8), "pass sums")) ||
         (rc = check(pass_counts.ensure(nclusters * 8), "pass counts")) ||
@@ -266,7 +275,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         ioff = 0;
         for (size_t j = 0; j < njobs; j++) {
             if (!jobs[j].x_on_device) continue;
-            if ((rc = check(launch_gather_rows(jobs[j].x, EMH_ROWS_F32, jobs[j].d, init_d.as<uint32_t>() + ioff,
+            if ((rc = check(launch_gather_rows(jobs[j].x, rows, jobs[j].d, init_d.as<uint32_t>() + ioff,
                                                jobs[j].k, cen_model.as<float>() + cen_base[j], h->stream),
                             "initial centroids")))
                 return rc;
@@ -337,6 +346,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.cen_op = (uint8_t *)cen_op.p + op_base[j];
         t.norm32 = norm32.as<float>() + cpad_base[j];
         t.hn = hn.as<float>() + cpad_base[j];
+        t.cn = op == OP_I8 ? cn.as<int64_t>() + cpad_base[j] : nullptr;
         t.stats = stats_d.as<unsigned>() + 4 * j;
         t.labels = lab_d.as<uint32_t>() + row_base[j];
         t.labels_all = labels_all.as<uint32_t>() + nrow_base[j];
@@ -388,8 +398,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess) ce = launch_jobs_search(op, M, sa_dev, (const uint2 *)(mp + m_wg), wg_all.size(), h->stream);
     if (ce == hipSuccess) ce = last_nodes(search_first);
     if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
-    if (ce == hipSuccess && op == OP_F32 && part_words)
-        ce = launch_jobs_merge_f32(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    if (ce == hipSuccess && op != OP_BF16 && part_words)
+        ce = launch_jobs_merge(op, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess)
         ce = launch_jobs_keys(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
                               nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
@@ -398,8 +408,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<hipGraphNode_t> dist_deps;
     if (ce == hipSuccess) ce = last_nodes(dist_deps);
     if (ce == hipSuccess)
-        ce = launch_jobs_sums(h->acc, jobs_dev, slots, (const uint32_t *)(in_dp + in_ids), (const uint32_t *)(mp + m_j),
-                              nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+        ce = launch_jobs_sums(h->acc, jobs_dev, rows, d_max, slots, (const uint32_t *)(in_dp + in_ids),
+                              (const uint32_t *)(mp + m_j), nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(),
+                              h->stream);
     if (ce == hipSuccess)
         ce = launch_jobs_ema(jobs_dev, (const uint32_t *)(mp + m_j), nclusters, stats_d.as<unsigned>(), 4 * njobs,
                              h->stream);
@@ -426,11 +437,25 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<char> live(njobs);
     std::vector<std::vector<std::pair<size_t, size_t>>> jpairs(njobs);
     JobDev *jd_h = (JobDev *)(in_hp + in_job);
+    // I8 jobs run the packed-key search only, whose scores need
+    // |sum c8^2 + round(bias)| < 2^26 - d 2^15 (sum c8^2 is in [0, d 2^14]);
+    // a bias past that (the single-problem driver then takes the i64 search)
+    // stops the batch with an error
+    std::vector<char> out_of_range(njobs, 0);
+    const auto i8_bias_fits = [&](const JobRun &J) {
+        const double lim = (double)(((int64_t)1 << 26) - (int64_t)J.d * 32768), top = (double)J.d * 16384.0;
+        for (float b : J.bias) {
+            const double r = std::round((double)b);
+            if (!(r + top < lim && r > -lim)) return false;
+        }
+        return true;
+    };
     const std::function<void(size_t)> stage = [&](size_t j) {
         JobRun &J = run[j];
         JobDev t = jd[j];
         live[j] = J.next();
         if (live[j]) {
+            if (J.penalized && op == OP_I8 && !i8_bias_fits(J)) out_of_range[j] = 1;
             if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
             memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
             t.m = (uint32_t)J.nb;
@@ -499,9 +524,18 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         finish(j);
         stage(j);
     };
+    const auto range_error = [&]() -> int {
+        for (size_t j = 0; j < njobs; j++)
+            if (out_of_range[j])
+                return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
+                                                   "'s balance bias leaves the packed I8 score range "
+                                                   "(|sum c8^2 + round(bias)| < 2^26 - d 2^15); emh_train takes it");
+        return EMH_OK;
+    };
     auto th = clk::now();
     pool.run(njobs, stage);
     host_s += since(th);
+    if ((rc = range_error())) return rc;
     for (;;) {
         if (std::find(live.begin(), live.end(), (char)1) == live.end()) break;
         th = clk::now();
@@ -526,6 +560,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         th = clk::now();
         pool.run(njobs, finish_stage);
         host_s += since(th);
+        if ((rc = range_error())) break;
         uint32_t *tp = pairs_h.as<uint32_t>();
         size_t ntrip = 0;
         for (size_t j = 0; j < njobs; j++)

@@ -98,13 +98,14 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 // fields before each replay of the step graph. Pointers address this job's
 // slot of the shared per-step arrays.
 struct JobDev {
-    const float *x;          // resident rows, n x d f32
+    const void *x;           // resident rows, n x d (f32; i8 for the I8 search)
     const uint32_t *ids;     // this step's batch row positions
     float *cen;              // the job's device-resident centroids, k x d
     const float *bias;       // this step's balance bias, k (read when penalized)
     float *usage;            // the job's device-resident cluster masses, k
     uint8_t *cen_op;         // tile-major operand table (register-resident search)
     float *norm32, *hn;      // k_pad each
+    int64_t *cn;             // I8: k_pad integer score offsets sum c8^2 + round(bias)
     unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias (f32 bits)
     uint32_t *labels;        // batch slot
     float *dist;             // batch slot (true distances once the keys kernel has run)
@@ -163,6 +164,7 @@ template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size
     EMH_G(cen_op);
     EMH_G(norm32);
     EMH_G(hn);
+    EMH_G(cn);
     EMH_G(stats);
     EMH_G(labels);
     EMH_G(dist);
@@ -187,9 +189,9 @@ hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint
 // rescoring of every job's candidates (merging a split job's slice lists
 // first): rmap holds (job, row) per batch slot.
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
-// FP32 jobs with a split search: each slot's slice results merged into its
-// label and distance.
-hipError_t launch_jobs_merge_f32(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
+// FP32 and I8 jobs with a split search: each slot's slice results merged
+// into its label and distance.
+hipError_t launch_jobs_merge(int op, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
 // Centroid-range split of one job's search: slices of consecutive tiles
 // sized so that every workgroup sweeps about the same operand volume.
 struct JobSlices {
@@ -253,10 +255,11 @@ hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rm
                             const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
                             hipStream_t s);
 // Second half: one stable sort, then per-job member counts (counts_all,
-// zeroed by the first half), f32 sums and FP64 pass totals.
-hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, size_t slots, const uint32_t *ids,
-                            const uint32_t *cjob, size_t nclusters, const uint32_t *keys, uint32_t *counts_all,
-                            hipStream_t s);
+// zeroed by the first half), f32 sums and FP64 pass totals. rows: the jobs'
+// row type (f32 or i8, one for all); d_max: the widest job's row length.
+hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, int rows, size_t d_max, size_t slots,
+                            const uint32_t *ids, const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
+                            uint32_t *counts_all, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
 
 // Error reporting shared by the C ABI.

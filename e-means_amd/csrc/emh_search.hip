@@ -224,10 +224,11 @@ __global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, c
     }
 }
 
-// FP32 jobs whose search ran as several centroid ranges: the lowest
-// (score, index) of the slices' results, as merge_slices_kernel does, one
-// batch slot per thread.
-__global__ void jobs_merge_f32_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
+// FP32 and I8 jobs whose search ran as several centroid ranges: the best
+// (score, index) of the slices' results, as merge_slices_kernel does (FP32:
+// the lowest score; I8: the highest packed key), one batch slot per thread.
+template <int OP>
+__global__ void jobs_merge_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= slots) return;
     const uint2 r = rmap[i];
@@ -237,8 +238,15 @@ __global__ void jobs_merge_f32_kernel(const JobDev *jobs, const uint2 *rmap, siz
     uint4 best = *(const uint4 *)pp;
     for (uint32_t s = 1; s < j.nslice; s++) {
         const uint4 p = *(const uint4 *)(pp + 4 * s);
-        const float ps = __uint_as_float(p.x), bs = __uint_as_float(best.x);
-        if (ps < bs || (ps == bs && p.y < best.y)) best = p;
+        bool better;
+        if constexpr (OP == OP_I8) {
+            const int32_t ps = (int32_t)p.x, bs = (int32_t)best.x;
+            better = ps > bs || (ps == bs && p.y < best.y);
+        } else {
+            const float ps = __uint_as_float(p.x), bs = __uint_as_float(best.x);
+            better = ps < bs || (ps == bs && p.y < best.y);
+        }
+        if (better) best = p;
     }
     j.labels[r.y] = best.y == NO_LABEL ? 0u : best.y;
     j.dist[r.y] = __uint_as_float(best.z);
@@ -373,12 +381,15 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
 // (|c|^2 + bias) / 2 with this step's bias, as prep_centroids_kernel +
 // set_bias_kernel do for one job. The jobs' stats (max |c|^2, max -bias)
 // take one atomic per job per workgroup: cmap lists each job's entries
-// together.
+// together. I8: the rounded, clamped i8 centroid (src/cpu_i8.rs:83) as the
+// operand, its exact sum c8^2 plus the rounded bias (src/cpu_i8.rs:112-117)
+// as the integer offset, 2^26 for pad centroids (as prep_centroids_kernel
+// and set_bias_kernel write them); no stats.
 constexpr int PREP_ENTRIES = 16;
 
 template <int OP>
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
-    constexpr int EB = OP == OP_F32 ? 4 : 2, EPQ = 16 / EB;  // element bytes, elements per quad
+    constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
     const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
     __shared__ uint32_t s_job[PREP_ENTRIES], s_norm[PREP_ENTRIES], s_bias[PREP_ENTRIES];
     const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
@@ -394,6 +405,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
             uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
             const size_t nq = jb.row_bytes / 16;
+            int64_t part64 = 0;
             for (size_t q = gl; q < nq; q += 16) {
                 uint32_t w[4];
 #pragma unroll
@@ -401,13 +413,37 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
 #pragma unroll
                 for (int i = 0; i < EPQ; i++) {
                     const size_t el = q * EPQ + i;
-                    const float v = (live && el < d) ? -cp[el] : 0.0f;
-                    if constexpr (OP == OP_F32) w[i] = __float_as_uint(v);
-                    else w[i / 2] |= (uint32_t)bf16_rne(v) << (16 * (i & 1));
+                    const bool in = live && el < d;
+                    if constexpr (OP == OP_I8) {
+                        int32_t qv = 0;
+                        if (in) {
+                            float r = roundf(cp[el]);
+                            r = r != r ? 0.0f : fminf(fmaxf(r, -128.0f), 127.0f);
+                            qv = (int32_t)r;
+                        }
+                        w[i / 4] |= ((uint32_t)qv & 0xFFu) << (8 * (i & 3));
+                        part64 += (int64_t)qv * qv;
+                    } else {
+                        const float v = in ? -cp[el] : 0.0f;
+                        if constexpr (OP == OP_F32) w[i] = __float_as_uint(v);
+                        else w[i / 2] |= (uint32_t)bf16_rne(v) << (16 * (i & 1));
+                    }
                 }
                 *(uint4 *)(tbase + q * tb * 16) = make_uint4(w[0], w[1], w[2], w[3]);
             }
-            if (!live) {
+            if constexpr (OP == OP_I8) {
+                for (int off = 8; off > 0; off >>= 1) part64 += shfl_xor_i64(part64, off);
+                int64_t v = (int64_t)1 << 26;
+                if (live) {
+                    v = part64;
+                    if (jb.penalized) {
+                        double r = round((double)jb.bias[jc]);
+                        r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
+                        v += (int64_t)r;
+                    }
+                }
+                if (gl == 0) jb.cn[jc] = v;
+            } else if (!live) {
                 if (gl == 0) jb.hn[jc] = INFINITY;
             } else {
                 // |c|^2 in f32 as src/cpu.rs:170-180 folds it
@@ -434,6 +470,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             }
         }
     }
+    if constexpr (OP == OP_I8) return;  // whole workgroup: no stats
     if (gl == 0) {
         s_job[slot] = job;
         s_norm[slot] = nbits;
@@ -762,8 +799,10 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     a.row_bytes = g.row_bytes;
     a.cen_op = j.cen_op;
     a.hn = j.hn;
+    a.cn = j.cn;
     a.stats = j.stats;
-    a.vec = (g.d * 4) % 16 == 0;  // rows are hipMalloc'd f32 tables
+    // whole 16-byte quads per row from a 16-byte aligned table
+    a.vec = (g.d * (g.op == OP_I8 ? 1 : 4)) % 16 == 0 && ((uintptr_t)j.x & 15) == 0;
     a.labels = j.labels;
     a.dist = j.dist;
     a.cand = j.cand;
@@ -794,9 +833,12 @@ size_t job_part_words(int op, int M, uint32_t nslice) {
     return (size_t)nslice * (op == OP_BF16 ? 2 * (size_t)M : 4);
 }
 
-hipError_t launch_jobs_merge_f32(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
+hipError_t launch_jobs_merge(int op, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
     if (!slots) return hipSuccess;
-    hipLaunchKernelGGL(jobs_merge_f32_kernel, dim3((unsigned)((slots + 255) / 256)), dim3(256), 0, s, jobs, rmap, slots);
+    const dim3 grid((unsigned)((slots + 255) / 256));
+    if (op == OP_F32) hipLaunchKernelGGL(jobs_merge_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, rmap, slots);
+    else if (op == OP_I8) hipLaunchKernelGGL(jobs_merge_kernel<OP_I8>, grid, dim3(256), 0, s, jobs, rmap, slots);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -805,6 +847,7 @@ hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_
     dim3 grid((unsigned)((entries + PREP_ENTRIES - 1) / PREP_ENTRIES));
     if (op == OP_F32) hipLaunchKernelGGL(prep_jobs_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_BF16) hipLaunchKernelGGL(prep_jobs_kernel<OP_BF16>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    else if (op == OP_I8) hipLaunchKernelGGL(prep_jobs_kernel<OP_I8>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

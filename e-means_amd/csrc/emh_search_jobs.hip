@@ -1,5 +1,5 @@
 // The batched search's instantiations: many independent problems in one
-// launch (search_reg_jobs_kernel), FP32 and BF16.
+// launch (search_reg_jobs_kernel), FP32, BF16 and I8.
 #include "emh_search_kernels.h"
 
 namespace emh {
@@ -15,6 +15,10 @@ hipError_t launch_jobs_search_op(int op, int M, const void *search_args, const u
     if (op == OP_F32) {
         if (M != 1) return hipErrorInvalidValue;
         return launch_jobs_search_m<OP_F32, 1>(search_args, wg, nwg, s);
+    }
+    if (op == OP_I8) {
+        if (M != 1) return hipErrorInvalidValue;
+        return launch_jobs_search_m<OP_I8, 1>(search_args, wg, nwg, s);
     }
     if (op != OP_BF16) return hipErrorInvalidValue;
     switch (M) {

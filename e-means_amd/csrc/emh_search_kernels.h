@@ -1262,6 +1262,7 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
     a.x = global_ptr_s(a.x);
     a.ids = global_ptr_s(a.ids);
     a.hn = global_ptr_s(a.hn);
+    a.cn = global_ptr_s(a.cn);
     a.stats = global_ptr_s(a.stats);
     a.part = global_ptr_s(a.part);
     a.cand = global_ptr_s(a.cand);

@@ -362,18 +362,22 @@ class BatchStats(C.Structure):
 def train_batch(problems, assigner: HipAssigner | None = None):
     """Trains independent problems in lockstep, one captured-graph replay per
     training step (emh_train_batch). problems: (x, k, cfg) tuples with f32
-    rows (host arrays, or torch CUDA tensors used in place). Each result
-    equals train(x, k, cfg, assigner) on the same assigner precision.
-    Returns (list of Output, BatchStats)."""
+    rows (i8 rows for the I8 assigner), host arrays or torch CUDA tensors
+    used in place. Each result equals train(x, k, cfg, assigner) on the same
+    assigner precision. Returns (list of Output, BatchStats)."""
     a = assigner or HipAssigner(FP32)
+    want = np.dtype(np.int8) if a.precision == I8 else np.dtype(np.float32)
+    kind = ROWS[want]
     keep, jobs = [], (Job * len(problems))()
     any_dev = False
     for i, (x, k, cfg) in enumerate(problems):
         dev = _device_rows(x)
         if dev is None:
-            x = np.ascontiguousarray(x, np.float32)
-        elif dev[1] != ROWS_F32:
-            raise TypeError("train_batch takes f32 rows")
+            if want == np.int8 and np.asarray(x).dtype != want:
+                raise TypeError("train_batch with the I8 assigner takes i8 rows")
+            x = np.ascontiguousarray(x, want)
+        elif dev[1] != kind:
+            raise TypeError(f"train_batch takes {want.name} rows with this assigner")
         any_dev = any_dev or dev is not None
         n, d = x.shape
         cent = np.empty((k, d), np.float32)
@@ -381,7 +385,7 @@ def train_batch(problems, assigner: HipAssigner | None = None):
         out = TrainOut()
         keep.append((x, cent, labels, out))
         ptr = _p(x) if dev is None else C.c_void_p(dev[0])
-        jobs[i] = Job(ptr, ROWS_F32, n, d, k, cfg or Config(), _p(cent), _p(labels), C.pointer(out),
+        jobs[i] = Job(ptr, kind, n, d, k, cfg or Config(), _p(cent), _p(labels), C.pointer(out),
                       int(dev is not None))
     if any_dev:
         import torch

@@ -278,11 +278,14 @@ int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice);
  * training step of all jobs still running is one replay of a captured
  * hipGraph: upload of every job's centroid snapshot, bias and batch ids,
  * then the gather + search (+ FP32 rescore) + per-cluster sums of all jobs,
- * then one download. Needs f32 rows and operand rows of at most 512 bytes
- * (FP32: d <= 128, BF16: d <= 256); precision FP32 or BF16_RESCORE. */
+ * then one download. Precision FP32 or BF16_RESCORE on f32 rows, or I8 on
+ * i8 rows; operand rows of at most 512 bytes (FP32: d <= 128, BF16:
+ * d <= 256, I8: d <= 512). I8 jobs run the packed-key search only: a
+ * balance bias with |sum c8^2 + round(bias)| >= 2^26 - d 2^15 stops the call
+ * with EMH_E_UNSUPPORTED (emh_train then takes the i64 search). */
 typedef struct emh_job {
     const void *x; /* n x d rows, row-major */
-    int rows;      /* enum emh_rows: EMH_ROWS_F32 */
+    int rows;      /* enum emh_rows: EMH_ROWS_F32 (EMH_ROWS_I8 for the I8 search) */
     uint64_t n, d, k;
     emh_config cfg;
     float *centroids_out; /* k x d */

@@ -259,3 +259,66 @@ def test_batched_device_rows_equal_host_rows(hip):
     for g, r in zip(got, ref):
         assert np.array_equal(g.labels, r.labels)
         assert np.array_equal(g.centroids, r.centroids)
+
+
+def test_batched_i8_jobs_equal_oracle(hip):
+    """int8 jobs in one lockstep batch (the packed integer search of every
+    job in one launch, i8 rows summed on the device): each job's centroids,
+    last-pass labels, objective and revivals equal the CPU oracle's int8
+    training bit for bit. Shapes cover row lengths off the 16-byte quad
+    (d = 7, 100), every operand width class up to 512 bytes (d = 300 and 512
+    also take the sums past 256 dimensions), a job whose centroid tiles
+    split into slices (k = 3000), clusters of hundreds of members per step
+    (k = 3, batches of 2048) and the balance penalty."""
+    specs = [(6000, 48, 40, 0.1, None), (900, 7, 30, 0.0, None), (4000, 100, 64, 0.1, None),
+             (3000, 200, 25, 0.0, None), (2500, 300, 20, 0.1, None), (2000, 512, 16, 0.0, None),
+             (9000, 64, 3000, 0.0, None), (12000, 24, 3, 0.1, 2048)]
+    problems = []
+    for i, (n, d, k, bal, bs) in enumerate(specs):
+        x = _i8_blobs(n, d, max(2, 2 * k), 80 + i)
+        problems.append((x, k, (bal, bs, 90 + i)))
+    batch = [(x, k, E.Config(nepochs=2 + (i % 2), balance=bal, seed=sd, batch_size=bs))
+             for i, (x, k, (bal, bs, sd)) in enumerate(problems)]
+    print(f"batch i8: {len(batch)} jobs", flush=True)
+    got, st = E.train_batch(batch, E.HipAssigner(E.I8))
+    assert st.steps == max(E.resolve_recipe(x.shape[0], x.shape[1], k, c).total_steps for x, k, c in batch)
+    for (x, k, cfg), g in zip(batch, got):
+        ocfg = O.Config(nepochs=cfg.nepochs, balance=cfg.balance, seed=cfg.seed,
+                        batch_size=cfg.batch_size if cfg.has_batch_size else None)
+        ref = O.train(x, k, ocfg, assigner=O.ASSIGNER_I8)
+        assert g.recipe.batch_size == ref.recipe.batch_size
+        assert np.array_equal(g.labels, ref.labels)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+        assert np.array_equal(g.centroids, ref.centroids)
+
+
+def test_batched_i8_device_rows_equal_host_rows(hip):
+    """A batched int8 job whose rows already sit in HBM (initial centroids
+    gathered and widened on the device) ends as its host-rows twin."""
+    import torch
+
+    xs = [_i8_blobs(5000, 40, 60, 3), _i8_blobs(3000, 130, 40, 4)]
+    cfgs = [E.Config(nepochs=2, seed=11), E.Config(nepochs=2, balance=0.1, seed=12)]
+    host = [(x, 30, c) for x, c in zip(xs, cfgs)]
+    dev = [(torch.from_numpy(xs[0]).cuda(), 30, cfgs[0]), host[1]]
+    ref, _ = E.train_batch(host, E.HipAssigner(E.I8))
+    got, _ = E.train_batch(dev, E.HipAssigner(E.I8))
+    for g, r in zip(got, ref):
+        assert np.array_equal(g.labels, r.labels)
+        assert np.array_equal(g.centroids, r.centroids)
+        assert g.objective == r.objective
+
+
+def test_batched_i8_bias_out_of_packed_range_is_an_error(hip):
+    """A balance penalty so strong that the rounded bias leaves the packed
+    int8 score range stops the batch with a clear error (the single-problem
+    driver takes the i64 search instead); the kernels all run in bounds."""
+    x = _i8_blobs(3000, 512, 20, 6)
+    cfg = E.Config(nepochs=2, balance=1000.0, seed=1)
+    with pytest.raises(E.EmeansError, match="packed I8 score range"):
+        E.train_batch([(x, 10, cfg)], E.HipAssigner(E.I8))
+    # the same problem on the single-problem driver: the oracle's result
+    ref = O.train(x, 10, O.Config(nepochs=2, balance=1000.0, seed=1), assigner=O.ASSIGNER_I8)
+    got = E.train(x, 10, cfg, E.HipAssigner(E.I8))
+    assert np.array_equal(got.labels, ref.labels) and np.array_equal(got.centroids, ref.centroids)
