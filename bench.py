@@ -680,11 +680,12 @@ def train_large_line(args, x, k, nepochs, prec, m):
 
     n, d = x.shape
     log(f"train {prec} warmup")
-    # the same k and batch size for two batches: the search's per-shape
-    # split choice and first-call costs stay out of the timed call
+    # the same k and batch sizes (two full batches and the short one closing
+    # an epoch): the search's per-shape tile choices and first-call costs
+    # stay out of the timed call
     bs = E.resolve_recipe(n, d, k, E.Config(nepochs=nepochs)).batch_size
     for p, mm in ((prec, m), ("fp32", 1)):
-        E.train(x[: min(n, max(2 * bs, k))], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(p, rescore_m=mm))
+        E.train(x[: warmup_rows(n, k, bs)], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(p, rescore_m=mm))
     runs = {}
     for name, p, mm in ((prec, prec, m), ("fp32", "fp32", 1)):
         log(f"train {name}")
@@ -731,6 +732,16 @@ def train_large_line(args, x, k, nepochs, prec, m):
     return res
 
 
+def warmup_rows(n, k, bs):
+    """Rows of a one-epoch warm-up training call whose batches have the timed
+    run's sizes: at least two full batches and the short batch n % bs (the
+    search picks its tiles per batch size, once per process)."""
+    w = 2 * bs + n % bs
+    while w < k:
+        w += bs
+    return min(n, w)
+
+
 def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     """GPU training (rows resident in HBM; per batch: id upload, gather +
     search + per-cluster sums on the device, EMA/revival on the host) vs the
@@ -743,9 +754,10 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     n, d = xh.shape
     cfg = E.Config(nepochs=nepochs)
     log(f"train {prec} warmup")
-    # the same k and batch size: first-call costs stay out of the timed call
+    # the same k and batch sizes (two full batches and the short one closing
+    # an epoch): tile choices and first-call costs stay out of the timed call
     bs = E.resolve_recipe(n, d, k, cfg).batch_size
-    E.train(x[: min(n, max(2 * bs, k))], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(prec))
+    E.train(x[: warmup_rows(n, k, bs)], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner(prec))
     log(f"train {prec}")
     t = time.perf_counter()
     out = E.train(x, k, cfg, E.HipAssigner(prec))  # rows already in HBM

@@ -175,10 +175,11 @@ __global__ __launch_bounds__(256) void digit_row_scan_kernel(uint32_t *hist, siz
 // offs holds each digit's row scanned over the blocks (digit_row_scan_kernel);
 // the digit's base (the total of the smaller digits) is added here from the
 // digit totals.
+// starts (or null): block 0 writes each digit's first sorted position.
 __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
                                                             int shift, const uint32_t *offs, size_t nblocks,
                                                             const uint32_t *totals, uint32_t *keys_out,
-                                                            uint32_t *vals_out) {
+                                                            uint32_t *vals_out, uint32_t *starts) {
     __shared__ uint32_t run[256];      // equal digits in the rounds before this one
     __shared__ uint32_t wcnt[4][256];  // per wave, this round
     __shared__ uint32_t sbase[256];    // total of the smaller digits
@@ -187,6 +188,7 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
     {
         uint32_t all;
         sbase[tid] = block_exclusive_scan(totals[tid], ssm, &all);
+        if (starts && blockIdx.x == 0) starts[tid] = sbase[tid];
     }
     run[tid] = 0;
     const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
@@ -241,7 +243,7 @@ __global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *ke
 __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
                                                                  const uint32_t *offs, size_t nblocks,
                                                                  const uint32_t *totals, uint32_t *keys_out,
-                                                                 uint32_t *vals_out) {
+                                                                 uint32_t *vals_out, uint32_t *starts) {
     constexpr int PER = WIDE_BINS / 256;
     __shared__ uint32_t run[WIDE_BINS];
     __shared__ uint32_t wcnt[4][WIDE_BINS];
@@ -261,6 +263,7 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             sbase[tid * PER + i] = base + loc[i];
+            if (starts && blockIdx.x == 0) starts[tid * PER + i] = base + loc[i];
             run[tid * PER + i] = 0;
 #pragma unroll
             for (int q = 0; q < 4; q++) wcnt[q][tid * PER + i] = 0;
@@ -321,6 +324,7 @@ hipError_t accum_reserve(AccumScratch &a, size_t m, size_t k) {
         if ((e = hipMalloc(&a.sort_tmp, a.sort_tmp_bytes)) != hipSuccess) return e;
         a.cap_rows = m;
     }
+    if (!a.starts && (e = hipMalloc(&a.starts, WIDE_BINS * 4)) != hipSuccess) return e;
     return e;
 }
 
@@ -331,12 +335,16 @@ void accum_free(AccumScratch &a) {
     hipFree(a.vals_alt);
     hipFree(a.sort_tmp);
     hipFree(a.big);
+    hipFree(a.starts);
     a = AccumScratch();
 }
 
-// vals_in: the value carried with each key (nullptr: its position)
+// vals_in: the value carried with each key (nullptr: its position).
+// counts_out (or null): for a one-pass sort (bits <= 11) the number of each
+// key (the pass's digit totals) and a.starts each key's first position.
 hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uint32_t *vals_in, size_t m, int bits,
-                             const uint32_t **keys_out, const uint32_t **vals_out, hipStream_t s) {
+                             const uint32_t **keys_out, const uint32_t **vals_out, hipStream_t s,
+                             const uint32_t **counts_out = nullptr) {
     const size_t nblocks = (m + SORT_BLOCK - 1) / SORT_BLOCK;
     uint32_t *hist = (uint32_t *)a.sort_tmp;
     // ping-pong: pass 0 reads the input keys and values
@@ -349,17 +357,20 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
                            hist + (size_t)WIDE_BINS * nblocks);
         hipLaunchKernelGGL(digit_scatter_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m,
                            (const uint32_t *)hist, nblocks, (const uint32_t *)(hist + (size_t)WIDE_BINS * nblocks),
-                           kb[0], vb[0]);
+                           kb[0], vb[0], counts_out ? a.starts : nullptr);
         *keys_out = kb[0];
         *vals_out = vb[0];
+        if (counts_out) *counts_out = hist + (size_t)WIDE_BINS * nblocks;
         return hipGetLastError();
     }
     for (int shift = 0; shift < bits; shift += 8) {
         hipLaunchKernelGGL(digit_hist_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, shift, hist, nblocks);
         hipLaunchKernelGGL(digit_row_scan_kernel, dim3(256), dim3(256), 0, s, hist, nblocks, hist + 256 * nblocks);
+        const bool one_pass = counts_out && bits <= 8;
         hipLaunchKernelGGL(digit_scatter_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m, shift,
                            (const uint32_t *)hist, nblocks, (const uint32_t *)(hist + 256 * nblocks), kb[cur],
-                           vb[cur]);
+                           vb[cur], one_pass ? a.starts : nullptr);
+        if (one_pass) *counts_out = hist + 256 * nblocks;
         kin = kb[cur];
         vin = vb[cur];
         cur ^= 1;
@@ -688,9 +699,34 @@ __global__ __launch_bounds__(256) void big_sum_kernel(const void *x, size_t d, c
     }
 }
 
+// One problem whose labels sort in one pass (k <= 2048): the sort's key
+// counts and first positions say where each cluster's members lie, so wave
+// w folds dimensions [64 (w % G), + 64) of cluster w / G (G = ceil(d / 64))
+// over its members in batch order, with no search for cluster boundaries;
+// it writes the cluster's count (also 0) and, for clusters with members,
+// its sums.
+template <int ROWS>
+__global__ __launch_bounds__(256) void cluster_sum_kernel(const void *x, size_t d, const uint32_t *src,
+                                                          const uint32_t *starts, const uint32_t *nkey, size_t k,
+                                                          uint32_t groups, float *sums, uint32_t *counts) {
+    const int lane = threadIdx.x & 63;
+    const size_t w = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const size_t c = w / groups;
+    if (c >= k) return;
+    const size_t tg = (w - c * groups) * 64;
+    const uint32_t n = nkey[c], p0 = starts[c];
+    if (tg == 0 && lane == 0) counts[c] = n;
+    if (n == 0) return;
+    auto src_of = [&](uint32_t q) -> size_t { return (size_t)src[p0 + q]; };
+    float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    fold_members<ROWS, 1, 64>(x, d, tg, n, lane, src_of, acc);
+    if (tg + lane < d) sums[c * d + tg + lane] = acc[0];
+}
+
 // Per-cluster counts and f32 sums of one batch, each cluster's rows folded
 // in batch order (src/lib.rs:431-441): labels sorted stably, carrying their
-// rows (ids[i], or i), then the sorted stream; empty clusters keep zeros.
+// rows (ids[i], or i), then per cluster (k <= 2048) or the sorted stream;
+// empty clusters keep zeros.
 hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m, size_t d, const uint32_t *ids,
                              const uint32_t *labels, size_t k, float *sums, uint32_t *counts, hipStream_t s,
                              bool cleared) {
@@ -705,8 +741,25 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if (m == 0) return hipSuccess;
-    const uint32_t *sorted_keys, *sorted_src;
-    if ((e = launch_sort_pairs(a, labels, ids, m, key_bits(k), &sorted_keys, &sorted_src, s)) != hipSuccess) return e;
+    const uint32_t *sorted_keys, *sorted_src, *nkey = nullptr;
+    const int bits = key_bits(k);
+    if ((e = launch_sort_pairs(a, labels, ids, m, bits, &sorted_keys, &sorted_src, s, bits <= 11 ? &nkey : nullptr)) !=
+        hipSuccess)
+        return e;
+    if (nkey) {
+        const uint32_t groups = (uint32_t)((d + 63) / 64);
+        const dim3 cgrid((unsigned)((k * groups + 3) / 4));
+        if (rows == EMH_ROWS_F32)
+            hipLaunchKernelGGL(cluster_sum_kernel<EMH_ROWS_F32>, cgrid, dim3(256), 0, s, x, d, sorted_src, a.starts,
+                               nkey, k, groups, sums, counts);
+        else if (rows == EMH_ROWS_I8)
+            hipLaunchKernelGGL(cluster_sum_kernel<EMH_ROWS_I8>, cgrid, dim3(256), 0, s, x, d, sorted_src, a.starts,
+                               nkey, k, groups, sums, counts);
+        else
+            hipLaunchKernelGGL(cluster_sum_kernel<EMH_ROWS_U8>, cgrid, dim3(256), 0, s, x, d, sorted_src, a.starts,
+                               nkey, k, groups, sums, counts);
+        return hipGetLastError();
+    }
     const dim3 grid((unsigned)(((m + SUM_SPAN - 1) / SUM_SPAN + 3) / 4)), bgrid(BIG_SUM_BLOCKS);
     switch (rows) {
         case EMH_ROWS_F32:
@@ -881,11 +934,20 @@ hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t 
     return hipGetLastError();
 }
 
-__global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint32_t *pairs, size_t npairs, size_t d) {
+__global__ __launch_bounds__(64) void revive_split_kernel(float *cen, const uint32_t *pairs, size_t npairs, size_t d,
+                                                          float *usage) {
     const size_t t = blockIdx.x;
     const int lane = threadIdx.x;
     if (t >= npairs) return;
-    float *cs = cen + (size_t)pairs[2 * t] * d, *cd = cen + (size_t)pairs[2 * t + 1] * d;
+    const uint32_t src = pairs[2 * t], dst = pairs[2 * t + 1];
+    // the pairs' clusters are distinct (a donor is used once, and is never
+    // an unused cluster), so each pair's masses are its own
+    if (usage && lane == 0) {
+        const float half = __fdiv_rn(usage[src], 2.0f);
+        usage[src] = half;
+        usage[dst] = half;
+    }
+    float *cs = cen + (size_t)src * d, *cd = cen + (size_t)dst * d;
     const float lo = 1.0f - 1e-4f, hi = 1.0f + 1e-4f;
     bool same = true;
     for (size_t i = lane; i < d; i += 64) {
@@ -937,9 +999,104 @@ hipError_t launch_jobs_revive(const JobDev *jobs, const uint32_t *triples, size_
     return hipGetLastError();
 }
 
-hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s) {
+hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s,
+                               float *usage) {
     if (!npairs) return hipSuccess;
-    hipLaunchKernelGGL(revive_split_kernel, dim3((unsigned)npairs), dim3(64), 0, s, cen, pairs, npairs, d);
+    hipLaunchKernelGGL(revive_split_kernel, dim3((unsigned)npairs), dim3(64), 0, s, cen, pairs, npairs, d, usage);
+    return hipGetLastError();
+}
+
+// ---- one problem's training step replayed as a graph ----
+// The step's kernels take everything that changes from step to step from
+// the device: ctr[0] is the next step's index, ctr[1] the running step's
+// (written by the stage kernel, read by the later kernels of the step), and
+// params[step] its batch position, EMA rate and last-epoch flag.
+
+// One workgroup: takes the next step index and copies the step's batch ids
+// (its slice of the epoch order) to a fixed array the step's kernels read.
+__global__ __launch_bounds__(1024) void step_stage_kernel(const uint32_t *order, const StepParam *params,
+                                                          uint32_t *ctr, uint32_t *step_ids, size_t nb) {
+    __shared__ uint32_t c;
+    if (threadIdx.x == 0) {
+        c = ctr[0];
+        ctr[0] = c + 1u;
+        ctr[1] = c;
+    }
+    __syncthreads();
+    const uint32_t *src = order + params[c].b0;
+    for (size_t i = threadIdx.x; i < nb; i += blockDim.x) step_ids[i] = src[i];
+}
+
+// A step that ran without the graph still takes its index.
+__global__ void step_advance_kernel(uint32_t *ctr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += 1u;
+}
+
+// ema_apply_kernel with the running step's rate and last-epoch flag.
+__global__ __launch_bounds__(256) void step_ema_kernel(float *cen, float *usage, const float *sums, uint32_t *counts,
+                                                       size_t k, size_t d, const StepParam *params,
+                                                       const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
+                                                       uint64_t *pass_counts) {
+    const StepParam p = params[ctr[1]];
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_word = 0u;
+    const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= k) return;
+    const int lane = threadIdx.x & 63;
+    if (p.is_last) {
+        const uint32_t n = counts[j];
+        if (n) {
+            for (size_t t = lane; t < d; t += 64) pass_sums[j * d + t] += (double)sums[j * d + t];
+            if (lane == 0) pass_counts[j] += n;
+        }
+    }
+    ema_cluster(cen, usage, sums, counts, j, d, p.alpha, p.decay, (int)p.replace, lane);
+    if (lane == 0) counts[j] = 0u;
+}
+
+// The step's labels and distances into pinned host slot (step % nslot):
+// labels at [0, nb), distances at [batch, batch + nb) of the slot, and
+// before a revival check the masses at [2 batch, 2 batch + k).
+__global__ __launch_bounds__(256) void step_out_kernel(const uint32_t *labels, const float *dist, size_t nb,
+                                                       size_t batch, const float *usage, size_t k,
+                                                       const StepParam *params, const uint32_t *ctr, StepSlots slots) {
+    const uint32_t c = ctr[1];
+    uint32_t *dst = slots.p[c % (uint32_t)slots.n];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
+        dst[i] = labels[i];
+        dst[batch + i] = __float_as_uint(dist[i]);
+    }
+    if (params[c].revive)
+        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride)
+            dst[2 * batch + i] = __float_as_uint(usage[i]);
+}
+
+hipError_t launch_step_stage(const uint32_t *order, const StepParam *params, uint32_t *ctr, uint32_t *step_ids,
+                             size_t nb, hipStream_t s) {
+    hipLaunchKernelGGL(step_stage_kernel, dim3(1), dim3(1024), 0, s, order, params, ctr, step_ids, nb);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_advance(uint32_t *ctr, hipStream_t s) {
+    hipLaunchKernelGGL(step_advance_kernel, dim3(1), dim3(64), 0, s, ctr);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_ema(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
+                           const StepParam *params, const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
+                           uint64_t *pass_counts, hipStream_t s) {
+    hipLaunchKernelGGL(step_ema_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
+                       params, ctr, clear_word, pass_sums, pass_counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_step_out(const uint32_t *labels, const float *dist, size_t nb, size_t batch, const float *usage,
+                           size_t k, const StepParam *params, const uint32_t *ctr, const StepSlots &slots,
+                           hipStream_t s) {
+    if (slots.n < 1 || slots.n > 8) return hipErrorInvalidValue;
+    const unsigned blocks = (unsigned)std::min<size_t>((std::max(nb, k) + 255) / 256, 256);
+    hipLaunchKernelGGL(step_out_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, s, labels, dist, nb, batch, usage, k,
+                       params, ctr, slots);
     return hipGetLastError();
 }
 

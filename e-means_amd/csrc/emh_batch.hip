@@ -25,18 +25,6 @@ namespace emh {
 
 namespace {
 
-size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
-
-// Byte layout of a block made of aligned segments.
-struct Layout {
-    size_t bytes = 0;
-    size_t add(size_t n) {
-        const size_t at = align_up(bytes, 256);
-        bytes = at + (n ? n : 1);
-        return at;
-    }
-};
-
 // Runs fn(0..n-1) on a few persistent host threads plus the caller; each
 // index runs exactly once, and run() returns when all have. Between runs the
 // workers poll (yielding the core) instead of sleeping: a step's host halves

@@ -208,6 +208,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j);
 struct AccumScratch {
     uint32_t *keys = nullptr, *keys_alt = nullptr, *vals = nullptr, *vals_alt = nullptr;
     uint32_t *big = nullptr;  // batched sums: [0] count, then (cluster, first position, members) triples
+    uint32_t *starts = nullptr;  // one-pass sorts (keys below 2048): each key's first sorted position
     void *sort_tmp = nullptr;
     size_t sort_tmp_bytes = 0;
     size_t cap_rows = 0;
@@ -233,8 +234,11 @@ hipError_t launch_ema_apply(float *cen, float *usage, const float *sums, const u
                             float alpha, hipStream_t s, uint32_t *clear_counts = nullptr,
                             uint32_t *clear_word = nullptr, double *pass_sums = nullptr,
                             uint64_t *pass_counts = nullptr);
-// The revival row split of src/lib.rs:330-340 for (donor, unused) pairs.
-hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s);
+// The revival row split of src/lib.rs:330-340 for (donor, unused) pairs;
+// with `usage`, also the masses' halving (src/lib.rs:334-336: both take half
+// the donor's mass). pairs may be host-coherent pinned memory.
+hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs, size_t d, hipStream_t s,
+                               float *usage = nullptr);
 // out (m x d f32) = rows ids of x (f32 / i8 / u8), widened
 hipError_t launch_gather_rows(const void *x, int rows, size_t d, const uint32_t *ids, size_t m, float *out,
                               hipStream_t s);
@@ -261,6 +265,39 @@ hipError_t launch_jobs_sums(AccumScratch &a, const JobDev *jobs, int rows, size_
                             const uint32_t *ids, const uint32_t *cjob, size_t nclusters, const uint32_t *keys,
                             uint32_t *counts_all, hipStream_t s);
 hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s);
+// ---- one problem's training step as a replayed graph (emh_accum.hip) ----
+// Per-step values of a run, fixed by the recipe before the first step.
+struct StepParam {
+    uint32_t b0, nb;     // the batch's position in the epoch order, its rows
+    float alpha, decay;  // the EMA rate and 1 - rate (f32, as the host computes them)
+    uint32_t replace;    // alpha == 1
+    uint32_t is_last;    // last epoch: the step's sums go into the FP64 pass totals
+    uint32_t revive;     // a revival check follows: the masses go to the host slot too
+    uint32_t pad;
+};
+// Pinned host slots the labels and distances of step s go to (slot s % n).
+struct StepSlots {
+    uint32_t *p[8];
+    int n;
+};
+// ctr[0]: the next step's index, ctr[1]: the running step's. The stage
+// kernel takes ctr[0] into ctr[1], advances ctr[0] and copies the step's
+// ids (order + params[step].b0, nb of them) to step_ids; a step run
+// without the graph advances ctr[0] with launch_step_advance.
+hipError_t launch_step_stage(const uint32_t *order, const StepParam *params, uint32_t *ctr, uint32_t *step_ids,
+                             size_t nb, hipStream_t s);
+hipError_t launch_step_advance(uint32_t *ctr, hipStream_t s);
+// launch_ema_apply with the running step's rate and last-epoch flag; it
+// also zeroes each count once read and *clear_word.
+hipError_t launch_step_ema(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
+                           const StepParam *params, const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
+                           uint64_t *pass_counts, hipStream_t s);
+// labels at [0, nb) and distances at [batch, batch + nb) of the slot; on
+// steps flagged `revive` the k masses at [2 batch, 2 batch + k).
+hipError_t launch_step_out(const uint32_t *labels, const float *dist, size_t nb, size_t batch, const float *usage,
+                           size_t k, const StepParam *params, const uint32_t *ctr, const StepSlots &slots,
+                           hipStream_t s);
+
 
 // Error reporting shared by the C ABI.
 void set_error(const std::string &msg);

@@ -192,10 +192,30 @@ static float widen(const void *x, int rows, size_t idx) {
     return ((const float *)x)[idx];
 }
 
+// Byte layout of a block made of 256-byte aligned segments.
+struct Layout {
+    size_t bytes = 0;
+    size_t add(size_t n) {
+        const size_t at = (bytes + 255) / 256 * 256;
+        bytes = at + (n ? n : 1);
+        return at;
+    }
+};
+
+// A segment of such a block.
+struct Seg {
+    void *p = nullptr;
+    template <class T> T *as() const { return (T *)p; }
+};
+
 // Pinned host buffer (hipHostMalloc).
 struct HostBuf {
     void *p = nullptr;
     hipError_t alloc(size_t bytes) { return hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocDefault); }
+    // host-coherent: kernels read it in place, seeing the host's latest writes
+    hipError_t alloc_coherent(size_t bytes) {
+        return hipHostMalloc(&p, bytes ? bytes : 16, hipHostMallocCoherent | hipHostMallocMapped);
+    }
     ~HostBuf() {
         if (p) (void)hipHostFree(p);
     }

@@ -261,6 +261,50 @@ def test_batched_device_rows_equal_host_rows(hip):
         assert np.array_equal(g.centroids, r.centroids)
 
 
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8", "i8", "i16"])
+def test_step_graph_equals_step_launches(hip, prec, monkeypatch):
+    """Training without the balance penalty replays each full batch's step as
+    one captured graph (step index and per-step rates read on the device);
+    the result equals the same training with every step launched kernel by
+    kernel (EMH_TRAIN_GRAPH=0) bit for bit: labels, centroids, objective and
+    revivals, over a short batch closing each epoch and a fractional epoch."""
+    if prec in ("i8", "i16"):
+        x = _i8_blobs(9000, 40, 90, 7)
+    else:
+        x = _job_data(17, 9000, 72, 180)
+    cfg = E.Config(nepochs=3.5, seed=21, batch_size=1024)
+    print(f"step graph {prec}", flush=True)
+    got = E.train(x, 70, cfg, E.HipAssigner(prec))
+    monkeypatch.setenv("EMH_TRAIN_GRAPH", "0")
+    ref = E.train(x, 70, cfg, E.HipAssigner(prec))
+    assert ref.recipe.total_steps == got.recipe.total_steps and got.n_splits == ref.n_splits
+    assert np.array_equal(got.labels, ref.labels)
+    assert np.array_equal(got.centroids, ref.centroids)
+    assert got.objective == ref.objective
+
+
+def test_step_graph_recaptured_when_search_scratch_moves(hip, monkeypatch, tmp_path):
+    """Regression: a captured step addresses the assigner's search scratch
+    (the per-slice results of a split search). When a later batch size (the
+    short batch closing an epoch) tunes a wider split, that buffer grows and
+    moves; the full-batch graph must then be captured again, not replayed
+    with the freed address. The full batch's tile comes from the tile cache
+    (tuned by an earlier call in this process), the short batch's is tuned
+    mid-run, and the result equals the run without graphs."""
+    monkeypatch.setenv("EMH_TUNE_CACHE", str(tmp_path / "tiles.json"))
+    x = _job_data(23, 20000, 96, 600)
+    bs, k = 4096, 1024
+    print("step graph recapture", flush=True)
+    E.train(x[: 2 * bs], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner("fp32"))  # tunes the full batch
+    cfg = E.Config(nepochs=2, seed=3, batch_size=bs)
+    got = E.train(x, k, cfg, E.HipAssigner("fp32"))
+    monkeypatch.setenv("EMH_TRAIN_GRAPH", "0")
+    ref = E.train(x, k, cfg, E.HipAssigner("fp32"))
+    assert np.array_equal(got.labels, ref.labels)
+    assert np.array_equal(got.centroids, ref.centroids)
+    assert got.objective == ref.objective and got.n_splits == ref.n_splits
+
+
 def test_batched_i8_jobs_equal_oracle(hip):
     """int8 jobs in one lockstep batch (the packed integer search of every
     job in one launch, i8 rows summed on the device): each job's centroids,
