@@ -240,14 +240,21 @@ __global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *ke
 // As digit_scatter_kernel; with 2048 digits the per-wave counts are not
 // cleared wholesale each round: the leader of a digit group adds its group
 // to the digit's running count and clears its own entry after the round.
+// RAW: offs is the unscanned histogram (digit_hist_wide_kernel) and each
+// block sums the earlier blocks' counts of its digits itself (a few blocks:
+// no scan launch); block 0 also writes the digit totals to totals_out.
+// Otherwise offs and totals come from digit_row_scan_kernel.
+template <bool RAW>
 __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
                                                                  const uint32_t *offs, size_t nblocks,
                                                                  const uint32_t *totals, uint32_t *keys_out,
-                                                                 uint32_t *vals_out, uint32_t *starts) {
+                                                                 uint32_t *vals_out, uint32_t *starts,
+                                                                 uint32_t *totals_out) {
     constexpr int PER = WIDE_BINS / 256;
     __shared__ uint32_t run[WIDE_BINS];
     __shared__ uint32_t wcnt[4][WIDE_BINS];
     __shared__ uint32_t sbase[WIDE_BINS];
+    __shared__ uint32_t pre[RAW ? WIDE_BINS : 1];
     __shared__ uint32_t ssm[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     {
@@ -256,7 +263,28 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             loc[i] = tsum;
-            tsum += totals[tid * PER + i];
+            uint32_t tot;
+            if constexpr (RAW) {
+                // the digit's row, 8 loads in flight at a time
+                const uint32_t *row = offs + (size_t)(tid * PER + i) * nblocks;
+                uint32_t p = 0;
+                tot = 0;
+                for (size_t b0 = 0; b0 < nblocks; b0 += 8) {
+                    uint32_t v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v[u] = b0 + u < nblocks ? row[b0 + u] : 0u;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        if (b0 + u < blockIdx.x) p += v[u];
+                        tot += v[u];
+                    }
+                }
+                pre[tid * PER + i] = p;
+                if (blockIdx.x == 0) totals_out[tid * PER + i] = tot;
+            } else {
+                tot = totals[tid * PER + i];
+            }
+            tsum += tot;
         }
         uint32_t all;
         const uint32_t base = block_exclusive_scan(tsum, ssm, &all);
@@ -287,7 +315,9 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
         if (leader) wcnt[w][dg] = cnt;
         __syncthreads();
         if (live) {
-            uint32_t pos = sbase[dg] + offs[(size_t)dg * nblocks + blockIdx.x] + run[dg] + before;
+            uint32_t pos = sbase[dg] + run[dg] + before;
+            if constexpr (RAW) pos += pre[dg];
+            else pos += offs[(size_t)dg * nblocks + blockIdx.x];
             for (int q = 0; q < w; q++) pos += wcnt[q][dg];
             keys_out[pos] = key;
             vals_out[pos] = vals ? vals[i] : (uint32_t)i;
@@ -352,12 +382,20 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
     uint32_t *kb[2] = {a.keys, a.keys_alt}, *vb[2] = {a.vals, a.vals_alt};
     int cur = 0;
     if (bits > 8 && bits <= 11) {
+        uint32_t *totals = hist + (size_t)WIDE_BINS * nblocks;
         hipLaunchKernelGGL(digit_hist_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist, nblocks);
-        hipLaunchKernelGGL(digit_row_scan_kernel, dim3(WIDE_BINS), dim3(256), 0, s, hist, nblocks,
-                           hist + (size_t)WIDE_BINS * nblocks);
-        hipLaunchKernelGGL(digit_scatter_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m,
-                           (const uint32_t *)hist, nblocks, (const uint32_t *)(hist + (size_t)WIDE_BINS * nblocks),
-                           kb[0], vb[0], counts_out ? a.starts : nullptr);
+        if (nblocks <= 64) {
+            // up to 64 blocks (64 Ki keys): each scatter block sums its digits'
+            // counts over the blocks itself, two launches instead of three
+            hipLaunchKernelGGL(digit_scatter_wide_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m,
+                               (const uint32_t *)hist, nblocks, (const uint32_t *)nullptr, kb[0], vb[0],
+                               counts_out ? a.starts : nullptr, totals);
+        } else {
+            hipLaunchKernelGGL(digit_row_scan_kernel, dim3(WIDE_BINS), dim3(256), 0, s, hist, nblocks, totals);
+            hipLaunchKernelGGL(digit_scatter_wide_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin,
+                               m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
+                               counts_out ? a.starts : nullptr, (uint32_t *)nullptr);
+        }
         *keys_out = kb[0];
         *vals_out = vb[0];
         if (counts_out) *counts_out = hist + (size_t)WIDE_BINS * nblocks;
@@ -1008,23 +1046,18 @@ hipError_t launch_revive_split(float *cen, const uint32_t *pairs, size_t npairs,
 
 // ---- one problem's training step replayed as a graph ----
 // The step's kernels take everything that changes from step to step from
-// the device: ctr[0] is the next step's index, ctr[1] the running step's
-// (written by the stage kernel, read by the later kernels of the step), and
-// params[step] its batch position, EMA rate and last-epoch flag.
+// the device: ctr[0] is the step's index (advanced by the step's last
+// kernel), ctr[1] a copy the stage kernel makes for the later kernels, and
+// params[step] the step's batch position, EMA rate and flags.
 
-// One workgroup: takes the next step index and copies the step's batch ids
-// (its slice of the epoch order) to a fixed array the step's kernels read.
-__global__ __launch_bounds__(1024) void step_stage_kernel(const uint32_t *order, const StepParam *params,
-                                                          uint32_t *ctr, uint32_t *step_ids, size_t nb) {
-    __shared__ uint32_t c;
-    if (threadIdx.x == 0) {
-        c = ctr[0];
-        ctr[0] = c + 1u;
-        ctr[1] = c;
-    }
-    __syncthreads();
-    const uint32_t *src = order + params[c].b0;
-    for (size_t i = threadIdx.x; i < nb; i += blockDim.x) step_ids[i] = src[i];
+// Copies the step's batch ids (its slice of the epoch order) to a fixed
+// array the step's kernels read, one per thread.
+__global__ __launch_bounds__(256) void step_stage_kernel(const uint32_t *order, const StepParam *params,
+                                                         uint32_t *ctr, uint32_t *step_ids, size_t nb) {
+    const uint32_t c = ctr[0];
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[1] = c;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nb) step_ids[i] = order[params[c].b0 + i];
 }
 
 // A step that ran without the graph still takes its index.
@@ -1032,12 +1065,27 @@ __global__ void step_advance_kernel(uint32_t *ctr) {
     if (threadIdx.x == 0 && blockIdx.x == 0) ctr[0] += 1u;
 }
 
-// ema_apply_kernel with the running step's rate and last-epoch flag.
-__global__ __launch_bounds__(256) void step_ema_kernel(float *cen, float *usage, const float *sums, uint32_t *counts,
-                                                       size_t k, size_t d, const StepParam *params,
-                                                       const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
-                                                       uint64_t *pass_counts) {
-    const StepParam p = params[ctr[1]];
+// The step's last kernel: the EMA step with the running step's rate and
+// last-epoch flag (ema_apply_kernel's, one wave per cluster) and the step's labels and distances into pinned host slot
+// (step % nslot): labels at [0, nb), distances at [batch, batch + nb) of
+// the slot, and before a revival check each cluster's new mass at
+// [2 batch + j].
+__global__ __launch_bounds__(256) void step_finish_kernel(float *cen, float *usage, const float *sums,
+                                                          uint32_t *counts, size_t k, size_t d,
+                                                          const StepParam *params, uint32_t *ctr,
+                                                          uint32_t *clear_word, double *pass_sums,
+                                                          uint64_t *pass_counts, const uint32_t *labels,
+                                                          const float *dist, size_t nb, size_t batch,
+                                                          StepSlots slots) {
+    const uint32_t c = ctr[1];
+    const StepParam p = params[c];
+    uint32_t *dst = slots.p[c % (uint32_t)slots.n];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = c + 1u;  // the next step's index
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
+        dst[i] = labels[i];
+        dst[batch + i] = __float_as_uint(dist[i]);
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) *clear_word = 0u;
     const size_t j = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (j >= k) return;
@@ -1050,30 +1098,16 @@ __global__ __launch_bounds__(256) void step_ema_kernel(float *cen, float *usage,
         }
     }
     ema_cluster(cen, usage, sums, counts, j, d, p.alpha, p.decay, (int)p.replace, lane);
-    if (lane == 0) counts[j] = 0u;
-}
-
-// The step's labels and distances into pinned host slot (step % nslot):
-// labels at [0, nb), distances at [batch, batch + nb) of the slot, and
-// before a revival check the masses at [2 batch, 2 batch + k).
-__global__ __launch_bounds__(256) void step_out_kernel(const uint32_t *labels, const float *dist, size_t nb,
-                                                       size_t batch, const float *usage, size_t k,
-                                                       const StepParam *params, const uint32_t *ctr, StepSlots slots) {
-    const uint32_t c = ctr[1];
-    uint32_t *dst = slots.p[c % (uint32_t)slots.n];
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
-        dst[i] = labels[i];
-        dst[batch + i] = __float_as_uint(dist[i]);
+    if (lane == 0) {
+        counts[j] = 0u;
+        if (p.revive) dst[2 * batch + j] = __float_as_uint(usage[j]);
     }
-    if (params[c].revive)
-        for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < k; i += stride)
-            dst[2 * batch + i] = __float_as_uint(usage[i]);
 }
 
 hipError_t launch_step_stage(const uint32_t *order, const StepParam *params, uint32_t *ctr, uint32_t *step_ids,
                              size_t nb, hipStream_t s) {
-    hipLaunchKernelGGL(step_stage_kernel, dim3(1), dim3(1024), 0, s, order, params, ctr, step_ids, nb);
+    hipLaunchKernelGGL(step_stage_kernel, dim3((unsigned)std::max<size_t>((nb + 255) / 256, 1)), dim3(256), 0, s,
+                       order, params, ctr, step_ids, nb);
     return hipGetLastError();
 }
 
@@ -1082,21 +1116,14 @@ hipError_t launch_step_advance(uint32_t *ctr, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_step_ema(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
-                           const StepParam *params, const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
-                           uint64_t *pass_counts, hipStream_t s) {
-    hipLaunchKernelGGL(step_ema_kernel, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, s, cen, usage, sums, counts, k, d,
-                       params, ctr, clear_word, pass_sums, pass_counts);
-    return hipGetLastError();
-}
-
-hipError_t launch_step_out(const uint32_t *labels, const float *dist, size_t nb, size_t batch, const float *usage,
-                           size_t k, const StepParam *params, const uint32_t *ctr, const StepSlots &slots,
-                           hipStream_t s) {
+hipError_t launch_step_finish(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
+                              const StepParam *params, uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
+                              uint64_t *pass_counts, const uint32_t *labels, const float *dist, size_t nb,
+                              size_t batch, const StepSlots &slots, hipStream_t s) {
     if (slots.n < 1 || slots.n > 8) return hipErrorInvalidValue;
-    const unsigned blocks = (unsigned)std::min<size_t>((std::max(nb, k) + 255) / 256, 256);
-    hipLaunchKernelGGL(step_out_kernel, dim3(blocks ? blocks : 1), dim3(256), 0, s, labels, dist, nb, batch, usage, k,
-                       params, ctr, slots);
+    const size_t blocks = std::max<size_t>((k + 3) / 4, std::min<size_t>((nb + 255) / 256, 256));
+    hipLaunchKernelGGL(step_finish_kernel, dim3((unsigned)blocks), dim3(256), 0, s, cen, usage, sums, counts, k, d,
+                       params, ctr, clear_word, pass_sums, pass_counts, labels, dist, nb, batch, slots);
     return hipGetLastError();
 }
 

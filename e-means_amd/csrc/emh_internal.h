@@ -280,23 +280,22 @@ struct StepSlots {
     uint32_t *p[8];
     int n;
 };
-// ctr[0]: the next step's index, ctr[1]: the running step's. The stage
-// kernel takes ctr[0] into ctr[1], advances ctr[0] and copies the step's
-// ids (order + params[step].b0, nb of them) to step_ids; a step run
-// without the graph advances ctr[0] with launch_step_advance.
+// ctr[0]: the step's index, ctr[1]: its copy for the step's later kernels.
+// The stage kernel copies ctr[0] to ctr[1] and the step's ids (order +
+// params[step].b0, nb of them) to step_ids; the step's last kernel
+// (launch_step_finish) advances ctr[0], and a step run without the graph
+// advances it with launch_step_advance.
 hipError_t launch_step_stage(const uint32_t *order, const StepParam *params, uint32_t *ctr, uint32_t *step_ids,
                              size_t nb, hipStream_t s);
 hipError_t launch_step_advance(uint32_t *ctr, hipStream_t s);
-// launch_ema_apply with the running step's rate and last-epoch flag; it
-// also zeroes each count once read and *clear_word.
-hipError_t launch_step_ema(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
-                           const StepParam *params, const uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
-                           uint64_t *pass_counts, hipStream_t s);
-// labels at [0, nb) and distances at [batch, batch + nb) of the slot; on
-// steps flagged `revive` the k masses at [2 batch, 2 batch + k).
-hipError_t launch_step_out(const uint32_t *labels, const float *dist, size_t nb, size_t batch, const float *usage,
-                           size_t k, const StepParam *params, const uint32_t *ctr, const StepSlots &slots,
-                           hipStream_t s);
+// The step's last kernel: launch_ema_apply with the running step's rate
+// and last-epoch flag (each count zeroed once read, and *clear_word), and
+// the step's labels at [0, nb) and distances at [batch, batch + nb) of its
+// slot; on steps flagged `revive` also the k new masses at [2 batch, + k).
+hipError_t launch_step_finish(float *cen, float *usage, const float *sums, uint32_t *counts, size_t k, size_t d,
+                              const StepParam *params, uint32_t *ctr, uint32_t *clear_word, double *pass_sums,
+                              uint64_t *pass_counts, const uint32_t *labels, const float *dist, size_t nb,
+                              size_t batch, const StepSlots &slots, hipStream_t s);
 
 
 // Error reporting shared by the C ABI.

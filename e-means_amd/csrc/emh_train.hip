@@ -295,9 +295,9 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                     (rc = check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), "begin capture")))
                     return rc;
                 uint32_t *sids = step_ids_d.as<uint32_t>();
-                const uint32_t *ctr = ctr_d.as<uint32_t>();
-                int crc = check(launch_step_stage(ids_d.as<uint32_t>(), params_d.as<StepParam>(), ctr_d.as<uint32_t>(),
-                                                  sids, nb, h->stream),
+                uint32_t *ctr = ctr_d.as<uint32_t>();
+                int crc = check(launch_step_stage(ids_d.as<uint32_t>(), params_d.as<StepParam>(), ctr, sids, nb,
+                                                  h->stream),
                                 "step stage");
                 if (!crc) crc = set_centroids_impl(h, cent_d.as<float>(), k, d, true, true);
                 if (!crc) crc = run_search(h, rows_dev, rows, nb, sids, lab_dev, dist_dev);
@@ -306,14 +306,11 @@ static int train_impl(emh_assigner *h, const void *x, int rows, size_t n, size_t
                                                   counts_d.as<uint32_t>(), h->stream, true),
                                 "accumulate");
                 if (!crc)
-                    crc = check(launch_step_ema(cent_d.as<float>(), usage_d.as<float>(), sums_d.as<float>(),
-                                                counts_d.as<uint32_t>(), k, d, params_d.as<StepParam>(), ctr, h->acc.big,
-                                                pass_d.as<double>(), passn_d.as<uint64_t>(), h->stream),
+                    crc = check(launch_step_finish(cent_d.as<float>(), usage_d.as<float>(), sums_d.as<float>(),
+                                                   counts_d.as<uint32_t>(), k, d, params_d.as<StepParam>(), ctr,
+                                                   h->acc.big, pass_d.as<double>(), passn_d.as<uint64_t>(), lab_dev,
+                                                   dist_dev, nb, batch, slots_dev, h->stream),
                                 "EMA step");
-                if (!crc)
-                    crc = check(launch_step_out(lab_dev, dist_dev, nb, batch, usage_d.as<float>(), k,
-                                                params_d.as<StepParam>(), ctr, slots_dev, h->stream),
-                                "labels and distances");
                 const hipError_t ee = hipStreamEndCapture(h->stream, &sg.graph);
                 if (crc) return crc;
                 if ((rc = check(ee, "end capture")) ||
