@@ -210,37 +210,83 @@ __device__ __forceinline__ uint32_t umed3(uint32_t a, uint32_t b, uint32_t c) {
 }
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
-// The two smallest keys of one 32x32 block's 16 values of a lane, built in
-// 8 phases so a caller can spread them between MFMAs: phases 0-4 take
-// triples (3p, 3p+1, 3p+2) to (min, median); phase 5 merges triples 3 and 4
-// with value 15; phase 6 merges triples 0-2; phase 7 merges the two halves.
-// (Of pairs lo_i <= hi_i, the second smallest is min(second smallest lo,
-// hi of the smallest lo), and every other hi is at least the second
-// smallest lo: hence min(med3(lo), min3(hi)).)
+// The fold's integer ops as single instructions. Left to itself the
+// compiler spells med3 as max/min chains wherever it shares their halves
+// with a min3, and splits a key into an and + or pair (a VOP3 on gfx9 takes
+// no 32-bit literal); the candidate fold is bound by these vector ops. min3
+// and med3 are written out: they read only vector-ALU results. A key reads
+// an MFMA result, whose wait states the compiler inserts only for code it
+// sees, so the key stays C++ and gets its single v_and_or_b32 from a mask
+// the compiler cannot fold: an SGPR written by an opaque s_mov.
+__device__ __forceinline__ uint32_t vmin3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_min3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+__device__ __forceinline__ uint32_t vmed3(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t d;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+// 0xFFFFFFE0 (the key mask of cand_key) in an SGPR
+__device__ __forceinline__ uint32_t cand_mask() {
+    uint32_t m;
+    asm volatile("s_mov_b32 %0, 0xffffffe0" : "=s"(m));
+    return m;
+}
+// cand_key(v, POS) given mask = cand_mask()
+template <int POS>
+__device__ __forceinline__ uint32_t vkey(float v, uint32_t mask) {
+    static_assert(POS >= 0 && POS < 32, "position");
+    return (__float_as_uint(v) & mask) | (uint32_t)POS;
+}
+
+// The two smallest keys of one 32x32 block's 16 values of a lane (positions
+// POS0 .. POS0 + 15), built in 8 phases so a caller can spread them between
+// MFMAs: phases 0-4 take triples (3p, 3p+1, 3p+2) to (min, median); phase 5
+// merges triples 3 and 4 with value 15; phase 6 merges triples 0-2; phase 7
+// merges the two halves. (Of pairs lo_i <= hi_i, the second smallest is
+// min(second smallest lo, hi of the smallest lo), and every other hi is at
+// least the second smallest lo: hence min(med3(lo), min3(hi)).) 16 key ops
+// and 20 min/med ops per block.
 struct Top2 {
     uint32_t pl[5], ph[5], ql[2], qh[2], lo, hi;
 };
-template <bool PIN>
-__device__ __forceinline__ void top2_phase(Top2 &s, const f32x16 &a, uint32_t pos0, int p) {
-    if (p < 5) {
-        const uint32_t k0 = cand_key(a[3 * p], pos0 + 3 * p), k1 = cand_key(a[3 * p + 1], pos0 + 3 * p + 1);
-        const uint32_t k2 = cand_key(a[3 * p + 2], pos0 + 3 * p + 2);
-        s.pl[p] = umin3(k0, k1, k2);
-        s.ph[p] = umed3(k0, k1, k2);
-        if constexpr (PIN) asm volatile("" : "+v"(s.pl[p]), "+v"(s.ph[p]));
-    } else if (p == 5) {
-        const uint32_t k = cand_key(a[15], pos0 + 15);
-        s.ql[1] = umin3(s.pl[3], s.pl[4], k);
-        s.qh[1] = umin3(umed3(s.pl[3], s.pl[4], k), s.ph[3], s.ph[4]);
+template <bool PIN, int POS0, int P>
+__device__ __forceinline__ void top2_step(Top2 &s, const f32x16 &a, uint32_t mask) {
+    if constexpr (P < 5) {
+        const uint32_t k0 = vkey<POS0 + 3 * P>(a[3 * P], mask), k1 = vkey<POS0 + 3 * P + 1>(a[3 * P + 1], mask);
+        const uint32_t k2 = vkey<POS0 + 3 * P + 2>(a[3 * P + 2], mask);
+        s.pl[P] = vmin3(k0, k1, k2);
+        s.ph[P] = vmed3(k0, k1, k2);
+        if constexpr (PIN) asm volatile("" : "+v"(s.pl[P]), "+v"(s.ph[P]));
+    } else if constexpr (P == 5) {
+        const uint32_t k = vkey<POS0 + 15>(a[15], mask);
+        s.ql[1] = vmin3(s.pl[3], s.pl[4], k);
+        s.qh[1] = vmin3(vmed3(s.pl[3], s.pl[4], k), s.ph[3], s.ph[4]);
         if constexpr (PIN) asm volatile("" : "+v"(s.ql[1]), "+v"(s.qh[1]));
-    } else if (p == 6) {
-        s.ql[0] = umin3(s.pl[0], s.pl[1], s.pl[2]);
-        s.qh[0] = min(umed3(s.pl[0], s.pl[1], s.pl[2]), umin3(s.ph[0], s.ph[1], s.ph[2]));
+    } else if constexpr (P == 6) {
+        s.ql[0] = vmin3(s.pl[0], s.pl[1], s.pl[2]);
+        s.qh[0] = min(vmin3(vmed3(s.pl[0], s.pl[1], s.pl[2]), s.ph[0], s.ph[1]), s.ph[2]);
         if constexpr (PIN) asm volatile("" : "+v"(s.ql[0]), "+v"(s.qh[0]));
     } else {
         s.lo = min(s.ql[0], s.ql[1]);
-        s.hi = umin3(max(s.ql[0], s.ql[1]), s.qh[0], s.qh[1]);
+        s.hi = vmin3(max(s.ql[0], s.ql[1]), s.qh[0], s.qh[1]);
         if constexpr (PIN) asm volatile("" : "+v"(s.lo), "+v"(s.hi));
+    }
+}
+// phase p (a constant after unrolling) of the tree above
+template <bool PIN, int POS0>
+__device__ __forceinline__ void top2_phase(Top2 &s, const f32x16 &a, uint32_t mask, int p) {
+    switch (p) {
+        case 0: top2_step<PIN, POS0, 0>(s, a, mask); break;
+        case 1: top2_step<PIN, POS0, 1>(s, a, mask); break;
+        case 2: top2_step<PIN, POS0, 2>(s, a, mask); break;
+        case 3: top2_step<PIN, POS0, 3>(s, a, mask); break;
+        case 4: top2_step<PIN, POS0, 4>(s, a, mask); break;
+        case 5: top2_step<PIN, POS0, 5>(s, a, mask); break;
+        case 6: top2_step<PIN, POS0, 6>(s, a, mask); break;
+        default: top2_step<PIN, POS0, 7>(s, a, mask); break;
     }
 }
 
@@ -255,7 +301,7 @@ __device__ __forceinline__ void cand_insert(uint32_t (&bs)[M], uint32_t (&bi)[M]
 #pragma unroll
     for (int t = M - 1; t > 0; t--) {
         bi[t] = c[t - 1] ? bi[t - 1] : (c[t] ? vi : bi[t]);
-        bs[t] = umed3(bs[t - 1], bs[t], v);
+        bs[t] = vmed3(bs[t - 1], bs[t], v);
     }
     bi[0] = c[0] ? vi : bi[0];
     bs[0] = min(bs[0], v);
@@ -714,15 +760,17 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
             topm_insert<M, uint32_t>(bs[fr], bi[fr], g, gi);
         } else if constexpr (OP == OP_BF16 || OP == OP_FP8) {
             // the tile's two smallest keys, then the exact list update
+            static_assert(NB <= 2, "a candidate tile has at most two blocks");
+            const uint32_t mask = cand_mask();
             Top2 t[NB];
 #pragma unroll
-            for (int nb = 0; nb < NB; nb++)
-#pragma unroll
-                for (int p = 0; p < 8; p++) top2_phase<false>(t[nb], acc[fr][nb], (uint32_t)(nb * 16), p);
+            for (int p = 0; p < 8; p++) top2_phase<false, 0>(t[0], acc[fr][0], mask, p);
             uint32_t m1 = t[0].lo, m2 = t[0].hi;
             if constexpr (NB == 2) {
-                m1 = min(t[0].lo, t[1].lo);
-                m2 = umin3(max(t[0].lo, t[1].lo), t[0].hi, t[1].hi);
+#pragma unroll
+                for (int p = 0; p < 8; p++) top2_phase<false, 16>(t[NB - 1], acc[fr][NB - 1], mask, p);
+                m1 = min(t[0].lo, t[NB - 1].lo);
+                m2 = vmin3(max(t[0].lo, t[NB - 1].lo), t[0].hi, t[NB - 1].hi);
             }
             cand_settle<M, NB>(acc[fr], m1, m2, h, c0, bs[fr], bi[fr]);
         } else if constexpr (OP == OP_F32) {
@@ -799,6 +847,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
     };
     uint32_t g[FR];
     Top2 t0[FR];  // M > 1: block 0's two smallest keys
+    const uint32_t mask = cand_mask();
     {  // block 0
         uint4 ops[NS];
 #pragma unroll
@@ -846,7 +895,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
                 for (int fr = 0; fr < FR; fr++)
 #pragma unroll
                     for (int p = 0; p < 8; p++)
-                        if ((p * STEPS) / 8 == st) top2_phase<true>(t0[fr], acc[fr][0], 0u, p);
+                        if ((p * STEPS) / 8 == st) top2_phase<true, 0>(t0[fr], acc[fr][0], mask, p);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -868,9 +917,9 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
         } else {
             Top2 t1;
 #pragma unroll
-            for (int p = 0; p < 8; p++) top2_phase<false>(t1, acc[fr][1], 16u, p);
+            for (int p = 0; p < 8; p++) top2_phase<false, 16>(t1, acc[fr][1], mask, p);
             const uint32_t m1 = min(t0[fr].lo, t1.lo);
-            const uint32_t m2 = umin3(max(t0[fr].lo, t1.lo), t0[fr].hi, t1.hi);
+            const uint32_t m2 = vmin3(max(t0[fr].lo, t1.lo), t0[fr].hi, t1.hi);
             cand_settle<M, 2>(acc[fr], m1, m2, h, c0, bs[fr], bi[fr]);
         }
     }
