@@ -166,9 +166,11 @@ def run_search(prec, m, x, c_host, steps, warmup):
     n = x.shape[0]
     labels = torch.empty(n, dtype=torch.int32, device=x.device)
     dist = torch.empty(n, dtype=torch.float32, device=x.device)
-    for _ in range(warmup):
+    def one():
         a.assign_device(x, labels, dist)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+
+    wcalls = warm(one, warmup)
     a.set_timing(True)
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
@@ -184,6 +186,7 @@ def run_search(prec, m, x, c_host, steps, warmup):
     fr, ns = a.last_tile()
     return {
         "tile": {"rows_per_workgroup": 128 * fr, "centroid_slices": ns},
+        "warmup_calls": wcalls,
         "ms_per_step": ms,
         "rows_per_s": n / (ms * 1e-3),
         "search_ms": search / max(calls, 1),
@@ -420,9 +423,11 @@ def quantize_line(args, n, d):
     shift = torch.empty(d, dtype=torch.float32, device=dev)
     codes = torch.empty(n, d, dtype=torch.int8, device=dev)
     scale = E.quant_fit_device(x, shift)
-    for _ in range(args.warmup):
+    def enc():
         E.quant_encode_device(x, shift, scale, codes)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+
+    warm(enc, args.warmup)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(args.steps):
@@ -497,9 +502,11 @@ def report_nearest_line(args, k, d):
     nearest = torch.empty(k, dtype=torch.int32, device=dev)
     dist = torch.empty(k, dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream().cuda_stream
-    for _ in range(args.warmup):
+    def rep():
         E.report_nearest_device(c, sizes, keys, nearest, dist, stream)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+
+    warm(rep, args.warmup)
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(args.steps):
@@ -601,8 +608,7 @@ def train_batch_line(args, njobs, prec, m):
     log("batched warmup")
     # one identical call first: first-call costs (code-object loads,
     # allocations, graph captures of every shape) stay out of the timed call
-    for _ in range(2):
-        E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
+    warm(lambda: E.train_batch(probs, E.HipAssigner(prec, rescore_m=m)), 2)
     log(f"batched {prec} ({njobs} jobs)")
     torch_sync()
     t = time.perf_counter()
@@ -688,8 +694,7 @@ def train_large_line(args, x, k, nepochs, prec, m):
     bs = E.resolve_recipe(n, d, k, E.Config(nepochs=nepochs)).batch_size
     # the headline precision: one identical call; the FP32 comparison (tens
     # of seconds a call): its batch sizes for one epoch
-    for _ in range(2):
-        E.train(x, k, E.Config(nepochs=nepochs), E.HipAssigner(prec, rescore_m=m))
+    warm(lambda: E.train(x, k, E.Config(nepochs=nepochs), E.HipAssigner(prec, rescore_m=m)), 1)
     E.train(x[: warmup_rows(n, k, bs)], k, E.Config(nepochs=1, batch_size=bs), E.HipAssigner("fp32"))
     runs = {}
     for name, p, mm in ((prec, prec, m), ("fp32", "fp32", 1)):
@@ -744,6 +749,23 @@ def torch_sync():
     torch.cuda.synchronize()
 
 
+# Untimed warm-up: at least this long of the same work before any timed
+# region. The GPU's clock ramps up over the first tens of milliseconds of
+# load; a warm-up of a few short launches leaves the timed ones on the ramp.
+WARM_SECONDS = 0.3
+
+
+def warm(fn, min_calls):
+    """Calls fn() at least min_calls times and until WARM_SECONDS have passed."""
+    t = time.perf_counter()
+    calls = 0
+    while calls < min_calls or time.perf_counter() - t < WARM_SECONDS:
+        fn()
+        calls += 1
+    torch_sync()
+    return calls
+
+
 def warmup_rows(n, k, bs):
     """Rows of a one-epoch warm-up training call whose batches have the timed
     run's sizes: at least two full batches and the short batch n % bs (the
@@ -772,8 +794,7 @@ def train_line(args, x, k, nepochs, prec, oracle_assigner, mse_x=None):
     # allocations of these sizes, and the GPU's clock ramp (a 200k-row run
     # lasts milliseconds; the first two calls of a process measure about
     # twice as long as the third) stay out of the timed call
-    for _ in range(3):
-        E.train(x, k, cfg, E.HipAssigner(prec))
+    warm(lambda: E.train(x, k, cfg, E.HipAssigner(prec)), 3)
     log(f"train {prec}")
     torch_sync()
     t = time.perf_counter()
