@@ -146,66 +146,6 @@ static int search_launch(emh_assigner *h, int M, const void *x_dev, int rows, si
                  "search kernel");
 }
 
-// search_launch over rows [lo, lo + m) of a call: every row-indexed pointer
-// offset by lo (the ids, or the rows themselves when there are no ids)
-static int search_launch_rows(emh_assigner *h, int M, const void *x_dev, int rows, size_t lo, size_t m,
-                              const uint32_t *ids_dev, const uint8_t *x8, const float *rs, const float *rn,
-                              uint32_t *labels_dev, float *dist_dev, uint32_t *cand, Tile t, uint32_t *part) {
-    const SearchGeom &g = h->g;
-    const size_t eb = rows == EMH_ROWS_F32 ? 4 : 1;
-    const void *x = ids_dev ? x_dev : (const void *)((const uint8_t *)x_dev + lo * g.d * eb);
-    return search_launch(h, M, x, rows, m, ids_dev ? ids_dev + lo : nullptr, x8 ? x8 + lo * g.d_pad : nullptr,
-                         rs ? rs + lo : nullptr, rn ? rn + lo : nullptr, labels_dev ? labels_dev + lo : nullptr,
-                         dist_dev ? dist_dev + lo : nullptr, cand ? cand + lo * (size_t)M : nullptr, t, part);
-}
-
-// The grid's last round. A launch of R row blocks on S = 2 per CU resident
-// workgroups (the register-resident kernels' occupancy) runs ceil(R / S)
-// rounds of equal workgroups. When the last round is at most half full,
-// its row blocks run as a second launch whose centroid tiles are split over
-// ns slices (merged in slice order, so the results are those of the
-// unsplit launch bit for bit): the last round then takes about 1/ns of a
-// round. EMH_TAIL_SPLIT=0 keeps one launch.
-struct TailPlan {
-    size_t m_main;  // rows of the first launch
-    int ns;         // slices of the second (1: no second launch)
-};
-static TailPlan tail_plan(const emh_assigner *h, size_t m, Tile t) {
-    TailPlan p{m, 1};
-    const char *e = getenv("EMH_TAIL_SPLIT");
-    if ((e && e[0] == '0') || !h->g.reg || t.ns > 1) return p;
-    const size_t rpb = 128 * (size_t)t.fr;
-    const size_t rb = (m + rpb - 1) / rpb;
-    const size_t slots = 2 * (size_t)h->cu_count;
-    const int nt = reg_tiles(h->g);
-    const size_t tail = rb % slots;
-    if (rb <= slots || nt < 4 || tail == 0 || tail > slots / 2) return p;
-    int ns = 1;
-    while (2 * ns <= nt / 2 && tail * (size_t)(2 * ns) <= slots) ns *= 2;
-    if (ns < 2) return p;
-    p.m_main = (rb - tail) * rpb;
-    p.ns = ns;
-    return p;
-}
-
-// The search of a call's rows with tile t: one launch, or the main launch
-// and the split last round when `tail_part` holds the second launch's slice
-// results (tail_plan).
-static int search_launch_all(emh_assigner *h, int M, const void *x_dev, int rows, size_t m, const uint32_t *ids_dev,
-                             const uint8_t *x8, const float *rs, const float *rn, uint32_t *labels_dev,
-                             float *dist_dev, uint32_t *cand, Tile t, uint32_t *part, uint32_t *tail_part) {
-    const TailPlan p = tail_part ? tail_plan(h, m, t) : TailPlan{m, 1};
-    int rc = search_launch_rows(h, M, x_dev, rows, 0, p.m_main, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand, t,
-                                part);
-    if (rc || p.ns == 1) return rc;
-    Tile tt = t;
-    tt.ns = p.ns;
-    rc = search_launch_rows(h, M, x_dev, rows, p.m_main, m - p.m_main, ids_dev, x8, rs, rn, labels_dev, dist_dev, cand,
-                            tt, tail_part);
-    h->last_tile = t;
-    return rc;
-}
-
 // fr = 1 is instantiated for f32 rows of the FP32 and BF16 searches, and
 // for the FP8 and I8 searches
 static bool fr1_available(const emh_assigner *h, int rows) {
@@ -310,7 +250,6 @@ struct RowScratch {
     float *rs = nullptr;       // FP8: m row scales
     float *rn = nullptr;       // FP8: m squared row norms
     uint32_t *part = nullptr;  // nslice > 1: per-slice results
-    uint32_t *tail_part = nullptr;  // the split last round's slice results (tail_plan), if any
 };
 
 // Encoding (FP8), search and rescore (BF16 / FP8) of one call on the
@@ -323,15 +262,14 @@ static int search_stages(emh_assigner *h, const void *x_dev, int rows, size_t m,
     int rc;
     if (h->op == OP_F32 || h->op == OP_I8 || h->op == OP_I16) {
         if ((rc = mk(1))) return rc;
-        if ((rc = search_launch_all(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev,
-                                    nullptr, t, b.part, b.tail_part)))
+        if ((rc = search_launch(h, 1, x_dev, rows, m, ids_dev, nullptr, nullptr, nullptr, labels_dev, dist_dev,
+                                nullptr, t, b.part)))
             return rc;
         if ((rc = mk(2))) return rc;
         return mk(3);
     }
     if ((rc = mk(1))) return rc;
-    if ((rc = search_launch_all(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, t,
-                                b.part, b.tail_part)))
+    if ((rc = search_launch(h, h->M, x_dev, rows, m, ids_dev, b.x8, b.rs, b.rn, nullptr, nullptr, b.cand, t, b.part)))
         return rc;
     if ((rc = mk(2))) return rc;
     if ((rc = check(launch_rescore(h->M, x_dev, rows, m, g.d, ids_dev, h->cen(),
@@ -379,13 +317,6 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
     if (t.ns > 1) {
         if ((rc = check(h->part.ensure(slice_part_bytes(M, m, t.ns, h->op)), "split buffer"))) return rc;
         b.part = h->part.as<uint32_t>();
-    } else {
-        const TailPlan p = tail_plan(h, m, t);
-        if (p.ns > 1) {
-            if ((rc = check(h->part.ensure(slice_part_bytes(M, m - p.m_main, p.ns, h->op)), "split buffer")))
-                return rc;
-            b.tail_part = h->part.as<uint32_t>();
-        }
     }
     return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, t, ev, true);
 }

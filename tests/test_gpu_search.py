@@ -434,42 +434,3 @@ def test_tile_choices_persist(hip, tmp_path):
     fr1, ns1, s1 = map(int, first.stdout.split())
     fr2, ns2, s2 = map(int, second.stdout.split())
     assert (fr2, ns2) == (tile["fr"], tile["nslice"]) == (fr1, ns1) and s1 == s2
-
-
-@pytest.mark.parametrize("prec,gather", [("fp32", False), ("bf16", False), ("fp8", True), ("i8", False),
-                                         ("bf16", True)])
-def test_split_last_round_equals_one_launch(hip, prec, gather, monkeypatch):
-    """A launch whose last round of workgroups is at most half full runs that
-    round as a second launch with its centroid tiles split (merged exactly):
-    labels, distances and candidate lists equal the single launch's bit for
-    bit, with the rows used in place or gathered by id."""
-    import torch
-
-    dev = torch.device("cuda:0")
-    slots = 2 * torch.cuda.get_device_properties(0).multi_processor_count
-    n, k, d = slots * 256 + 300, 2048, 64  # two row blocks past whole rounds
-    rng = np.random.default_rng(21)
-    if prec == "i8":
-        xh = rng.integers(-128, 128, size=(n, d)).astype(np.int8)
-        c = rng.normal(scale=40, size=(k, d)).astype(np.float32)
-    else:
-        xh = _mixture(n, d, 600, 22)
-        c = xh[O.permutation(98, n)[:k]]
-    x = torch.from_numpy(xh).to(dev)
-    ids = torch.from_numpy(O.permutation(97, n).astype(np.int32)).to(dev) if gather else None
-    out = {}
-    for split in ("0", "1"):
-        monkeypatch.setenv("EMH_TAIL_SPLIT", split)
-        a = E.HipAssigner(prec, rescore_m=4 if prec == "bf16" else 8)
-        a.set_centroids(c)
-        labels = torch.empty(n, dtype=torch.int32, device=dev)
-        dist = torch.empty(n, dtype=torch.float32, device=dev)
-        a.assign_device(x, labels, dist, ids)
-        a.synchronize()
-        out[split] = (labels.cpu().numpy(), dist.cpu().numpy())
-    assert np.array_equal(out["0"][0], out["1"][0]) and out["0"][1].tobytes() == out["1"][1].tobytes()
-    if prec == "i8":
-        rows = np.sort(O.permutation(96, n)[:3000])
-        src = rows if ids is None else ids.cpu().numpy()[rows]
-        cl, cd = O.i8_assign(xh[src], c)
-        assert np.array_equal(out["1"][0].view(np.uint32)[rows], cl) and np.array_equal(out["1"][1][rows], cd)
