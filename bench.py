@@ -646,7 +646,7 @@ def train_batch_line(args, njobs, prec, m):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "fp32": "f32"}[prec],
+        "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32"}[prec],
         "data": "synthetic (generated on the GPU with torch, per-job seeds)",
         "config": {"workload": args.workload_desc, "jobs": njobs, "precision": f"{prec}" + ("+fp32 rescore" if prec != "fp32" else ""),
                    "rescore_m": m, "rows_total": int(sum(x.shape[0] for x, _, _ in probs)),

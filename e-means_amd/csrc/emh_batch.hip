@@ -116,9 +116,10 @@ struct GraphGuard {
 static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats) {
     if (!jobs || njobs == 0) return fail(EMH_E_ARG, "train_batch: need at least one job");
     if (njobs > 0xFFFFu) return fail(EMH_E_ARG, "train_batch: at most 65535 jobs per call");
-    if (h->op != OP_F32 && h->op != OP_BF16 && h->op != OP_I8)
-        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32, BF16 and I8 searches");
-    const int op = h->op, M = op == OP_BF16 ? h->M : 1;
+    if (h->op != OP_F32 && h->op != OP_BF16 && h->op != OP_FP8 && h->op != OP_I8)
+        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32, BF16, FP8 and I8 searches");
+    const int op = h->op, M = (op == OP_BF16 || op == OP_FP8) ? h->M : 1;
+    const bool cand_search = op == OP_BF16 || op == OP_FP8;
     // the I8 search takes i8 rows (AssignRows<i8>), the float searches f32
     const int rows = op == OP_I8 ? EMH_ROWS_I8 : EMH_ROWS_F32;
     const size_t rb = op == OP_I8 ? 1 : 4;
@@ -153,16 +154,17 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         if (init_rc[j] != EMH_OK) return fail(init_rc[j], init_err[j]);
     std::vector<SearchGeom> geo(njobs);
     std::vector<size_t> row_base(njobs), cl_base(njobs), cen_base(njobs), cpad_base(njobs), op_base(njobs),
-        part_base(njobs), nrow_base(njobs);
+        part_base(njobs), nrow_base(njobs), x8_base(njobs);
     std::vector<JobSlices> slices(njobs);
-    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0, rows_total = 0, d_max = 0;
+    size_t slots = 0, nclusters = 0, cen_floats = 0, cpad = 0, op_bytes = 0, part_words = 0, rows_total = 0, d_max = 0,
+           x8_bytes = 0;
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         geo[j] = make_geom(op, jb.k, jb.d);
         if (!geo[j].reg)
             return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
                                                " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256, "
-                                               "I8 d <= 512)");
+                                               "FP8 and I8 d <= 512)");
         d_max = std::max(d_max, (size_t)jb.d);
         row_base[j] = slots;
         slots += run[j].R.batch_size;
@@ -179,6 +181,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         slices[j] = job_slices(geo[j]);
         part_base[j] = part_words;
         part_words += run[j].R.batch_size * job_part_words(op, M, slices[j].nslice);
+        // FP8: the step's encoded batch rows, d_pad bytes a slot
+        x8_base[j] = x8_bytes;
+        if (op == OP_FP8) x8_bytes += run[j].R.batch_size * geo[j].d_pad;
     }
     if (slots > 0xFFFFFFFFull || nclusters >= 0xFFFFFFFFull)
         return fail(EMH_E_ARG, "train_batch: more than 2^32 - 1 batch rows or clusters in one call");
@@ -208,7 +213,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     Layout outl;
     const size_t out_dist = outl.add(slots * 4), out_use = outl.add(nclusters * 4);
     DevBuf in_d, out_d, cen_model, sums_d, counts_d, cen_op, norm32, hn, cn, stats_d, cand, keys, pass_sums,
-        pass_counts, maps, pairs_d, part_d, lab_d, labels_all;
+        pass_counts, maps, pairs_d, part_d, lab_d, labels_all, x8_d, rscale_d, rnorm_d;
     HostBuf in_h, out_h, pairs_h;
     if ((rc = check(in_d.ensure(in.bytes), "step input")) || (rc = check(out_d.ensure(outl.bytes), "step output")) ||
         (rc = check(in_h.alloc(in.bytes), "pinned step input")) ||
@@ -220,6 +225,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
         (rc = check(hn.ensure(cpad * 4), "offsets")) ||
         (op == OP_I8 && (rc = check(cn.ensure(cpad * 8), "integer offsets"))) ||
+        (op == OP_FP8 && ((rc = check(x8_d.ensure(x8_bytes), "fp8 rows")) ||
+                          (rc = check(rscale_d.ensure(slots * 4), "fp8 row scales")) ||
+                          (rc = check(rnorm_d.ensure(slots * 4), "fp8 row norms")))) ||
         (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
         (rc = check(cand.ensure(slots * 4 * (size_t)M), "candidates")) ||
         (rc = check(keys.ensure(slots * 4), "keys")) || (rc = check(pass_sums.ensure(cen_floats * 8), "pass sums")) ||
@@ -353,6 +361,11 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.nslice = slices[j].nslice;
         t.tiles_per_slice = slices[j].tiles_per_slice;
         t.part = slices[j].nslice > 1 ? part_d.as<uint32_t>() + part_base[j] : nullptr;
+        if (op == OP_FP8) {
+            t.x8 = x8_d.as<uint8_t>() + x8_base[j];
+            t.row_scale = rscale_d.as<float>() + row_base[j];
+            t.row_norm = rnorm_d.as<float>() + row_base[j];
+        }
         t.row_base = (uint32_t)row_base[j];
         t.cluster_base = (uint32_t)cl_base[j];
     }
@@ -383,10 +396,13 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     std::vector<hipGraphNode_t> search_first;
     hipError_t ce = hipMemcpyAsync(in_dp, in_hp, in_ids, hipMemcpyHostToDevice, h->stream);
     if (ce == hipSuccess) ce = launch_jobs_prep(op, jobs_dev, (const uint2 *)(mp + m_c), cpad, h->stream);
+    // FP8: the batch rows' encoding is the first reader of the batch ids
+    if (ce == hipSuccess && op == OP_FP8) ce = launch_jobs_rows_fp8(jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    if (ce == hipSuccess && op == OP_FP8) ce = last_nodes(search_first);
     if (ce == hipSuccess) ce = launch_jobs_search(op, M, sa_dev, (const uint2 *)(mp + m_wg), wg_all.size(), h->stream);
-    if (ce == hipSuccess) ce = last_nodes(search_first);
-    if (ce == hipSuccess && op == OP_BF16) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
-    if (ce == hipSuccess && op != OP_BF16 && part_words)
+    if (ce == hipSuccess && op != OP_FP8) ce = last_nodes(search_first);
+    if (ce == hipSuccess && cand_search) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    if (ce == hipSuccess && !cand_search && part_words)
         ce = launch_jobs_merge(op, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess)
         ce = launch_jobs_keys(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),

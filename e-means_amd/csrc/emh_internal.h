@@ -106,7 +106,7 @@ struct JobDev {
     uint8_t *cen_op;         // tile-major operand table (register-resident search)
     float *norm32, *hn;      // k_pad each
     int64_t *cn;             // I8: k_pad integer score offsets sum c8^2 + round(bias)
-    unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias (f32 bits)
+    unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias, [2] FP8: max finite |c| (f32 bits)
     uint32_t *labels;        // batch slot
     float *dist;             // batch slot (true distances once the keys kernel has run)
     uint32_t *labels_all;    // n: the last pass's label of every row (device-resident)
@@ -116,6 +116,9 @@ struct JobDev {
     double *pass_sums;       // k x d
     uint64_t *pass_counts;   // k
     uint32_t *part;          // centroid-range split: per (row, slice) results, batch slot x nslice
+    uint8_t *x8;             // FP8: this step's batch rows as e4m3, batch slot x d_pad
+    float *row_scale;        // FP8: batch slot (power-of-two encoding scale)
+    float *row_norm;         // FP8: batch slot (|x|^2 of the f32 row)
     uint32_t d, k, k_pad, d_pad, row_bytes, tb;
     uint32_t nslice, tiles_per_slice;  // the search's centroid-range split (1: none)
     uint32_t row_base, cluster_base;  // offsets in the concatenated batch / cluster ranges
@@ -175,14 +178,21 @@ template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size
     EMH_G(pass_sums);
     EMH_G(pass_counts);
     EMH_G(part);
+    EMH_G(x8);
+    EMH_G(row_scale);
+    EMH_G(row_norm);
 #undef EMH_G
     return j;
 }
 
 // prep: every (job, centroid < k_pad) entry of cmap (cmap lists each job's
 // entries together); operand, norms, offsets with this step's bias, stats
-// (zeroed before).
+// (zeroed before). FP8 first takes each job's largest finite |c| (the
+// centroid encoding scale) in a pass of its own.
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s);
+// FP8: every slot's batch row (rmap: (job, row) per slot) encoded as
+// rows_fp8_kernel encodes a single problem's rows.
+hipError_t launch_jobs_rows_fp8(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
 // search of every job in one launch: wg holds (job, first row | slice) per
 // workgroup (first rows are multiples of 256, slices below 256).
 hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint2 *wg, size_t nwg, hipStream_t s);

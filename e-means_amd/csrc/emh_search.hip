@@ -406,12 +406,29 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
             const size_t nq = jb.row_bytes / 16;
             int64_t part64 = 0;
+            // FP8: the encoding scale of the job's largest finite |c| (stats[2])
+            [[maybe_unused]] const float cs = OP == OP_FP8 ? fp8_scale_for(__uint_as_float(jb.stats[2])) : 1.0f;
             for (size_t q = gl; q < nq; q += 16) {
                 uint32_t w[4];
 #pragma unroll
                 for (int i = 0; i < 4; i++) w[i] = 0u;
+                if constexpr (OP == OP_FP8) {
+                    // e4m3(-c s_C), four values a word, as prep_centroids_kernel
 #pragma unroll
-                for (int i = 0; i < EPQ; i++) {
+                    for (int wi = 0; wi < 4; wi++) {
+                        float v[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            const size_t el = q * EPQ + 4 * wi + t;
+                            v[t] = (live && el < d) ? -cp[el] * cs : 0.0f;
+                        }
+                        int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+                        pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+                        w[wi] = (uint32_t)pk;
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < (OP == OP_FP8 ? 0 : EPQ); i++) {
                     const size_t el = q * EPQ + i;
                     const bool in = live && el < d;
                     if constexpr (OP == OP_I8) {
@@ -423,7 +440,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
                         }
                         w[i / 4] |= ((uint32_t)qv & 0xFFu) << (8 * (i & 3));
                         part64 += (int64_t)qv * qv;
-                    } else {
+                    } else if constexpr (OP == OP_F32 || OP == OP_BF16) {
                         const float v = in ? -cp[el] : 0.0f;
                         if constexpr (OP == OP_F32) w[i] = __float_as_uint(v);
                         else w[i / 2] |= (uint32_t)bf16_rne(v) << (16 * (i & 1));
@@ -493,6 +510,49 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
                 mn = max(mn, s_norm[t]);
                 mb = max(mb, s_bias[t]);
             }
+        }
+    }
+}
+
+// FP8 jobs: each job's largest finite |c| over its k x d centroids (what
+// absmax_kernel takes for one problem: the centroid encoding scale), as f32
+// bits into the job's stats[2] (zeroed before); 16 lanes per (job, centroid)
+// entry of cmap, one atomic per job per workgroup.
+__global__ __launch_bounds__(256) void jobs_absmax_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
+    const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    __shared__ uint32_t s_job[PREP_ENTRIES], s_max[PREP_ENTRIES];
+    const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
+    uint32_t job = NO_LABEL;
+    float mx = 0.0f;
+    if (e < entries) {
+        const uint2 cm = cmap[e];
+        const JobDev jb = job_at<false>(jobs, cm.x);
+        if (jb.m != 0 && cm.y < jb.k) {  // a finished job: nothing reads its tables this step
+            const float *cp = jb.cen + (size_t)cm.y * jb.d;
+            for (size_t i = gl; i < jb.d; i += 16) {
+                const float a = fabsf(cp[i]);
+                if (a > mx && isfinite(a)) mx = a;
+            }
+            job = cm.x;
+        }
+    }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    if (gl == 0) {
+        s_job[slot] = job;
+        s_max[slot] = __float_as_uint(mx);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t cur = NO_LABEL, m = 0u;
+        for (int t = 0; t <= PREP_ENTRIES; t++) {
+            const uint32_t jt = t < PREP_ENTRIES ? s_job[t] : NO_LABEL;
+            if (jt != cur) {
+                if (cur != NO_LABEL && m) atomicMax(&jobs[cur].stats[2], m);
+                cur = jt;
+                m = 0u;
+            }
+            if (t < PREP_ENTRIES && jt != NO_LABEL) m = max(m, s_max[t]);
         }
     }
 }
@@ -591,72 +651,103 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
     }
 }
 
-// Per-row e4m3 encoding with a power-of-two scale: 16 lanes per row, four
-// rows per wave. Pass 1 takes the row's largest finite |v| and |x|^2, pass 2
-// (the row again, now from cache) writes the scaled, packed codes.
+// One row's e4m3 encoding with a power-of-two scale, by a 16-lane group
+// (lane gl of it): pass 1 takes the row's largest finite |v| and |x|^2, pass
+// 2 (the row again, now from cache) writes the scaled, packed codes into o8
+// (d_pad bytes) and lane 0 the scale and |x|^2. Every lane of the wave calls
+// it (the group sums are shuffles); a group with live = false writes nothing.
+template <int ROWS>
+__device__ inline void encode_row_fp8(const void *x, size_t src, size_t d, size_t d_pad, bool vec, bool live, int gl,
+                                      uint8_t *o8, float *scale_out, float *norm_out) {
+    constexpr int G = 16;
+    float mx = 0.0f, sq = 0.0f;
+    auto take = [&](float v) {
+        const float a = fabsf(v);
+        if (a > mx && isfinite(a)) mx = a;
+        sq = fmaf(v, v, sq);
+    };
+    if (live) {
+        if (vec) {
+            const float *xr = (const float *)x + src * d;
+            for (size_t e = 4 * gl; e < d; e += 4 * G) {
+                const float4 v = *(const float4 *)(xr + e);
+                take(v.x);
+                take(v.y);
+                take(v.z);
+                take(v.w);
+            }
+        } else {
+            for (size_t e = gl; e < d; e += G) take(src_elem<ROWS>(x, src * d + e));
+        }
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+        sq += __shfl_xor(sq, off, 64);
+    }
+    if (!live) return;
+    const float s = fp8_scale_for(mx);
+    uint32_t *o = (uint32_t *)o8;
+    for (size_t w = gl; w < d_pad / 4; w += G) {
+        float v[4];
+        if (vec && 4 * w + 3 < d) {
+            const float4 q = *(const float4 *)((const float *)x + src * d + 4 * w);
+            v[0] = q.x * s;
+            v[1] = q.y * s;
+            v[2] = q.z * s;
+            v[3] = q.w * s;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const size_t e = 4 * w + q;
+                v[q] = e < d ? src_elem<ROWS>(x, src * d + e) * s : 0.0f;
+            }
+        }
+        int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
+        pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
+        o[w] = (uint32_t)pk;
+    }
+    if (gl == 0) {
+        *norm_out = sq;
+        *scale_out = s;
+    }
+}
+
+// Per-row e4m3 encoding of a call's rows: 16 lanes per row, four rows per
+// wave.
 template <int ROWS>
 __global__ __launch_bounds__(256) void rows_fp8_kernel(const void *x, size_t m, size_t d, size_t d_pad,
                                                        const uint32_t *ids, uint8_t *x8, float *row_scale,
                                                        float *row_norm) {
-    constexpr int G = 16;
-    const int gl = threadIdx.x & (G - 1);
+    const int gl = threadIdx.x & 15;
     const bool vec = ROWS == EMH_ROWS_F32 && (d % 4) == 0 && ((uintptr_t)x % 16) == 0;
     const size_t stride = (size_t)gridDim.x * 16;
     for (size_t base = (size_t)blockIdx.x * 16; base < m; base += stride) {
         const size_t r = base + (threadIdx.x >> 4);
         const bool live = r < m;
         const size_t src = live ? (ids ? (size_t)ids[r] : r) : 0;
-        float mx = 0.0f, sq = 0.0f;
-        auto take = [&](float v) {
-            const float a = fabsf(v);
-            if (a > mx && isfinite(a)) mx = a;
-            sq = fmaf(v, v, sq);
-        };
-        if (live) {
-            if (vec) {
-                const float *xr = (const float *)x + src * d;
-                for (size_t e = 4 * gl; e < d; e += 4 * G) {
-                    const float4 v = *(const float4 *)(xr + e);
-                    take(v.x);
-                    take(v.y);
-                    take(v.z);
-                    take(v.w);
-                }
-            } else {
-                for (size_t e = gl; e < d; e += G) take(src_elem<ROWS>(x, src * d + e));
-            }
-        }
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) {
-            mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-            sq += __shfl_xor(sq, off, 64);
-        }
-        if (!live) continue;
-        const float s = fp8_scale_for(mx);
-        uint32_t *o = (uint32_t *)(x8 + r * d_pad);
-        for (size_t w = gl; w < d_pad / 4; w += G) {
-            float v[4];
-            if (vec && 4 * w + 3 < d) {
-                const float4 q = *(const float4 *)((const float *)x + src * d + 4 * w);
-                v[0] = q.x * s;
-                v[1] = q.y * s;
-                v[2] = q.z * s;
-                v[3] = q.w * s;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const size_t e = 4 * w + q;
-                    v[q] = e < d ? src_elem<ROWS>(x, src * d + e) * s : 0.0f;
-                }
-            }
-            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0], v[1], 0, false);
-            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2], v[3], pk, true);
-            o[w] = (uint32_t)pk;
-        }
-        if (gl == 0) {
-            row_norm[r] = sq;
-            row_scale[r] = s;
-        }
+        encode_row_fp8<ROWS>(x, src, d, d_pad, vec, live, gl, x8 + (live ? r : 0) * d_pad, row_scale + (live ? r : 0),
+                             row_norm + (live ? r : 0));
+    }
+}
+
+// FP8 jobs: every batch slot's row (rmap: job, row of its batch) encoded as
+// rows_fp8_kernel encodes one problem's rows, into the job's slot; 16 lanes
+// per slot. A slot past its job's rows this step (or of a finished job)
+// writes nothing.
+__global__ __launch_bounds__(256) void jobs_rows_fp8_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
+    const int gl = threadIdx.x & 15;
+    const size_t stride = (size_t)gridDim.x * 16;
+    for (size_t base = (size_t)blockIdx.x * 16; base < slots; base += stride) {
+        const size_t sl = base + (threadIdx.x >> 4);
+        const uint2 r = sl < slots ? rmap[sl] : make_uint2(0u, 0u);
+        const JobDev j = job_at<false>(jobs, r.x);  // groups of a wave may belong to different jobs
+        const bool live = sl < slots && r.y < j.m;
+        const size_t src = live ? (size_t)j.ids[r.y] : 0;
+        const size_t row = live ? r.y : 0;
+        const bool vec = (j.d % 4) == 0 && ((uintptr_t)j.x % 16) == 0;
+        encode_row_fp8<EMH_ROWS_F32>(j.x, src, j.d, j.d_pad, vec, live, gl, j.x8 + row * j.d_pad, j.row_scale + row,
+                                     j.row_norm + row);
     }
 }
 
@@ -809,6 +900,12 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     a.nslice = (int)j.nslice;
     a.tiles_per_slice = (int)j.tiles_per_slice;
     a.part = j.part;
+    if (g.op == OP_FP8) {
+        a.x8 = j.x8;
+        a.row_scale = j.row_scale;
+        a.row_norm = j.row_norm;
+        a.cmax_bits = j.stats + 2;
+    }
     memcpy(dst, &a, sizeof a);
 }
 
@@ -830,7 +927,7 @@ JobSlices job_slices(const SearchGeom &g) {
 
 size_t job_part_words(int op, int M, uint32_t nslice) {
     if (nslice < 2) return 0;
-    return (size_t)nslice * (op == OP_BF16 ? 2 * (size_t)M : 4);
+    return (size_t)nslice * ((op == OP_BF16 || op == OP_FP8) ? 2 * (size_t)M : 4);
 }
 
 hipError_t launch_jobs_merge(int op, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
@@ -847,8 +944,19 @@ hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_
     dim3 grid((unsigned)((entries + PREP_ENTRIES - 1) / PREP_ENTRIES));
     if (op == OP_F32) hipLaunchKernelGGL(prep_jobs_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_BF16) hipLaunchKernelGGL(prep_jobs_kernel<OP_BF16>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    else if (op == OP_FP8) {
+        hipLaunchKernelGGL(jobs_absmax_kernel, grid, dim3(256), 0, s, jobs, cmap, entries);
+        hipLaunchKernelGGL(prep_jobs_kernel<OP_FP8>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    }
     else if (op == OP_I8) hipLaunchKernelGGL(prep_jobs_kernel<OP_I8>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_jobs_rows_fp8(const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
+    if (!slots) return hipSuccess;
+    const dim3 grid((unsigned)std::min<size_t>((slots + 15) / 16, 16384));
+    hipLaunchKernelGGL(jobs_rows_fp8_kernel, grid, dim3(256), 0, s, jobs, rmap, slots);
     return hipGetLastError();
 }
 

@@ -1317,6 +1317,12 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
     a.cand = global_ptr_s(a.cand);
     a.labels = global_ptr_s(a.labels);
     a.dist = global_ptr_s(a.dist);
+    if constexpr (OP == OP_FP8) {
+        a.x8 = global_ptr_s(a.x8);
+        a.row_scale = global_ptr_s(a.row_scale);
+        a.row_norm = global_ptr_s(a.row_norm);
+        a.cmax_bits = global_ptr_s(a.cmax_bits);
+    }
     const size_t row0 = w.y & ~255u;
     if (row0 >= a.m) return;  // whole workgroup: before any barrier
     const int s = (int)(w.y & 255u);  // the slice

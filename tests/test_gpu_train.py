@@ -154,14 +154,14 @@ def _job_data(seed, n, d, blobs):
     return (centres[rng.integers(0, blobs, n)] + rng.normal(size=(n, d))).astype(np.float32)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_batched_training_equals_single(hip, prec):
     """Every job of a lockstep batch (one graph replay per step, uneven n, d,
     k and step counts, balance on some) ends with exactly the result of its
     own single-problem training on the same assigner."""
     specs = [(3000, 24, 40, 0.0), (9000, 64, 200, 0.1), (700, 96, 64, 0.0), (20000, 128, 300, 0.1),
              (5000, 33, 17, 0.0)]
-    if prec == "bf16":
+    if prec in ("bf16", "fp8"):
         specs.append((4000, 256, 90, 0.1))
     problems = []
     for i, (n, d, k, bal) in enumerate(specs):
@@ -177,14 +177,15 @@ def test_batched_training_equals_single(hip, prec):
         assert g.n_splits == ref.n_splits
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_batched_slices_and_big_clusters_equal_single(hip, prec):
     """The batched step's own paths: a job whose centroid tiles split into
     two slices (k = 2100, d = 64), jobs whose clusters get hundreds of
     members per step (k = 3 and 5 with batches of 2048: the per-64-dimension
     folds of big clusters, with balance on one), beside an ordinary job; each
     ends with exactly its single-problem result."""
-    specs = [(8000, 64, 2100, 0.0, None), (20000, 24, 3, 0.1, 2048), (12000, 200 if prec == "bf16" else 100, 5, 0.0, 2048),
+    specs = [(8000, 64, 2100, 0.0, None), (20000, 24, 3, 0.1, 2048),
+             (12000, 200 if prec in ("bf16", "fp8") else 100, 5, 0.0, 2048),
              (3000, 40, 50, 0.0, None)]
     problems = []
     for i, (n, d, k, bal, bs) in enumerate(specs):
@@ -200,13 +201,13 @@ def test_batched_slices_and_big_clusters_equal_single(hip, prec):
         assert g.n_splits == ref.n_splits
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_batched_edge_jobs_equal_single(hip, prec):
     """Edge shapes in one batch: a single cluster, as many rows as clusters,
     three- and five-wide rows (row lengths not a multiple of four), a row
     width whose operand rows round up to 384 bytes (BF16; FP32 stays at
     d <= 128), tiny batches beside a large one."""
-    wide = 130 if prec == "bf16" else 120
+    wide = {"bf16": 130, "fp8": 300}.get(prec, 120)
     specs = [(50, 3, 1, 0.0), (64, 7, 64, 0.0), (1000, wide, 700, 0.1), (300, 5, 2, 0.1), (30000, 32, 40, 0.0)]
     problems = [(_job_data(70 + i, n, d, max(2, 2 * k)), k, E.Config(nepochs=2, balance=bal, seed=60 + i))
                 for i, (n, d, k, bal) in enumerate(specs)]
