@@ -615,6 +615,7 @@ def train_batch_line(args, njobs, prec, m):
     got, st = E.train_batch(probs, E.HipAssigner(prec, rescore_m=m))
     bs = time.perf_counter() - t
     rel, seed_rel, ls, loop_steps = [0.0], [0.0], float("nan"), 0
+    chk = None
     if not args.no_compare:
         log("per-job fp32 loop")
         for x, k, cfg in probs:  # the same warm-up: each job once, identical
@@ -625,8 +626,14 @@ def train_batch_line(args, njobs, prec, m):
         ls = time.perf_counter() - t
         log("exact re-assignment MSE")
         rel, seed_rel = [], []
+        chk = {"checked_rows": 0, "label_disagreements": 0, "true_label_errors": 0, "dist_err_max_rel_1pd": 0.0}
         for (x, k, cfg), g, r in zip(probs, got, ref):
-            a = exact_reassign(x, g.centroids, 0)[0]
+            # the final centroids' exact re-assignment, judged by FP64 brute
+            # force on a fixed sample of each job's rows
+            a, q = exact_reassign(x, g.centroids, 2000)
+            for key in ("checked_rows", "label_disagreements", "true_label_errors"):
+                chk[key] += q[key]
+            chk["dist_err_max_rel_1pd"] = max(chk["dist_err_max_rel_1pd"], q["dist_err_max_rel_1pd"])
             b = exact_reassign(x, r.centroids, 0)[0]
             rel.append(a / b - 1.0)
             # the same FP32 loop with the next seed: how far two k-means
@@ -662,6 +669,9 @@ def train_batch_line(args, njobs, prec, m):
         "speedup_vs_fp32_loop": round(ls / bs, 3),
         "mse_rel_vs_fp32_loop": {"mean": float(np.mean(rel)), "max": float(np.max(rel)), "min": float(np.min(rel)),
                                  "abs_mean": float(np.mean(np.abs(rel)))},
+        "final_assign_vs_fp64": (dict(chk, label_disagreement=chk["label_disagreements"] / max(1, chk["checked_rows"]),
+                                      true_label_error_rate=chk["true_label_errors"] / max(1, chk["checked_rows"]))
+                                 if not args.no_compare else None),
         "mse_rel_fp32_next_seed": ({"mean": float(np.mean(seed_rel)), "max": float(np.max(seed_rel)),
                                     "min": float(np.min(seed_rel)), "abs_mean": float(np.mean(np.abs(seed_rel)))}
                                    if not args.no_compare else None),
