@@ -724,6 +724,84 @@ __global__ __launch_bounds__(256) void stream_sum_kernel(const void *x, size_t d
     }
 }
 
+// Integer rows (i8, u8), clusters of at most SUM_BIG members: every partial
+// sum of the reference's f32 fold in batch order (src/lib.rs:431-441) is an
+// integer of magnitude at most 128 x 255 < 2^24, so each f32 add is exact
+// and the fold's result is the exact integer sum in any order. One wave
+// adds a cluster's member rows in int32, 16 bytes a lane: lane (slot, chunk)
+// = (lane >> L, lane & (2^L - 1)) takes chunk `chunk` of every 2^(6-L)-th
+// member from member `slot` on (2^L = d / 16 chunks a row), the slots are
+// summed by shuffles, and slot 0 writes the f32 sums; the cluster's count
+// goes to counts.
+template <int ROWS>
+__device__ __forceinline__ void int_cluster_sum(const uint8_t *x, size_t d, int L, const uint32_t *src, size_t p,
+                                                uint32_t n, uint32_t key, float *sums, uint32_t *counts, int lane) {
+    const int nch = 1 << L, R = 64 >> L;
+    const int slot = lane >> L, ch = lane & (nch - 1);
+    int32_t acc[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) acc[e] = 0;
+    auto add = [&](const uint4 w) {
+        const uint32_t v[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                if constexpr (ROWS == EMH_ROWS_I8) acc[4 * q + b] += (int32_t)(int8_t)(v[q] >> (8 * b));
+                else acc[4 * q + b] += (int32_t)((v[q] >> (8 * b)) & 255u);
+            }
+    };
+    // four members a lane in flight
+    for (uint32_t it = 0; it < n; it += 4 * (uint32_t)R) {
+        uint4 w[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t q = it + (uint32_t)(u * R + slot);
+            w[u] = make_uint4(0u, 0u, 0u, 0u);
+            if (q < n) w[u] = *(const uint4 *)(x + (size_t)src[p + q] * d + (size_t)ch * 16);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) add(w[u]);
+    }
+    for (int off = 32; off >= nch; off >>= 1)
+#pragma unroll
+        for (int e = 0; e < 16; e++) acc[e] += __shfl_xor(acc[e], off, 64);
+    if (slot == 0) {
+        float4 *o = (float4 *)(sums + (size_t)key * d + (size_t)ch * 16);
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            o[q] = make_float4((float)acc[4 * q], (float)acc[4 * q + 1], (float)acc[4 * q + 2], (float)acc[4 * q + 3]);
+    }
+    if (lane == 0) counts[key] = n;
+}
+
+// stream_sum_kernel for integer rows whose d is 16 times a power of two (at
+// most 1024) and whose rows are 16-byte aligned: the same clusters per wave
+// (those whose first sorted member lies in the wave's SUM_SPAN positions),
+// each summed by int_cluster_sum over all its dimensions at once; big
+// clusters go to the list for big_sum_kernel.
+template <int ROWS>
+__global__ __launch_bounds__(256) void stream_sum_int_kernel(const void *x, size_t d, int L, const uint32_t *keys,
+                                                             const uint32_t *src, size_t m, size_t k, float *sums,
+                                                             uint32_t *counts, uint32_t *big) {
+    const int lane = threadIdx.x & 63;
+    size_t p = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * SUM_SPAN;
+    if (p >= m) return;
+    const size_t stop = p + SUM_SPAN < m ? p + SUM_SPAN : m;
+    if (p > 0) p = first_change(keys, p, stop, keys[p - 1], lane);
+    while (p < stop) {
+        const uint32_t key = keys[p];
+        if (key >= k) break;
+        const size_t n = first_change(keys, p, m, key, lane) - p;
+        if (n > SUM_BIG) {
+            if (lane == 0) list_big(big, &counts[key], key, p, n);
+        } else {
+            int_cluster_sum<ROWS>((const uint8_t *)x, d, L, src, p, (uint32_t)n, key, sums, counts, lane);
+        }
+        p += n;
+    }
+}
+
 template <int ROWS>
 __global__ __launch_bounds__(256) void big_sum_kernel(const void *x, size_t d, const uint32_t *src, size_t k,
                                                       float *sums, uint32_t *counts, const uint32_t *big) {
@@ -799,6 +877,26 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
         return hipGetLastError();
     }
     const dim3 grid((unsigned)(((m + SUM_SPAN - 1) / SUM_SPAN + 3) / 4)), bgrid(BIG_SUM_BLOCKS);
+    // integer rows of 16 x 2^L bytes (L <= 6), 16-byte aligned: whole rows
+    // at once, int32 sums
+    int L = -1;
+    if (rows != EMH_ROWS_F32 && d % 16 == 0 && ((uintptr_t)x & 15) == 0)
+        for (int l = 0; l <= 6; l++)
+            if (d == ((size_t)16 << l)) L = l;
+    if (L >= 0) {
+        if (rows == EMH_ROWS_I8) {
+            hipLaunchKernelGGL(stream_sum_int_kernel<EMH_ROWS_I8>, grid, dim3(256), 0, s, x, d, L, sorted_keys,
+                               sorted_src, m, k, sums, counts, a.big);
+            hipLaunchKernelGGL(big_sum_kernel<EMH_ROWS_I8>, bgrid, dim3(256), 0, s, x, d, sorted_src, k, sums, counts,
+                               (const uint32_t *)a.big);
+        } else {
+            hipLaunchKernelGGL(stream_sum_int_kernel<EMH_ROWS_U8>, grid, dim3(256), 0, s, x, d, L, sorted_keys,
+                               sorted_src, m, k, sums, counts, a.big);
+            hipLaunchKernelGGL(big_sum_kernel<EMH_ROWS_U8>, bgrid, dim3(256), 0, s, x, d, sorted_src, k, sums, counts,
+                               (const uint32_t *)a.big);
+        }
+        return hipGetLastError();
+    }
     switch (rows) {
         case EMH_ROWS_F32:
             hipLaunchKernelGGL(stream_sum_kernel<EMH_ROWS_F32>, grid, dim3(256), 0, s, x, d, sorted_keys, sorted_src, m,

@@ -148,6 +148,48 @@ def test_device_accumulate_is_batch_order_f32(hip, n, m, k, d):
     assert np.array_equal(sums.cpu().numpy(), want)
 
 
+@pytest.mark.parametrize("n,m,k,d,dt", [(20000, 15000, 4096, 512, "i8"), (9000, 8000, 3000, 16, "u8"),
+                                        (9000, 8000, 2500, 1024, "i8"), (9000, 8000, 2500, 96, "i8"),
+                                        (9000, 8000, 2500, 37, "u8"), (9000, 8000, 300, 64, "i8"),
+                                        (12000, 11000, 2600, 256, "u8")])
+def test_device_accumulate_is_batch_order_int(hip, n, m, k, d, dt):
+    """Integer rows: the same bit-identity with the sequential f32 host loop
+    (src/lib.rs:431-441), over the whole-row integer fold (d = 16 x 2^L),
+    the 256-dimension stream (other d), the per-cluster fold (k <= 2048),
+    full-range values, a cluster of 600 members (folded as a big one) and
+    an empty cluster."""
+    import torch
+
+    print(f"accumulate int n={n} m={m} k={k} d={d} {dt}", flush=True)
+    rng = np.random.default_rng(4)
+    if dt == "i8":
+        x = rng.integers(-128, 128, size=(n, d)).astype(np.int8)
+    else:
+        x = rng.integers(0, 256, size=(n, d)).astype(np.uint8)
+    ids = rng.permutation(n)[:m].astype(np.uint32)
+    labels = rng.integers(0, k, m).astype(np.uint32)
+    labels[labels == 7] = 8  # an empty cluster
+    labels[rng.permutation(m)[:600]] = 3  # a big cluster, members spread over the batch
+    a = E.HipAssigner(E.FP32)
+    a.set_centroids(np.zeros((k, d), np.float32))
+    xd = torch.from_numpy(x).cuda()
+    idd = torch.from_numpy(ids.view(np.int32)).cuda()
+    ld = torch.from_numpy(labels.view(np.int32)).cuda()
+    sums = torch.full((k, d), 7.0, device="cuda")
+    counts = torch.full((k,), 9, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    a.accumulate_device(xd, ld, sums, counts, ids=idd)
+    a.synchronize()
+    want = np.zeros((k, d), np.float32)
+    want_c = np.zeros(k, np.int64)
+    xf = x.astype(np.float32)
+    for r in range(m):
+        want[labels[r]] += xf[ids[r]]
+        want_c[labels[r]] += 1
+    assert np.array_equal(counts.cpu().numpy(), want_c)
+    assert np.array_equal(sums.cpu().numpy(), want)
+
+
 def _job_data(seed, n, d, blobs):
     rng = np.random.default_rng(seed)
     centres = rng.normal(scale=4, size=(blobs, d)).astype(np.float32)
