@@ -219,42 +219,47 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
     }
 }
 
-// The same sort in one pass over 11-bit digits (keys below 2048), for the
-// cluster labels of problems with k <= 2048: three launches instead of six.
-constexpr int WIDE_BINS = 2048;
+// The same sort in one pass over 11- or 12-bit digits (keys below BINS =
+// 2048 or 4096), for the cluster labels of problems with k <= 4096: two or
+// three launches instead of six.
+constexpr int WIDE_BINS = 4096;  // the widest digit: scratch sizes
 
+template <int BINS>
 __global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *keys, size_t m, uint32_t *hist,
                                                               size_t nblocks) {
-    __shared__ uint32_t h[WIDE_BINS];
-    for (int i = threadIdx.x; i < WIDE_BINS; i += 256) h[i] = 0;
+    __shared__ uint32_t h[BINS];
+    for (int i = threadIdx.x; i < BINS; i += 256) h[i] = 0;
     __syncthreads();
     const size_t b0 = (size_t)blockIdx.x * SORT_BLOCK;
     for (int r = 0; r < SORT_BLOCK / 256; r++) {
         const size_t i = b0 + (size_t)r * 256 + threadIdx.x;
-        if (i < m) atomicAdd(&h[keys[i] & (WIDE_BINS - 1)], 1u);
+        if (i < m) atomicAdd(&h[keys[i] & (BINS - 1)], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < WIDE_BINS; i += 256) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
+    for (int i = threadIdx.x; i < BINS; i += 256) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
 }
 
-// As digit_scatter_kernel; with 2048 digits the per-wave counts are not
-// cleared wholesale each round: the leader of a digit group adds its group
-// to the digit's running count and clears its own entry after the round.
+// As digit_scatter_kernel; with BINS digits the per-wave counts (at most 64,
+// one byte each) are not cleared wholesale each round: the leader of a digit
+// group adds its group to the digit's running count and clears its own
+// entry after the round.
 // RAW: offs is the unscanned histogram (digit_hist_wide_kernel) and each
 // block sums the earlier blocks' counts of its digits itself (a few blocks:
 // no scan launch); block 0 also writes the digit totals to totals_out.
 // Otherwise offs and totals come from digit_row_scan_kernel.
-template <bool RAW>
+template <bool RAW, int BINS>
 __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
                                                                  const uint32_t *offs, size_t nblocks,
                                                                  const uint32_t *totals, uint32_t *keys_out,
                                                                  uint32_t *vals_out, uint32_t *starts,
                                                                  uint32_t *totals_out) {
-    constexpr int PER = WIDE_BINS / 256;
-    __shared__ uint32_t run[WIDE_BINS];
-    __shared__ uint32_t wcnt[4][WIDE_BINS];
-    __shared__ uint32_t sbase[WIDE_BINS];
-    __shared__ uint32_t pre[RAW ? WIDE_BINS : 1];
+    constexpr int PER = BINS / 256, DB = BINS == 4096 ? 12 : 11;
+    static_assert(BINS == 2048 || BINS == 4096, "digit width");
+    static_assert(!RAW || BINS == 2048, "the block-prefix form keeps 2048 digits (LDS)");
+    __shared__ uint32_t run[BINS];
+    __shared__ uint8_t wcnt[4][BINS];
+    __shared__ uint32_t sbase[BINS];
+    __shared__ uint32_t pre[RAW ? BINS : 1];
     __shared__ uint32_t ssm[4];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     {
@@ -304,15 +309,15 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
         const size_t i = b0 + (size_t)r * 256 + tid;
         const bool live = i < m;
         const uint32_t key = live ? keys[i] : 0u;
-        const uint32_t dg = key & (WIDE_BINS - 1);
+        const uint32_t dg = key & (BINS - 1);
         uint64_t same = __ballot(live);
-        for (int bit = 0; bit < 11; bit++) {
+        for (int bit = 0; bit < DB; bit++) {
             const uint64_t bl = __ballot(live && ((dg >> bit) & 1u));
             same &= ((dg >> bit) & 1u) ? bl : ~bl;
         }
         const uint32_t before = (uint32_t)__popcll(same & lt), cnt = (uint32_t)__popcll(same);
         const bool leader = live && before == 0;
-        if (leader) wcnt[w][dg] = cnt;
+        if (leader) wcnt[w][dg] = (uint8_t)cnt;
         __syncthreads();
         if (live) {
             uint32_t pos = sbase[dg] + run[dg] + before;
@@ -325,7 +330,7 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
         __syncthreads();
         if (leader) {
             atomicAdd(&run[dg], cnt);
-            wcnt[w][dg] = 0;
+            wcnt[w][dg] = 0u;
         }
         __syncthreads();
     }
@@ -382,23 +387,39 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
     uint32_t *kb[2] = {a.keys, a.keys_alt}, *vb[2] = {a.vals, a.vals_alt};
     int cur = 0;
     if (bits > 8 && bits <= 11) {
-        uint32_t *totals = hist + (size_t)WIDE_BINS * nblocks;
-        hipLaunchKernelGGL(digit_hist_wide_kernel, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist, nblocks);
+        uint32_t *totals = hist + (size_t)2048 * nblocks;
+        hipLaunchKernelGGL(digit_hist_wide_kernel<2048>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist,
+                           nblocks);
         if (nblocks <= 64) {
             // up to 64 blocks (64 Ki keys): each scatter block sums its digits'
             // counts over the blocks itself, two launches instead of three
-            hipLaunchKernelGGL(digit_scatter_wide_kernel<true>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin, m,
-                               (const uint32_t *)hist, nblocks, (const uint32_t *)nullptr, kb[0], vb[0],
+            hipLaunchKernelGGL((digit_scatter_wide_kernel<true, 2048>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
+                               vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)nullptr, kb[0], vb[0],
                                counts_out ? a.starts : nullptr, totals);
         } else {
-            hipLaunchKernelGGL(digit_row_scan_kernel, dim3(WIDE_BINS), dim3(256), 0, s, hist, nblocks, totals);
-            hipLaunchKernelGGL(digit_scatter_wide_kernel<false>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, vin,
-                               m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
+            hipLaunchKernelGGL(digit_row_scan_kernel, dim3(2048), dim3(256), 0, s, hist, nblocks, totals);
+            hipLaunchKernelGGL((digit_scatter_wide_kernel<false, 2048>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
+                               vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
                                counts_out ? a.starts : nullptr, (uint32_t *)nullptr);
         }
         *keys_out = kb[0];
         *vals_out = vb[0];
-        if (counts_out) *counts_out = hist + (size_t)WIDE_BINS * nblocks;
+        if (counts_out) *counts_out = hist + (size_t)2048 * nblocks;
+        return hipGetLastError();
+    }
+    if (bits == 12) {
+        // 12-bit keys in one pass: histogram, the digits' rows scanned over
+        // the blocks, scatter (three launches instead of two passes' six)
+        uint32_t *totals = hist + (size_t)4096 * nblocks;
+        hipLaunchKernelGGL(digit_hist_wide_kernel<4096>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist,
+                           nblocks);
+        hipLaunchKernelGGL(digit_row_scan_kernel, dim3(4096), dim3(256), 0, s, hist, nblocks, totals);
+        hipLaunchKernelGGL((digit_scatter_wide_kernel<false, 4096>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
+                           vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
+                           counts_out ? a.starts : nullptr, (uint32_t *)nullptr);
+        *keys_out = kb[0];
+        *vals_out = vb[0];
+        if (counts_out) *counts_out = totals;
         return hipGetLastError();
     }
     for (int shift = 0; shift < bits; shift += 8) {

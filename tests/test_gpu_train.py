@@ -113,10 +113,12 @@ def test_f32_training_tracks_oracle(hip):
 
 @pytest.mark.parametrize("n,m,k,d", [(5000, 3000, 50, 37), (20000, 7001, 300, 37), (70000, 66000, 70000, 37),
                                      (9000, 1, 1, 37), (20000, 15000, 7, 37), (6000, 5000, 40, 300),
-                                     (20000, 17001, 2048, 37), (9000, 4097, 257, 96)])
+                                     (20000, 17001, 2048, 37), (9000, 4097, 257, 96), (70000, 66001, 3000, 37),
+                                     (70000, 66001, 1500, 37)])
 def test_device_accumulate_is_batch_order_f32(hip, n, m, k, d):
     """The stable sort + streamed fold equals the sequential host loop bit
-    for bit: one, two and three 8-bit sort passes, blocks with a ragged end,
+    for bit: one, two and three 8-bit sort passes, one 11-bit pass (with and
+    without the scan launch) and one 12-bit pass, blocks with a ragged end,
     a single row, empty clusters, clusters of thousands of rows (folded one
     wave per 64 dimensions) and rows wider than 256 (streamed 256 dimensions
     at a time)."""
