@@ -47,8 +47,8 @@ for _p in (ROOT, os.path.join(ROOT, "e-means_amd")):
 
 # MI355X dense peaks (AMD published): TFLOP/s (TOP/s for int8)
 PEAK = {"fp32": 157.3, "bf16": 2516.6, "fp8": 5033.2, "i8": 5033.2, "i16": 5033.2}
-# i8 MFMA operations per algorithmic multiply-add pair (I16 runs three i8 planes)
-OPS_PER_PRODUCT = {"i16": 3}
+# i8 MFMA operations per algorithmic multiply-add pair (I16 runs two i8 planes)
+OPS_PER_PRODUCT = {"i16": 2}
 METRIC = "assigned rows/s & % roofline, BF16/FP8+rescore vs FP32, at matched MSE"
 WORKLOADS = ["search", "search-i8", "search-fp8-large-k", "train-c1", "train-c3", "train-c4", "train-c5",
              "quantize", "report-nearest"]
@@ -344,7 +344,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         "tile": head["tile"],
     }
     if prec in OPS_PER_PRODUCT:
-        # I16 runs three i8 MFMA planes per product: the executed rate beside
+        # I16 runs two i8 MFMA planes per product: the executed rate beside
         # the algorithmic one
         ex = OPS_PER_PRODUCT[prec] * achieved
         roofline["executed"] = {"achieved": round(ex, 2), "frac": round(ex / PEAK[prec], 4),
@@ -364,7 +364,7 @@ def search_line(args, x, c, prec, m, compare, i8_rows=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32", "i8": "int8", "i16": "int8 (3 planes)"}[prec],
+        "dtype": {"bf16": "bf16", "fp8": "fp8", "fp32": "f32", "i8": "int8", "i16": "int8 (2 planes)"}[prec],
         "data": "synthetic (generated on the GPU with torch, seed 0)",
         "config": {"workload": args.workload_desc, "n": n, "d": d, "k": k,
                    "precision": prec + ("+fp32 rescore" if prec in ("bf16", "fp8") else ""),

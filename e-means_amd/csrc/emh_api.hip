@@ -643,7 +643,7 @@ int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice) {
 int emh_name(const emh_assigner *h, char *buf, size_t cap) {
     if (!h || !buf || cap == 0) return fail(EMH_E_ARG, "emh_name: bad arguments");
     static const char *kind[] = {"FP32 fused MFMA argmin", "BF16 MFMA + FP32 rescore", "FP8-e4m3 MFMA + FP32 rescore",
-                                 "I8 MFMA exact", "I16 (three i8 MFMA planes) exact"};
+                                 "I8 MFMA exact", "I16 (two i8 MFMA planes) exact"};
     char tmp[512];
     int n;
     if (h->op == OP_BF16 || h->op == OP_FP8)

@@ -43,16 +43,19 @@ struct SearchGeom {
 
 SearchGeom make_geom(int op, size_t k, size_t d);
 
+// I16 centroids as i8 planes: c16 = 2^8 hi + (lo + 128), hi the signed top
+// byte, lo the low byte less 128, so x.c16 = 2^8 x.hi + x.lo + 128 sum(x).
+constexpr int I16_PLANES = 2;
 // Centroids per tile of the register-resident search (one or two 32-wide
 // MFMA blocks).
-// (I16: 32 centroids, whose three planes are the tile's three MFMA blocks.)
+// (I16: 32 centroids, whose two planes are the tile's two MFMA blocks.)
 inline int reg_tile(const SearchGeom &g) { return g.op == OP_I16 ? 32 : g.row_bytes <= 256 ? 64 : 32; }
 // Bytes of the operand centroid buffer: the row-major table (k_pad rows of
 // d_pad elements; row_bytes is only the LDS segment when rows stream in
 // chunks), followed on the register-resident path (where a row is
 // row_bytes) by the same table in tile-major order (reg_operand).
 inline size_t operand_table_bytes(const SearchGeom &g) {
-    return g.k_pad * g.d_pad * (size_t)op_elem_bytes(g.op) * (g.op == OP_I16 ? 3 : 1);
+    return g.k_pad * g.d_pad * (size_t)op_elem_bytes(g.op) * (g.op == OP_I16 ? I16_PLANES : 1);
 }
 inline size_t operand_bytes(const SearchGeom &g) {
     return g.reg ? 2 * operand_table_bytes(g) : operand_table_bytes(g);

@@ -20,9 +20,9 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
     g.k = k;
     g.k_pad = (k + BN - 1) / BN * BN;
     if (op == OP_I16) {
-        // three i8 planes, rows padded to 128 bytes; shift = largest s <= 7
+        // two i8 planes, rows padded to 128 bytes; shift = largest s <= 7
         // with d * 2^(14 + s) < 2^31 (src/cpu_i8.rs:97). Rows of at most 512
-        // bytes take the register-resident kernel (tiles of 32 centroids x 3
+        // bytes take the register-resident kernel (tiles of 32 centroids x 2
         // planes), wider ones the LDS kernel, d streamed in 128-byte chunks.
         g.d_pad = (d + CHUNK_BYTES - 1) / CHUNK_BYTES * CHUNK_BYTES;
         g.shift = 0;
@@ -32,7 +32,7 @@ SearchGeom make_geom(int op, size_t k, size_t d) {
                 break;
             }
         g.resident = false;
-        g.reg = g.d_pad <= 512u && 3 * g.k_pad * g.d_pad <= 0x7FFFFFFFu;
+        g.reg = g.d_pad <= 512u && I16_PLANES * g.k_pad * g.d_pad <= 0x7FFFFFFFu;
         g.row_bytes = g.reg ? (int)g.d_pad : CHUNK_BYTES;
         return g;
     }
@@ -558,9 +558,9 @@ __global__ __launch_bounds__(256) void jobs_absmax_kernel(const JobDev *jobs, co
 }
 
 // I16 centroids (src/cpu_i8.rs:95-107): c16 = round(c * 2^shift) clamped to
-// i16 (NaN -> 0, as Rust's `as`), stored as three i8 planes of k_pad x d_pad
-// bytes each (low 7 bits, next 7 bits, the signed top), and sum c16^2 in i64.
-// One wave per centroid.
+// i16 (NaN -> 0, as Rust's `as`), stored as two i8 planes of k_pad x d_pad
+// bytes each (the low byte less 128, then the signed top byte: c16 =
+// 2^8 hi + lo + 128), and sum c16^2 in i64. One wave per centroid.
 __global__ __launch_bounds__(256) void prep_i16_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                        size_t k_pad, int shift, uint8_t *cen_op, int64_t *norm64) {
     const int lane = threadIdx.x & 63;
@@ -578,9 +578,8 @@ __global__ __launch_bounds__(256) void prep_i16_kernel(const float *c, size_t k,
             r = r != r ? 0.0f : fminf(fmaxf(r, -32768.0f), 32767.0f);
             q = (int32_t)r;
         }
-        o[e] = (int8_t)(q & 127);
-        o[plane + e] = (int8_t)((q >> 7) & 127);
-        o[2 * plane + e] = (int8_t)(q >> 14);
+        o[e] = (int8_t)((q & 255) - 128);
+        o[plane + e] = (int8_t)(q >> 8);
         part += (int64_t)q * q;
     }
     for (int off = 32; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
@@ -615,7 +614,7 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
     if (bias && jc < k && bias[jc] < 0.0f) atomicMax(&stats[1], __float_as_uint(-bias[jc]));
     if (op == OP_I16) {
         // offset = sum c16^2 + round(bias * 4^shift) (src/cpu_i8.rs:112-117);
-        // pad centroids (zero planes) score INT64_MAX, which never wins the
+        // pad centroids (c16 = 0) score INT64_MAX, which never wins the
         // register kernel's strict '<'; the LDS kernel skips indices >= k
         int64_t v = INT64_MAX;
         if (jc < k) {
@@ -853,7 +852,7 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;
     // the register-resident path's tile-major copy, written by the same
-    // kernel (I16's three planes go through tile_layout_kernel)
+    // kernel (I16's two planes go through tile_layout_kernel)
     uint8_t *tile = g.reg && g.op != OP_I16 ? (uint8_t *)reg_operand(g, cen_op) : nullptr;
     const int tbc = g.reg ? reg_tile(g) : 1;
     switch (g.op) {
@@ -866,7 +865,7 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !g.reg || g.op != OP_I16) return e;
-    const int nq = g.row_bytes / 16, tb = reg_tile(g), planes = g.op == OP_I16 ? 3 : 1;
+    const int nq = g.row_bytes / 16, tb = reg_tile(g), planes = g.op == OP_I16 ? I16_PLANES : 1;
     const size_t quads = (size_t)planes * g.k_pad * (size_t)nq;
     hipLaunchKernelGGL(tile_layout_kernel, dim3((unsigned)((quads + 255) / 256)), dim3(256), 0, s,
                        (const uint4 *)cen_op, (uint4 *)reg_operand(g, cen_op), g.k_pad, nq, tb, planes);

@@ -369,11 +369,11 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     using T = OpT<OP>;
     constexpr int ES = T::ES, LB = T::LB;
     constexpr bool INT = OP == OP_I8 || OP == OP_I16;
-    // I16: three i8 planes per centroid, c16 = p0 + 2^7 p1 + 2^14 p2 with
-    // p0, p1 in [0, 127] and p2 in [-2, 1]; x.c16 = x.p0 + 2^7 x.p1 + 2^14 x.p2
-    // computed with wrapping int32 adds, which is the reference's wrapping
-    // i32 dot product (vpdpwssd lanes) exactly.
-    constexpr int NPL = OP == OP_I16 ? 3 : 1;
+    // I16: two i8 planes per centroid, c16 = 2^8 hi + lo + 128 with hi, lo in
+    // [-128, 127]; x.c16 = 2^8 x.hi + x.lo + 128 sum(x) computed with wrapping
+    // int32 adds, which is the reference's wrapping i32 dot product
+    // (vpdpwssd lanes) exactly.
+    constexpr int NPL = OP == OP_I16 ? I16_PLANES : 1;
     using Score = typename std::conditional<INT, int64_t, float>::type;
     extern __shared__ __align__(16) uint8_t smem[];
 
@@ -424,7 +424,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
     }
     // Squared row norm (f32 for FP32, exact int for I8), accumulated from LDS.
     float xn32 = 0.0f;
-    int32_t xn_i = 0;
+    int32_t xn_i = 0, xs_i = 0;  // I16: the row's element sum (half-row partial)
     float srs = 1.0f;
     if constexpr (OP == OP_FP8) {
         float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
@@ -446,6 +446,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                     for (int bb = 0; bb < 4; bb++) {
                         int e = (int)(int8_t)((w[q] >> (8 * bb)) & 0xff);
                         xn_i += e * e;
+                        xs_i += e;
                     }
             }
         }
@@ -529,7 +530,10 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
             }
             if (ch == nchunks - 1) {
                 // Epilogue: fold the 2 x 16 scores of this lane into its top-m,
-                // in ascending centroid index.
+                // in ascending centroid index. (I16: the row sum of every
+                // chunk is in by now, the first centroid block's chunks
+                // having added theirs.)
+                [[maybe_unused]] const uint32_t xs128 = 128u * (uint32_t)(xs_i + __shfl_xor(xs_i, 32, 64));
 #pragma unroll
                 for (int nb = 0; nb < 2; nb++) {
 #pragma unroll
@@ -543,8 +547,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
                         } else if constexpr (OP == OP_I16) {
                             const i32x16 *pp = nb == 0 ? pacc0 : pacc1;
                             // x.c16 (mod 2^32, as the reference's i32 sum)
-                            const uint32_t dv = (uint32_t)pp[0][r] + ((uint32_t)pp[1][r] << 7) +
-                                                ((uint32_t)pp[2 % NPL][r] << 14);
+                            const uint32_t dv = (uint32_t)pp[0][r] + ((uint32_t)pp[1 % NPL][r] << 8) + xs128;
                             s = sCn[il] - ((int64_t)2 << a.shift) * (int64_t)(int32_t)dv;
                         } else if constexpr (OP == OP_FP8) {
                             // operands are -c * s_C and x * s_row: acc ~ -s_row s_C x.c
@@ -643,8 +646,8 @@ template <> struct RegScore<OP_I16> { using T = int64_t; };
 template <int OP> using RegAcc = typename std::conditional<OP == OP_I8 || OP == OP_I16, i32x16, f32x16>::type;
 
 // Centroids per tile of the register-resident search: the TB = 32 NB MFMA
-// rows, except for I16, whose tile holds the three planes of 32 centroids
-// as its three 32-row blocks (block b = plane b).
+// rows, except for I16, whose tile holds the two planes of 32 centroids as
+// its two 32-row blocks (block b = plane b).
 template <int OP, int NB> constexpr int reg_centroids() { return OP == OP_I16 ? 32 : 32 * NB; }
 
 // Operand quads a lane reads ahead of the MFMAs that use them.
@@ -746,7 +749,8 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, con
 // The epilogue half: fold a tile's scores into each row's running result.
 template <int OP, int M, int FR, int NB>
 __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const float *sN, int h, uint32_t c0,
-                                         int shift, typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
+                                         int shift, const uint32_t (&xs128)[FR],
+                                         typename RegScore<OP>::T (&bs)[FR][M], uint32_t (&bi)[FR][M]) {
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
         if constexpr ((OP == OP_BF16 || OP == OP_FP8) && M == 1) {
@@ -784,14 +788,14 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
                     bi[fr][0] = u ? c0 + nb * 32 + 8 * (r >> 2) + 4 * h + (r & 3) : bi[fr][0];
                 }
         } else if constexpr (OP == OP_I16) {
-            // score = cn - 2^(shift+1) x.c16, x.c16 = p0 + 2^7 p1 + 2^14 p2 taken
-            // mod 2^32 as the reference's i32 sum; strict '<' in ascending index
+            // score = cn - 2^(shift+1) x.c16, x.c16 = x.lo + 2^8 x.hi + 128 sum(x)
+            // taken mod 2^32 as the reference's i32 sum; strict '<' in
+            // ascending index
             const int64_t *sC = (const int64_t *)sN;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int cl = 8 * (r >> 2) + 4 * h + (r & 3);
-                const uint32_t dv = (uint32_t)acc[fr][0][r] + ((uint32_t)acc[fr][1][r] << 7) +
-                                    ((uint32_t)acc[fr][2][r] << 14);
+                const uint32_t dv = (uint32_t)acc[fr][0][r] + ((uint32_t)acc[fr][1][r] << 8) + xs128[fr];
                 const int64_t sc = sC[cl] - ((int64_t)2 << shift) * (int64_t)(int32_t)dv;
                 const bool u = sc < bs[fr][0];
                 bs[fr][0] = u ? sc : bs[fr][0];
@@ -942,7 +946,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
     constexpr bool CAND = OP == OP_BF16 || OP == OP_FP8;
     using Score = typename RegScore<OP>::T;
     static_assert(NS % 2 == 0 && PER_T >= 1 && TILE % 4096 == 0, "shape");
-    static_assert(OP != OP_I16 || NB == 3, "an I16 tile is three planes");
+    static_assert(OP != OP_I16 || NB == I16_PLANES, "an I16 tile is its centroids' planes");
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -953,6 +957,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
     int sb[FR];
     float xn[FR];
     int32_t xni[FR];
+    uint32_t xs128[FR];  // I16: 128 x the row's element sum (mod 2^32)
     float nmax = 0.0f;
 #pragma unroll
     for (int fr = 0; fr < FR; fr++) {
@@ -965,7 +970,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         // for I8, the f32 values for FP32, the bf16-rounded values for BF16
         // (only the positivity offset uses it), the encoding pass's for FP8
         float part = 0.0f;
-        int32_t ipart = 0;
+        int32_t ipart = 0, spart = 0;
         if constexpr (OP == OP_FP8) {
             if (valid && h == 0) part = a.row_norm[r];
         } else {
@@ -985,6 +990,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
                         for (int b = 0; b < 4; b++) {
                             const int32_t v = (int32_t)(int8_t)(w[u] >> (8 * b));
                             ipart += v * v;
+                            spart += v;
                         }
                     }
                 }
@@ -994,6 +1000,8 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         ipart += __shfl_xor(ipart, 32, 64);
         xn[fr] = part;
         xni[fr] = ipart;
+        xs128[fr] = 0u;
+        if constexpr (OP == OP_I16) xs128[fr] = 128u * (uint32_t)(spart + __shfl_xor(spart, 32, 64));
         nmax = fmaxf(nmax, part);
         sb[fr] = 127;
         if constexpr (OP == OP_FP8) {
@@ -1116,7 +1124,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
             reg_tile_cand2<OP, M, NS, FR>(BC, BH, xb, sb, sa, h, j, (uint32_t)((T) * CT), acc, bs, bi); \
         } else {                                                                  \
             reg_mfma<OP, NS, FR, NB>(BC, BH, xb, sb, sa, h, j, acc);              \
-            reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * CT), a.shift, bs, bi); \
+            reg_fold<OP, M, FR, NB>(acc, BH, h, (uint32_t)((T) * CT), a.shift, xs128, bs, bi); \
         }                                                                         \
         __builtin_amdgcn_sched_barrier(0); /* keep the wait after the compute */  \
         __builtin_amdgcn_s_waitcnt(WAIT_ONE);                                     \
@@ -1340,7 +1348,7 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
 // ------------------------------------------------------------- launchers ---
 
 inline size_t search_lds(const SearchGeom &g) {
-    const size_t planes = g.op == OP_I16 ? 3 : 1;
+    const size_t planes = g.op == OP_I16 ? I16_PLANES : 1;
     return (size_t)(BM + planes * BN) * (g.row_bytes + 16) + BN * 8;
 }
 
@@ -1379,7 +1387,7 @@ static hipError_t launch_m(const SearchGeom &g, int M, const SearchArgs &a, hipS
 // FR: 32-row fragments per wave (a workgroup sweeps 128 FR rows)
 template <int OP, int ROWS, int M, int NS, int FR = 2>
 static hipError_t launch_reg(const SearchArgs &a, hipStream_t s) {
-    constexpr int NB = OP == OP_I16 ? 3 : (NS <= 8 ? 2 : 1);
+    constexpr int NB = OP == OP_I16 ? I16_PLANES : (NS <= 8 ? 2 : 1);
     dim3 grid((unsigned)((a.m + 128 * FR - 1) / (128 * FR)), (unsigned)a.nslice);
     // I8 on 512-byte rows needs 269 registers at full speed; held to 256 (44
     // bytes per lane kept in scratch) it runs two waves per SIMD and measures

@@ -8,7 +8,7 @@ hipError_t launch_reg_i8(const SearchGeom &g, int fr, const SearchArgs &a, hipSt
     return launch_reg_ns<OP_I8, EMH_ROWS_I8, 1>(g, a, s);
 }
 
-// I16 rows are multiples of 128 bytes (a tile of three planes is then a
+// I16 rows are multiples of 128 bytes (a tile of two planes is then a
 // whole number of 4 KiB workgroup loads)
 hipError_t launch_reg_i16(const SearchGeom &g, const SearchArgs &a, hipStream_t s) {
     switch (g.row_bytes / 32) {
