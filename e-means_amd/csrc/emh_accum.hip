@@ -224,7 +224,9 @@ __global__ __launch_bounds__(256) void digit_scatter_kernel(const uint32_t *keys
 // three launches instead of six.
 constexpr int WIDE_BINS = 4096;  // the widest digit: scratch sizes
 
-template <int BINS>
+// BM: the histogram block-major (hist[block * BINS + digit], for the scatter
+// that sums the earlier blocks' counts itself), else digit-major.
+template <int BINS, bool BM>
 __global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *keys, size_t m, uint32_t *hist,
                                                               size_t nblocks) {
     __shared__ uint32_t h[BINS];
@@ -236,16 +238,20 @@ __global__ __launch_bounds__(256) void digit_hist_wide_kernel(const uint32_t *ke
         if (i < m) atomicAdd(&h[keys[i] & (BINS - 1)], 1u);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < BINS; i += 256) hist[(size_t)i * nblocks + blockIdx.x] = h[i];
+    for (int i = threadIdx.x; i < BINS; i += 256) {
+        if constexpr (BM) hist[(size_t)blockIdx.x * BINS + i] = h[i];
+        else hist[(size_t)i * nblocks + blockIdx.x] = h[i];
+    }
 }
 
 // As digit_scatter_kernel; with BINS digits the per-wave counts (at most 64,
 // one byte each) are not cleared wholesale each round: the leader of a digit
 // group adds its group to the digit's running count and clears its own
 // entry after the round.
-// RAW: offs is the unscanned histogram (digit_hist_wide_kernel) and each
-// block sums the earlier blocks' counts of its digits itself (a few blocks:
-// no scan launch); block 0 also writes the digit totals to totals_out.
+// RAW: offs is the unscanned block-major histogram (digit_hist_wide_kernel)
+// and each block sums the earlier blocks' counts of its digits itself (a few
+// blocks: no scan launch), every load of it in flight at once; block 0 also
+// writes the digit totals to totals_out.
 // Otherwise offs and totals come from digit_row_scan_kernel.
 template <bool RAW, int BINS>
 __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t *keys, const uint32_t *vals, size_t m,
@@ -264,32 +270,50 @@ __global__ __launch_bounds__(256) void digit_scatter_wide_kernel(const uint32_t 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     {
         // exclusive scan of the digit totals, PER consecutive digits a thread
-        uint32_t loc[PER], tsum = 0;
+        uint32_t loc[PER], tot[PER], tsum = 0;
+        if constexpr (RAW) {
+            // this thread's PER digits of every block: PER / 4 16-byte loads
+            // a block, eight blocks' loads issued before any is added
+            uint32_t pr[PER];
+#pragma unroll
+            for (int i = 0; i < PER; i++) pr[i] = tot[i] = 0u;
+            const uint32_t *col = offs + (size_t)tid * PER;
+            constexpr int BB = 8;
+            for (size_t b0 = 0; b0 < nblocks; b0 += BB) {
+                uint4 v[BB][PER / 4];
+#pragma unroll
+                for (int u = 0; u < BB; u++)
+#pragma unroll
+                    for (int q = 0; q < PER / 4; q++)
+                        v[u][q] = b0 + u < nblocks ? *(const uint4 *)(col + (b0 + u) * BINS + 4 * q)
+                                                   : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                for (int u = 0; u < BB; u++) {
+                    const bool before = b0 + u < blockIdx.x;
+#pragma unroll
+                    for (int q = 0; q < PER / 4; q++) {
+                        const uint32_t c4[4] = {v[u][q].x, v[u][q].y, v[u][q].z, v[u][q].w};
+#pragma unroll
+                        for (int c = 0; c < 4; c++) {
+                            tot[4 * q + c] += c4[c];
+                            if (before) pr[4 * q + c] += c4[c];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < PER; i++) {
+                pre[tid * PER + i] = pr[i];
+                if (blockIdx.x == 0) totals_out[tid * PER + i] = tot[i];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < PER; i++) tot[i] = totals[tid * PER + i];
+        }
 #pragma unroll
         for (int i = 0; i < PER; i++) {
             loc[i] = tsum;
-            uint32_t tot;
-            if constexpr (RAW) {
-                // the digit's row, 8 loads in flight at a time
-                const uint32_t *row = offs + (size_t)(tid * PER + i) * nblocks;
-                uint32_t p = 0;
-                tot = 0;
-                for (size_t b0 = 0; b0 < nblocks; b0 += 8) {
-                    uint32_t v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) v[u] = b0 + u < nblocks ? row[b0 + u] : 0u;
-#pragma unroll
-                    for (int u = 0; u < 8; u++) {
-                        if (b0 + u < blockIdx.x) p += v[u];
-                        tot += v[u];
-                    }
-                }
-                pre[tid * PER + i] = p;
-                if (blockIdx.x == 0) totals_out[tid * PER + i] = tot;
-            } else {
-                tot = totals[tid * PER + i];
-            }
-            tsum += tot;
+            tsum += tot[i];
         }
         uint32_t all;
         const uint32_t base = block_exclusive_scan(tsum, ssm, &all);
@@ -388,15 +412,17 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
     int cur = 0;
     if (bits > 8 && bits <= 11) {
         uint32_t *totals = hist + (size_t)2048 * nblocks;
-        hipLaunchKernelGGL(digit_hist_wide_kernel<2048>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist,
-                           nblocks);
         if (nblocks <= 64) {
+            hipLaunchKernelGGL((digit_hist_wide_kernel<2048, true>), dim3((unsigned)nblocks), dim3(256), 0, s, kin, m,
+                               hist, nblocks);
             // up to 64 blocks (64 Ki keys): each scatter block sums its digits'
             // counts over the blocks itself, two launches instead of three
             hipLaunchKernelGGL((digit_scatter_wide_kernel<true, 2048>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
                                vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)nullptr, kb[0], vb[0],
                                counts_out ? a.starts : nullptr, totals);
         } else {
+            hipLaunchKernelGGL((digit_hist_wide_kernel<2048, false>), dim3((unsigned)nblocks), dim3(256), 0, s, kin, m,
+                               hist, nblocks);
             hipLaunchKernelGGL(digit_row_scan_kernel, dim3(2048), dim3(256), 0, s, hist, nblocks, totals);
             hipLaunchKernelGGL((digit_scatter_wide_kernel<false, 2048>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
                                vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
@@ -411,8 +437,8 @@ hipError_t launch_sort_pairs(AccumScratch &a, const uint32_t *keys_in, const uin
         // 12-bit keys in one pass: histogram, the digits' rows scanned over
         // the blocks, scatter (three launches instead of two passes' six)
         uint32_t *totals = hist + (size_t)4096 * nblocks;
-        hipLaunchKernelGGL(digit_hist_wide_kernel<4096>, dim3((unsigned)nblocks), dim3(256), 0, s, kin, m, hist,
-                           nblocks);
+        hipLaunchKernelGGL((digit_hist_wide_kernel<4096, false>), dim3((unsigned)nblocks), dim3(256), 0, s, kin, m,
+                           hist, nblocks);
         hipLaunchKernelGGL(digit_row_scan_kernel, dim3(4096), dim3(256), 0, s, hist, nblocks, totals);
         hipLaunchKernelGGL((digit_scatter_wide_kernel<false, 4096>), dim3((unsigned)nblocks), dim3(256), 0, s, kin,
                            vin, m, (const uint32_t *)hist, nblocks, (const uint32_t *)totals, kb[0], vb[0],
