@@ -430,9 +430,24 @@ emh_assigner::~emh_assigner() {
 }
 
 // I8: whether every score fits the packed int32 key of the register-resident
-// kernel (flag written by the bias kernel); otherwise the i64 kernel runs.
+// kernel; I16: whether every offset fits its exact f64 keys (flags written by
+// the bias kernel). Otherwise the LDS kernel's i64 scores run.
 static int refresh_i8_range(emh_assigner *h, bool with_bias) {
     h->g.i8_packed = false;
+    h->g.i16_f64 = false;
+    if (h->op == OP_I16 && h->g.reg) {
+        // I16 on the register kernel: without a bias cn = sum c16^2 <= d 2^30
+        // < 2^46 (d <= 512 there); with one, the bias kernel's flag says
+        if (!with_bias) {
+            h->g.i16_f64 = true;
+            return EMH_OK;
+        }
+        uint32_t flag = 1;
+        int rc = check(hipMemcpy(&flag, h->stats.as<uint32_t>() + 2, 4, hipMemcpyDeviceToHost), "read range flag");
+        if (rc) return rc;
+        h->g.i16_f64 = flag == 0;
+        return EMH_OK;
+    }
     if (h->op != OP_I8 || !h->g.reg || (int64_t)h->g.d * 32768 >= ((int64_t)1 << 26)) return EMH_OK;
     // without a bias cn = sum c8^2 <= d 2^14, inside the limit 2^26 - d 2^15
     // whenever d 3 2^14 < 2^26: no need to read the flag back (no sync)

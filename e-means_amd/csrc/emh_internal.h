@@ -34,6 +34,7 @@ struct SearchGeom {
     int op;
     bool reg;           // rows fit the register-resident kernel (operand row <= 512 bytes)
     bool i8_packed = false;  // I8: every |2 x.c - cn| < 2^26, so scores fit the packed int32 key
+    bool i16_f64 = false;    // I16: every |cn| < 2^46, so the register kernel's f64 keys are exact
     bool resident;      // the workgroup's rows stay in LDS for the whole centroid sweep
     size_t d, d_pad;    // elements; d_pad * elem_bytes is the operand row stride
     size_t k, k_pad;

@@ -624,6 +624,9 @@ __global__ void set_bias_kernel(int op, size_t k, size_t k_pad, const float *nor
                 r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
                 v += (int64_t)r;
             }
+            // past 2^46 the register kernel's f64 keys lose exactness: the
+            // LDS kernel's i64 scores run instead
+            if (v >= ((int64_t)1 << 46) || v <= -((int64_t)1 << 46)) atomicMax(&stats[2], 1u);
         }
         cn[jc] = v;
         return;
@@ -815,7 +818,9 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
                 break;
             case OP_I16:
                 if (M != 1 || rows != EMH_ROWS_I8) return hipErrorInvalidValue;
-                return launch_reg_i16(g, r, s);
+                // offsets past the f64 keys' exact range take the LDS kernel
+                if (g.i16_f64) return launch_reg_i16(g, r, s);
+                break;
         }
     }
     // LDS kernel (rows resident in LDS, or streamed in d-chunks)

@@ -641,7 +641,7 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 template <int OP> struct RegScore { using T = uint32_t; };
 template <> struct RegScore<OP_F32> { using T = float; };
 template <> struct RegScore<OP_I8> { using T = int32_t; };
-template <> struct RegScore<OP_I16> { using T = int64_t; };
+template <> struct RegScore<OP_I16> { using T = double; };  // exact integers
 
 template <int OP> using RegAcc = typename std::conditional<OP == OP_I8 || OP == OP_I16, i32x16, f32x16>::type;
 
@@ -789,18 +789,27 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
                 }
         } else if constexpr (OP == OP_I16) {
             // score = cn - 2^(shift+1) x.c16, x.c16 = x.lo + 2^8 x.hi + 128 sum(x)
-            // taken mod 2^32 as the reference's i32 sum; strict '<' in
-            // ascending index
-            const int64_t *sC = (const int64_t *)sN;
+            // taken mod 2^32 as the reference's i32 sum. As an f64 key
+            // 32 score + position (the tile's offsets are stored as 32 cn +
+            // position): |cn| < 2^46 and |2^(shift+1) x.c16| <= 2^39 keep
+            // every key an exact integer below 2^52, so one f64 min per value
+            // takes the lowest score, then the lowest index; the tile's best
+            // replaces the running one only with a strictly lower score
+            // (an earlier tile keeps a tie).
+            const double *sK = (const double *)sN;
+            const double step = -(double)(2 << shift) * 32.0;
+            double kmin = INFINITY;
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const int cl = 8 * (r >> 2) + 4 * h + (r & 3);
                 const uint32_t dv = (uint32_t)acc[fr][0][r] + ((uint32_t)acc[fr][1][r] << 8) + xs128[fr];
-                const int64_t sc = sC[cl] - ((int64_t)2 << shift) * (int64_t)(int32_t)dv;
-                const bool u = sc < bs[fr][0];
-                bs[fr][0] = u ? sc : bs[fr][0];
-                bi[fr][0] = u ? c0 + (uint32_t)cl : bi[fr][0];
+                kmin = fmin(kmin, fma(step, (double)(int32_t)dv, sK[cl]));
             }
+            const double sc = floor(kmin * 0.03125);
+            const bool u = sc < bs[fr][0];
+            const double pos = u ? kmin - sc * 32.0 : 0.0;
+            bs[fr][0] = u ? sc : bs[fr][0];
+            bi[fr][0] = u ? c0 + (uint32_t)pos : bi[fr][0];
         } else {  // OP_I8: maximise key = (dot << 6) + ((-cn) << 5) + 31 - position
             const int32_t *sK = (const int32_t *)sN;
             int32_t g = INT32_MIN;
@@ -1043,15 +1052,17 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
     // Per-centroid score offsets of a tile, loaded raw: any arithmetic on the
     // loaded value would make the compiler wait for every outstanding load of
     // the wave (the tile DMA included) right there. Converted at the store:
-    // float hn + B, or the I8 key part ((-cn) << 5) + 31 - position.
+    // float hn + B, the I8 key part ((-cn) << 5) + 31 - position, or the
+    // I16 f64 key part 32 cn + position.
     // I8 loads the low word of cn only (|cn| <= 2^26 on this path): a dead
     // high word still being written by a load would make the compiler wait
     // for it, and with it for the tile DMA, before reusing its register.
-    // I16 loads both words of its 32 i64 offsets, one word per thread.
-    using HRaw = typename std::conditional<OP == OP_I8 || OP == OP_I16, int32_t, float>::type;
+    // I16 loads each of its 32 i64 offsets whole (both words are used).
+    using HRaw = typename std::conditional<OP == OP_I16, int64_t,
+                                           typename std::conditional<OP == OP_I8, int32_t, float>::type>::type;
     auto hload = [&](int t) -> HRaw {
         // every thread loads (threads past TB repeat an entry): no branch
-        if constexpr (OP == OP_I16) return ((const int32_t *)a.cn)[(size_t)t * 64 + (tid & 63)];
+        if constexpr (OP == OP_I16) return a.cn[(size_t)t * 32 + (tid & 31)];
         const size_t ci = (size_t)t * TB + (tid & (TB - 1));
         if constexpr (OP == OP_I8) return ((const int32_t *)a.cn)[2 * ci];  // little-endian low word
         else return a.hn[ci];
@@ -1060,7 +1071,9 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
     // compiler's wait tracking stays exact across the stage
     auto hstore = [&](float *dst, HRaw v) {
         if constexpr (OP == OP_I16) {
-            dst[tid] = __int_as_float(v);  // words 0..63: the tile's offsets, as int64
+            // the tile's 32 offsets as f64 keys 32 cn + position (exact for
+            // |cn| < 2^46); pad centroids (INT64_MAX) never win
+            ((double *)dst)[tid & 31] = v == INT64_MAX ? (double)INFINITY : (double)v * 32.0 + (double)(tid & 31);
         } else if constexpr (OP == OP_I8) {
             // centroid c of the tile is value r of block nb for lanes of half
             // (c >> 2) & 1, with position nb * 16 + r in the fold
@@ -1081,7 +1094,7 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
 #pragma unroll
         for (int t = 0; t < M; t++) {
             if constexpr (OP == OP_I8) bs[fr][t] = INT32_MIN;
-            else if constexpr (OP == OP_I16) bs[fr][t] = INT64_MAX;
+            else if constexpr (OP == OP_I16) bs[fr][t] = INFINITY;
             else if constexpr (OP == OP_F32) bs[fr][t] = INFINITY;
             else bs[fr][t] = 0xFFFFFFFFu;
             bi[fr][t] = NO_LABEL;
@@ -1166,7 +1179,8 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
             }
         } else {
             Score ps;
-            if constexpr (OP == OP_I16) ps = shfl_xor_i64(bs[fr][0], 32);
+            if constexpr (OP == OP_I16)
+                ps = __builtin_bit_cast(double, shfl_xor_i64(__builtin_bit_cast(int64_t, bs[fr][0]), 32));
             else ps = __shfl_xor(bs[fr][0], 32, 64);
             const uint32_t pi = (uint32_t)__shfl_xor((int)bi[fr][0], 32, 64);
             const bool better = OP == OP_I8 ? (ps > bs[fr][0] || (ps == bs[fr][0] && pi < bi[fr][0]))
@@ -1196,8 +1210,10 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
                     uint32_t *pp = a.part + (r * a.nslice + slice) * 4;
                     uint32_t lo, hi = 0u;
                     if constexpr (OP == OP_I16) {
-                        lo = (uint32_t)(uint64_t)bs[fr][0];
-                        hi = (uint32_t)((uint64_t)bs[fr][0] >> 32);
+                        // the exact integer score as int64 (INT64_MAX: no centroid)
+                        const int64_t s64 = bs[fr][0] < 9.2e18 ? (int64_t)bs[fr][0] : INT64_MAX;
+                        lo = (uint32_t)(uint64_t)s64;
+                        hi = (uint32_t)((uint64_t)s64 >> 32);
                     } else {
                         lo = __builtin_bit_cast(uint32_t, bs[fr][0]);  // f32 or int32 score bits
                     }

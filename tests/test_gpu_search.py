@@ -138,6 +138,27 @@ def test_i8_large_bias_takes_the_i64_path(hip, d):
         assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias scale {scale}"
 
 
+@pytest.mark.parametrize("d", [96, 512])
+def test_i16_large_bias_takes_the_i64_path(hip, d):
+    """I16 offsets past 2^46 (sum c16^2 + round(bias 4^shift)) leave the
+    register kernel's exact f64 keys for the LDS kernel's i64 scores; both
+    stay bit-exact, with and without a split of the centroid range."""
+    rng = np.random.default_rng(10)
+    x = rng.integers(-128, 128, size=(700, d)).astype(np.int8)
+    c = rng.normal(scale=40, size=(300, d)).astype(np.float32)
+    for scale in (1.0, 1e10):
+        bias = (rng.uniform(0, 1, size=300) * scale).astype(np.float32)
+        bias[3] = -bias[3]
+        cl, cd = O.i8_assign(x, c, bits=O.BITS_I16, bias=bias)
+        for split in (0, 4):
+            a = E.HipAssigner(E.I16)
+            a.set_split(split)
+            a.set_centroids(c)
+            a.set_bias(bias)
+            gl, gd = a.assign(x)
+            assert np.array_equal(gl, cl) and np.array_equal(gd, cd), f"d={d} bias scale {scale} split {split}"
+
+
 @pytest.mark.parametrize("n,k,d", I8_SHAPES)
 def test_i16_bit_exact(hip, n, k, d):
     """src/cpu_i8.rs:298-324 with CentroidBits::I16: integer centroids give
