@@ -1211,7 +1211,8 @@ __global__ void step_advance_kernel(uint32_t *ctr) {
 }
 
 // The step's last kernel: the EMA step with the running step's rate and
-// last-epoch flag (ema_apply_kernel's, one wave per cluster) and the step's labels and distances into pinned host slot
+// last-epoch flag (ema_apply_kernel's, one wave per cluster) and the step's
+// labels (last epoch only) and distances into pinned host slot
 // (step % nslot): labels at [0, nb), distances at [batch, batch + nb) of
 // the slot, and before a revival check each cluster's new mass at
 // [2 batch + j].
@@ -1227,8 +1228,10 @@ __global__ __launch_bounds__(256) void step_finish_kernel(float *cen, float *usa
     uint32_t *dst = slots.p[c % (uint32_t)slots.n];
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     if (blockIdx.x == 0 && threadIdx.x == 0) ctr[0] = c + 1u;  // the next step's index
+    // the host reads a step's labels only in the last epoch (the last
+    // pass's labels), its distances always (the objective)
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += stride) {
-        dst[i] = labels[i];
+        if (p.is_last) dst[i] = labels[i];
         dst[batch + i] = __float_as_uint(dist[i]);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) *clear_word = 0u;
