@@ -161,10 +161,6 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         geo[j] = make_geom(op, jb.k, jb.d);
-        if (!geo[j].reg)
-            return fail(EMH_E_UNSUPPORTED, "train_batch: job " + std::to_string(j) +
-                                               " has operand rows over 512 bytes (FP32 d <= 128, BF16 d <= 256, "
-                                               "FP8 and I8 d <= 512)");
         d_max = std::max(d_max, (size_t)jb.d);
         row_base[j] = slots;
         slots += run[j].R.batch_size;
@@ -177,8 +173,9 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         cpad_base[j] = cpad;
         cpad += geo[j].k_pad;
         op_base[j] = op_bytes;
-        op_bytes += geo[j].k_pad * (size_t)geo[j].row_bytes;
-        slices[j] = job_slices(geo[j]);
+        op_bytes += geo[j].k_pad * op_row_bytes(geo[j]);
+        // jobs with operand rows past 512 bytes run the LDS kernel, unsplit
+        slices[j] = geo[j].reg ? job_slices(geo[j]) : JobSlices{1u, 1u};
         part_base[j] = part_words;
         part_words += run[j].R.batch_size * job_part_words(op, M, slices[j].nslice);
         // FP8: the step's encoded batch rows, d_pad bytes a slot
@@ -292,10 +289,17 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     // slice's centroid tiles. Padding entries (first row past every batch)
     // exit at once.
     constexpr size_t XCDS = 8, XCD_RUN = 8;
-    std::vector<uint2> wg_all, cmap(cpad), rmap(slots);
+    std::vector<uint2> wg_all, wg_lds, cmap(cpad), rmap(slots);
+    size_t lds_max = 0;  // the LDS kernel's dynamic LDS for the widest-row job
+    for (size_t j = 0; j < njobs; j++) {
+        if (geo[j].reg) continue;
+        lds_max = std::max(lds_max, search_lds_bytes(geo[j]));
+        for (size_t r0 = 0; r0 < run[j].R.batch_size; r0 += BM) wg_lds.push_back(make_uint2((unsigned)j, (unsigned)r0));
+    }
     {
-        std::vector<size_t> order(njobs);
-        for (size_t j = 0; j < njobs; j++) order[j] = j;
+        std::vector<size_t> order;
+        for (size_t j = 0; j < njobs; j++)
+            if (geo[j].reg) order.push_back(j);
         auto work = [&](size_t j) { return (size_t)slices[j].tiles_per_slice * reg_tile(geo[j]) * geo[j].d_pad; };
         std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
         std::vector<uint2> list;
@@ -319,13 +323,16 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     }
     Layout ml;
     const size_t m_wg = ml.add(wg_all.size() * 8), m_c = ml.add(cpad * 8), m_r = ml.add(slots * 8),
-                 m_j = ml.add(nclusters * 4);
+                 m_j = ml.add(nclusters * 4), m_lwg = ml.add(wg_lds.size() * 8);
     if ((rc = check(maps.ensure(ml.bytes), "maps"))) return rc;
     uint8_t *mp = (uint8_t *)maps.p;
     if ((rc = check(hipMemcpyAsync(mp + m_wg, wg_all.data(), wg_all.size() * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
         (rc = check(hipMemcpyAsync(mp + m_c, cmap.data(), cpad * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
         (rc = check(hipMemcpyAsync(mp + m_r, rmap.data(), slots * 8, hipMemcpyHostToDevice, h->stream), "maps")) ||
         (rc = check(hipMemcpyAsync(mp + m_j, cjob.data(), nclusters * 4, hipMemcpyHostToDevice, h->stream), "maps")) ||
+        (!wg_lds.empty() &&
+         (rc = check(hipMemcpyAsync(mp + m_lwg, wg_lds.data(), wg_lds.size() * 8, hipMemcpyHostToDevice, h->stream),
+                     "maps"))) ||
         (rc = check(hipStreamSynchronize(h->stream), "upload")))
         return rc;
 
@@ -356,8 +363,10 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.k = (uint32_t)jobs[j].k;
         t.k_pad = (uint32_t)geo[j].k_pad;
         t.d_pad = (uint32_t)geo[j].d_pad;
-        t.row_bytes = (uint32_t)geo[j].row_bytes;
-        t.tb = (uint32_t)reg_tile(geo[j]);
+        // the register-resident search reads a tile-major table, the LDS
+        // kernel a row-major one (tiles of one centroid)
+        t.row_bytes = (uint32_t)op_row_bytes(geo[j]);
+        t.tb = geo[j].reg ? (uint32_t)reg_tile(geo[j]) : 1u;
         t.nslice = slices[j].nslice;
         t.tiles_per_slice = slices[j].tiles_per_slice;
         t.part = slices[j].nslice > 1 ? part_d.as<uint32_t>() + part_base[j] : nullptr;
@@ -401,6 +410,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess && op == OP_FP8) ce = last_nodes(search_first);
     if (ce == hipSuccess) ce = launch_jobs_search(op, M, sa_dev, (const uint2 *)(mp + m_wg), wg_all.size(), h->stream);
     if (ce == hipSuccess && op != OP_FP8) ce = last_nodes(search_first);
+    if (ce == hipSuccess)
+        ce = launch_jobs_search_lds(op, M, sa_dev, (const uint2 *)(mp + m_lwg), wg_lds.size(), lds_max, h->stream);
     if (ce == hipSuccess && cand_search) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess && !cand_search && part_words)
         ce = launch_jobs_merge(op, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
@@ -459,7 +470,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         JobDev t = jd[j];
         live[j] = J.next();
         if (live[j]) {
-            if (J.penalized && op == OP_I8 && !i8_bias_fits(J)) out_of_range[j] = 1;
+            if (J.penalized && op == OP_I8 && geo[j].reg && !i8_bias_fits(J)) out_of_range[j] = 1;
             if (J.penalized) memcpy(in_hp + in_bias + cl_base[j] * 4, J.bias.data(), J.k * 4);
             memcpy(in_hp + in_ids + row_base[j] * 4, J.ids, J.nb * 4);
             t.m = (uint32_t)J.nb;

@@ -61,6 +61,11 @@ inline size_t operand_table_bytes(const SearchGeom &g) {
 inline size_t operand_bytes(const SearchGeom &g) {
     return g.reg ? 2 * operand_table_bytes(g) : operand_table_bytes(g);
 }
+// Bytes of one operand row in the table a search reads: the register-
+// resident kernel's row, or the LDS kernel's whole padded row.
+inline size_t op_row_bytes(const SearchGeom &g) {
+    return g.reg ? (size_t)g.row_bytes : g.d_pad * (size_t)op_elem_bytes(g.op);
+}
 inline const void *reg_operand(const SearchGeom &g, const void *cen_op) {
     return (const uint8_t *)cen_op + operand_table_bytes(g);
 }
@@ -200,6 +205,13 @@ hipError_t launch_jobs_rows_fp8(const JobDev *jobs, const uint2 *rmap, size_t sl
 // search of every job in one launch: wg holds (job, first row | slice) per
 // workgroup (first rows are multiples of 256, slices below 256).
 hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint2 *wg, size_t nwg, hipStream_t s);
+// the same for the jobs whose operand rows are too wide for it, on the LDS
+// kernel: wg holds (job, first row) per workgroup (first rows are multiples
+// of BM); lds_bytes: the largest such job's search_lds_bytes.
+hipError_t launch_jobs_search_lds(int op, int M, const void *search_args, const uint2 *wg, size_t nwg,
+                                  size_t lds_bytes, hipStream_t s);
+// dynamic LDS of the LDS kernel for a geometry
+size_t search_lds_bytes(const SearchGeom &g);
 // rescoring of every job's candidates (merging a split job's slice lists
 // first): rmap holds (job, row) per batch slot.
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);

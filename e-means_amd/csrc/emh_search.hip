@@ -898,6 +898,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     a.stats = j.stats;
     // whole 16-byte quads per row from a 16-byte aligned table
     a.vec = (g.d * (g.op == OP_I8 ? 1 : 4)) % 16 == 0 && ((uintptr_t)j.x & 15) == 0;
+    a.resident = g.resident ? 1 : 0;
     a.labels = j.labels;
     a.dist = j.dist;
     a.cand = j.cand;
@@ -968,6 +969,14 @@ hipError_t launch_jobs_search(int op, int M, const void *search_args, const uint
     if (!nwg) return hipSuccess;
     return launch_jobs_search_op(op, M, search_args, wg, nwg, s);
 }
+
+hipError_t launch_jobs_search_lds(int op, int M, const void *search_args, const uint2 *wg, size_t nwg,
+                                  size_t lds_bytes, hipStream_t s) {
+    if (!nwg) return hipSuccess;
+    return launch_jobs_search_lds_op(op, M, search_args, wg, nwg, lds_bytes, s);
+}
+
+size_t search_lds_bytes(const SearchGeom &g) { return search_lds(g); }
 
 hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
     if (!slots) return hipSuccess;

@@ -88,6 +88,7 @@ struct SearchArgs {
     const unsigned *stats;
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
     int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
+    int resident;           // LDS kernel: whole rows resident in LDS (else d-chunks)
     int shift;              // I16: score units are 4^-shift squared distance
     // Centroid-range split (register-resident search): workgroup (b, s)
     // sweeps tiles [s * tiles_per_slice, +tiles_per_slice) and, with more than
@@ -364,8 +365,10 @@ __device__ inline int64_t shfl_xor_i64(int64_t v, int mask) {
 
 // --------------------------------------------------------------- search ----
 
+// Rows [row0, row0 + BM) of a against every centroid, on the dynamic LDS
+// the launch provides (search_lds bytes).
 template <int OP, int ROWS, int M, bool RES>
-__global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
+__device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_t row0) {
     using T = OpT<OP>;
     constexpr int ES = T::ES, LB = T::LB;
     constexpr bool INT = OP == OP_I8 || OP == OP_I16;
@@ -379,7 +382,6 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = lane >> 5, j = lane & 31;
-    const size_t row0 = (size_t)blockIdx.x * BM;
     const int RB = a.row_bytes, RS = RB + 16;
     uint8_t *sR = smem;
     uint8_t *sC = smem + BM * RS;
@@ -602,6 +604,11 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
             for (int t = 0; t < M; t++) a.cand[my_row * M + t] = bi[t];
         }
     }
+}
+
+template <int OP, int ROWS, int M, bool RES>
+__global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
+    search_lds_core<OP, ROWS, M, RES>(a, (size_t)blockIdx.x * BM);
 }
 
 // ------------------------------------------- register-resident searches ----
@@ -1361,6 +1368,37 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
     }
 }
 
+// Jobs whose operand rows are too wide for the register-resident sweep, in
+// one launch of the LDS kernel: workgroup b sweeps rows [wg[b].y, + BM) of
+// job wg[b].x against all its centroids (the job's row-major operand table),
+// rows resident in LDS or streamed in d-chunks as the job's geometry says.
+// The dynamic LDS is the largest such job's.
+template <int OP, int M>
+__global__ __launch_bounds__(256) void search_lds_jobs_kernel(const SearchArgs *jobs, const uint2 *wg) {
+    const uint2 w = wg[blockIdx.x];
+    SearchArgs a = jobs[w.x];
+    a.x = global_ptr_s(a.x);
+    a.ids = global_ptr_s(a.ids);
+    a.cen_op = global_ptr_s(a.cen_op);
+    a.hn = global_ptr_s(a.hn);
+    a.cn = global_ptr_s(a.cn);
+    a.stats = global_ptr_s(a.stats);
+    a.cand = global_ptr_s(a.cand);
+    a.labels = global_ptr_s(a.labels);
+    a.dist = global_ptr_s(a.dist);
+    if constexpr (OP == OP_FP8) {
+        a.x8 = global_ptr_s(a.x8);
+        a.row_scale = global_ptr_s(a.row_scale);
+        a.row_norm = global_ptr_s(a.row_norm);
+        a.cmax_bits = global_ptr_s(a.cmax_bits);
+    }
+    const size_t row0 = w.y;
+    if (row0 >= a.m) return;  // whole workgroup: before any barrier
+    constexpr int ROWS = OP == OP_I8 ? EMH_ROWS_I8 : EMH_ROWS_F32;
+    if (__builtin_amdgcn_readfirstlane(a.resident)) search_lds_core<OP, ROWS, M, true>(a, row0);
+    else search_lds_core<OP, ROWS, M, false>(a, row0);
+}
+
 // ------------------------------------------------------------- launchers ---
 
 inline size_t search_lds(const SearchGeom &g) {
@@ -1455,5 +1493,8 @@ hipError_t launch_reg_i16(const SearchGeom &g, const SearchArgs &a, hipStream_t 
 hipError_t launch_lds(const SearchGeom &g, int rows, int M, const SearchArgs &a, hipStream_t s);
 // the batched search of many problems in one launch
 hipError_t launch_jobs_search_op(int op, int M, const void *search_args, const uint2 *wg, size_t nwg, hipStream_t s);
+// the batched LDS-kernel search of the jobs with wide operand rows
+hipError_t launch_jobs_search_lds_op(int op, int M, const void *search_args, const uint2 *wg, size_t nwg,
+                                     size_t lds_bytes, hipStream_t s);
 
 }  // namespace emh

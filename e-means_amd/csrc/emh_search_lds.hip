@@ -28,4 +28,38 @@ hipError_t launch_lds(const SearchGeom &g, int rows, int M, const SearchArgs &a,
     return hipErrorInvalidValue;
 }
 
+template <int OP, int M>
+static hipError_t launch_lds_jobs_m(const void *args, const uint2 *wg, size_t nwg, size_t lds, hipStream_t s) {
+    auto kern = search_lds_jobs_kernel<OP, M>;
+    static size_t attr_bytes = 0;
+    if (lds > 64 * 1024 && lds > attr_bytes) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        attr_bytes = lds;
+    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(256), lds, s, (const SearchArgs *)args, wg);
+    return hipGetLastError();
+}
+
+hipError_t launch_jobs_search_lds_op(int op, int M, const void *args, const uint2 *wg, size_t nwg, size_t lds,
+                                     hipStream_t s) {
+    if (op == OP_F32 || op == OP_I8) {
+        if (M != 1) return hipErrorInvalidValue;
+        return op == OP_F32 ? launch_lds_jobs_m<OP_F32, 1>(args, wg, nwg, lds, s)
+                            : launch_lds_jobs_m<OP_I8, 1>(args, wg, nwg, lds, s);
+    }
+    if (op != OP_BF16 && op != OP_FP8) return hipErrorInvalidValue;
+    switch (M) {
+        case 1: return op == OP_BF16 ? launch_lds_jobs_m<OP_BF16, 1>(args, wg, nwg, lds, s)
+                                     : launch_lds_jobs_m<OP_FP8, 1>(args, wg, nwg, lds, s);
+        case 2: return op == OP_BF16 ? launch_lds_jobs_m<OP_BF16, 2>(args, wg, nwg, lds, s)
+                                     : launch_lds_jobs_m<OP_FP8, 2>(args, wg, nwg, lds, s);
+        case 4: return op == OP_BF16 ? launch_lds_jobs_m<OP_BF16, 4>(args, wg, nwg, lds, s)
+                                     : launch_lds_jobs_m<OP_FP8, 4>(args, wg, nwg, lds, s);
+        case 8: return op == OP_BF16 ? launch_lds_jobs_m<OP_BF16, 8>(args, wg, nwg, lds, s)
+                                     : launch_lds_jobs_m<OP_FP8, 8>(args, wg, nwg, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 }  // namespace emh

@@ -278,11 +278,13 @@ int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice);
  * training step of all jobs still running is one replay of a captured
  * hipGraph: upload of every job's centroid snapshot, bias and batch ids,
  * then the gather + search (+ FP32 rescore) + per-cluster sums of all jobs,
- * then one download. Precision FP32 or BF16_RESCORE on f32 rows, or I8 on
- * i8 rows; operand rows of at most 512 bytes (FP32: d <= 128, BF16:
- * d <= 256, I8: d <= 512). I8 jobs run the packed-key search only: a
- * balance bias with |sum c8^2 + round(bias)| >= 2^26 - d 2^15 stops the call
- * with EMH_E_UNSUPPORTED (emh_train then takes the i64 search). */
+ * then one download. Precision FP32, BF16_RESCORE or FP8_RESCORE on f32
+ * rows, or I8 on i8 rows, any d: jobs whose operand rows pass 512 bytes
+ * (FP32: d > 128, BF16: d > 256, FP8 and I8: d > 512) run the LDS search
+ * kernel in the same step. I8 jobs of at most 512-byte rows run the
+ * packed-key search only: a balance bias with |sum c8^2 + round(bias)| >=
+ * 2^26 - d 2^15 stops the call with EMH_E_UNSUPPORTED (emh_train then takes
+ * the i64 search). */
 typedef struct emh_job {
     const void *x; /* n x d rows, row-major */
     int rows;      /* enum emh_rows: EMH_ROWS_F32 (EMH_ROWS_I8 for the I8 search) */

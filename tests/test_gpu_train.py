@@ -222,6 +222,26 @@ def test_batched_training_equals_single(hip, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
+def test_batched_wide_rows_equal_single(hip, prec):
+    """Jobs whose operand rows pass 512 bytes (FP32 d > 128, BF16 d > 256,
+    FP8 d > 512) run the LDS kernel inside the batched step, rows resident
+    in LDS or streamed in d-chunks, beside register-resident jobs; each ends
+    with exactly its single-problem result."""
+    wide = {"fp32": (200, 1000), "bf16": (300, 1400), "fp8": (600, 2100)}[prec]
+    specs = [(3000, wide[0], 60, 0.0), (2500, wide[1], 40, 0.1), (4000, 64, 90, 0.0), (2000, 24, 30, 0.1)]
+    problems = [(_job_data(140 + i, n, d, 2 * k), k, E.Config(nepochs=2, balance=bal, seed=150 + i))
+                for i, (n, d, k, bal) in enumerate(specs)]
+    print(f"batch wide rows {prec}", flush=True)
+    got, _ = E.train_batch(problems, E.HipAssigner(prec))
+    for (x, k, cfg), g in zip(problems, got):
+        ref = E.train(x, k, cfg, E.HipAssigner(prec))
+        assert np.array_equal(g.labels, ref.labels)
+        assert np.array_equal(g.centroids, ref.centroids)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_batched_slices_and_big_clusters_equal_single(hip, prec):
     """The batched step's own paths: a job whose centroid tiles split into
     two slices (k = 2100, d = 64), jobs whose clusters get hundreds of
@@ -376,6 +396,25 @@ def test_batched_i8_jobs_equal_oracle(hip):
                         batch_size=cfg.batch_size if cfg.has_batch_size else None)
         ref = O.train(x, k, ocfg, assigner=O.ASSIGNER_I8)
         assert g.recipe.batch_size == ref.recipe.batch_size
+        assert np.array_equal(g.labels, ref.labels)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+        assert np.array_equal(g.centroids, ref.centroids)
+
+
+def test_batched_i8_wide_rows_equal_oracle(hip):
+    """int8 jobs whose rows pass 512 bytes run the LDS kernel's i64 search
+    inside the batched step (rows resident in LDS, and streamed in d-chunks
+    for the widest), beside a register-resident job; each equals the CPU
+    oracle's int8 training bit for bit, balance on one."""
+    specs = [(1500, 600, 20, 0.1), (1200, 1100, 12, 0.0), (3000, 64, 30, 0.0)]
+    batch = []
+    for i, (n, d, k, bal) in enumerate(specs):
+        batch.append((_i8_blobs(n, d, 2 * k, 120 + i), k, E.Config(nepochs=2, balance=bal, seed=130 + i)))
+    print(f"batch i8 wide: {len(batch)} jobs", flush=True)
+    got, _ = E.train_batch(batch, E.HipAssigner(E.I8))
+    for (x, k, cfg), g in zip(batch, got):
+        ref = O.train(x, k, O.Config(nepochs=cfg.nepochs, balance=cfg.balance, seed=cfg.seed), assigner=O.ASSIGNER_I8)
         assert np.array_equal(g.labels, ref.labels)
         assert g.objective == ref.objective
         assert g.n_splits == ref.n_splits
