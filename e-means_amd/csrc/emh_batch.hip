@@ -116,13 +116,14 @@ struct GraphGuard {
 static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_stats *stats) {
     if (!jobs || njobs == 0) return fail(EMH_E_ARG, "train_batch: need at least one job");
     if (njobs > 0xFFFFu) return fail(EMH_E_ARG, "train_batch: at most 65535 jobs per call");
-    if (h->op != OP_F32 && h->op != OP_BF16 && h->op != OP_FP8 && h->op != OP_I8)
-        return fail(EMH_E_UNSUPPORTED, "train_batch: the batched driver runs the FP32, BF16, FP8 and I8 searches");
+    if (h->op != OP_F32 && h->op != OP_BF16 && h->op != OP_FP8 && h->op != OP_I8 && h->op != OP_I16)
+        return fail(EMH_E_UNSUPPORTED, "train_batch: unknown search precision");
     const int op = h->op, M = (op == OP_BF16 || op == OP_FP8) ? h->M : 1;
     const bool cand_search = op == OP_BF16 || op == OP_FP8;
     // the I8 search takes i8 rows (AssignRows<i8>), the float searches f32
-    const int rows = op == OP_I8 ? EMH_ROWS_I8 : EMH_ROWS_F32;
-    const size_t rb = op == OP_I8 ? 1 : 4;
+    const bool int_search = op == OP_I8 || op == OP_I16;
+    const int rows = int_search ? EMH_ROWS_I8 : EMH_ROWS_F32;
+    const size_t rb = int_search ? 1 : 4;
     const auto t0 = std::chrono::steady_clock::now();
     int rc;
     if ((rc = check(hipSetDevice(h->device), "hipSetDevice"))) return rc;
@@ -141,8 +142,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         const emh_job &jb = jobs[j];
         if (jb.rows != rows) {
             init_rc[j] = EMH_E_UNSUPPORTED;
-            init_err[j] = op == OP_I8 ? "train_batch: the I8 search takes i8 rows only"
-                                      : "train_batch: the float searches take f32 rows only";
+            init_err[j] = int_search ? "train_batch: the integer searches take i8 rows only"
+                                     : "train_batch: the float searches take f32 rows only";
             return;
         }
         init_rc[j] = run[j].init(jb.x, jb.rows, jb.n, jb.d, jb.k, jb.cfg, jb.centroids_out, jb.labels_out, jb.out,
@@ -161,6 +162,13 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     for (size_t j = 0; j < njobs; j++) {
         const emh_job &jb = jobs[j];
         geo[j] = make_geom(op, jb.k, jb.d);
+        if (op == OP_I16) {
+            // I16 jobs run the LDS kernel's exact i64 scores, d streamed in
+            // chunks (as a single problem's rows past 512 bytes)
+            geo[j].reg = false;
+            geo[j].resident = false;
+            geo[j].row_bytes = CHUNK_BYTES;
+        }
         d_max = std::max(d_max, (size_t)jb.d);
         row_base[j] = slots;
         slots += run[j].R.batch_size;
@@ -173,7 +181,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         cpad_base[j] = cpad;
         cpad += geo[j].k_pad;
         op_base[j] = op_bytes;
-        op_bytes += geo[j].k_pad * op_row_bytes(geo[j]);
+        op_bytes += geo[j].reg ? geo[j].k_pad * (size_t)geo[j].row_bytes : operand_table_bytes(geo[j]);
         // jobs with operand rows past 512 bytes run the LDS kernel, unsplit
         slices[j] = geo[j].reg ? job_slices(geo[j]) : JobSlices{1u, 1u};
         part_base[j] = part_words;
@@ -221,7 +229,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(sums_d.ensure(cen_floats * 4), "sums")) || (rc = check(counts_d.ensure(nclusters * 4), "counts")) ||
         (rc = check(cen_op.ensure(op_bytes), "operand tables")) || (rc = check(norm32.ensure(cpad * 4), "norms")) ||
         (rc = check(hn.ensure(cpad * 4), "offsets")) ||
-        (op == OP_I8 && (rc = check(cn.ensure(cpad * 8), "integer offsets"))) ||
+        (int_search && (rc = check(cn.ensure(cpad * 8), "integer offsets"))) ||
         (op == OP_FP8 && ((rc = check(x8_d.ensure(x8_bytes), "fp8 rows")) ||
                           (rc = check(rscale_d.ensure(slots * 4), "fp8 row scales")) ||
                           (rc = check(rnorm_d.ensure(slots * 4), "fp8 row norms")))) ||
@@ -349,7 +357,8 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.cen_op = (uint8_t *)cen_op.p + op_base[j];
         t.norm32 = norm32.as<float>() + cpad_base[j];
         t.hn = hn.as<float>() + cpad_base[j];
-        t.cn = op == OP_I8 ? cn.as<int64_t>() + cpad_base[j] : nullptr;
+        t.cn = int_search ? cn.as<int64_t>() + cpad_base[j] : nullptr;
+        t.shift = (uint32_t)geo[j].shift;
         t.stats = stats_d.as<unsigned>() + 4 * j;
         t.labels = lab_d.as<uint32_t>() + row_base[j];
         t.labels_all = labels_all.as<uint32_t>() + nrow_base[j];

@@ -129,6 +129,7 @@ struct JobDev {
     float *row_scale;        // FP8: batch slot (power-of-two encoding scale)
     float *row_norm;         // FP8: batch slot (|x|^2 of the f32 row)
     uint32_t d, k, k_pad, d_pad, row_bytes, tb;
+    uint32_t shift;          // I16: centroids are round(c 2^shift)
     uint32_t nslice, tiles_per_slice;  // the search's centroid-range split (1: none)
     uint32_t row_base, cluster_base;  // offsets in the concatenated batch / cluster ranges
     // per step

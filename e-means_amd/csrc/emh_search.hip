@@ -899,6 +899,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
     // whole 16-byte quads per row from a 16-byte aligned table
     a.vec = (g.d * (g.op == OP_I8 ? 1 : 4)) % 16 == 0 && ((uintptr_t)j.x & 15) == 0;
     a.resident = g.resident ? 1 : 0;
+    a.shift = g.shift;
     a.labels = j.labels;
     a.dist = j.dist;
     a.cand = j.cand;
@@ -944,6 +945,56 @@ hipError_t launch_jobs_merge(int op, const JobDev *jobs, const uint2 *rmap, size
     return hipGetLastError();
 }
 
+// I16 jobs (the LDS kernel's row-major planes, as prep_i16_kernel writes a
+// single problem's): c16 = round(c 2^shift) clamped to i16 (NaN -> 0), the
+// low byte less 128 in plane 0 and the signed top byte in plane 1; the
+// offset sum c16^2 + round(bias 4^shift) (src/cpu_i8.rs:95-117), INT64_MAX
+// for pad centroids. A 16-lane group per centroid, 4 bytes a lane.
+__global__ __launch_bounds__(256) void prep_jobs_i16_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
+    const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
+    const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
+    if (e >= entries) return;
+    const uint2 cm = cmap[e];
+    const JobDev jb = job_at<false>(jobs, cm.x);
+    if (jb.m == 0) return;  // a finished job: nothing reads its tables this step
+    const size_t jc = cm.y, d = jb.d, d_pad = jb.d_pad;
+    const bool live = jc < jb.k;
+    const float *cp = jb.cen + jc * d;
+    const float scale = (float)(1u << jb.shift);
+    uint8_t *o = jb.cen_op + jc * d_pad;
+    const size_t plane = (size_t)jb.k_pad * d_pad;
+    int64_t part = 0;
+    for (size_t e4 = 4 * (size_t)gl; e4 < d_pad; e4 += 64) {
+        uint32_t lo = 0u, hi = 0u;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            int32_t q = 0;
+            if (live && e4 + t < d) {
+                float r = roundf(cp[e4 + t] * scale);
+                r = r != r ? 0.0f : fminf(fmaxf(r, -32768.0f), 32767.0f);
+                q = (int32_t)r;
+            }
+            lo |= ((uint32_t)((q & 255) - 128) & 0xFFu) << (8 * t);
+            hi |= ((uint32_t)(q >> 8) & 0xFFu) << (8 * t);
+            part += (int64_t)q * q;
+        }
+        *(uint32_t *)(o + e4) = lo;
+        *(uint32_t *)(o + plane + e4) = hi;
+    }
+    for (int off = 8; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
+    if (gl != 0) return;
+    int64_t v = INT64_MAX;
+    if (live) {
+        v = part;
+        if (jb.penalized) {
+            double r = round(ldexp((double)jb.bias[jc], 2 * (int)jb.shift));
+            r = r != r ? 0.0 : fmin(fmax(r, -9.2e18), 9.2e18);
+            v += (int64_t)r;
+        }
+    }
+    jb.cn[jc] = v;
+}
+
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s) {
     if (!entries) return hipSuccess;
     dim3 grid((unsigned)((entries + PREP_ENTRIES - 1) / PREP_ENTRIES));
@@ -954,6 +1005,7 @@ hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_
         hipLaunchKernelGGL(prep_jobs_kernel<OP_FP8>, grid, dim3(256), 0, s, jobs, cmap, entries);
     }
     else if (op == OP_I8) hipLaunchKernelGGL(prep_jobs_kernel<OP_I8>, grid, dim3(256), 0, s, jobs, cmap, entries);
+    else if (op == OP_I16) hipLaunchKernelGGL(prep_jobs_i16_kernel, grid, dim3(256), 0, s, jobs, cmap, entries);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }

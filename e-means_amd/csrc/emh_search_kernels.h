@@ -1394,9 +1394,13 @@ __global__ __launch_bounds__(256) void search_lds_jobs_kernel(const SearchArgs *
     }
     const size_t row0 = w.y;
     if (row0 >= a.m) return;  // whole workgroup: before any barrier
-    constexpr int ROWS = OP == OP_I8 ? EMH_ROWS_I8 : EMH_ROWS_F32;
-    if (__builtin_amdgcn_readfirstlane(a.resident)) search_lds_core<OP, ROWS, M, true>(a, row0);
-    else search_lds_core<OP, ROWS, M, false>(a, row0);
+    constexpr int ROWS = OP == OP_I8 || OP == OP_I16 ? EMH_ROWS_I8 : EMH_ROWS_F32;
+    if constexpr (OP == OP_I16) {
+        search_lds_core<OP, ROWS, M, false>(a, row0);  // I16 streams d in chunks
+    } else {
+        if (__builtin_amdgcn_readfirstlane(a.resident)) search_lds_core<OP, ROWS, M, true>(a, row0);
+        else search_lds_core<OP, ROWS, M, false>(a, row0);
+    }
 }
 
 // ------------------------------------------------------------- launchers ---

@@ -43,10 +43,11 @@ static hipError_t launch_lds_jobs_m(const void *args, const uint2 *wg, size_t nw
 
 hipError_t launch_jobs_search_lds_op(int op, int M, const void *args, const uint2 *wg, size_t nwg, size_t lds,
                                      hipStream_t s) {
-    if (op == OP_F32 || op == OP_I8) {
+    if (op == OP_F32 || op == OP_I8 || op == OP_I16) {
         if (M != 1) return hipErrorInvalidValue;
-        return op == OP_F32 ? launch_lds_jobs_m<OP_F32, 1>(args, wg, nwg, lds, s)
-                            : launch_lds_jobs_m<OP_I8, 1>(args, wg, nwg, lds, s);
+        if (op == OP_F32) return launch_lds_jobs_m<OP_F32, 1>(args, wg, nwg, lds, s);
+        if (op == OP_I8) return launch_lds_jobs_m<OP_I8, 1>(args, wg, nwg, lds, s);
+        return launch_lds_jobs_m<OP_I16, 1>(args, wg, nwg, lds, s);
     }
     if (op != OP_BF16 && op != OP_FP8) return hipErrorInvalidValue;
     switch (M) {

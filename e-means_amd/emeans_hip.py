@@ -362,11 +362,11 @@ class BatchStats(C.Structure):
 def train_batch(problems, assigner: HipAssigner | None = None):
     """Trains independent problems in lockstep, one captured-graph replay per
     training step (emh_train_batch). problems: (x, k, cfg) tuples with f32
-    rows (i8 rows for the I8 assigner), host arrays or torch CUDA tensors
-    used in place. Each result equals train(x, k, cfg, assigner) on the same
-    assigner precision. Returns (list of Output, BatchStats)."""
+    rows (i8 rows for the I8 and I16 assigners), host arrays or torch CUDA
+    tensors used in place. Each result equals train(x, k, cfg, assigner) on
+    the same assigner precision. Returns (list of Output, BatchStats)."""
     a = assigner or HipAssigner(FP32)
-    want = np.dtype(np.int8) if a.precision == I8 else np.dtype(np.float32)
+    want = np.dtype(np.int8) if a.precision in (I8, I16) else np.dtype(np.float32)
     kind = ROWS[want]
     keep, jobs = [], (Job * len(problems))()
     any_dev = False
@@ -374,7 +374,7 @@ def train_batch(problems, assigner: HipAssigner | None = None):
         dev = _device_rows(x)
         if dev is None:
             if want == np.int8 and np.asarray(x).dtype != want:
-                raise TypeError("train_batch with the I8 assigner takes i8 rows")
+                raise TypeError("train_batch with the integer assigners takes i8 rows")
             x = np.ascontiguousarray(x, want)
         elif dev[1] != kind:
             raise TypeError(f"train_batch takes {want.name} rows with this assigner")

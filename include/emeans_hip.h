@@ -279,9 +279,9 @@ int emh_last_tile(const emh_assigner *h, int *fragments, int *nslice);
  * hipGraph: upload of every job's centroid snapshot, bias and batch ids,
  * then the gather + search (+ FP32 rescore) + per-cluster sums of all jobs,
  * then one download. Precision FP32, BF16_RESCORE or FP8_RESCORE on f32
- * rows, or I8 on i8 rows, any d: jobs whose operand rows pass 512 bytes
- * (FP32: d > 128, BF16: d > 256, FP8 and I8: d > 512) run the LDS search
- * kernel in the same step. I8 jobs of at most 512-byte rows run the
+ * rows, or I8 or I16 on i8 rows, any d: jobs whose operand rows pass 512
+ * bytes (FP32: d > 128, BF16: d > 256, FP8 and I8: d > 512), and I16 jobs,
+ * run the LDS search kernel in the same step. I8 jobs of at most 512-byte rows run the
  * packed-key search only: a balance bias with |sum c8^2 + round(bias)| >=
  * 2^26 - d 2^15 stops the call with EMH_E_UNSUPPORTED (emh_train then takes
  * the i64 search). */

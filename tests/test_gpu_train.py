@@ -421,6 +421,25 @@ def test_batched_i8_wide_rows_equal_oracle(hip):
         assert np.array_equal(g.centroids, ref.centroids)
 
 
+def test_batched_i16_jobs_equal_oracle(hip):
+    """int8 rows with CentroidBits::I16 centroids in one lockstep batch (the
+    LDS kernel's exact i64 scores over two i8 planes): each job equals the
+    CPU oracle's I16 training bit for bit, balance on some, rows from 48 to
+    600 bytes."""
+    specs = [(3000, 48, 30, 0.1), (1500, 200, 20, 0.0), (1000, 600, 12, 0.1), (2500, 7, 9, 0.0)]
+    batch = []
+    for i, (n, d, k, bal) in enumerate(specs):
+        batch.append((_i8_blobs(n, d, 2 * k, 160 + i), k, E.Config(nepochs=2, balance=bal, seed=170 + i)))
+    print(f"batch i16: {len(batch)} jobs", flush=True)
+    got, _ = E.train_batch(batch, E.HipAssigner(E.I16))
+    for (x, k, cfg), g in zip(batch, got):
+        ref = O.train(x, k, O.Config(nepochs=cfg.nepochs, balance=cfg.balance, seed=cfg.seed), assigner=O.ASSIGNER_I16)
+        assert np.array_equal(g.labels, ref.labels)
+        assert g.objective == ref.objective
+        assert g.n_splits == ref.n_splits
+        assert np.array_equal(g.centroids, ref.centroids)
+
+
 def test_batched_i8_device_rows_equal_host_rows(hip):
     """A batched int8 job whose rows already sit in HBM (initial centroids
     gathered and widened on the device) ends as its host-rows twin."""
