@@ -63,7 +63,7 @@ struct emh_assigner {
     bool has_bias = false;
 
     // centroid side (set_centroids / set_bias)
-    emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cmax, stats;
+    emh::DevBuf cen_f32, cen_op, norm32, norm64, hn, cn, bias, cexp, stats;
     // the f32 centroid table the search reads: cen_f32, or (training's
     // per-step snapshot of a search without a rescore) the caller's device
     // table itself, valid until the next set_centroids

@@ -11,8 +11,8 @@
 
 namespace emh {
 
-// OP_I16: CentroidBits::I16, the centroid stored as three i8 planes (see
-// search_kernel).
+// OP_I16: CentroidBits::I16, the centroid stored as two i8 planes (see
+// search_lds_core).
 enum Op { OP_F32 = 0, OP_BF16 = 1, OP_FP8 = 2, OP_I8 = 3, OP_I16 = 4 };
 
 // Workgroup geometry of the search kernels: 4 waves, each owning 32 rows, and
@@ -71,15 +71,15 @@ inline const void *reg_operand(const SearchGeom &g, const void *cen_op) {
 }
 
 // Launchers (stream-ordered, no synchronisation).
-// stats (2 x u32, f32 bits): [0] max_j |c_j|^2 (set by the prep kernel),
-// [1] max(0, -min_j bias_j) (set by the bias kernel); both start at 0.
+// stats (u32 words, starting at 0): [0] unused, [1] max(0, -min_j bias_j)
+// as f32 bits (set by the bias kernel), [2] the I8 / I16 range flag.
 // cen_op holds operand_bytes(g): the prep launcher fills both layouts.
 // hn_out / cn_out: also write the unbiased score offsets (what
 // launch_set_bias without a bias writes) in the same pass.
+// cexp (FP8, k_pad ints): each centroid's MFMA scale (E8M0 of 1 / s_c).
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax_bits, unsigned *stats, hipStream_t s, float *hn_out = nullptr,
+                                 int *cexp, unsigned *stats, hipStream_t s, float *hn_out = nullptr,
                                  int64_t *cn_out = nullptr);
-hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s);
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
                            float *hn, int64_t *cn, unsigned *stats, hipStream_t s);
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
@@ -93,7 +93,7 @@ bool reg_path(const SearchGeom &g);
 // labels/dist or cand. The results do not depend on nslice.
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
+                         const float *row_scale, const float *row_norm, const int *cexp,
                          const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
                          int nslice = 1, uint32_t *part = nullptr, int fr = 2);
 int reg_tiles(const SearchGeom &g);
@@ -115,7 +115,8 @@ struct JobDev {
     uint8_t *cen_op;         // tile-major operand table (register-resident search)
     float *norm32, *hn;      // k_pad each
     int64_t *cn;             // I8: k_pad integer score offsets sum c8^2 + round(bias)
-    unsigned *stats;         // 4 words: [0] max |c|^2, [1] max -bias, [2] FP8: max finite |c| (f32 bits)
+    unsigned *stats;         // 4 words: [1] max -bias (f32 bits); the others unused
+    int *cexp;               // FP8: k_pad centroid MFMA scales (E8M0 of 1 / s_c)
     uint32_t *labels;        // batch slot
     float *dist;             // batch slot (true distances once the keys kernel has run)
     uint32_t *labels_all;    // n: the last pass's label of every row (device-resident)

@@ -276,14 +276,31 @@ __device__ inline float centroid_norm32(const float *cp, size_t d, int lane) {
 // of the rounded i8 centroid (src/cpu_i8.rs:83-93).
 constexpr int PREP_CENTROIDS = 16;
 
+// FP8: the power-of-two encoding scale s_c of one centroid, from its largest
+// finite |c| (every lane of the centroid's 16-lane group returns it; a dead
+// or all-zero centroid gets 1). Each centroid has its own scale, so a small
+// centroid keeps e4m3's full relative precision beside a large one.
+__device__ inline float centroid_fp8_scale(const float *cp, size_t d, bool live, int gl) {
+    float mx = 0.0f;
+    if (live)
+        for (size_t i = gl; i < d; i += 16) {
+            const float a = fabsf(cp[i]);
+            if (a > mx && isfinite(a)) mx = a;
+        }
+#pragma unroll
+    for (int off = 8; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    return fp8_scale_for(mx);
+}
+
 // tile_op (register-resident path, else nullptr): the same quads in the
 // tile-major order, quad q of centroid c of tile t at quad (t nq + q) tb + c.
 // hn_out / cn_out (else nullptr): the unbiased score offsets set_bias_kernel
 // would write, (|c|^2)/2 (+inf for pad centroids) or sum c8^2 (2^26).
+// cexp (FP8): each centroid's MFMA scale, the E8M0 value of 1 / s_c.
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
-                                                             int64_t *norm64, const unsigned *cmax_bits,
+                                                             int64_t *norm64, int *cexp,
                                                              unsigned *stats, uint8_t *tile_op, int tb,
                                                              float *hn_out, int64_t *cn_out) {
     constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
@@ -292,7 +309,10 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     const bool live = jc < k;
     const float *cp = c + jc * d;
     float cs = 1.0f;
-    if constexpr (OP == OP_FP8) cs = fp8_scale_for(__uint_as_float(*cmax_bits));
+    if constexpr (OP == OP_FP8) {
+        cs = centroid_fp8_scale(cp, d, live, gl);
+        if (gl == 0) cexp[jc] = e8m0_inverse(cs);
+    }
     // operand row: the float searches store -c, so that an MFMA accumulator
     // started at |c|^2/2 ends at |c|^2/2 - x.c
     uint8_t *orow = cen_op + jc * d_pad * EB;
@@ -358,18 +378,6 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
             if (gl == 0) norm32[jc] = nv;
         }
         if (gl == 0 && hn_out) hn_out[jc] = live ? __fmul_rn(0.5f, nv) : INFINITY;
-        // max |c|^2 (the candidate searches' positivity offset): one atomic
-        // per workgroup, not per centroid; NaN norms are skipped
-        if constexpr (OP == OP_BF16 || OP == OP_FP8) {
-            __shared__ unsigned gmax[PREP_CENTROIDS];
-            if (gl == 0) gmax[threadIdx.x >> 4] = (live && nv >= 0.0f) ? __float_as_uint(nv) : 0u;
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                unsigned mx = 0u;
-                for (int t = 0; t < PREP_CENTROIDS; t++) mx = max(mx, gmax[t]);
-                if (mx) atomicMax(&stats[0], mx);
-            }
-        }
     }
 }
 
@@ -379,9 +387,9 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
 // the order of centroid_norm32 (lane l: elements l, l + 16, ...; then the 16
 // partial sums in order; then the tail), and sets the score offset
 // (|c|^2 + bias) / 2 with this step's bias, as prep_centroids_kernel +
-// set_bias_kernel do for one job. The jobs' stats (max |c|^2, max -bias)
-// take one atomic per job per workgroup: cmap lists each job's entries
-// together. I8: the rounded, clamped i8 centroid (src/cpu_i8.rs:83) as the
+// set_bias_kernel do for one job; FP8: each centroid's own encoding scale.
+// The jobs' stats word [1] (max -bias) takes one atomic per job per
+// workgroup: cmap lists each job's entries together. I8: the rounded, clamped i8 centroid (src/cpu_i8.rs:83) as the
 // operand, its exact sum c8^2 plus the rounded bias (src/cpu_i8.rs:112-117)
 // as the integer offset, 2^26 for pad centroids (as prep_centroids_kernel
 // and set_bias_kernel write them); no stats.
@@ -391,9 +399,9 @@ template <int OP>
 __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
     constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
     const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    __shared__ uint32_t s_job[PREP_ENTRIES], s_norm[PREP_ENTRIES], s_bias[PREP_ENTRIES];
+    __shared__ uint32_t s_job[PREP_ENTRIES], s_bias[PREP_ENTRIES];
     const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
-    uint32_t job = NO_LABEL, nbits = 0u, bbits = 0u;
+    uint32_t job = NO_LABEL, bbits = 0u;
     if (e < entries) {
         const uint2 cm = cmap[e];
         const JobDev jb = job_at<false>(jobs, cm.x);  // groups of a wave may belong to different jobs
@@ -406,8 +414,12 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
             const size_t nq = jb.row_bytes / 16;
             int64_t part64 = 0;
-            // FP8: the encoding scale of the job's largest finite |c| (stats[2])
-            [[maybe_unused]] const float cs = OP == OP_FP8 ? fp8_scale_for(__uint_as_float(jb.stats[2])) : 1.0f;
+            // FP8: the centroid's own encoding scale s_c
+            [[maybe_unused]] float cs = 1.0f;
+            if constexpr (OP == OP_FP8) {
+                cs = centroid_fp8_scale(cp, d, live, gl);
+                if (gl == 0) jb.cexp[jc] = e8m0_inverse(cs);
+            }
             for (size_t q = gl; q < nq; q += 16) {
                 uint32_t w[4];
 #pragma unroll
@@ -473,7 +485,6 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
                 for (size_t i = body; i < d; i++) tail = __fadd_rn(tail, __fmul_rn(cp[i], cp[i]));
                 nv = __fadd_rn(nv, tail);
                 job = cm.x;
-                nbits = nv >= 0.0f ? __float_as_uint(nv) : 0u;  // NaN norms are skipped
                 float v = nv;
                 if (jb.penalized) {
                     const float bv = jb.bias[jc];
@@ -490,69 +501,19 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
     if constexpr (OP == OP_I8) return;  // whole workgroup: no stats
     if (gl == 0) {
         s_job[slot] = job;
-        s_norm[slot] = nbits;
         s_bias[slot] = bbits;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t cur = NO_LABEL, mn = 0u, mb = 0u;
+        uint32_t cur = NO_LABEL, mb = 0u;
         for (int t = 0; t <= PREP_ENTRIES; t++) {
             const uint32_t jt = t < PREP_ENTRIES ? s_job[t] : NO_LABEL;
             if (jt != cur) {
-                if (cur != NO_LABEL) {
-                    if (mn) atomicMax(&jobs[cur].stats[0], mn);
-                    if (mb) atomicMax(&jobs[cur].stats[1], mb);
-                }
+                if (cur != NO_LABEL && mb) atomicMax(&jobs[cur].stats[1], mb);
                 cur = jt;
-                mn = mb = 0u;
+                mb = 0u;
             }
-            if (t < PREP_ENTRIES) {
-                mn = max(mn, s_norm[t]);
-                mb = max(mb, s_bias[t]);
-            }
-        }
-    }
-}
-
-// FP8 jobs: each job's largest finite |c| over its k x d centroids (what
-// absmax_kernel takes for one problem: the centroid encoding scale), as f32
-// bits into the job's stats[2] (zeroed before); 16 lanes per (job, centroid)
-// entry of cmap, one atomic per job per workgroup.
-__global__ __launch_bounds__(256) void jobs_absmax_kernel(const JobDev *jobs, const uint2 *cmap, size_t entries) {
-    const int gl = threadIdx.x & 15, slot = threadIdx.x >> 4;
-    __shared__ uint32_t s_job[PREP_ENTRIES], s_max[PREP_ENTRIES];
-    const size_t e = (size_t)blockIdx.x * PREP_ENTRIES + slot;
-    uint32_t job = NO_LABEL;
-    float mx = 0.0f;
-    if (e < entries) {
-        const uint2 cm = cmap[e];
-        const JobDev jb = job_at<false>(jobs, cm.x);
-        if (jb.m != 0 && cm.y < jb.k) {  // a finished job: nothing reads its tables this step
-            const float *cp = jb.cen + (size_t)cm.y * jb.d;
-            for (size_t i = gl; i < jb.d; i += 16) {
-                const float a = fabsf(cp[i]);
-                if (a > mx && isfinite(a)) mx = a;
-            }
-            job = cm.x;
-        }
-    }
-#pragma unroll
-    for (int off = 8; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if (gl == 0) {
-        s_job[slot] = job;
-        s_max[slot] = __float_as_uint(mx);
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t cur = NO_LABEL, m = 0u;
-        for (int t = 0; t <= PREP_ENTRIES; t++) {
-            const uint32_t jt = t < PREP_ENTRIES ? s_job[t] : NO_LABEL;
-            if (jt != cur) {
-                if (cur != NO_LABEL && m) atomicMax(&jobs[cur].stats[2], m);
-                cur = jt;
-                m = 0u;
-            }
-            if (t < PREP_ENTRIES && jt != NO_LABEL) m = max(m, s_max[t]);
+            if (t < PREP_ENTRIES) mb = max(mb, s_bias[t]);
         }
     }
 }
@@ -584,23 +545,6 @@ __global__ __launch_bounds__(256) void prep_i16_kernel(const float *c, size_t k,
     }
     for (int off = 32; off > 0; off >>= 1) part += shfl_xor_i64(part, off);
     if (lane == 0 && live) norm64[jc] = part;
-}
-
-// max |v| over the finite values, one atomic per workgroup (256 threads)
-__global__ __launch_bounds__(256) void absmax_kernel(const float *v, size_t count, unsigned *out_bits) {
-    __shared__ float wmax[4];
-    float mx = 0.0f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x) {
-        float a = fabsf(v[i]);
-        if (a > mx && isfinite(a)) mx = a;
-    }
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        mx = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        if (mx > 0.0f) atomicMax(out_bits, __float_as_uint(mx));
-    }
 }
 
 // hn_j = (|c_j|^2 + bias_j) / 2 for the float searches; for I8
@@ -761,7 +705,7 @@ size_t slice_part_bytes(int M, size_t m, int nslice, int op) {
 
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const float *row_norm, const unsigned *cmax_bits,
+                         const float *row_scale, const float *row_norm, const int *cexp,
                          const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
                          int nslice, uint32_t *part, int fr) {
     if (m == 0) return hipSuccess;
@@ -794,7 +738,7 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
     a.cn = cn;
     a.x8 = x8;
     a.row_scale = row_scale;
-    a.cmax_bits = cmax_bits;
+    a.cexp = cexp;
     a.labels = labels;
     a.dist = dist;
     a.cand = cand;
@@ -852,7 +796,7 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 }
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 const unsigned *cmax, unsigned *stats, hipStream_t s, float *hn_out,
+                                 int *cexp, unsigned *stats, hipStream_t s, float *hn_out,
                                  int64_t *cn_out) {
     const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;
@@ -861,10 +805,10 @@ hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_
     uint8_t *tile = g.reg && g.op != OP_I16 ? (uint8_t *)reg_operand(g, cen_op) : nullptr;
     const int tbc = g.reg ? reg_tile(g) : 1;
     switch (g.op) {
-        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
-        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
-        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
-        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cmax, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_F32: hipLaunchKernelGGL(prep_centroids_kernel<OP_F32>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cexp, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_BF16: hipLaunchKernelGGL(prep_centroids_kernel<OP_BF16>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cexp, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_FP8: hipLaunchKernelGGL(prep_centroids_kernel<OP_FP8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cexp, stats, tile, tbc, hn_out, cn_out); break;
+        case OP_I8: hipLaunchKernelGGL(prep_centroids_kernel<OP_I8>, grid, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, op, norm32, norm64, cexp, stats, tile, tbc, hn_out, cn_out); break;
         case OP_I16: hipLaunchKernelGGL(prep_i16_kernel, grid_wave, dim3(256), 0, s, c, g.k, g.d, g.d_pad, g.k_pad, g.shift, op, norm64); break;
         default: return hipErrorInvalidValue;
     }
@@ -910,7 +854,7 @@ void fill_search_args(void *dst, const SearchGeom &g, const JobDev &j) {
         a.x8 = j.x8;
         a.row_scale = j.row_scale;
         a.row_norm = j.row_norm;
-        a.cmax_bits = j.stats + 2;
+        a.cexp = j.cexp;
     }
     memcpy(dst, &a, sizeof a);
 }
@@ -1000,10 +944,7 @@ hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_
     dim3 grid((unsigned)((entries + PREP_ENTRIES - 1) / PREP_ENTRIES));
     if (op == OP_F32) hipLaunchKernelGGL(prep_jobs_kernel<OP_F32>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_BF16) hipLaunchKernelGGL(prep_jobs_kernel<OP_BF16>, grid, dim3(256), 0, s, jobs, cmap, entries);
-    else if (op == OP_FP8) {
-        hipLaunchKernelGGL(jobs_absmax_kernel, grid, dim3(256), 0, s, jobs, cmap, entries);
-        hipLaunchKernelGGL(prep_jobs_kernel<OP_FP8>, grid, dim3(256), 0, s, jobs, cmap, entries);
-    }
+    else if (op == OP_FP8) hipLaunchKernelGGL(prep_jobs_kernel<OP_FP8>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_I8) hipLaunchKernelGGL(prep_jobs_kernel<OP_I8>, grid, dim3(256), 0, s, jobs, cmap, entries);
     else if (op == OP_I16) hipLaunchKernelGGL(prep_jobs_i16_kernel, grid, dim3(256), 0, s, jobs, cmap, entries);
     else return hipErrorInvalidValue;
@@ -1040,13 +981,6 @@ hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, siz
         case 8: hipLaunchKernelGGL(rescore_jobs_kernel<8>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_absmax(const float *v, size_t count, unsigned *out_bits, hipStream_t s) {
-    unsigned blocks = (unsigned)std::min<size_t>((count + 255) / 256, 512);
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(absmax_kernel, dim3(blocks), dim3(256), 0, s, v, count, out_bits);
     return hipGetLastError();
 }
 

@@ -73,6 +73,10 @@ __device__ inline float fp8_scale_for(float amax) {
     return ldexpf(1.0f, p);
 }
 
+// The E8M0 MFMA scale value of 1 / s for a power-of-two s (a normal float):
+// 127 - log2(s), as the biased exponent 254 - exponent bits of s.
+__device__ inline int e8m0_inverse(float s) { return 254 - (int)((__float_as_uint(s) >> 23) & 0xff); }
+
 struct SearchArgs {
     const void *x;
     const uint32_t *ids;
@@ -83,8 +87,8 @@ struct SearchArgs {
     const int64_t *cn;
     const uint8_t *x8;
     const float *row_scale;
-    const unsigned *cmax_bits;
-    // [0]: max_j |c_j|^2, [1]: max(0, -min_j bias_j), both as f32 bits
+    const int *cexp;  // FP8: k_pad E8M0 MFMA scales, 1 / s_c of each centroid's encoding scale
+    // [1]: max(0, -min_j bias_j) as f32 bits
     const unsigned *stats;
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
     int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
@@ -387,6 +391,7 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
     uint8_t *sC = smem + BM * RS;
     float *sHn = (float *)(sC + NPL * BN * RS);
     int64_t *sCn = (int64_t *)(sC + NPL * BN * RS);
+    int *sCe = (int *)(sHn + BN);  // FP8: the block's centroid MFMA scales
     const int segs = RB / 16;
     const int nchunks = RES ? 1 : (int)((a.d_pad * ES) / RB);
     const size_t my_row = row0 + wave * 32 + j;
@@ -412,6 +417,7 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
         if (tid < BN) {
             if constexpr (INT) sCn[tid] = a.cn[c0 + tid];
             else sHn[tid] = a.hn[c0 + tid];
+            if constexpr (OP == OP_FP8) sCe[tid] = a.cexp[c0 + tid];
         }
     };
 
@@ -427,11 +433,8 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
     // Squared row norm (f32 for FP32, exact int for I8), accumulated from LDS.
     float xn32 = 0.0f;
     int32_t xn_i = 0, xs_i = 0;  // I16: the row's element sum (half-row partial)
-    float srs = 1.0f;
-    if constexpr (OP == OP_FP8) {
-        float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
-        srs = (my_row < a.m ? a.row_scale[my_row] : 1.0f) * cs;
-    }
+    // FP8: the E8M0 scale 1 / s_row of this lane's row (its MFMA B column)
+    [[maybe_unused]] const int sbr = OP == OP_FP8 && my_row < a.m ? e8m0_inverse(a.row_scale[my_row]) : 127;
     auto norm_part = [&]() {
         const uint8_t *p = sR + (wave * 32 + j) * RS + h * (RB / 2);
         for (int b = 0; b < RB / 2; b += 16) {
@@ -518,7 +521,8 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
                         pacc1[pl] = __builtin_amdgcn_mfma_i32_32x32x32_i8(cb, xb, pacc1[pl], 0, 0, 0);
                     }
                 }
-            } else {  // OP_FP8
+            } else {  // OP_FP8: scales 1 / s_C (this lane's centroid) and 1 / s_row
+                const int sa0 = sCe[j], sa1 = sCe[32 + j];
                 for (int b = 0; b < RB / 2; b += LB) {
                     uint4 x0 = *(const uint4 *)(pb + b), x1 = *(const uint4 *)(pb + b + 16);
                     uint4 a0 = *(const uint4 *)(pa0 + b), a1 = *(const uint4 *)(pa0 + b + 16);
@@ -526,8 +530,8 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
                     i32x8 xb = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
                     i32x8 ca = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
                     i32x8 cb = {(int)b0.x, (int)b0.y, (int)b0.z, (int)b0.w, (int)b1.x, (int)b1.y, (int)b1.z, (int)b1.w};
-                    acc0 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ca, xb, acc0, 0, 0, 0, 127, 0, 127);
-                    acc1 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cb, xb, acc1, 0, 0, 0, 127, 0, 127);
+                    acc0 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(ca, xb, acc0, 0, 0, 0, sa0, 0, sbr);
+                    acc1 = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cb, xb, acc1, 0, 0, 0, sa1, 0, sbr);
                 }
             }
             if (ch == nchunks - 1) {
@@ -551,12 +555,9 @@ __device__ __forceinline__ void search_lds_core(const SearchArgs &a, const size_
                             // x.c16 (mod 2^32, as the reference's i32 sum)
                             const uint32_t dv = (uint32_t)pp[0][r] + ((uint32_t)pp[1 % NPL][r] << 8) + xs128;
                             s = sCn[il] - ((int64_t)2 << a.shift) * (int64_t)(int32_t)dv;
-                        } else if constexpr (OP == OP_FP8) {
-                            // operands are -c * s_C and x * s_row: acc ~ -s_row s_C x.c
-                            float dv = nb == 0 ? acc0[r] : acc1[r];
-                            s = fmaf(srs, sHn[il], dv);
                         } else {
-                            // operands are -c and x: acc = -x.c
+                            // operands are -c and x (FP8: -c s_c and x s_row,
+                            // scaled back by the MFMA): acc ~ -x.c
                             float dv = nb == 0 ? acc0[r] : acc1[r];
                             s = sHn[il] + dv;
                         }
@@ -628,11 +629,16 @@ __global__ __launch_bounds__(256) void search_kernel(SearchArgs a) {
 //
 // BF16 / FP8 (candidate search): the accumulators start at
 // |c|^2/2 + bias/2 + B, where B is a per-workgroup offset keeping every score
-// positive (B >= max |x|^2/2 plus a bound on the operands' rounding error),
-// so a score's f32 bits order like the score. Each lane replaces the low 5
-// bits of its 16*NB scores of a tile by their position, takes one integer
-// min (value and position at once) and inserts that tile minimum into its
-// running top-m; the rescoring kernel settles the exact FP32 order.
+// positive, so a score's f32 bits order like the score. The operands'
+// rounding is relative (FP8: each row and each centroid has its own
+// power-of-two scale), so |x~||c~| <= (1 + e) |x||c| with e = 2^-7 for
+// BF16 (2^-8 suffices) and 2^-2 for FP8 (2^-4 per e4m3 operand, plus its
+// subnormals, below 2^-3): over all |c| = t the score is at least
+// t^2/2 - (1 + e)|x|t >= -(1 + e)^2 |x|^2 / 2, and B = (1 + e)^2 (1 + 2^-6)
+// max|x|^2 / 2 + max(-bias) / 2 holds whatever the centroids' sizes.
+// Each lane replaces the low 5 bits of its 16*NB scores of a tile by their
+// position and keeps the m smallest such keys of its half of the row; the
+// rescoring kernel settles the exact FP32 order.
 //
 // FP32 (exact): accumulators start at (|c|^2 + bias)/2 and the per-lane
 // running argmin takes every score with a strict '<' in ascending index.
@@ -657,6 +663,10 @@ template <int OP> using RegAcc = typename std::conditional<OP == OP_I8 || OP == 
 // its two 32-row blocks (block b = plane b).
 template <int OP, int NB> constexpr int reg_centroids() { return OP == OP_I16 ? 32 : 32 * NB; }
 
+// Words of a tile's offset slot in LDS: 256 score offsets (TB <= 256), and
+// for FP8 then 256 centroid scales.
+template <int OP> constexpr int reg_offset_slot() { return OP == OP_FP8 ? 512 : 256; }
+
 // Operand quads a lane reads ahead of the MFMAs that use them.
 template <int OP> struct RegPrefetch { static constexpr int D = (OP == OP_BF16 || OP == OP_I8 || OP == OP_I16) ? 2 : 1; };
 
@@ -664,7 +674,8 @@ template <int OP> struct RegPrefetch { static constexpr int D = (OP == OP_BF16 |
 // fragments. sT is the tile in LDS, sH its per-centroid offsets.
 template <int OP, int NS, int FR, int NB>
 __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, const uint4 (&xb)[FR][NS],
-                                         const int (&sb)[FR], int sa, int h, int j, RegAcc<OP> (&acc)[FR][NB]) {
+                                         const int (&sb)[FR], const int (&sa)[NB], int h, int j,
+                                         RegAcc<OP> (&acc)[FR][NB]) {
     constexpr int TB = 32 * NB;
     constexpr int QS = TB * 16;  // bytes from one quad of a centroid to its next
     using Acc = RegAcc<OP>;
@@ -747,7 +758,7 @@ __device__ __forceinline__ void reg_mfma(const uint8_t *sT, const float *sH, con
 #pragma unroll
                 for (int nb = 0; nb < NB; nb++)
                     acc[fr][nb] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv[nb], xv, s == 0 ? init[nb] : acc[fr][nb],
-                                                                                  0, 0, 0, sa, 0, sb[fr]);
+                                                                                  0, 0, 0, sa[nb], 0, sb[fr]);
             }
         }
     }
@@ -845,7 +856,7 @@ __device__ __forceinline__ void reg_fold(const RegAcc<OP> (&acc)[FR][NB], const 
 // reg_mfma + reg_fold.
 template <int OP, int M, int NS, int FR>
 __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *sH, const uint4 (&xb)[FR][NS],
-                                               const int (&sb)[FR], int sa, int h, int j, uint32_t c0,
+                                               const int (&sb)[FR], const int (&sa)[2], int h, int j, uint32_t c0,
                                                f32x16 (&acc)[FR][2], uint32_t (&bs)[FR][M],
                                                uint32_t (&bi)[FR][M]) {
     constexpr int TB = 64, QS = TB * 16;
@@ -854,12 +865,12 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
     const uint8_t *ab = sT + (h * NS * TB + j) * 16;
     auto ld = [&](int q, int nb) -> uint4 { return *(const uint4 *)(ab + q * QS + nb * 512); };
     auto key = [&](float v, uint32_t pos) -> uint32_t { return (__float_as_uint(v) & 0xFFFFFFE0u) | pos; };
-    auto mfma = [&](int st, int fr, const uint4 (&ops)[NS], const f32x16 &c) -> f32x16 {
+    auto mfma = [&](int st, int fr, int nb, const uint4 (&ops)[NS], const f32x16 &c) -> f32x16 {
         if constexpr (OP == OP_FP8) {
             const uint4 a0 = ops[2 * st], a1 = ops[2 * st + 1], x0 = xb[fr][2 * st], x1 = xb[fr][2 * st + 1];
             const i32x8 cv = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w, (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
             const i32x8 xv = {(int)x0.x, (int)x0.y, (int)x0.z, (int)x0.w, (int)x1.x, (int)x1.y, (int)x1.z, (int)x1.w};
-            return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, xv, c, 0, 0, 0, sa, 0, sb[fr]);
+            return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(cv, xv, c, 0, 0, 0, sa[nb], 0, sb[fr]);
         } else {
             return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ops[st]),
                                                            __builtin_bit_cast(bf16x8, xb[fr][st]), c, 0, 0, 0);
@@ -878,7 +889,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
 #pragma unroll
         for (int st = 0; st < STEPS; st++)
 #pragma unroll
-            for (int fr = 0; fr < FR; fr++) acc[fr][0] = mfma(st, fr, ops, st == 0 ? init : acc[fr][0]);
+            for (int fr = 0; fr < FR; fr++) acc[fr][0] = mfma(st, fr, 0, ops, st == 0 ? init : acc[fr][0]);
     }
     {  // block 1, block 0 folded in between
         uint4 ops[NS];
@@ -892,7 +903,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
 #pragma unroll
         for (int st = 0; st < STEPS; st++) {
 #pragma unroll
-            for (int fr = 0; fr < FR; fr++) acc[fr][1] = mfma(st, fr, ops, st == 0 ? init : acc[fr][1]);
+            for (int fr = 0; fr < FR; fr++) acc[fr][1] = mfma(st, fr, 1, ops, st == 0 ? init : acc[fr][1]);
             if constexpr (M == 1) {
 #pragma unroll
                 for (int fr = 0; fr < FR; fr++)
@@ -947,7 +958,7 @@ __device__ __forceinline__ void reg_tile_cand2(const uint8_t *sT, const float *s
 
 // One workgroup's sweep: rows [row0, row0 + 128 FR) of a.x against every
 // centroid tile. LDS: three tile slots sC0..2 (TILE bytes each), three
-// offset slots sH0..2 (256 floats) and sRed (4 floats).
+// offset slots sH0..2 (reg_offset_slot words) and sRed (4 floats).
 template <int OP, int ROWS, int M, int NS, int FR, int NB>
 __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_t row0, const int t0, const int t1,
                                                 const int slice, uint8_t *sC0, uint8_t *sC1, uint8_t *sC2,
@@ -1027,21 +1038,16 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         }
     }
     float B = 0.0f;
-    int sa = 127;
     if constexpr (CAND) {
         for (int off = 16; off > 0; off >>= 1) nmax = fmaxf(nmax, __shfl_xor(nmax, off, 64));
         if (lane == 0) sRed[wave] = nmax;
         __syncthreads();
         nmax = fmaxf(fmaxf(sRed[0], sRed[1]), fmaxf(sRed[2], sRed[3]));
-        const float cmaxn = __uint_as_float(a.stats[0]);
         const float negb = __uint_as_float(a.stats[1]);
-        constexpr float coef = OP == OP_FP8 ? 0.25f : 0.0078125f;
-        // (1 + 2^-6) covers the bf16 rounding of the norm itself
-        B = 0.5078125f * nmax + coef * sqrtf(nmax * cmaxn) + 0.5f * negb + 1.0f;
-        if constexpr (OP == OP_FP8) {
-            const float cs = fp8_scale_for(__uint_as_float(*a.cmax_bits));
-            sa = 254 - (int)((__float_as_uint(cs) >> 23) & 0xff);
-        }
+        constexpr float e = OP == OP_FP8 ? 0.25f : 0.0078125f;
+        // (1 + 2^-6) covers the bf16 rounding of the norm itself; FLT_MIN
+        // keeps B positive for all-zero rows
+        B = 0.5078125f * (1.0f + e) * (1.0f + e) * nmax + 0.5f * negb + 1.17549435e-38f;
     }
 
     // Tile DMA: wave-load q of wave w copies tile bytes [(4q + w) KiB, +1 KiB)
@@ -1073,6 +1079,15 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         const size_t ci = (size_t)t * TB + (tid & (TB - 1));
         if constexpr (OP == OP_I8) return ((const int32_t *)a.cn)[2 * ci];  // little-endian low word
         else return a.hn[ci];
+    };
+    // FP8: each centroid's E8M0 scale 1 / s_c, loaded and stored raw the
+    // same way, into words [256, 512) of the offset slot
+    auto eload = [&](int t) -> int {
+        if constexpr (OP == OP_FP8) return a.cexp[(size_t)t * TB + (tid & (TB - 1))];
+        else return 0;
+    };
+    auto estore = [&](float *dst, int v) {
+        if constexpr (OP == OP_FP8) ((int *)dst)[256 + tid] = v;
     };
     // every thread stores (slots past TB are never read): no branch, so the
     // compiler's wait tracking stays exact across the stage
@@ -1113,7 +1128,9 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
     // this wave's outstanding loads, which complete in order).
     using Acc = RegAcc<OP>;
     Acc acc[FR][NB];
-    constexpr int WAIT_ONE = (PER_T + 1) | 0x0F70;  // vmcnt(PER_T + 1): the newest tile may stay in flight
+    // vmcnt(PER_T + 1, FP8 + 2): the newest tile's DMA and offset (and
+    // scale) loads may stay in flight
+    constexpr int WAIT_ONE = (PER_T + (OP == OP_FP8 ? 2 : 1)) | 0x0F70;
     // Workgroup barrier that waits for LDS traffic only: __syncthreads()'s
     // release fence would also wait for the newest tile's DMA. The DMA each
     // wave needs next has landed by its own vmcnt wait before this point.
@@ -1124,22 +1141,29 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         asm volatile("" ::: "memory");
     };
     HRaw hv0 = hload(t0), hv1 = 0, hv2 = 0;
+    int ev0 = eload(t0), ev1 = 0, ev2 = 0;
     dma(sC0, t0);
     dma(sC1, t0 + 1 < t1 ? t0 + 1 : t0);
     hv1 = hload(t0 + 1 < t1 ? t0 + 1 : t0);
+    ev1 = eload(t0 + 1 < t1 ? t0 + 1 : t0);
     __builtin_amdgcn_s_waitcnt(WAIT_ONE);
     hstore(sH0, hv0);
+    estore(sH0, ev0);
     __syncthreads();
     // One stage: compute tile T from (BC, BH); start tile T+2 into BN2;
     // publish tile T+1's offsets into BH1. The LDS arrays are named directly
     // so the compiler can tell the buffer being read from the ones in flight.
     // Every stage issues the same loads (past the end it re-fetches the last
     // tile into the free buffer), so the wait counts are the same each time.
-#define EMH_STAGE(T, BC, BH, BN2, BH1, HV1, HV2)                                   \
+#define EMH_STAGE(T, BC, BH, BN2, BH1, HV1, HV2, EV1, EV2)                         \
     {                                                                             \
         const int tn = (T) + 2 < t1 ? (T) + 2 : t1 - 1;                           \
         dma(BN2, tn);                                                             \
         HV2 = hload(tn);                                                          \
+        EV2 = eload(tn);                                                          \
+        int sa[NB];                                                               \
+        _Pragma("unroll") for (int nb = 0; nb < NB; nb++)                         \
+            sa[nb] = OP == OP_FP8 ? ((const int *)(BH))[256 + nb * 32 + j] : 127; \
         if constexpr (CAND && NB == 2) {                                          \
             reg_tile_cand2<OP, M, NS, FR>(BC, BH, xb, sb, sa, h, j, (uint32_t)((T) * CT), acc, bs, bi); \
         } else {                                                                  \
@@ -1149,14 +1173,15 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         __builtin_amdgcn_sched_barrier(0); /* keep the wait after the compute */  \
         __builtin_amdgcn_s_waitcnt(WAIT_ONE);                                     \
         hstore(BH1, HV1);                                                         \
+        estore(BH1, EV1);                                                         \
         tile_barrier();                                                           \
     }
     for (int t = t0; t < t1; t += 3) {
-        EMH_STAGE(t, sC0, sH0, sC2, sH1, hv1, hv2);
+        EMH_STAGE(t, sC0, sH0, sC2, sH1, hv1, hv2, ev1, ev2);
         if (t + 1 >= t1) break;
-        EMH_STAGE(t + 1, sC1, sH1, sC0, sH2, hv2, hv0);
+        EMH_STAGE(t + 1, sC1, sH1, sC0, sH2, hv2, hv0, ev2, ev0);
         if (t + 2 >= t1) break;
-        EMH_STAGE(t + 2, sC2, sH2, sC1, sH0, hv0, hv1);
+        EMH_STAGE(t + 2, sC2, sH2, sC1, sH0, hv0, hv1, ev0, ev1);
     }
 #undef EMH_STAGE
 
@@ -1242,7 +1267,7 @@ __device__ __forceinline__ void search_reg_body(const SearchArgs &a, const size_
     __shared__ __align__(16) uint8_t sC0[TILE];
     __shared__ __align__(16) uint8_t sC1[TILE];
     __shared__ __align__(16) uint8_t sC2[TILE];
-    __shared__ __align__(16) float sH0[256], sH1[256], sH2[256];
+    __shared__ __align__(16) float sH0[reg_offset_slot<OP>()], sH1[reg_offset_slot<OP>()], sH2[reg_offset_slot<OP>()];
     __shared__ float sRed[4];
     search_reg_core<OP, ROWS, M, NS, FR, NB>(a, row0, t0, t1, slice, sC0, sC1, sC2, sH0, sH1, sH2, sRed);
 }
@@ -1322,7 +1347,7 @@ constexpr int JOBS_TILE_MAX = 16384;  // 64 centroids x 256 B, or 32 x 512 B
 
 template <int OP, int M, int NS>
 __device__ __forceinline__ void jobs_sweep(const SearchArgs &a, size_t row0, int s, uint8_t (*sC)[JOBS_TILE_MAX],
-                                           float (*sH)[256], float *sRed) {
+                                           float (*sH)[reg_offset_slot<OP>()], float *sRed) {
     constexpr int FR = 2, NB = NS <= 8 ? 2 : 1;
     static_assert(32 * NB * 32 * NS <= JOBS_TILE_MAX, "tile");
     const int ntiles = (int)(a.k_pad / (32 * NB));
@@ -1334,7 +1359,7 @@ __device__ __forceinline__ void jobs_sweep(const SearchArgs &a, size_t row0, int
 template <int OP, int M>
 __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *jobs, const uint2 *wg) {
     __shared__ __align__(16) uint8_t sC[3][JOBS_TILE_MAX];
-    __shared__ __align__(16) float sH[3][256];
+    __shared__ __align__(16) float sH[3][reg_offset_slot<OP>()];
     __shared__ float sRed[4];
     const uint2 w = wg[blockIdx.x];
     // one job per workgroup: its table pointers are uniform and global
@@ -1352,7 +1377,7 @@ __global__ __launch_bounds__(256) void search_reg_jobs_kernel(const SearchArgs *
         a.x8 = global_ptr_s(a.x8);
         a.row_scale = global_ptr_s(a.row_scale);
         a.row_norm = global_ptr_s(a.row_norm);
-        a.cmax_bits = global_ptr_s(a.cmax_bits);
+        a.cexp = global_ptr_s(a.cexp);
     }
     const size_t row0 = w.y & ~255u;
     if (row0 >= a.m) return;  // whole workgroup: before any barrier
@@ -1390,7 +1415,7 @@ __global__ __launch_bounds__(256) void search_lds_jobs_kernel(const SearchArgs *
         a.x8 = global_ptr_s(a.x8);
         a.row_scale = global_ptr_s(a.row_scale);
         a.row_norm = global_ptr_s(a.row_norm);
-        a.cmax_bits = global_ptr_s(a.cmax_bits);
+        a.cexp = global_ptr_s(a.cexp);
     }
     const size_t row0 = w.y;
     if (row0 >= a.m) return;  // whole workgroup: before any barrier

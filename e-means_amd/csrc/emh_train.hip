@@ -65,7 +65,7 @@ static std::vector<StepParam> step_params(const emh_recipe &R) {
 // holds a freed address: it is captured again once any of these has moved.
 static std::vector<const void *> step_buffers(const emh_assigner *h) {
     return {h->cen_f32.p, h->cen_op.p,    h->norm32.p, h->norm64.p,    h->hn.p,       h->cn.p,
-            h->cmax.p,    h->stats.p,     h->bias.p,   h->cand.p,      h->x8.p,       h->row_scale.p,
+            h->cexp.p,    h->stats.p,     h->bias.p,   h->cand.p,      h->x8.p,       h->row_scale.p,
             h->row_norm.p, h->part.p,     h->acc.keys, h->acc.keys_alt, h->acc.vals,  h->acc.vals_alt,
             h->acc.big,   h->acc.sort_tmp, h->acc.starts};
 }

@@ -347,6 +347,34 @@ def test_fp8_rows_of_extreme_magnitude(hip, mag):
     assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
 
 
+@pytest.mark.parametrize("d,split", [(64, 0), (64, 3), (256, 0), (600, 0)])
+def test_fp8_small_centroids_beside_large(hip, d, split):
+    """FP8 centroids of very different magnitudes: 128 of scale 1e3 and 128
+    of scale 1e-3, rows drawn near the small ones. Each centroid has its own
+    power-of-two e4m3 scale, so the small ones keep their 3 mantissa bits
+    (one scale from the largest |c| would flush every small centroid to zero
+    and leave their scores tied at |c|^2 / 2). Labels match FP64 brute force.
+    (d = 600 rows are wider than 512 bytes and take the LDS kernel.)"""
+    print(f"fp8 mixed-magnitude centroids d={d} split={split}", flush=True)
+    rng = np.random.default_rng(d + 5)
+    small = (rng.normal(size=(128, d)) * 1e-3).astype(np.float32)
+    big = (rng.normal(size=(128, d)) * 1e3).astype(np.float32)
+    c = np.empty((256, d), np.float32)
+    c[0::2], c[1::2] = small, big
+    pick = rng.integers(0, 128, size=2000)
+    x = (small[pick] + rng.normal(size=(2000, d)) * 1e-4).astype(np.float32)
+    a = E.HipAssigner("fp8", rescore_m=8)
+    a.set_split(split)
+    a.set_centroids(c)
+    labels, dist = a.assign(x)
+    bl, bd = O.brute_f64(x, c)
+    got = O.dist_f64(x, c, labels)
+    wrong = np.nonzero(got > bd + 1e-6 * (1 + bd))[0]
+    assert len(wrong) == 0, f"{len(wrong)} true label errors, e.g. rows {wrong[:10]}"
+    assert np.array_equal(bl, 2 * pick)
+    assert np.all(np.abs(dist - got) <= 1e-4 * (1 + got))
+
+
 def test_bf16_odd_shapes(hip):
     for n, k, d in SHAPES:
         x = O.lcg_f32(n * d, 1).reshape(n, d)
