@@ -141,7 +141,7 @@ static int search_launch(emh_assigner *h, int M, const void *x_dev, int rows, si
     const SearchGeom &g = h->g;
     h->last_tile = t;
     return check(launch_search(g, M, x_dev, rows, m, ids_dev, h->cen_op.p, h->hn.as<float>(), h->cn.as<int64_t>(), x8,
-                               rs, rn, h->cexp.as<int>(), h->stats.as<unsigned>(), labels_dev, dist_dev, cand,
+                               rs, rn, h->cexp.as<uint8_t>(), h->stats.as<unsigned>(), labels_dev, dist_dev, cand,
                                h->stream, t.ns, t.ns > 1 ? part : nullptr, t.fr),
                  "search kernel");
 }
@@ -483,7 +483,7 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
     if ((rc = check(h->norm64.ensure(g.k_pad * 8), "norm buffer"))) return rc;
     if ((rc = check(h->hn.ensure(g.k_pad * 4), "score buffer"))) return rc;
     if ((rc = check(h->cn.ensure(g.k_pad * 8), "score buffer"))) return rc;
-    if ((rc = check(h->cexp.ensure(g.k_pad * 4), "scale buffer"))) return rc;
+    if ((rc = check(h->cexp.ensure(g.k_pad), "scale buffer"))) return rc;
     if ((rc = check(h->stats.ensure(12), "stats buffer"))) return rc;
     h->g = g;
     h->has_centroids = false;
@@ -499,7 +499,7 @@ int set_centroids_impl(emh_assigner *h, const float *centroids, size_t k, size_t
     const bool fused = h->op == OP_F32 || (h->op == OP_I8 && (int64_t)d * 49152 < ((int64_t)1 << 26));
     if (!fused && (rc = check(launch_fill_u32(h->stats.as<uint32_t>(), 3, 0u, h->stream), "fill"))) return rc;
     if ((rc = check(launch_prep_centroids(g, h->cen(), h->cen_op.p, h->norm32.as<float>(), h->norm64.as<int64_t>(),
-                                          h->cexp.as<int>(), h->stats.as<unsigned>(), h->stream,
+                                          h->cexp.as<uint8_t>(), h->stats.as<unsigned>(), h->stream,
                                           fused && h->op == OP_F32 ? h->hn.as<float>() : nullptr,
                                           fused && h->op == OP_I8 ? h->cn.as<int64_t>() : nullptr),
                     "centroid prep")))

@@ -231,7 +231,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         (rc = check(hn.ensure(cpad * 4), "offsets")) ||
         (int_search && (rc = check(cn.ensure(cpad * 8), "integer offsets"))) ||
         (op == OP_FP8 && ((rc = check(x8_d.ensure(x8_bytes), "fp8 rows")) ||
-                          (rc = check(cexp_d.ensure(cpad * 4), "fp8 centroid scales")) ||
+                          (rc = check(cexp_d.ensure(cpad), "fp8 centroid scales")) ||
                           (rc = check(rscale_d.ensure(slots * 4), "fp8 row scales")) ||
                           (rc = check(rnorm_d.ensure(slots * 4), "fp8 row norms")))) ||
         (rc = check(stats_d.ensure(njobs * 16), "stats")) ||
@@ -359,7 +359,7 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
         t.norm32 = norm32.as<float>() + cpad_base[j];
         t.hn = hn.as<float>() + cpad_base[j];
         t.cn = int_search ? cn.as<int64_t>() + cpad_base[j] : nullptr;
-        t.cexp = op == OP_FP8 ? cexp_d.as<int>() + cpad_base[j] : nullptr;
+        t.cexp = op == OP_FP8 ? cexp_d.as<uint8_t>() + cpad_base[j] : nullptr;
         t.shift = (uint32_t)geo[j].shift;
         t.stats = stats_d.as<unsigned>() + 4 * j;
         t.labels = lab_d.as<uint32_t>() + row_base[j];

@@ -76,9 +76,9 @@ inline const void *reg_operand(const SearchGeom &g, const void *cen_op) {
 // cen_op holds operand_bytes(g): the prep launcher fills both layouts.
 // hn_out / cn_out: also write the unbiased score offsets (what
 // launch_set_bias without a bias writes) in the same pass.
-// cexp (FP8, k_pad ints): each centroid's MFMA scale (E8M0 of 1 / s_c).
+// cexp (FP8, k_pad bytes): each centroid's MFMA scale (E8M0 of 1 / s_c).
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 int *cexp, unsigned *stats, hipStream_t s, float *hn_out = nullptr,
+                                 uint8_t *cexp, unsigned *stats, hipStream_t s, float *hn_out = nullptr,
                                  int64_t *cn_out = nullptr);
 hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64_t *norm64, const float *bias,
                            float *hn, int64_t *cn, unsigned *stats, hipStream_t s);
@@ -93,7 +93,7 @@ bool reg_path(const SearchGeom &g);
 // labels/dist or cand. The results do not depend on nslice.
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const float *row_norm, const int *cexp,
+                         const float *row_scale, const float *row_norm, const uint8_t *cexp,
                          const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
                          int nslice = 1, uint32_t *part = nullptr, int fr = 2);
 int reg_tiles(const SearchGeom &g);
@@ -116,7 +116,7 @@ struct JobDev {
     float *norm32, *hn;      // k_pad each
     int64_t *cn;             // I8: k_pad integer score offsets sum c8^2 + round(bias)
     unsigned *stats;         // 4 words: [1] max -bias (f32 bits); the others unused
-    int *cexp;               // FP8: k_pad centroid MFMA scales (E8M0 of 1 / s_c)
+    uint8_t *cexp;           // FP8: k_pad centroid MFMA scales (E8M0 of 1 / s_c)
     uint32_t *labels;        // batch slot
     float *dist;             // batch slot (true distances once the keys kernel has run)
     uint32_t *labels_all;    // n: the last pass's label of every row (device-resident)

@@ -300,7 +300,7 @@ __device__ inline float centroid_fp8_scale(const float *cp, size_t d, bool live,
 template <int OP>
 __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, size_t k, size_t d, size_t d_pad,
                                                              size_t k_pad, uint8_t *cen_op, float *norm32,
-                                                             int64_t *norm64, int *cexp,
+                                                             int64_t *norm64, uint8_t *cexp,
                                                              unsigned *stats, uint8_t *tile_op, int tb,
                                                              float *hn_out, int64_t *cn_out) {
     constexpr int EB = OP == OP_F32 ? 4 : OP == OP_BF16 ? 2 : 1, EPQ = 16 / EB;  // element bytes, per quad
@@ -311,7 +311,7 @@ __global__ __launch_bounds__(256) void prep_centroids_kernel(const float *c, siz
     float cs = 1.0f;
     if constexpr (OP == OP_FP8) {
         cs = centroid_fp8_scale(cp, d, live, gl);
-        if (gl == 0) cexp[jc] = e8m0_inverse(cs);
+        if (gl == 0) cexp[jc] = (uint8_t)e8m0_inverse(cs);
     }
     // operand row: the float searches store -c, so that an MFMA accumulator
     // started at |c|^2/2 ends at |c|^2/2 - x.c
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             [[maybe_unused]] float cs = 1.0f;
             if constexpr (OP == OP_FP8) {
                 cs = centroid_fp8_scale(cp, d, live, gl);
-                if (gl == 0) jb.cexp[jc] = e8m0_inverse(cs);
+                if (gl == 0) jb.cexp[jc] = (uint8_t)e8m0_inverse(cs);
             }
             for (size_t q = gl; q < nq; q += 16) {
                 uint32_t w[4];
@@ -705,7 +705,7 @@ size_t slice_part_bytes(int M, size_t m, int nslice, int op) {
 
 hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, size_t m, const uint32_t *ids,
                          const void *cen_op, const float *hn, const int64_t *cn, const uint8_t *x8,
-                         const float *row_scale, const float *row_norm, const int *cexp,
+                         const float *row_scale, const float *row_norm, const uint8_t *cexp,
                          const unsigned *stats, uint32_t *labels, float *dist, uint32_t *cand, hipStream_t s,
                          int nslice, uint32_t *part, int fr) {
     if (m == 0) return hipSuccess;
@@ -796,7 +796,7 @@ hipError_t launch_rescore(int M, const void *x, int rows, size_t m, size_t d, co
 }
 
 hipError_t launch_prep_centroids(const SearchGeom &g, const float *c, void *cen_op, float *norm32, int64_t *norm64,
-                                 int *cexp, unsigned *stats, hipStream_t s, float *hn_out,
+                                 uint8_t *cexp, unsigned *stats, hipStream_t s, float *hn_out,
                                  int64_t *cn_out) {
     const dim3 grid((unsigned)((g.k_pad + PREP_CENTROIDS - 1) / PREP_CENTROIDS)), grid_wave((unsigned)((g.k_pad + 3) / 4));
     uint8_t *op = (uint8_t *)cen_op;

@@ -87,7 +87,7 @@ struct SearchArgs {
     const int64_t *cn;
     const uint8_t *x8;
     const float *row_scale;
-    const int *cexp;  // FP8: k_pad E8M0 MFMA scales, 1 / s_c of each centroid's encoding scale
+    const uint8_t *cexp;  // FP8: k_pad E8M0 MFMA scales, 1 / s_c of each centroid's encoding scale
     // [1]: max(0, -min_j bias_j) as f32 bits
     const unsigned *stats;
     const float *row_norm;  // FP8: |x|^2 of each call row, from the encoding pass
