@@ -35,7 +35,7 @@ typedef struct emh_assigner emh_assigner;
 typedef enum {
     EMH_FP32 = 0,          /* fused FP32 |x|^2 - 2x.c + |c|^2 argmin: the drop-in */
     EMH_BF16_RESCORE = 1,  /* BF16 MFMA, FP32 accumulate, top-m kept, FP32 rescore */
-    EMH_FP8_RESCORE = 2,   /* FP8-e4m3 (per-row scaled) MFMA, top-m, FP32 rescore */
+    EMH_FP8_RESCORE = 2,   /* FP8-e4m3 (per-row and per-centroid scaled) MFMA, top-m, FP32 rescore */
     EMH_I8 = 3,            /* i8 MFMA, int32 dots: bit-exact CpuI8Assigner(I8) */
     EMH_I16 = 4            /* i8 MFMA on split i16 centroids: CpuI8Assigner(I16) */
 } emh_precision;
