@@ -481,43 +481,15 @@ hipError_t launch_fill_f64(double *p, size_t count, hipStream_t s) {
 
 // ---- many jobs per launch ----
 
-// key of batch slot i: its job's first global cluster + label, or nclusters
-// for a slot past the job's rows this step (sorts last, never folded); the
-// slot's true distance (penalized jobs) and, in the last epoch, its row's
-// label. The first thread clears the big-cluster count for the sums.
+// key of batch slot i (slot_key); the first thread clears the big-cluster
+// count for the sums.
 __global__ void jobs_keys_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots, uint32_t nclusters,
                                  uint32_t *keys, uint32_t *big_count) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *big_count = 0;  // read by the sums after the sort
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < slots; i += (size_t)gridDim.x * blockDim.x) {
         const uint2 r = rmap[i];
-        const JobDev j = job_at<false>(jobs, r.x);
-        uint32_t key = nclusters;
-        if (r.y < j.m) {
-            const uint32_t l = j.labels[r.y];
-            if (l < j.k) {
-                key = j.cluster_base + l;
-                if (j.penalized) {
-                    // the true squared distance: the score less the bias
-                    const float v = __fsub_rn(j.dist[r.y], j.bias[l]);
-                    j.dist[r.y] = v > 0.0f ? v : 0.0f;
-                }
-            }
-            if (j.last_epoch) j.labels_all[j.ids[r.y]] = l;
-        }
-        keys[i] = key;
+        keys[i] = slot_key(job_at<false>(jobs, r.x), r.y, nclusters);
     }
-}
-
-// The last epoch's first batch zeroes its job's FP64 totals.
-__global__ void jobs_pass_clear_kernel(const JobDev *jobs, const uint32_t *cjob, size_t nclusters) {
-    const size_t g = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    if (g >= nclusters) return;
-    const JobDev j = job_at<true>(jobs, cjob[g]);
-    if (!j.clear_pass || j.m == 0) return;
-    const size_t c = g - j.cluster_base;
-    for (size_t t = lane; t < j.d; t += 64) j.pass_sums[c * j.d + t] = 0.0;
-    if (lane == 0) j.pass_counts[c] = 0;
 }
 
 // ---- batched per-cluster sums: the sorted members as one stream ----------
@@ -968,14 +940,12 @@ hipError_t launch_accumulate(AccumScratch &a, const void *x, int rows, size_t m,
 }
 
 hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                            const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                            bool keys_written, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
                             hipStream_t s) {
     hipError_t e;
     if ((e = accum_reserve(a, slots > 0 ? slots : 1, nclusters)) != hipSuccess) return e;
-    dim3 cgrid((unsigned)((nclusters + 3) / 4));
-    hipLaunchKernelGGL(jobs_pass_clear_kernel, cgrid, dim3(256), 0, s, jobs, cjob, nclusters);
     (void)counts_all;  // zero here: set up so, and left so by jobs_ema_kernel
-    if (slots == 0) return hipSuccess;
+    if (slots == 0 || keys_written) return hipSuccess;
     unsigned blocks = (unsigned)std::min<size_t>((slots + 255) / 256, 4096);
     hipLaunchKernelGGL(jobs_keys_kernel, dim3(blocks), dim3(256), 0, s, jobs, rmap, slots, (uint32_t)nclusters, keys,
                        a.big);

@@ -423,12 +423,15 @@ static int batch_impl(emh_assigner *h, emh_job *jobs, size_t njobs, emh_batch_st
     if (ce == hipSuccess && op != OP_FP8) ce = last_nodes(search_first);
     if (ce == hipSuccess)
         ce = launch_jobs_search_lds(op, M, sa_dev, (const uint2 *)(mp + m_lwg), wg_lds.size(), lds_max, h->stream);
-    if (ce == hipSuccess && cand_search) ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
+    // the candidate searches' rescore writes the sort keys in the same pass
+    if (ce == hipSuccess && cand_search)
+        ce = launch_jobs_rescore(M, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream, (uint32_t)nclusters,
+                                 keys.as<uint32_t>(), (uint32_t *)h->acc.big);
     if (ce == hipSuccess && !cand_search && part_words)
         ce = launch_jobs_merge(op, jobs_dev, (const uint2 *)(mp + m_r), slots, h->stream);
     if (ce == hipSuccess)
-        ce = launch_jobs_keys(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, (const uint32_t *)(mp + m_j),
-                              nclusters, keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
+        ce = launch_jobs_keys(h->acc, jobs_dev, (const uint2 *)(mp + m_r), slots, cand_search, nclusters,
+                              keys.as<uint32_t>(), counts_d.as<uint32_t>(), h->stream);
     // the distances are final here (true distances): an event record node
     // follows these nodes once the capture ends
     std::vector<hipGraphNode_t> dist_deps;

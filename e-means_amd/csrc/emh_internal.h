@@ -192,14 +192,44 @@ template <bool UNIFORM> __device__ inline JobDev job_at(const JobDev *jobs, size
     EMH_G(x8);
     EMH_G(row_scale);
     EMH_G(row_norm);
+    EMH_G(cexp);
 #undef EMH_G
     return j;
 }
 
+// The last epoch's first batch zeroes its job's FP64 totals: centroid c's
+// row, by the 16 lanes gl of the group that preps that centroid.
+__device__ inline void clear_pass_row(const JobDev &j, size_t c, int gl) {
+    if (!j.clear_pass || c >= j.k) return;
+    for (size_t t = gl; t < j.d; t += 16) j.pass_sums[c * j.d + t] = 0.0;
+    if (gl == 0) j.pass_counts[c] = 0;
+}
+
+// The sort key of a batch slot holding row `row` of job j's batch: the job's
+// first global cluster + the row's label, or nclusters for a slot past the
+// job's rows this step (sorts last, never folded). Once the labels are in:
+// a penalized job's distance becomes the true one (the score less the
+// bias), and in the last epoch the row's label is kept.
+__device__ inline uint32_t slot_key(const JobDev &j, uint32_t row, uint32_t nclusters) {
+    uint32_t key = nclusters;
+    if (row < j.m) {
+        const uint32_t l = j.labels[row];
+        if (l < j.k) {
+            key = j.cluster_base + l;
+            if (j.penalized) {
+                const float v = __fsub_rn(j.dist[row], j.bias[l]);
+                j.dist[row] = v > 0.0f ? v : 0.0f;
+            }
+        }
+        if (j.last_epoch) j.labels_all[j.ids[row]] = l;
+    }
+    return key;
+}
+
 // prep: every (job, centroid < k_pad) entry of cmap (cmap lists each job's
 // entries together); operand, norms, offsets with this step's bias, stats
-// (zeroed before). FP8 first takes each job's largest finite |c| (the
-// centroid encoding scale) in a pass of its own.
+// (zeroed before), FP8 centroid scales; in the last epoch's first batch the
+// centroid's FP64 pass totals are zeroed too (clear_pass_row).
 hipError_t launch_jobs_prep(int op, const JobDev *jobs, const uint2 *cmap, size_t entries, hipStream_t s);
 // FP8: every slot's batch row (rmap: (job, row) per slot) encoded as
 // rows_fp8_kernel encodes a single problem's rows.
@@ -215,8 +245,11 @@ hipError_t launch_jobs_search_lds(int op, int M, const void *search_args, const 
 // dynamic LDS of the LDS kernel for a geometry
 size_t search_lds_bytes(const SearchGeom &g);
 // rescoring of every job's candidates (merging a split job's slice lists
-// first): rmap holds (job, row) per batch slot.
-hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
+// first): rmap holds (job, row) per batch slot. With keys given, each slot's
+// sort key (slot_key) is written in the same pass and *big_count zeroed,
+// which launch_jobs_keys then leaves out.
+hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s,
+                               uint32_t nclusters = 0, uint32_t *keys = nullptr, uint32_t *big_count = nullptr);
 // FP32 and I8 jobs with a split search: each slot's slice results merged
 // into its label and distance.
 hipError_t launch_jobs_merge(int op, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s);
@@ -283,8 +316,10 @@ hipError_t launch_jobs_ema(const JobDev *jobs, const uint32_t *cjob, size_t nclu
 // counts_all (all zero on entry); penalized jobs' distances lose their bias (the
 // true squared distances, src/lib.rs:425-430); last-epoch labels are stored
 // at their rows in labels_all.
+// keys_written: the rescore pass has written the keys already (only the
+// scratch is reserved here).
 hipError_t launch_jobs_keys(AccumScratch &a, const JobDev *jobs, const uint2 *rmap, size_t slots,
-                            const uint32_t *cjob, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
+                            bool keys_written, size_t nclusters, uint32_t *keys, uint32_t *counts_all,
                             hipStream_t s);
 // Second half: one stable sort, then per-job member counts (counts_all,
 // zeroed by the first half), f32 sums and FP64 pass totals. rows: the jobs'

@@ -205,9 +205,13 @@ __global__ __launch_bounds__(256) void rescore_kernel(const void *x, size_t m, s
 // rmap[i].x (live when below that job's row count this step). (Giving each
 // XCD a contiguous run of slots, for L2 reuse of the candidates, measured
 // slower than this grid-stride order.)
+// keys (else nullptr): each slot's sort key too (slot_key, from the label
+// and distance this lane has just written), and *big_count zeroed.
 template <int M>
-__global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots) {
+__global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, const uint2 *rmap, size_t slots,
+                                                           uint32_t nclusters, uint32_t *keys, uint32_t *big_count) {
     const int gl = threadIdx.x & 15;
+    if (keys && blockIdx.x == 0 && threadIdx.x == 0) *big_count = 0;  // read by the sums after the sort
     const size_t stride = (size_t)gridDim.x * RESCORE_ROWS;
     for (size_t base = (size_t)blockIdx.x * RESCORE_ROWS; base < slots; base += stride) {
         const size_t i = base + (threadIdx.x >> 4);
@@ -221,6 +225,7 @@ __global__ __launch_bounds__(256) void rescore_jobs_kernel(const JobDev *jobs, c
         // shuffles are within the 16 lanes of the group only
         rescore_row<EMH_ROWS_F32, M>(j.x, j.d, j.k, j.ids, j.cen, j.penalized ? j.bias : nullptr, j.cand, j.labels,
                                      j.dist, r.y, live, gl, j.part, (int)j.nslice);
+        if (keys && gl == 0 && i < slots) keys[i] = slot_key(j, r.y, nclusters);
     }
 }
 
@@ -409,6 +414,7 @@ __global__ __launch_bounds__(256) void prep_jobs_kernel(const JobDev *jobs, cons
             const size_t jc = cm.y, d = jb.d;
             const bool live = jc < jb.k;
             const float *cp = jb.cen + jc * d;
+            clear_pass_row(jb, jc, gl);
             // tile-major: quad q of centroid jc lives in tile jc / tb at quad row q
             const size_t tb = jb.tb, tile = jc / tb, cl = jc - tile * tb;
             uint8_t *tbase = jb.cen_op + tile * tb * jb.row_bytes + cl * 16;
@@ -904,6 +910,7 @@ __global__ __launch_bounds__(256) void prep_jobs_i16_kernel(const JobDev *jobs, 
     const size_t jc = cm.y, d = jb.d, d_pad = jb.d_pad;
     const bool live = jc < jb.k;
     const float *cp = jb.cen + jc * d;
+    clear_pass_row(jb, jc, gl);
     const float scale = (float)(1u << jb.shift);
     uint8_t *o = jb.cen_op + jc * d_pad;
     const size_t plane = (size_t)jb.k_pad * d_pad;
@@ -971,14 +978,15 @@ hipError_t launch_jobs_search_lds(int op, int M, const void *search_args, const 
 
 size_t search_lds_bytes(const SearchGeom &g) { return search_lds(g); }
 
-hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s) {
+hipError_t launch_jobs_rescore(int M, const JobDev *jobs, const uint2 *rmap, size_t slots, hipStream_t s,
+                               uint32_t nclusters, uint32_t *keys, uint32_t *big_count) {
     if (!slots) return hipSuccess;
     dim3 grid((unsigned)std::min<size_t>((slots + RESCORE_ROWS - 1) / RESCORE_ROWS, 16384));
     switch (M) {
-        case 1: hipLaunchKernelGGL(rescore_jobs_kernel<1>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
-        case 2: hipLaunchKernelGGL(rescore_jobs_kernel<2>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
-        case 4: hipLaunchKernelGGL(rescore_jobs_kernel<4>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
-        case 8: hipLaunchKernelGGL(rescore_jobs_kernel<8>, grid, dim3(256), 0, s, jobs, rmap, slots); break;
+        case 1: hipLaunchKernelGGL(rescore_jobs_kernel<1>, grid, dim3(256), 0, s, jobs, rmap, slots, nclusters, keys, big_count); break;
+        case 2: hipLaunchKernelGGL(rescore_jobs_kernel<2>, grid, dim3(256), 0, s, jobs, rmap, slots, nclusters, keys, big_count); break;
+        case 4: hipLaunchKernelGGL(rescore_jobs_kernel<4>, grid, dim3(256), 0, s, jobs, rmap, slots, nclusters, keys, big_count); break;
+        case 8: hipLaunchKernelGGL(rescore_jobs_kernel<8>, grid, dim3(256), 0, s, jobs, rmap, slots, nclusters, keys, big_count); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
