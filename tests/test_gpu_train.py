@@ -221,6 +221,36 @@ def test_batched_training_equals_single(hip, prec):
         assert g.n_splits == ref.n_splits
 
 
+def test_batched_fp8_mixed_magnitudes(hip):
+    """FP8 jobs whose clusters sit at scales 1e3 and 1e-3 side by side: the
+    batched prep sets each centroid's own e4m3 scale as the single-problem
+    prep does (each job ends with exactly its single-problem result), and
+    the small clusters stay resolved: the final centroids' FP8 top-8 search
+    labels every small-cluster row as FP64 brute force does."""
+    problems = []
+    for i, (n, d, k) in enumerate([(4000, 32, 40), (3000, 64, 24)]):
+        rng = np.random.default_rng(300 + i)
+        small = rng.normal(size=(k // 2, d)) * 1e-3
+        big = rng.normal(size=(k // 2, d)) * 1e3
+        lab = rng.integers(0, k, n)
+        centres = np.concatenate([small, big])
+        noise = np.where((lab < k // 2)[:, None], 1e-4, 1e2) * rng.normal(size=(n, d))
+        problems.append(((centres[lab] + noise).astype(np.float32), k, E.Config(nepochs=2, seed=310 + i)))
+    print("batch fp8 mixed magnitudes", flush=True)
+    got, _ = E.train_batch(problems, E.HipAssigner("fp8", rescore_m=8))
+    for (x, k, cfg), g in zip(problems, got):
+        ref = E.train(x, k, cfg, E.HipAssigner("fp8", rescore_m=8))
+        assert np.array_equal(g.labels, ref.labels)
+        assert np.array_equal(g.centroids, ref.centroids)
+        rows = x[np.abs(x).max(axis=1) < 1.0]  # the small clusters' rows
+        a = E.HipAssigner("fp8", rescore_m=8)
+        a.set_centroids(g.centroids)
+        labels, _ = a.assign(rows)
+        _, bd = O.brute_f64(rows, g.centroids)
+        got_d = O.dist_f64(rows, g.centroids, labels)
+        assert len(rows) > 1000 and np.all(got_d <= bd + 1e-6 * (1 + bd))
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp8"])
 def test_batched_wide_rows_equal_single(hip, prec):
     """Jobs whose operand rows pass 512 bytes (FP32 d > 128, BF16 d > 256,
