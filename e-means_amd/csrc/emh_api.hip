@@ -311,7 +311,8 @@ int run_search(emh_assigner *h, const void *x_dev, int rows, size_t m, const uin
         if ((rc = check(h->cand.ensure(m * h->M * 4), "candidate buffer"))) return rc;
         b.cand = h->cand.as<uint32_t>();
     }
-    if (h->op == OP_FP8) {
+    // FP8: the encoded rows, unless the register-resident kernel encodes them itself
+    if (h->op == OP_FP8 && !fp8_encodes_in_search(g, rows)) {
         if ((rc = check(h->x8.ensure(m * g.d_pad), "fp8 row buffer"))) return rc;
         if ((rc = check(h->row_scale.ensure(m * 4), "row scale buffer"))) return rc;
         if ((rc = check(h->row_norm.ensure(m * 4), "row norm buffer"))) return rc;
@@ -403,7 +404,8 @@ int run_search_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, c
     noalloc_layout(h, m, t.ns, base, &b);
     int rc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    if (h->op == OP_FP8 && (rc = encode_rows(h, x_dev, rows, m, ids_dev, b))) return rc;
+    if (h->op == OP_FP8 && !fp8_encodes_in_search(h->g, rows) && (rc = encode_rows(h, x_dev, rows, m, ids_dev, b)))
+        return rc;
     return search_stages(h, x_dev, rows, m, ids_dev, labels_dev, dist_dev, b, t, ev, false);
 }
 

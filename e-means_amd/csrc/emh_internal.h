@@ -85,6 +85,12 @@ hipError_t launch_set_bias(const SearchGeom &g, const float *norm32, const int64
 hipError_t launch_rows_fp8(const void *x, int rows, size_t m, size_t d, size_t d_pad, const uint32_t *ids,
                            uint8_t *x8, float *row_scale, float *row_norm, hipStream_t s);
 bool reg_path(const SearchGeom &g);
+// FP8 on f32 rows through the register-resident kernel: the kernel encodes
+// its rows itself (no separate encoding pass, x8 / row_scale / row_norm
+// unused).
+inline bool fp8_encodes_in_search(const SearchGeom &g, int rows) {
+    return g.op == OP_FP8 && g.reg && rows == EMH_ROWS_F32;
+}
 // fr (register-resident path): 32-row fragments per wave, 1 or 2 (128 or 256
 // rows per workgroup); the results do not depend on it.
 // nslice > 1 (register-resident path only): the centroid tiles are split

@@ -753,6 +753,7 @@ hipError_t launch_search(const SearchGeom &g, int M, const void *x, int rows, si
         // the register-resident kernels read the tile-major copy
         SearchArgs r = a;
         r.cen_op = (const uint8_t *)reg_operand(g, cen_op);
+        r.enc = fp8_encodes_in_search(g, rows) ? 1 : 0;
         switch (g.op) {
             case OP_F32:
                 if (M != 1) return hipErrorInvalidValue;

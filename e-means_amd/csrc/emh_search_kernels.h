@@ -94,6 +94,7 @@ struct SearchArgs {
     int vec;                // rows are 16-byte aligned and d fills whole 16-byte quads
     int resident;           // LDS kernel: whole rows resident in LDS (else d-chunks)
     int shift;              // I16: score units are 4^-shift squared distance
+    int enc;                // FP8 register-resident kernel: encode the f32 rows x in the prologue (x8 unused)
     // Centroid-range split (register-resident search): workgroup (b, s)
     // sweeps tiles [s * tiles_per_slice, +tiles_per_slice) and, with more than
     // one slice, writes its per-row result to part[row * nslice + s] for
@@ -991,15 +992,67 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         const size_t r = row0 + (size_t)(wave * FR + fr) * 32 + j;
         const bool valid = r < a.m;
         const size_t src = valid ? (a.ids ? (size_t)a.ids[r] : r) : 0;
+        if (OP != OP_FP8 || !a.enc) {
 #pragma unroll
-        for (int q = 0; q < NS; q++) xb[fr][q] = row_segment<OP, ROWS>(a, r, src, (size_t)(h * HB + 16 * q) / ES, valid);
+            for (int q = 0; q < NS; q++) xb[fr][q] = row_segment<OP, ROWS>(a, r, src, (size_t)(h * HB + 16 * q) / ES, valid);
+        }
         // |x|^2 from the fragment just loaded (pad elements are zero): exact
         // for I8, the f32 values for FP32, the bf16-rounded values for BF16
         // (only the positivity offset uses it), the encoding pass's for FP8
         float part = 0.0f;
         int32_t ipart = 0, spart = 0;
         if constexpr (OP == OP_FP8) {
-            if (valid && h == 0) part = a.row_norm[r];
+            if (a.enc) {
+                // the row's e4m3 encoding as the encoding pass makes it: its
+                // largest finite |v| over both lane halves, the power-of-two
+                // scale, then this half's codes (the row read again, now from
+                // cache); |x|^2 for the positivity offset
+                const float *xr = (const float *)a.x + src * a.d;
+                float mx = 0.0f, sq = 0.0f;
+                auto quad = [&](size_t e, float (&v)[4]) {
+                    if (a.vec && e + 4 <= a.d) {
+                        const float4 t = *(const float4 *)(xr + e);
+                        v[0] = t.x, v[1] = t.y, v[2] = t.z, v[3] = t.w;
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < 4; u++) v[u] = e + u < a.d ? xr[e + u] : 0.0f;
+                    }
+                };
+                if (valid) {
+#pragma unroll
+                    for (int q = 0; q < 4 * NS; q++) {
+                        float v[4];
+                        quad((size_t)(h * HB + 4 * q), v);
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float av = fabsf(v[u]);
+                            if (av > mx && isfinite(av)) mx = av;
+                            sq = fmaf(v[u], v[u], sq);
+                        }
+                    }
+                }
+                mx = fmaxf(mx, __shfl_xor(mx, 32, 64));
+                const float s = fp8_scale_for(mx);
+#pragma unroll
+                for (int q = 0; q < NS; q++) {
+                    uint32_t w[4] = {0u, 0u, 0u, 0u};
+                    if (valid) {
+#pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            float v[4];
+                            quad((size_t)(h * HB + 16 * q + 4 * u), v);
+                            int pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[0] * s, v[1] * s, 0, false);
+                            pk = __builtin_amdgcn_cvt_pk_fp8_f32(v[2] * s, v[3] * s, pk, true);
+                            w[u] = (uint32_t)pk;
+                        }
+                    }
+                    xb[fr][q] = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+                part = sq;  // halves summed below
+                sb[fr] = valid ? e8m0_inverse(s) : 127;
+            } else if (valid && h == 0) {
+                part = a.row_norm[r];
+            }
         } else {
 #pragma unroll
             for (int q = 0; q < NS; q++) {
@@ -1030,11 +1083,13 @@ __device__ __forceinline__ void search_reg_core(const SearchArgs &a, const size_
         xs128[fr] = 0u;
         if constexpr (OP == OP_I16) xs128[fr] = 128u * (uint32_t)(spart + __shfl_xor(spart, 32, 64));
         nmax = fmaxf(nmax, part);
-        sb[fr] = 127;
+        if (OP != OP_FP8 || !a.enc) sb[fr] = 127;
         if constexpr (OP == OP_FP8) {
             // E8M0 scale 1 / s_row: undoes the row's power-of-two encoding scale
-            const uint32_t eb = valid ? ((__float_as_uint(a.row_scale[r]) >> 23) & 0xff) : 127u;
-            sb[fr] = 254 - (int)eb;
+            if (!a.enc) {
+                const uint32_t eb = valid ? ((__float_as_uint(a.row_scale[r]) >> 23) & 0xff) : 127u;
+                sb[fr] = 254 - (int)eb;
+            }
         }
     }
     float B = 0.0f;
