@@ -359,7 +359,7 @@ static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
 // Scratch layout for m rows and ns slices: cand | x8 | row scales | row
 // norms | slice results; returns the bytes used.
-static size_t noalloc_layout(const emh_assigner *h, size_t m, int ns, uint8_t *base, RowScratch *b) {
+static size_t noalloc_layout(const emh_assigner *h, int rows, size_t m, int ns, uint8_t *base, RowScratch *b) {
     const SearchGeom &g = h->g;
     size_t off = 0;
     auto take = [&](size_t bytes) -> uint8_t * {
@@ -368,7 +368,7 @@ static size_t noalloc_layout(const emh_assigner *h, size_t m, int ns, uint8_t *b
         return p;
     };
     if (h->op == OP_BF16 || h->op == OP_FP8) b->cand = (uint32_t *)take(m * h->M * 4);
-    if (h->op == OP_FP8) {
+    if (h->op == OP_FP8 && !fp8_encodes_in_search(g, rows)) {  // else the kernel encodes its rows
         b->x8 = take(m * g.d_pad);
         b->rs = (float *)take(m * 4);
         b->rn = (float *)take(m * 4);
@@ -383,7 +383,7 @@ int run_search_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, c
     uint8_t *base = h->device >= 0 && h->device < MAX_DEVICES ? (uint8_t *)g_scratch[h->device] : nullptr;
     if (!base) return fail(EMH_E_STATE, "no-allocation search: the scratch array of this device is not set up");
     RowScratch probe;
-    const size_t need1 = noalloc_layout(h, m, 1, nullptr, &probe);
+    const size_t need1 = noalloc_layout(h, rows, m, 1, nullptr, &probe);
     if (need1 > NOALLOC_SCRATCH_BYTES) {
         const size_t per_row = need1 / m + 1;
         return fail(EMH_E_ARG, "no-allocation search: " + std::to_string(m) + " rows need " + std::to_string(need1) +
@@ -398,10 +398,10 @@ int run_search_noalloc(emh_assigner *h, const void *x_dev, int rows, size_t m, c
         const int per = (nt + t.ns - 1) / t.ns;
         t.ns = per > 0 ? (nt + per - 1) / per : 1;  // the slices launch_search forms
         RowScratch p2;
-        if (!h->g.reg || noalloc_layout(h, m, t.ns, nullptr, &p2) > NOALLOC_SCRATCH_BYTES) t.ns = 1;
+        if (!h->g.reg || noalloc_layout(h, rows, m, t.ns, nullptr, &p2) > NOALLOC_SCRATCH_BYTES) t.ns = 1;
     }
     RowScratch b;
-    noalloc_layout(h, m, t.ns, base, &b);
+    noalloc_layout(h, rows, m, t.ns, base, &b);
     int rc;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     if (h->op == OP_FP8 && !fp8_encodes_in_search(h->g, rows) && (rc = encode_rows(h, x_dev, rows, m, ids_dev, b)))

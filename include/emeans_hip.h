@@ -103,8 +103,9 @@ int emh_synchronize(emh_assigner *h);
  * row-side buffers (candidate lists, FP8 row encodings, centroid-split
  * results) live in one static device array of EMH_NOALLOC_SCRATCH_BYTES per
  * GPU, held until the process ends. A call whose rows need more than that
- * fails with EMH_E_ARG (BF16 top-4: 16 bytes per row; FP8 top-8 at d = 256:
- * 296 bytes per row; FP32 / I8 / I16 need none unless split). Every call on
+ * fails with EMH_E_ARG (BF16 top-4: 16 bytes per row; FP8 top-8: 32 bytes
+ * per row, plus d_pad + 8 for rows over 512 bytes or i8/u8 rows, whose
+ * encodings are staged; FP32 / I8 / I16 need none unless split). Every call on
  * one GPU shares that GPU's array: calls that overlap on one GPU (other
  * streams, other threads, other handles) race, so order them. No timing
  * events are recorded. */

@@ -91,11 +91,12 @@ def test_noalloc_allocates_nothing(hip):
 
 
 def test_noalloc_rejects_rows_past_the_scratch(hip):
-    """FP8 top-8 at d = 512 needs 552 bytes per row: 200,000 rows exceed
-    the 64 MiB scratch and are refused before any launch."""
+    """FP8 top-8 at d = 600 (rows over 512 bytes, encoded into the scratch)
+    needs 32 + 640 + 8 bytes per row: 200,000 rows exceed the 64 MiB
+    scratch and are refused before any launch."""
     import torch
 
-    n, d = 200_000, 512
+    n, d = 200_000, 600
     x = torch.zeros(n, d, device="cuda")
     labels = torch.empty(n, dtype=torch.int32, device="cuda")
     dist = torch.empty(n, dtype=torch.float32, device="cuda")
